@@ -1,0 +1,229 @@
+// Harness around the reference CPU simplifier (ParaFROST src/cpu).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is compiled together with the
+// reference's own src/cpu sources by oracle/build_ref.sh into
+// oracle/_ref/pfref.  It is used in this container to (1) validate the
+// oracle restatement bit-for-bit, (2) write the golden fixtures under
+// tests/golden/, and (3) time the reference CPU simplifier.  Nothing in
+// the product library links or calls it.
+//
+// It drives the reference's own simplifier stages in the exact order of
+// Solver::sigmifying (src/cpu/simplify.cpp:157-210) so that the simplified
+// clause list, the new root units, the eliminated-variable stack and the
+// variable states can be dumped before the host write-back
+// (newBeginning / map, simplify.cpp:227-228) consumes them.  All the
+// simplification work is done by the reference functions (createOT, prop,
+// LCVE, sortOT, SUB, VE, BCE, ERE, countAll, filterPVs, shrinkSimp).
+//
+// Usage: pfref <in.cnf> <input.pfd> <output.pfd> [reference options...]
+//   input.pfd : the simplifier input (clauses after parse + root BCP +
+//               rootify/shrinkTop, simplify.cpp:163-164)
+//   output.pfd: the simplifier output (see oracle/pfd_format.md)
+// Per-stage CPU times (seconds) are printed on stdout as one JSON line.
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <map>
+#include <iostream>
+#include <fstream>
+#include <sstream>
+
+#define protected public
+#define private public
+#include "solver.h"
+#undef protected
+#undef private
+
+bool quiet_en = true;
+bool competition_en = false;
+int verbose = 0;
+
+using namespace ParaFROST;
+
+namespace {
+
+std::string g_out_path;
+bool g_done = false;
+uint32_t g_maxvar = 0;
+
+void put_u32(FILE* f, uint32_t v) { fwrite(&v, 4, 1, f); }
+void put_u64(FILE* f, uint64_t v) { fwrite(&v, 8, 1, f); }
+
+// Section writer shared by input and output dumps.  Clause flags byte:
+// bit0 learnt, bit1 added (resolvent), bits 4-5 usage.
+struct ClauseDump {
+  std::vector<uint32_t> sizes, lbds;
+  std::vector<uint8_t> flags;
+  std::vector<uint32_t> lits;
+};
+
+void write_pfd(const std::string& path, uint32_t status, uint32_t maxvar,
+               const ClauseDump& cd, const std::vector<uint32_t>& units,
+               const std::vector<uint32_t>& resolved,
+               const std::vector<uint8_t>& vstate) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) { perror("pfref: open output"); exit(2); }
+  fwrite("PFD1", 1, 4, f);
+  put_u32(f, status);
+  put_u32(f, maxvar);
+  put_u64(f, cd.sizes.size());
+  fwrite(cd.sizes.data(), 4, cd.sizes.size(), f);
+  fwrite(cd.flags.data(), 1, cd.flags.size(), f);
+  fwrite(cd.lbds.data(), 4, cd.lbds.size(), f);
+  put_u64(f, cd.lits.size());
+  fwrite(cd.lits.data(), 4, cd.lits.size(), f);
+  put_u64(f, units.size());
+  fwrite(units.data(), 4, units.size(), f);
+  put_u64(f, resolved.size());
+  fwrite(resolved.data(), 4, resolved.size(), f);
+  put_u64(f, vstate.size());
+  fwrite(vstate.data(), 1, vstate.size(), f);
+  fclose(f);
+}
+
+void on_exit_unsat() {
+  // The reference kills the process (learnEmpty + killSolver) when a
+  // simplification stage proves the formula UNSAT.  Record that outcome.
+  if (g_done || g_out_path.empty()) return;
+  ClauseDump empty;
+  std::vector<uint32_t> none;
+  std::vector<uint8_t> nov;
+  write_pfd(g_out_path, 0u, g_maxvar, empty, none, none, nov);
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    fprintf(stderr, "usage: pfref <in.cnf> <input.pfd> <output.pfd> [options]\n");
+    return 2;
+  }
+  std::string cnf = argv[1], in_path = argv[2];
+  g_out_path = argv[3];
+  std::vector<char*> av;
+  av.push_back(argv[0]);
+  av.push_back(argv[1]);
+  for (int i = 4; i < argc; i++) av.push_back(argv[i]);
+  int ac = (int)av.size();
+  parseArguments(ac, av.data());
+  quiet_en = true;
+  verbose = 0;
+  atexit(on_exit_unsat);
+
+  Solver* s = new Solver(cnf);  // parse + root BCP (src/cpu/solver.cpp:28-48)
+  solver = s;
+  g_maxvar = inf.maxVar;
+
+  // --- simplify.cpp:163-164: root-level cleanup before extraction
+  s->rootify();
+  s->shrinkTop(false);
+
+  // --- dump the simplifier input (the clauses extract() will read)
+  {
+    ClauseDump cd;
+    for (uint32_t i = 0; i < s->orgs.size(); i++) {
+      const C_REF r = s->orgs[i];
+      if (s->cm.deleted(r)) continue;
+      CLAUSE& c = s->cm[r];
+      cd.sizes.push_back(c.size());
+      cd.flags.push_back(0);
+      cd.lbds.push_back(0);
+      for (int k = 0; k < c.size(); k++) cd.lits.push_back(c[k]);
+    }
+    std::vector<uint32_t> units;
+    for (uint32_t i = 0; i < s->trail.size(); i++) units.push_back(s->trail[i]);
+    std::vector<uint8_t> vst(inf.maxVar + 1, 0);
+    for (uint32_t v = 1; v <= inf.maxVar; v++) vst[v] = s->sp->vstate[v].state;
+    std::vector<uint32_t> none;
+    write_pfd(in_path, 2u, inf.maxVar, cd, units, none, vst);
+  }
+  const uint32_t trail_start = s->trail.size();
+
+  std::map<std::string, double> t;
+  auto T = [&](const char* k, double d) { t[k] += d; };
+  double t0 = now_s(), a;
+  uint32_t status = 2;  // UNSOLVED
+  int rounds = 0;
+  std::vector<uint32_t> elected_per_round;
+
+  // --- sigmify (simplify.cpp:115-129) -> sigmifying (simplify.cpp:157-210)
+  if (!s->orgs.empty()) {
+    s->stats.sigma.calls++;
+    a = now_s(); s->awaken(); T("extract", now_s() - a);
+    if (s->simpstate == AWAKEN_FAIL) { fprintf(stderr, "pfref: AWAKEN_FAIL\n"); return 3; }
+    int64 litsbefore = inf.nLiterals, diff = INT64_MAX;
+    while (inf.nClauses && inf.nLiterals) {
+      a = now_s(); s->resizeCNF(); T("gc", now_s() - a);
+      a = now_s(); s->createOT(); T("ot", now_s() - a);
+      a = now_s();
+      if (!s->prop()) { status = 0; break; }
+      T("bcp", now_s() - a);
+      a = now_s(); bool ok = s->LCVE(); T("lcve", now_s() - a);
+      elected_per_round.push_back(s->PVs.size());
+      rounds++;
+      if (!ok) break;
+      a = now_s(); s->sortOT(); T("sort", now_s() - a);
+      if (s->stop(diff)) { a = now_s(); s->ERE(); T("ere", now_s() - a); break; }
+      a = now_s(); s->SUB(); T("sub", now_s() - a);
+      a = now_s(); s->VE(); T("ve", now_s() - a);
+      a = now_s(); s->BCE(); T("bce", now_s() - a);
+      a = now_s(); s->countAll(); s->filterPVs(); T("count", now_s() - a);
+      inf.nClauses = inf.n_cls_after, inf.nLiterals = inf.n_lits_after;
+      diff = litsbefore - inf.nLiterals, litsbefore = inf.nLiterals;
+      s->phase++, s->mu_inc++;
+      s->mu_inc += s->phase == s->opts.phases;
+    }
+    if (status) {
+      a = now_s();
+      if (!s->prop()) status = 0;
+      T("bcp", now_s() - a);
+    }
+    if (status) {
+      s->occurs.clear(true), s->ot.clear(true);
+      a = now_s(); s->countFinal(); s->shrinkSimp(); T("gc", now_s() - a);
+    }
+  }
+  double total = now_s() - t0;
+
+  ClauseDump cd;
+  std::vector<uint32_t> units, resolved;
+  std::vector<uint8_t> vst;
+  if (status) {
+    for (uint32_t i = 0; i < s->scnf.size(); i++) {
+      SCLAUSE& c = *s->scnf[i];
+      cd.sizes.push_back(c.size());
+      uint8_t fl = (c.learnt() ? 1 : 0) | (c.added() ? 2 : 0) | ((c.usage() & 3) << 4);
+      cd.flags.push_back(fl);
+      cd.lbds.push_back(c.learnt() ? c.lbd() : 0);
+      for (int k = 0; k < c.size(); k++) cd.lits.push_back(c[k]);
+    }
+    for (uint32_t i = trail_start; i < s->trail.size(); i++) units.push_back(s->trail[i]);
+    for (uint32_t i = 0; i < s->model.resolved.size(); i++) resolved.push_back(s->model.resolved[i]);
+    vst.assign(inf.maxVar + 1, 0);
+    for (uint32_t v = 1; v <= inf.maxVar; v++) vst[v] = s->sp->vstate[v].state;
+    // all clauses removed => SAT (simplify.cpp:217-226)
+    if (!inf.unassigned || !inf.nClauses) status = 1;
+  }
+  write_pfd(g_out_path, status, inf.maxVar, cd, units, resolved, vst);
+  g_done = true;
+
+  printf("{\"total_s\": %.6f, \"rounds\": %d, \"elected\": [", total, rounds);
+  for (size_t i = 0; i < elected_per_round.size(); i++)
+    printf("%s%u", i ? ", " : "", elected_per_round[i]);
+  printf("]");
+  for (auto& kv : t) printf(", \"%s_s\": %.6f", kv.first.c_str(), kv.second);
+  printf("}\n");
+  fflush(stdout);
+  _exit(0);  // skip the reference destructors' logging
+}

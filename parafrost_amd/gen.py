@@ -1,0 +1,167 @@
+"""Seeded synthetic CNF workloads (host C++ generators in libpfgen.so).
+
+Formulas are returned as ``Cnf`` objects holding numpy arrays in the
+simplifier encoding (literal = 2*v + sign, variables 1..maxvar).  The five
+workload families are the uniform random k-SAT, the Tseitin-encoded random
+circuit, the heavy-tailed k-SAT, batches of the latter two, and BMC-style
+unrollings (see ``parafrost_amd.bmc``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_LIB = None
+BASE_SEED = 0x5EED0000
+
+
+class _PfgCnf(ctypes.Structure):
+    _fields_ = [
+        ("maxvar", ctypes.c_uint32),
+        ("nclauses", ctypes.c_uint64),
+        ("nlits", ctypes.c_uint64),
+        ("offsets", ctypes.POINTER(ctypes.c_uint64)),
+        ("lits", ctypes.POINTER(ctypes.c_uint32)),
+    ]
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpfgen.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"libpfgen.so not built ({path}); run __graft_entry__.build()")
+        lib = ctypes.CDLL(path)
+        P = ctypes.POINTER(_PfgCnf)
+        lib.pfg_uniform_ksat.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, P]
+        lib.pfg_tseitin.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_double, ctypes.c_double, P]
+        lib.pfg_heavytail.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_uint32, P]
+        lib.pfg_prepare.argtypes = [P, P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.pfg_write_dimacs.argtypes = [P, ctypes.c_char_p]
+        lib.pfg_free.argtypes = [P]
+        _LIB = lib
+    return _LIB
+
+
+@dataclass
+class Cnf:
+    """A clause list in the simplifier encoding."""
+
+    maxvar: int
+    offsets: np.ndarray  # uint64, nclauses + 1
+    lits: np.ndarray  # uint32
+    flags: np.ndarray | None = None  # uint8 per clause: bit0 learnt, bits 4-5 usage
+    lbd: np.ndarray | None = None  # uint32 per clause
+    trail: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))  # root units
+
+    @property
+    def nclauses(self) -> int:
+        return int(self.offsets.shape[0] - 1)
+
+    @property
+    def nlits(self) -> int:
+        return int(self.lits.shape[0])
+
+    def clause(self, i: int) -> np.ndarray:
+        return self.lits[self.offsets[i]:self.offsets[i + 1]]
+
+
+def _take(c: _PfgCnf) -> Cnf:
+    n = int(c.nclauses)
+    off = np.ctypeslib.as_array(c.offsets, shape=(n + 1,)).copy()
+    lits = np.ctypeslib.as_array(c.lits, shape=(max(int(c.nlits), 1),))[: int(c.nlits)].copy()
+    out = Cnf(int(c.maxvar), off, lits)
+    _lib().pfg_free(ctypes.byref(c))
+    return out
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise ValueError(f"{what} failed with code {rc}")
+
+
+def uniform_ksat(nvars: int, nclauses: int, k: int = 3, seed: int = BASE_SEED + 1) -> Cnf:
+    c = _PfgCnf()
+    _check(_lib().pfg_uniform_ksat(seed, nvars, nclauses, k, ctypes.byref(c)), "uniform_ksat")
+    return _take(c)
+
+
+def tseitin(ngates: int, ninputs: int = 5000, noutputs: int = 1000, seed: int = BASE_SEED + 2,
+            pand: float = 0.6, pxor: float = 0.3) -> Cnf:
+    c = _PfgCnf()
+    _check(_lib().pfg_tseitin(seed, ninputs, ngates, noutputs, pand, pxor, ctypes.byref(c)), "tseitin")
+    return _take(c)
+
+
+def heavytail(nvars: int, nclauses: int, alpha: float = 2.0 / 3.0, maxlen: int = 30,
+              seed: int = BASE_SEED + 3) -> Cnf:
+    c = _PfgCnf()
+    _check(_lib().pfg_heavytail(seed, nvars, nclauses, alpha, maxlen, ctypes.byref(c)), "heavytail")
+    return _take(c)
+
+
+def prepare(cnf: Cnf) -> tuple[Cnf, bool]:
+    """Root-level unit propagation and cleanup, as a host solver does before
+    calling the simplifier.  Returns (prepared formula, refuted)."""
+    src = _PfgCnf(cnf.maxvar, cnf.nclauses, cnf.nlits,
+                  cnf.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                  cnf.lits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    dst = _PfgCnf()
+    trail = np.zeros(cnf.maxvar + 1, np.uint32)
+    ntrail = ctypes.c_uint32(0)
+    rc = _lib().pfg_prepare(ctypes.byref(src), ctypes.byref(dst), None,
+                            trail.ctypes.data, ctypes.addressof(ntrail))
+    out = _take(dst)
+    out.trail = trail[: ntrail.value].copy()
+    return out, rc == 1
+
+
+def write_dimacs(cnf: Cnf, path: str) -> None:
+    src = _PfgCnf(cnf.maxvar, cnf.nclauses, cnf.nlits,
+                  cnf.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                  cnf.lits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    _check(_lib().pfg_write_dimacs(ctypes.byref(src), path.encode()), "write_dimacs")
+
+
+def from_clauses(maxvar: int, clauses) -> Cnf:
+    """Build a Cnf from a list of DIMACS-style signed integer clauses."""
+    off = [0]
+    lits = []
+    for c in clauses:
+        for x in c:
+            lits.append(2 * abs(x) + (1 if x < 0 else 0))
+        off.append(len(lits))
+    return Cnf(maxvar, np.array(off, np.uint64), np.array(lits, np.uint32))
+
+
+# workload definitions -------------------------------------------------------
+def workload(name: str, scale: float = 1.0, seed: int | None = None) -> Cnf:
+    """The named workloads at full size (scale=1) or scaled down.
+
+    uniform3  : uniform random 3-SAT, 20,000 vars, 85,200 clauses
+    tseitin   : 2M-gate random circuit (60% AND, 30% XOR, 10% ITE), 5,000
+                inputs, 1,000 unit outputs (root-propagated)
+    heavytail : 10M vars, 40M clauses, length 2+Geom(1/2) capped at 30,
+                variable weights rank^(-2/3)
+    """
+    if name == "uniform3":
+        n = max(10, int(20000 * scale))
+        return uniform_ksat(n, int(round(n * 4.26)), 3, seed if seed is not None else BASE_SEED + 1)
+    if name == "tseitin":
+        g = max(100, int(2_000_000 * scale))
+        ni = max(10, int(5000 * min(1.0, scale * 10)))
+        no = max(1, min(g, int(1000 * scale)))
+        raw = tseitin(g, ni, no, seed if seed is not None else BASE_SEED + 2)
+        cnf, unsat = prepare(raw)
+        if unsat:
+            raise RuntimeError("tseitin workload refuted at the root")
+        return cnf
+    if name == "heavytail":
+        v = max(100, int(10_000_000 * scale))
+        return heavytail(v, 4 * v, seed=seed if seed is not None else BASE_SEED + 3)
+    raise KeyError(name)
