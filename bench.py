@@ -1,0 +1,199 @@
+"""Benchmark of the MI355X simplifier: one full inprocessing round.
+
+A step is one simplify call with phases=1 and BCE on, i.e. one round of
+SUB + BVE (with gate detection) + BCE followed by the stopping round's ERE
+(src/cpu/simplify.cpp:184-197), on a formula already resident in HBM: the
+device copy of the input is restored (pfs_rewind, device-to-device) and
+simplified inside the timed region.  The default workload is the
+Tseitin-encoded random circuit with 2M gates (60% AND, 30% XOR, 10% ITE,
+5,000 inputs, 1,000 unit outputs, root-propagated: about 6.8M clauses).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tseitin|uniform3|heavytail]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "simplify-round time & clauses/s @1 GPU; batched formulas/s + p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+OPTIONS = {"phases": 1, "bce": 1}
+REF_OPTIONS = ["--phases=1", "-bce"]
+
+
+def kernel_bytes(name, rounds, cnf):
+    """Algorithmic bytes per launch of the clause-parallel kernels: each reads
+    the live clause headers (offset, size: 8 B; status: 1 B per slot) and
+    literals (4 B) once and writes its output once.  Counts per launch are the
+    live clauses/literals at the start of the round the launch belongs to."""
+    c0, l0 = cnf.nclauses, cnf.nlits
+    starts = [(c0, l0)] + [(r["clauses_after"], r["literals_after"]) for r in rounds[:-1]]
+    V = cnf.maxvar
+    per = []
+    for c, l in starts:
+        if name == "hist_k":      # read headers + literals, write the 2V histogram
+            per.append(9 * c + 4 * l + 8 * V)
+        elif name == "scatter_k":  # read headers + literals + list offsets, write refs
+            per.append(9 * c + 4 * l + 8 * V + 4 * l)
+        elif name == "count_k":
+            per.append(9 * c)
+        elif name == "bcp_clean_k":
+            per.append(c)
+        else:
+            return None
+    return sum(per) / len(per)
+
+
+def cpu_baseline(cnf, name, budget_clauses=7_500_000):
+    """The reference CPU simplifier (compiled from its sources into
+    oracle/_ref/pfref) on the same workload, or a bounded sample of it, on one
+    host core; falls back to the oracle restatement when the reference
+    binary is absent."""
+    from parafrost_amd import gen
+    sample = cnf
+    note = "full workload"
+    if cnf.nclauses > budget_clauses:
+        scale = budget_clauses / cnf.nclauses
+        sample = gen.workload(name, scale=scale)
+        note = f"{name} workload scaled by {scale:.3f} ({sample.nclauses} clauses)"
+    ref = os.path.join(ROOT, "oracle", "_ref", "pfref")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "w.cnf")
+            # the reference runs its own parser and root propagation on the
+            # raw formula; the prepared formula has the same clauses and units
+            raw = sample
+            off_units = list(raw.trail)
+            if off_units:
+                import numpy as np
+                lits = np.concatenate([raw.lits, np.array(off_units, np.uint32)])
+                off = np.concatenate([raw.offsets, raw.offsets[-1] + 1 + np.arange(len(off_units), dtype=np.uint64)])
+                raw = gen.Cnf(raw.maxvar, off, lits)
+            gen.write_dimacs(raw, p)
+            r = subprocess.run(["taskset", "-c", "0", ref, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd")]
+                               + REF_OPTIONS, capture_output=True, text=True, timeout=600)
+            line = [x for x in r.stdout.splitlines() if x.startswith("{")][-1]
+            t = json.loads(line)["total_s"]
+        return {"value": sample.nclauses / t, "unit": "clauses/s", "cores": 1, "kind": "reference",
+                "sample": note + f"; reference CPU simplifier (ParaFROST src/cpu) round time {t:.3f} s, taskset 1 core",
+                "seconds": t}
+    from oracle import oracle
+    r = oracle.simplify(sample.maxvar, sample.offsets, sample.lits, nassigned=int(sample.trail.shape[0]), det=0,
+                        phases=1, bce_en=1)
+    return {"value": sample.nclauses / r.seconds, "unit": "clauses/s", "cores": 1, "kind": "port",
+            "sample": note + f"; oracle restatement round time {r.seconds:.3f} s", "seconds": r.seconds}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tseitin", choices=["tseitin", "uniform3", "heavytail"])
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(formulas shard by independent instances or elected variables; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from parafrost_amd import gen
+    from parafrost_amd.simplifier import Simplifier
+
+    t0 = time.time()
+    cnf = gen.workload(args.workload, scale=args.scale)
+    gen_s = time.time() - t0
+    print(f"bench: workload {args.workload} scale {args.scale}: {cnf.maxvar} vars, {cnf.nclauses} clauses, "
+          f"{cnf.nlits} literals ({gen_s:.1f}s to generate)", file=sys.stderr, flush=True)
+
+    s = Simplifier(0, keep_input=True, **OPTIONS)
+    s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+    status = None
+    for _ in range(args.warmup):
+        s.rewind()
+        status = s.simplify()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.rewind()
+        status = s.simplify()
+    sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    res = s.result(status)
+    print(f"bench: {args.steps} steps, {dt * 1e3:.2f} ms/step, result {res.status}, "
+          f"{res.sizes.shape[0]} clauses out, rounds {res.rounds}", file=sys.stderr, flush=True)
+
+    # one profiled step: per-kernel device time from HIP events on the
+    # context's stream
+    s.set_option("profilegpu", 1)
+    s.rewind()
+    s.simplify()
+    kt = s.kernel_times()
+    stages = s.stage_times()
+    s.set_option("profilegpu", 0)
+    rounds = s.result(status).rounds
+    dom = max(kt.items(), key=lambda kv: kv[1][0], default=None)
+    print("bench: kernels " + json.dumps({k: [round(v[0], 3), v[1]] for k, v in
+                                          sorted(kt.items(), key=lambda kv: -kv[1][0])[:12]}),
+          file=sys.stderr, flush=True)
+    print("bench: stages " + json.dumps({k: [round(v[0], 3), v[1]] for k, v in stages.items()}),
+          file=sys.stderr, flush=True)
+    hbm = max(((k, v) for k, v in kt.items() if kernel_bytes(k, rounds, cnf)), key=lambda kv: kv[1][0],
+              default=None)
+    roof_kernel = dom if dom and kernel_bytes(dom[0], rounds, cnf) else hbm
+    roofline = None
+    if roof_kernel:
+        name, (ms, n) = roof_kernel
+        avg_s = ms / n / 1e3
+        b = kernel_bytes(name, rounds, cnf)
+        ach = b / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(b),
+                    "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
+    s.close()
+
+    base = None
+    if not args.no_cpu_baseline:
+        try:
+            base = cpu_baseline(cnf, args.workload)
+        except Exception as e:  # the baseline is reported, never required
+            print(f"bench: cpu baseline failed: {e}", file=sys.stderr)
+    value = cnf.nclauses / dt
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "clauses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} (scale {args.scale})", "variables": cnf.maxvar,
+                   "clauses": cnf.nclauses, "literals": cnf.nlits, "options": "phases=1 bce=1 (SUB+BVE+BCE, ERE)",
+                   "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/* pfsimp.h -- C ABI of the MI355X-native GPU simplifier (SIGmA inprocessing
+ * engine).  A drop-in for the reference's simplifier boundary: the member
+ * Solver::simplify() of the GPU build (src/gpu/simplify.cu:57-75,
+ * src/gpu/solver.hpp:772) and Solver::sigmify() of the CPU build
+ * (src/cpu/simplify.cpp:115-129).  The host solver hands over its root-level
+ * clause database and variable state, runs the simplify rounds, and takes
+ * back the simplified clauses, the new root units, the eliminated-variable
+ * marks and the eliminated-variable stack for model reconstruction.
+ *
+ * Conventions: literals are 2*v + sign (v >= 1); plain pointers and sizes
+ * only; the library never keeps a caller pointer after a call returns and
+ * never exits the process; one thread uses a context at a time; any number
+ * of contexts may share a device.
+ */
+#ifndef PFSIMP_H
+#define PFSIMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pfs_ctx pfs_ctx;
+
+/* Status codes.  PFS_ALLOC_FAIL plays the role of the reference's
+ * AWAKEN_FAIL / CNFALLOC_FAIL / OTALLOC_FAIL simplifier states
+ * (src/gpu/constants.cuh:28-31): the caller keeps its own clauses and skips
+ * GPU simplification. */
+typedef enum {
+  PFS_OK = 0,
+  PFS_ALLOC_FAIL = 1,
+  PFS_BAD_INPUT = 2,
+  PFS_DEVICE_ERROR = 3,
+  PFS_NO_DEVICE = 4,
+  PFS_BAD_STATE = 5,
+  PFS_UNKNOWN_OPTION = 6
+} pfs_status;
+
+/* Outcome of pfs_simplify (the reference's cnfstate after sigmify,
+ * src/cpu/simplify.cpp:217-226 and learnEmpty on a refutation). */
+typedef enum { PFS_RESULT_UNSAT = 0, PFS_RESULT_SAT = 1, PFS_RESULT_UNSOLVED = 2 } pfs_result;
+
+typedef struct {
+  int32_t device;      /* HIP device ordinal */
+  uint64_t mem_budget; /* device bytes this context may hold; 0 = no limit */
+  int32_t keep_input;  /* keep a device copy of the input for pfs_rewind */
+} pfs_config;
+
+typedef struct {
+  uint32_t elected, eliminated, units, resolvents;
+  uint64_t clauses_after, literals_after;
+  uint32_t mis_iterations, sub_passes, ere_passes, reserved;
+} pfs_round_stats;
+
+/* Creates a context on a device.  Replaces the allocation part of the
+ * reference's awaken() (src/gpu/simplify.cu:77-134). */
+pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out);
+void pfs_destroy(pfs_ctx* ctx);
+
+/* Sets a simplifier option by its reference name (src/cpu/options.cpp:24-46,
+ * src/gpu/options.cpp:24-43): booleans "ve", "ve+", "sub", "bce", "ere",
+ * "all", "velbound", "profilegpu"; integers "phases", "mupos", "muneg",
+ * "vephasemin", "veclausemax", "xormaxarity", "submax", "bcemax", "eremax",
+ * "ereclausemax", "lcvemin", "gcfreq", "lbdtier1".  GPU-build aliases
+ * ("eliminatedlitsmin", "resolventmax", "submaxoccurs", "bcemaxoccurs",
+ * "eremaxoccurs", "electionsmin", "collectfreq", "velitsbound") are accepted. */
+pfs_status pfs_set_option(pfs_ctx* ctx, const char* name, const char* value);
+
+/* Hands over the root-level clause database, as the host solver has it after
+ * rootify()/shrinkTop() (src/cpu/simplify.cpp:163-164; the GPU build's
+ * extractCNF/reflectCNF, src/gpu/cnf.cu:166-184).
+ *   offsets[ncls+1], lits[offsets[ncls]] : clauses (any literal order)
+ *   flags[ncls]  (nullable): bit0 learnt, bits 4-5 usage
+ *   lbd[ncls]    (nullable): learnt clause LBD
+ *   vstate[maxvar+1] (nullable): 1 melted, 2 frozen (root-assigned), 4 substituted
+ *   assumed[maxvar+1] (nullable): assumption variables are never eliminated
+ *   vorg[maxvar+1]   (nullable): current -> original variable map for the stack
+ *   trail[ntrail]: the root units already assigned and propagated */
+pfs_status pfs_upload(pfs_ctx* ctx, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
+                      const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
+                      const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
+                      const uint32_t* trail, uint32_t ntrail);
+
+/* Restores the last uploaded input from its device copy (keep_input). */
+pfs_status pfs_rewind(pfs_ctx* ctx);
+
+/* Runs the simplify rounds (src/cpu/simplify.cpp:157-236; the GPU build's
+ * simplifying(), src/gpu/simplify.cu:136-241).  *result receives a pfs_result. */
+pfs_status pfs_simplify(pfs_ctx* ctx, int32_t* result);
+
+/* Result sizes: clauses, literals, new root units, stack words, rounds. */
+pfs_status pfs_result_sizes(pfs_ctx* ctx, uint64_t* ncls, uint64_t* nlits, uint64_t* nunits,
+                            uint64_t* nstack, uint32_t* nrounds);
+/* Simplified clauses in simplifier order (the reference's scnf order):
+ * offsets[ncls+1], lits, flags (bit0 learnt, bit1 added resolvent, bits 4-5
+ * usage), lbd.  Any pointer may be null.  Replaces cacheCNF/writeBackCNF
+ * (src/gpu/cnf.cu:186-236). */
+pfs_status pfs_download_cnf(pfs_ctx* ctx, uint64_t* offsets, uint32_t* lits, uint8_t* flags,
+                            uint32_t* lbd);
+/* New root units in assignment order (cacheUnits, src/gpu/transfer.cu:62-81). */
+pfs_status pfs_download_units(pfs_ctx* ctx, uint32_t* lits);
+/* Variable states (maxvar+1 bytes): 1 eliminated, 2 root-assigned
+ * (cacheEliminated/markEliminated, src/gpu/transfer.cu:42-75). */
+pfs_status pfs_download_vstate(pfs_ctx* ctx, uint8_t* vstate);
+/* Eliminated-variable stack words to append to the model's stack, in
+ * original numbering (model.resolved; cacheResolved, src/gpu/transfer.cu:83-96). */
+pfs_status pfs_download_stack(pfs_ctx* ctx, uint32_t* words);
+pfs_status pfs_round_stats_get(pfs_ctx* ctx, pfs_round_stats* out, uint32_t max, uint32_t* n);
+/* Per-stage device milliseconds (when "profilegpu" is on) and stage calls. */
+pfs_status pfs_stage_times(pfs_ctx* ctx, double* ms, uint64_t* calls, uint32_t n);
+const char* pfs_stage_name(uint32_t stage);
+uint32_t pfs_stage_count(void);
+/* Per-kernel device time of the last pfs_simplify when "profilegpu" is on:
+ * events recorded on the context's stream around every launch, summed per
+ * kernel (the MI355X counterpart of the reference's -profilegpu timers,
+ * src/gpu/timer.cuh:27-58). */
+uint32_t pfs_kernel_count(pfs_ctx* ctx);
+pfs_status pfs_kernel_stats(pfs_ctx* ctx, uint32_t i, const char** name, double* total_ms,
+                            uint64_t* launches);
+pfs_status pfs_memory(pfs_ctx* ctx, uint64_t* used, uint64_t* peak);
+const char* pfs_last_error(const pfs_ctx* ctx);
+const char* pfs_version(void);
+int32_t pfs_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFSIMP_H */

@@ -1,0 +1,344 @@
+// extern "C" entry points of libpfsimp.so (declared in include/pfsimp.h).
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/pfsimp.h"
+#include "pfs_driver.hpp"
+
+using namespace pfs;
+
+struct pfs_ctx {
+  std::unique_ptr<Simplifier> s;
+  std::string err;
+  bool uploaded = false, simplified = false;
+  int result = PFS_RESULT_UNSOLVED;
+  // options as given (before the normalisation of options.cpp:280-311)
+  int32_t phases = 5, mu_pos = 32, mu_neg = 32, lits_min = 500, ve_clause_limit = 100,
+          xor_max_arity = 10, sub_limit = 3000, bce_limit = 3000, ere_limit = 3000,
+          ere_max_resolvent = 1000, lcve_min = 1, shrink_rate = 2, lbd_tier1 = 2;
+  bool ve = true, ve_plus = true, sub = true, bce = false, ere = true, all = false, velbound = false,
+       profile = false;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+// src/cpu/options.cpp:280-311
+Options normalise(const pfs_ctx* c) {
+  Options o;
+  o.all_en = c->all;
+  o.ve_en = c->ve || c->ve_plus;
+  o.ve_plus_en = c->ve_plus;
+  o.ve_lbound_en = c->velbound;
+  o.ve_clause_limit = c->ve_clause_limit;
+  o.xor_max_arity = c->xor_max_arity;
+  o.bce_en = c->bce;
+  o.bce_limit = c->bce_limit;
+  o.ere_en = c->ere;
+  o.ere_limit = c->ere_limit;
+  o.ere_max_resolvent = c->ere_max_resolvent;
+  o.sub_en = c->sub;
+  o.sub_limit = c->sub_limit;
+  o.lcve_min = c->lcve_min;
+  o.lits_min = c->lits_min;
+  o.mu_pos = c->mu_pos;
+  o.mu_neg = c->mu_neg;
+  o.phases = c->phases;
+  o.shrink_rate = c->shrink_rate;
+  o.lbd_tier1 = c->lbd_tier1;
+  if (o.all_en) o.ve_en = 1, o.ve_plus_en = 1, o.bce_en = 1, o.ere_en = 1;
+  if (!o.phases && (o.ve_en || o.sub_en || o.bce_en)) o.phases = 1;
+  if (o.phases && !(o.ve_en || o.sub_en || o.bce_en)) o.phases = 0;
+  if (o.phases > 1 && !o.ve_en) o.phases = 1;
+  return o;
+}
+
+bool parse_bool(const char* v, bool& out) {
+  if (!v || !*v || !strcmp(v, "1") || !strcmp(v, "true") || !strcmp(v, "on")) { out = true; return true; }
+  if (!strcmp(v, "0") || !strcmp(v, "false") || !strcmp(v, "off")) { out = false; return true; }
+  return false;
+}
+bool parse_int(const char* v, int32_t lo, int32_t hi, int32_t& out) {
+  if (!v || !*v) return false;
+  char* end = nullptr;
+  const double d = strtod(v, &end);  // accepts 3e3 like the reference parser
+  if (*end || d < lo || d > hi) return false;
+  out = (int32_t)d;
+  return true;
+}
+
+template <class F>
+pfs_status guard(pfs_ctx* c, F&& f) {
+  try {
+    return f();
+  } catch (AllocFail& e) {
+    if (c) c->err = e.what;
+    return PFS_ALLOC_FAIL;
+  } catch (DeviceFail& e) {
+    if (c) c->err = e.what;
+    return PFS_DEVICE_ERROR;
+  } catch (std::invalid_argument& e) {
+    if (c) c->err = e.what();
+    return PFS_BAD_INPUT;
+  } catch (std::logic_error& e) {
+    if (c) c->err = e.what();
+    return PFS_BAD_STATE;
+  } catch (std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+    return PFS_ALLOC_FAIL;
+  } catch (std::exception& e) {
+    if (c) c->err = e.what();
+    return PFS_DEVICE_ERROR;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out) {
+  if (!out) return PFS_BAD_INPUT;
+  *out = nullptr;
+  const int dev = cfg ? cfg->device : 0;
+  if (pfs::device_count() <= dev || dev < 0) {
+    g_create_error = "no HIP device " + std::to_string(dev);
+    return PFS_NO_DEVICE;
+  }
+  pfs_ctx* c = new pfs_ctx();
+  const pfs_status s = guard(c, [&] {
+    const size_t budget = cfg && cfg->mem_budget ? (size_t)cfg->mem_budget : ~(size_t)0;
+    c->s.reset(new Simplifier(dev, budget, cfg ? cfg->keep_input != 0 : false));
+    return PFS_OK;
+  });
+  if (s != PFS_OK) {
+    g_create_error = c->err;
+    delete c;
+    return s;
+  }
+  *out = c;
+  return PFS_OK;
+}
+
+void pfs_destroy(pfs_ctx* c) {
+  if (!c) return;
+  try {
+    c->s.reset();
+  } catch (...) {
+  }
+  delete c;
+}
+
+pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
+  if (!c || !name) return PFS_BAD_INPUT;
+  const std::string n = name;
+  struct IntOpt { const char* a; const char* b; int32_t pfs_ctx::*f; int32_t lo, hi; };
+  static const IntOpt ints[] = {
+      {"phases", nullptr, &pfs_ctx::phases, 0, INT32_MAX},
+      {"mupos", nullptr, &pfs_ctx::mu_pos, 10, INT32_MAX},
+      {"muneg", nullptr, &pfs_ctx::mu_neg, 10, INT32_MAX},
+      {"vephasemin", "eliminatedlitsmin", &pfs_ctx::lits_min, 1, INT32_MAX},
+      {"veclausemax", "resolventmax", &pfs_ctx::ve_clause_limit, 0, INT32_MAX},
+      {"xormaxarity", nullptr, &pfs_ctx::xor_max_arity, 2, 20},
+      {"submax", "submaxoccurs", &pfs_ctx::sub_limit, 100, INT32_MAX},
+      {"bcemax", "bcemaxoccurs", &pfs_ctx::bce_limit, 100, INT32_MAX},
+      {"eremax", "eremaxoccurs", &pfs_ctx::ere_limit, 100, INT32_MAX},
+      {"ereclausemax", nullptr, &pfs_ctx::ere_max_resolvent, 0, INT32_MAX},
+      {"lcvemin", "electionsmin", &pfs_ctx::lcve_min, 1, INT32_MAX},
+      {"gcfreq", "collectfreq", &pfs_ctx::shrink_rate, 0, 5},
+      {"lbdtier1", nullptr, &pfs_ctx::lbd_tier1, 1, INT32_MAX},
+  };
+  for (const IntOpt& o : ints) {
+    if (n == o.a || (o.b && n == o.b)) {
+      int32_t v;
+      if (!parse_int(value, o.lo, o.hi, v)) {
+        c->err = "bad value for option " + n;
+        return PFS_BAD_INPUT;
+      }
+      c->*(o.f) = v;
+      return PFS_OK;
+    }
+  }
+  struct BoolOpt { const char* a; const char* b; bool pfs_ctx::*f; };
+  static const BoolOpt bools[] = {
+      {"ve", nullptr, &pfs_ctx::ve},   {"ve+", "veextend", &pfs_ctx::ve_plus},
+      {"sub", nullptr, &pfs_ctx::sub}, {"bce", nullptr, &pfs_ctx::bce},
+      {"ere", nullptr, &pfs_ctx::ere}, {"all", nullptr, &pfs_ctx::all},
+      {"velbound", "velitsbound", &pfs_ctx::velbound},
+      {"profilegpu", "profilesimp", &pfs_ctx::profile},
+  };
+  for (const BoolOpt& o : bools) {
+    if (n == o.a || (o.b && n == o.b)) {
+      bool v;
+      if (!parse_bool(value, v)) {
+        c->err = "bad value for option " + n;
+        return PFS_BAD_INPUT;
+      }
+      c->*(o.f) = v;
+      return PFS_OK;
+    }
+  }
+  c->err = "unknown option " + n;
+  return PFS_UNKNOWN_OPTION;
+}
+
+pfs_status pfs_upload(pfs_ctx* c, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
+                      const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
+                      const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
+                      const uint32_t* trail, uint32_t ntrail) {
+  if (!c || !offsets || (!lits && offsets[ncls]) || (!trail && ntrail) || !maxvar) return PFS_BAD_INPUT;
+  return guard(c, [&] {
+    for (uint64_t i = 0; i < ncls; i++)
+      if (offsets[i + 1] < offsets[i] + 1) throw std::invalid_argument("empty clause or bad offsets");
+    for (uint32_t i = 0; i < ntrail; i++)
+      if ((trail[i] >> 1) == 0 || (trail[i] >> 1) > maxvar) throw std::invalid_argument("trail literal out of range");
+    c->s->upload(maxvar, ncls, offsets, lits, flags, lbd, vstate, assumed, vorg, trail, ntrail);
+    c->uploaded = true;
+    c->simplified = false;
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_rewind(pfs_ctx* c) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    c->s->rewind();
+    c->simplified = false;
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_simplify(pfs_ctx* c, int32_t* result) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    c->s->opt = normalise(c);
+    c->s->profile = c->profile;
+    c->result = c->s->simplify(0);
+    c->simplified = true;
+    if (result) *result = c->result;
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_result_sizes(pfs_ctx* c, uint64_t* ncls, uint64_t* nlits, uint64_t* nunits, uint64_t* nstack,
+                            uint32_t* nrounds) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->simplified) {
+    c->err = "simplify has not run";
+    return PFS_BAD_STATE;
+  }
+  const bool unsat = c->result == PFS_RESULT_UNSAT;
+  if (ncls) *ncls = unsat ? 0 : c->s->out_ncls();
+  if (nlits) *nlits = unsat ? 0 : c->s->out_nlits();
+  if (nunits) *nunits = c->s->new_units().size();
+  if (nstack) *nstack = c->s->stack_words();
+  if (nrounds) *nrounds = (uint32_t)c->s->rounds().size();
+  return PFS_OK;
+}
+
+pfs_status pfs_download_cnf(pfs_ctx* c, uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->simplified) return PFS_BAD_STATE;
+  if (c->result == PFS_RESULT_UNSAT) {
+    if (offsets) offsets[0] = 0;
+    return PFS_OK;
+  }
+  return guard(c, [&] {
+    c->s->download_cnf(offsets, lits, flags, lbd);
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_download_units(pfs_ctx* c, uint32_t* lits) {
+  if (!c || !lits) return PFS_BAD_INPUT;
+  if (!c->simplified) return PFS_BAD_STATE;
+  const auto& u = c->s->new_units();
+  std::memcpy(lits, u.data(), u.size() * 4);
+  return PFS_OK;
+}
+
+pfs_status pfs_download_vstate(pfs_ctx* c, uint8_t* vstate) {
+  if (!c || !vstate) return PFS_BAD_INPUT;
+  if (!c->simplified) return PFS_BAD_STATE;
+  return guard(c, [&] {
+    c->s->download_vstate(vstate);
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_download_stack(pfs_ctx* c, uint32_t* words) {
+  if (!c || !words) return PFS_BAD_INPUT;
+  if (!c->simplified) return PFS_BAD_STATE;
+  return guard(c, [&] {
+    c->s->download_stack(words);
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_round_stats_get(pfs_ctx* c, pfs_round_stats* out, uint32_t max, uint32_t* n) {
+  if (!c) return PFS_BAD_INPUT;
+  const auto& r = c->s->rounds();
+  const uint32_t k = (uint32_t)std::min<size_t>(r.size(), max);
+  for (uint32_t i = 0; i < k; i++) {
+    out[i].elected = r[i].elected;
+    out[i].eliminated = r[i].eliminated;
+    out[i].units = r[i].units;
+    out[i].resolvents = r[i].resolvents;
+    out[i].clauses_after = r[i].clauses_after;
+    out[i].literals_after = r[i].literals_after;
+    out[i].mis_iterations = r[i].mis_iterations;
+    out[i].sub_passes = r[i].sub_passes;
+    out[i].ere_passes = r[i].ere_passes;
+    out[i].reserved = 0;
+  }
+  if (n) *n = (uint32_t)r.size();
+  return PFS_OK;
+}
+
+pfs_status pfs_stage_times(pfs_ctx* c, double* ms, uint64_t* calls, uint32_t n) {
+  if (!c) return PFS_BAD_INPUT;
+  for (uint32_t i = 0; i < n && i < (uint32_t)S_NSTAGES; i++) {
+    if (ms) ms[i] = c->s->stage_ms[i];
+    if (calls) calls[i] = c->s->stage_calls[i];
+  }
+  return PFS_OK;
+}
+
+const char* pfs_stage_name(uint32_t stage) { return stage < (uint32_t)S_NSTAGES ? kStageNames[stage] : ""; }
+uint32_t pfs_stage_count(void) { return (uint32_t)S_NSTAGES; }
+
+uint32_t pfs_kernel_count(pfs_ctx* c) { return c ? (uint32_t)c->s->kernel_times.size() : 0; }
+
+pfs_status pfs_kernel_stats(pfs_ctx* c, uint32_t i, const char** name, double* total_ms, uint64_t* launches) {
+  if (!c || i >= c->s->kernel_times.size()) return PFS_BAD_INPUT;
+  const KernelTime& k = c->s->kernel_times[i];
+  if (name) *name = k.name.c_str();
+  if (total_ms) *total_ms = k.ms;
+  if (launches) *launches = k.launches;
+  return PFS_OK;
+}
+
+pfs_status pfs_memory(pfs_ctx* c, uint64_t* used, uint64_t* peak) {
+  if (!c) return PFS_BAD_INPUT;
+  if (used) *used = c->s->mem_used();
+  if (peak) *peak = c->s->mem_peak();
+  return PFS_OK;
+}
+
+const char* pfs_last_error(const pfs_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+const char* pfs_version(void) { return "pfsimp 0.1 (gfx950)"; }
+int32_t pfs_device_count(void) { return pfs::device_count(); }
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Shared types and per-item helpers of the HIP simplifier.
+//
+// Everything in this header is plain C++ usable from device code (HIP
+// kernels) and host code; PF_HD marks functions compiled for both.
+//
+// Clause database layout in HBM (structure of arrays, one slot per clause
+// id; clause ids follow the simplifier's clause order, src/cpu/simplify.cpp
+// scnf, so the id doubles as the deterministic tie-break):
+//   c_off[c]   u32  start of the clause in the literal pool
+//   c_size[c]  u32  current number of literals (sorted ascending)
+//   c_sig[c]   u32  OR of 1 << (lit & 31)          (sclause.h:121-125)
+//   c_flags[c] u8   bits 0-1 status (0 original, 1 learnt, 2 deleted),
+//                   bit 2 molten, bit 3 added (resolvent), bits 4-5 usage
+//   c_lbd[c]   u32  literal block distance of learnt clauses
+//   lits[]     u32  literal pool, literal = 2*var + sign
+// Occurrence table: for literal l, ot_refs[ot_off[l] .. ot_off[l] +
+// ot_size[l]) are the clause ids of the list.  ot_size follows the size the
+// sequential simplifier would observe (entries of deleted clauses stay in a
+// list until the list is compacted or cleared).
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PF_HD __host__ __device__ inline
+#define PF_DEV __device__ inline
+#else
+#define PF_HD inline
+#define PF_DEV inline
+#endif
+
+namespace pfs {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef uint8_t u8;
+typedef int64_t i64;
+
+constexpr u8 ST_ORIGINAL = 0, ST_LEARNT = 1, ST_DELETED = 2, ST_MASK = 3;
+constexpr u8 F_MOLTEN = 4, F_ADDED = 8;
+constexpr u8 USAGE_SHIFT = 4, USAGE_MASK = 0x30;
+constexpr u8 VS_MELTED = 1, VS_FROZEN = 2, VS_SUBSTITUTED = 4;
+constexpr u32 NOPOS = 0xFFFFFFFFu;
+constexpr int HSE_MAX_CL_SIZE = 1000;  // src/cpu/subsume.h:25
+
+PF_HD u32 ABS(u32 l) { return l >> 1; }
+PF_HD u32 FLIP(u32 l) { return l ^ 1u; }
+PF_HD u32 NEGL(u32 l) { return l | 1u; }
+PF_HD u32 SIGN(u32 l) { return l & 1u; }
+PF_HD u32 V2L(u32 v) { return v << 1; }
+PF_HD u32 MAPHASH(u32 l) { return 1u << (l & 31u); }
+
+PF_HD bool st_original(u8 f) { return (f & ST_MASK) == ST_ORIGINAL; }
+PF_HD bool st_learnt(u8 f) { return (f & ST_LEARNT) != 0; }
+PF_HD bool st_deleted(u8 f) { return (f & ST_DELETED) != 0; }
+
+// Simplifier options (names and defaults of the reference CPU simplifier,
+// src/cpu/options.cpp:24-46, normalised as in options.cpp:280-311).
+struct Options {
+  int32_t phases = 5, mu_pos = 32, mu_neg = 32, lits_min = 500, ve_clause_limit = 100,
+          xor_max_arity = 10, sub_limit = 3000, bce_limit = 3000, ere_limit = 3000,
+          ere_max_resolvent = 1000, lcve_min = 1, shrink_rate = 2, lbd_tier1 = 2;
+  int32_t ve_en = 1, ve_plus_en = 1, sub_en = 1, bce_en = 0, ere_en = 1, all_en = 0,
+          ve_lbound_en = 0;
+};
+
+// Device view of the clause database and per-variable state.  Root
+// assignments are kept per variable (assign[v] = true literal + 1) so that
+// parallel unit propagation can claim a variable with one compare-and-swap.
+struct DB {
+  u32* lits;
+  u32* c_off;
+  u32* c_size;
+  u32* c_sig;
+  u8* c_flags;
+  u32* c_lbd;
+  u32* ot_off;
+  u32* ot_size;
+  u32* ot_refs;
+  u32* assign;    // per variable: 0 unassigned, else the true literal + 1
+  u8* vstate;     // per variable
+  u8* assumed;    // per variable
+  u32* vorg;      // per variable: original numbering
+  u32 nvars;
+  u32 ncls;
+};
+
+}  // namespace pfs

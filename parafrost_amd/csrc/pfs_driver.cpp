@@ -1,0 +1,823 @@
+// Round driver of the HIP simplifier.  Restates the control flow of the
+// reference CPU simplifier (src/cpu/simplify.cpp:157-236) over the device
+// stages; see pfs_items.hpp for the per-item semantics.
+#include "pfs_driver.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace pfs {
+
+const char* const kStageNames[S_NSTAGES] = {"gc", "ot", "bcp", "lcve", "sort",
+                                            "sub", "ve", "bce", "ere", "count"};
+static constexpr u32 INF = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------- memory
+void DeviceMemory::ensure(Buf& b, size_t bytes, size_t keep, Stream* st) {
+  if (bytes <= b.cap) return;
+  if (used_ - b.cap + bytes > budget_)
+    throw AllocFail{"device memory budget exceeded (" + std::to_string(used_ - b.cap + bytes) + " > " +
+                    std::to_string(budget_) + " bytes)"};
+  void* p = nullptr;
+  try {
+    p = dev_alloc(bytes);
+  } catch (std::bad_alloc&) {
+    throw AllocFail{"device allocation of " + std::to_string(bytes) + " bytes failed"};
+  }
+  if (b.p) {
+    if (keep) d2d(p, b.p, std::min(keep, b.cap), st);
+    stream_sync(st);
+    dev_free(b.p);
+    used_ -= b.cap;
+  }
+  b.p = p;
+  b.cap = bytes;
+  used_ += bytes;
+  peak_ = std::max(peak_, used_);
+}
+void DeviceMemory::release(Buf& b) {
+  if (b.p) {
+    dev_free(b.p);
+    used_ -= b.cap;
+  }
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+// ---------------------------------------------------------------- context
+Simplifier::Simplifier(int device, size_t budget, bool keep_input)
+    : mem_(budget), keep_input_(keep_input) {
+  st_ = stream_create(device);
+}
+
+Simplifier::~Simplifier() {
+  Buf* all[] = {&c_off_[0], &c_size_[0], &c_sig_[0], &c_flags_[0], &c_lbd_[0], &lits_[0],
+                &c_off_[1], &c_size_[1], &c_sig_[1], &c_flags_[1], &c_lbd_[1], &lits_[1],
+                &ot_off_, &ot_size_, &ot_refs_, &hist_, &assign_, &vstate_, &assumed_, &vorg_, &trail_,
+                &score_, &cflag_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_,
+                &def_, &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_ref_, &snap_flg_,
+                &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
+                &recs_, &ids_, &mark_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_, &in_c_size_,
+                &in_c_sig_, &in_c_flags_, &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_};
+  if (st_) stream_sync(st_);
+  for (Buf* b : all) mem_.release(*b);
+  stream_destroy(st_);
+}
+
+DB Simplifier::make_db(int w) {
+  DB db;
+  db.lits = P<u32>(lits_[w]);
+  db.c_off = P<u32>(c_off_[w]);
+  db.c_size = P<u32>(c_size_[w]);
+  db.c_sig = P<u32>(c_sig_[w]);
+  db.c_flags = P<u8>(c_flags_[w]);
+  db.c_lbd = P<u32>(c_lbd_[w]);
+  db.ot_off = P<u32>(ot_off_);
+  db.ot_size = P<u32>(ot_size_);
+  db.ot_refs = P<u32>(ot_refs_);
+  db.assign = P<u32>(assign_);
+  db.vstate = P<u8>(vstate_);
+  db.assumed = P<u8>(assumed_);
+  db.vorg = P<u32>(vorg_);
+  db.nvars = V_;
+  db.ncls = (u32)ncls_;
+  return db;
+}
+
+uint32_t Simplifier::read_u32(const void* dptr) {
+  u32 v;
+  d2h(&v, dptr, 4, st_);
+  stream_sync(st_);
+  return v;
+}
+
+void Simplifier::stage_begin(Stage s) {
+  stage_calls[s]++;
+  if (!profile) return;
+  prof_stage_ = s;
+  prof_ev_ = event_record(st_);
+}
+void Simplifier::stage_end() {
+  if (!profile || !prof_ev_) return;
+  Event* e = event_record(st_);
+  stage_ms[prof_stage_] += event_ms(prof_ev_, e);
+  event_free(prof_ev_);
+  event_free(e);
+  prof_ev_ = nullptr;
+}
+
+void Simplifier::alloc_state() {
+  const size_t V1 = (size_t)V_ + 1, L2 = 2 * (size_t)V_ + 3;
+  for (int w = 0; w < 2; w++) {
+    mem_.ensure(c_off_[w], ccap_ * 4, 0, st_);
+    mem_.ensure(c_size_[w], ccap_ * 4, 0, st_);
+    mem_.ensure(c_sig_[w], ccap_ * 4, 0, st_);
+    mem_.ensure(c_flags_[w], ccap_, 0, st_);
+    mem_.ensure(c_lbd_[w], ccap_ * 4, 0, st_);
+    mem_.ensure(lits_[w], lcap_ * 4, 0, st_);
+  }
+  mem_.ensure(ot_off_, (L2 + 1) * 4, 0, st_);
+  mem_.ensure(ot_size_, L2 * 4, 0, st_);
+  mem_.ensure(ot_refs_, lcap_ * 4, 0, st_);
+  mem_.ensure(hist_, L2 * 4, 0, st_);
+  mem_.ensure(assign_, V1 * 4, 0, st_);
+  mem_.ensure(vstate_, V1, 0, st_);
+  mem_.ensure(assumed_, V1, 0, st_);
+  mem_.ensure(vorg_, V1 * 4, 0, st_);
+  mem_.ensure(trail_, (V1 + 1) * 4, 0, st_);
+  for (Buf* b : {&score_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_, &def_,
+                 &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_np_, &snap_nn_})
+    mem_.ensure(*b, (V1 + 1) * 4, 0, st_);
+  mem_.ensure(cflag_, V1, 0, st_);
+  mem_.ensure(elits_, 2 * (V1 + 1) * 4, 0, st_);
+  mem_.ensure(unit_by_, L2 * 4, 0, st_);
+  mem_.ensure(recs_, 3 * (V1 + 1) * 4, 0, st_);
+  mem_.ensure(ids_, (V1 + 1) * 4, 0, st_);
+  mem_.ensure(mark_, ccap_, 0, st_);
+  const size_t big = std::max<size_t>({V1 + 2, L2 + 1, ccap_});
+  mem_.ensure(tmp_, std::max(scan_tmp_bytes(big), radix_tmp_bytes(V1 + 1)), 0, st_);
+  mem_.ensure(scal_, 256, 0, st_);
+  mem_.ensure(stack_, (size_t)1 << 20, 0, st_);
+}
+
+// ---------------------------------------------------------------- upload
+void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits,
+                        const uint8_t* flags, const uint32_t* lbd, const uint8_t* vstate,
+                        const uint8_t* assumed, const uint32_t* vorg, const uint32_t* trail,
+                        uint32_t ntrail) {
+  V_ = maxvar;
+  // extraction order: originals, then learnts (simplify.cpp:148-149)
+  std::vector<uint64_t> order;
+  order.reserve(ncls);
+  for (uint64_t i = 0; i < ncls; i++)
+    if (!(flags && (flags[i] & 1))) order.push_back(i);
+  for (uint64_t i = 0; i < ncls; i++)
+    if (flags && (flags[i] & 1)) order.push_back(i);
+  const uint64_t L = offsets[ncls] - offsets[0];
+  if (ncls >= INF / 2 || L >= INF / 4) throw AllocFail{"formula too large for 32-bit clause ids"};
+  std::vector<u32> h_off(ncls), h_size(ncls), h_lbd(ncls), h_lits(L);
+  std::vector<u8> h_flags(ncls);
+  uint64_t w = 0;
+  for (uint64_t k = 0; k < ncls; k++) {
+    const uint64_t i = order[k];
+    const uint64_t n = offsets[i + 1] - offsets[i];
+    h_off[k] = (u32)w;
+    h_size[k] = (u32)n;
+    const u8 f = flags ? flags[i] : 0;
+    const bool learnt = f & 1;
+    h_flags[k] = learnt ? (u8)(ST_LEARNT | (f & USAGE_MASK)) : (u8)ST_ORIGINAL;
+    h_lbd[k] = learnt && lbd ? (lbd[i] & 0x03FFFFFFu) : 0u;
+    for (uint64_t j = 0; j < n; j++) {
+      const u32 l = lits[offsets[i] + j];
+      if (ABS(l) == 0 || ABS(l) > maxvar) throw std::invalid_argument("literal out of range");
+      h_lits[w + j] = l;
+    }
+    w += n;
+  }
+  ccap_ = std::max<uint64_t>(2 * ncls + 1024, 4096);
+  lcap_ = std::max<uint64_t>(2 * L + 4096, 16384);
+  alloc_state();
+  cur_ = 0;
+  ncls_ = ncls;
+  lits_end_ = L;
+  h2d(c_off_[0].p, h_off.data(), ncls * 4, st_);
+  h2d(c_size_[0].p, h_size.data(), ncls * 4, st_);
+  h2d(c_flags_[0].p, h_flags.data(), ncls, st_);
+  h2d(c_lbd_[0].p, h_lbd.data(), ncls * 4, st_);
+  h2d(lits_[0].p, h_lits.data(), L * 4, st_);
+  // per-variable state
+  h_assign_.assign((size_t)V_ + 1, 0);
+  in_units_.assign(trail, trail + ntrail);
+  for (u32 i = 0; i < ntrail; i++) h_assign_[ABS(trail[i])] = trail[i] + 1;
+  std::vector<u8> vs((size_t)V_ + 1, 0), as((size_t)V_ + 1, 0);
+  std::vector<u32> vo((size_t)V_ + 1);
+  u32 melted = 0;
+  for (u32 v = 0; v <= V_; v++) {
+    vs[v] = v && vstate ? vstate[v] : 0;
+    as[v] = v && assumed ? assumed[v] : 0;
+    vo[v] = vorg ? vorg[v] : v;
+    melted += (vs[v] & (VS_MELTED | VS_SUBSTITUTED)) ? 1 : 0;
+  }
+  for (u32 i = 0; i < ntrail; i++) vs[ABS(trail[i])] = VS_FROZEN;
+  h2d(assign_.p, h_assign_.data(), ((size_t)V_ + 1) * 4, st_);
+  h2d(vstate_.p, vs.data(), (size_t)V_ + 1, st_);
+  h2d(assumed_.p, as.data(), (size_t)V_ + 1, st_);
+  h2d(vorg_.p, vo.data(), ((size_t)V_ + 1) * 4, st_);
+  h2d(trail_.p, trail, (size_t)ntrail * 4, st_);
+  trail_n_ = propagated_ = ntrail;
+  unassigned_ = V_ - ntrail - melted;
+  stack_n_ = 0;
+  units_out_.clear();
+  fill_u8(P<u8>(mark_), 0, ccap_, st_);
+  // extract: sort literals, signatures
+  k_prep(make_db(0), st_);
+  if (keep_input_) {
+    in_ncls_ = ncls;
+    in_nlits_ = L;
+    in_trail_ = ntrail;
+    in_unassigned_ = unassigned_;
+    mem_.ensure(in_c_off_, ncls * 4, 0, st_);
+    mem_.ensure(in_c_size_, ncls * 4, 0, st_);
+    mem_.ensure(in_c_sig_, ncls * 4, 0, st_);
+    mem_.ensure(in_c_flags_, ncls, 0, st_);
+    mem_.ensure(in_c_lbd_, ncls * 4, 0, st_);
+    mem_.ensure(in_lits_, L * 4, 0, st_);
+    mem_.ensure(in_assign_, ((size_t)V_ + 1) * 4, 0, st_);
+    mem_.ensure(in_vstate_, (size_t)V_ + 1, 0, st_);
+    mem_.ensure(in_trail_buf_, ((size_t)ntrail + 1) * 4, 0, st_);
+    d2d(in_c_off_.p, c_off_[0].p, ncls * 4, st_);
+    d2d(in_c_size_.p, c_size_[0].p, ncls * 4, st_);
+    d2d(in_c_sig_.p, c_sig_[0].p, ncls * 4, st_);
+    d2d(in_c_flags_.p, c_flags_[0].p, ncls, st_);
+    d2d(in_c_lbd_.p, c_lbd_[0].p, ncls * 4, st_);
+    d2d(in_lits_.p, lits_[0].p, L * 4, st_);
+    d2d(in_assign_.p, assign_.p, ((size_t)V_ + 1) * 4, st_);
+    d2d(in_vstate_.p, vstate_.p, (size_t)V_ + 1, st_);
+    d2d(in_trail_buf_.p, trail_.p, (size_t)ntrail * 4, st_);
+  }
+  stream_sync(st_);
+}
+
+void Simplifier::rewind() {
+  if (!keep_input_) throw std::logic_error("context was created without keep_input");
+  cur_ = 0;
+  ncls_ = in_ncls_;
+  lits_end_ = in_nlits_;
+  ensure_clause_room(0, 0);
+  d2d(c_off_[0].p, in_c_off_.p, in_ncls_ * 4, st_);
+  d2d(c_size_[0].p, in_c_size_.p, in_ncls_ * 4, st_);
+  d2d(c_sig_[0].p, in_c_sig_.p, in_ncls_ * 4, st_);
+  d2d(c_flags_[0].p, in_c_flags_.p, in_ncls_, st_);
+  d2d(c_lbd_[0].p, in_c_lbd_.p, in_ncls_ * 4, st_);
+  d2d(lits_[0].p, in_lits_.p, in_nlits_ * 4, st_);
+  d2d(assign_.p, in_assign_.p, ((size_t)V_ + 1) * 4, st_);
+  d2d(vstate_.p, in_vstate_.p, (size_t)V_ + 1, st_);
+  d2d(trail_.p, in_trail_buf_.p, in_trail_ * 4, st_);
+  trail_n_ = propagated_ = in_trail_;
+  unassigned_ = in_unassigned_;
+  stack_n_ = 0;
+  units_out_.clear();
+  std::fill(h_assign_.begin(), h_assign_.end(), 0u);
+  for (u32 u : in_units_) h_assign_[ABS(u)] = u + 1;
+}
+
+// ---------------------------------------------------------------- helpers
+void Simplifier::ensure_clause_room(uint64_t add_cls, uint64_t add_lits) {
+  const uint64_t need_c = ncls_ + add_cls, need_l = lits_end_ + add_lits;
+  if (need_c <= ccap_ && need_l <= lcap_) return;
+  const uint64_t nc = std::max(ccap_, need_c + need_c / 2), nl = std::max(lcap_, need_l + need_l / 2);
+  if (nc >= INF / 2 || nl >= INF / 4) throw AllocFail{"clause database exceeds 32-bit ids"};
+  for (int w = 0; w < 2; w++) {
+    const bool keep = (w == cur_);
+    mem_.ensure(c_off_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
+    mem_.ensure(c_size_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
+    mem_.ensure(c_sig_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
+    mem_.ensure(c_flags_[w], nc, keep ? ncls_ : 0, st_);
+    mem_.ensure(c_lbd_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
+    mem_.ensure(lits_[w], nl * 4, keep ? lits_end_ * 4 : 0, st_);
+  }
+  // the occurrence table is rebuilt every round; its refs array only needs
+  // the new capacity from the next round on, but it is resized here too
+  mem_.ensure(ot_refs_, nl * 4, lits_end_ * 4, st_);
+  const size_t old_mark = mark_.cap;
+  mem_.ensure(mark_, nc, old_mark, st_);
+  if (nc > old_mark) fill_u8(P<u8>(mark_) + old_mark, 0, nc - old_mark, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(nc)), 0, st_);
+  ccap_ = nc;
+  lcap_ = nl;
+}
+
+void Simplifier::ensure_stack(uint64_t words) {
+  if (words * 4 <= stack_.cap) return;
+  mem_.ensure(stack_, std::max(words * 4 + words * 2, (uint64_t)stack_.cap * 2), stack_n_ * 4, st_);
+}
+
+void Simplifier::commit_units(const std::vector<uint32_t>& lits) {
+  if (lits.empty()) return;
+  mem_.ensure(ids_, lits.size() * 4, 0, st_);
+  h2d(ids_.p, lits.data(), lits.size() * 4, st_);
+  k_assign_units(make_db(cur_), P<u32>(ids_), (u32)lits.size(), st_);
+  h2d(P<u32>(trail_) + trail_n_, lits.data(), lits.size() * 4, st_);
+  trail_n_ += lits.size();
+  stream_sync(st_);
+}
+
+void Simplifier::count(uint64_t& cls, uint64_t& lits) {
+  stage_begin(S_COUNT);
+  u64* d = (u64*)P<u8>(scal_);
+  k_count(make_db(cur_), d, st_);
+  u64 h[2];
+  d2h(h, d, 16, st_);
+  stream_sync(st_);
+  cls = h[0];
+  lits = h[1];
+  stage_end();
+}
+
+// stable compaction of the live clauses (simplify.cpp:238-249); clause ids
+// keep their relative order, so the schedule of collections never changes
+// the simplifier's result
+void Simplifier::gc() {
+  stage_begin(S_GC);
+  mem_.ensure(tmp2_, std::max<size_t>(ncls_, 1) * 4, 0, st_);
+  mem_.ensure(tmp3_, std::max<size_t>(ncls_, 1) * 4, 0, st_);
+  DB src = make_db(cur_), dst = make_db(1 - cur_);
+  u32* tot = P<u32>(scal_) + 8;
+  k_compact(src, dst, P<u32>(tmp2_), P<u32>(tmp3_), tmp_.p, tot, st_);
+  u32 h[2];
+  d2h(h, tot, 8, st_);
+  stream_sync(st_);
+  ncls_ = h[0];
+  lits_end_ = h[1];
+  cur_ = 1 - cur_;
+  stage_end();
+}
+
+// ---------------------------------------------------------------- OT (simplify.cpp:35-56)
+void Simplifier::create_ot() {
+  stage_begin(S_OT);
+  const size_t n2 = 2 * (size_t)V_ + 2;
+  DB db = make_db(cur_);
+  u32* hist = P<u32>(hist_);
+  fill_u32(hist, 0, n2, st_);
+  k_hist(db, hist, st_);
+  scan_u32(hist, db.ot_off, n2, db.ot_off + n2, tmp_.p, st_);
+  d2d(db.ot_size, hist, n2 * 4, st_);
+  fill_u32(hist, 0, n2, st_);
+  k_scatter(db, hist, st_);
+  stage_end();
+}
+
+// ---------------------------------------------------------------- BCP (elimination.cpp:64-99)
+bool Simplifier::prop() {
+  if (trail_n_ == propagated_) return true;
+  stage_begin(S_BCP);
+  DB db = make_db(cur_);
+  u32* trail = P<u32>(trail_);
+  u32* sc = P<u32>(scal_);
+  const uint64_t start_new = trail_n_;
+  uint64_t front = propagated_, nfront = trail_n_ - propagated_;
+  while (nfront) {
+    fill_u32(sc, 0, 2, st_);
+    k_bcp_visit(db, trail + front, (u32)nfront, trail + trail_n_, sc, sc + 1, st_);
+    u32 h[2];
+    d2h(h, sc, 8, st_);
+    stream_sync(st_);
+    if (h[1]) {
+      stage_end();
+      return false;
+    }
+    front = trail_n_;
+    nfront = h[0];
+    trail_n_ += h[0];
+  }
+  const uint64_t nnew = trail_n_ - start_new;
+  if (nnew) {
+    std::vector<u32> fresh(nnew);
+    d2h(fresh.data(), trail + start_new, nnew * 4, st_);
+    k_set_vstate(db, trail + start_new, (u32)nnew, VS_FROZEN, st_);
+    stream_sync(st_);
+    for (u32 u : fresh) {
+      h_assign_[ABS(u)] = u + 1;
+      units_out_.push_back(u);
+    }
+    unassigned_ -= (u32)nnew;
+    if (!rounds_.empty()) rounds_.back().units += (u32)nnew;
+  }
+  const u32 nprop = (u32)(trail_n_ - propagated_);
+  k_bcp_mark(db, trail + propagated_, nprop, P<u8>(mark_), st_);
+  k_clear_lists(db, trail + propagated_, nprop, st_);
+  k_bcp_clean(db, P<u8>(mark_), st_);
+  propagated_ = trail_n_;
+  if (!opt.sub_en) k_reduce_lists(db, st_);
+  stream_sync(st_);
+  stage_end();
+  return true;
+}
+
+// ---------------------------------------------------------------- LCVE (lcve.cpp)
+bool Simplifier::lcve() {
+  stage_begin(S_LCVE);
+  const size_t n2 = 2 * (size_t)V_ + 2;
+  DB db = make_db(cur_);
+  u32* hist = P<u32>(hist_);
+  fill_u32(hist, 0, n2, st_);
+  k_hist(db, hist, st_);
+  u32* sc = P<u32>(scal_);
+  u64* hard = (u64*)(sc + 16);
+  fill_u32((u32*)hard, INF, 2, st_);
+  LcveArgs a;
+  a.hist = hist;
+  a.score = P<u32>(score_);
+  a.hard = hard;
+  a.cflag = P<u8>(cflag_);
+  a.pt = (u32)opt.mu_pos << mu_inc_;
+  a.nt = (u32)opt.mu_neg << mu_inc_;
+  k_lcve_classify(db, a, st_);
+  k_lcve_select(db, a, P<u32>(flag_), P<u32>(cand_), P<u32>(key_), sc, tmp_.p, st_);
+  u32 h[4];
+  d2h(h, sc, 4, st_);
+  d2h(h + 2, hard, 8, st_);
+  stream_sync(st_);
+  const u32 ncand = h[0];
+  const u64 hk = ((u64)h[3] << 32) | h[2];
+  int bits = 32;
+  if (hk != ~0ull) {
+    const u32 hs = (u32)(hk >> 32);
+    bits = 0;
+    while (bits < 32 && (hs >> bits)) bits++;
+  }
+  radix_sort_pairs(P<u32>(key_), P<u32>(cand_), ncand, bits, tmp_.p, st_);
+  fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
+  k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
+  fill_u32(P<u32>(state_), 0, ncand, st_);
+  u32 iters = 0;
+  while (ncand) {
+    fill_u32(sc + 4, 0, 1, st_);
+    k_mis_step(db, P<u32>(cand_), ncand, P<u32>(pos_), P<u32>(state_), sc + 4, st_);
+    iters++;
+    if (iters > ncand + 2) throw DeviceFail{"LCVE election did not converge"};
+    if (!read_u32(sc + 4)) break;
+  }
+  fill_u32(sc, 0, 1, st_);
+  fill_u32(sc + 1, ncand, 1, st_);
+  k_mis_break(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(state_), sc + 1, st_);
+  k_mis_collect(P<u32>(cand_), ncand, P<u32>(state_), 0, P<u32>(flag_), P<u32>(pv_), sc, tmp_.p, st_);
+  E_ = read_u32(sc);
+  rounds_.back().elected = E_;
+  rounds_.back().mis_iterations = iters;
+  stage_end();
+  return E_ >= (u32)opt.lcve_min;
+}
+
+// ---------------------------------------------------------------- sortOT (simplify.cpp:81-93)
+void Simplifier::sort_ot() {
+  if (!E_) return;
+  stage_begin(S_SORT);
+  DB db = make_db(cur_);
+  k_pv_lits(P<u32>(pv_), E_, P<u32>(elits_), st_);
+  u32* sc = P<u32>(scal_);
+  k_max_list(db, P<u32>(elits_), 2 * E_, sc, st_);
+  const u32 mx = read_u32(sc);
+  if (mx > 2048) mem_.ensure(scratch_, std::max<size_t>(lits_end_, 1) * 4, 0, st_);
+  k_sort_lists(db, P<u32>(elits_), 2 * E_, mx, P<u32>(scratch_), st_);
+  stage_end();
+}
+
+// ---------------------------------------------------------------- SUB (elimination.cpp:110-129)
+bool Simplifier::sub() {
+  if (!(opt.sub_en || opt.ve_plus_en) || !E_) return true;
+  stage_begin(S_SUB);
+  DB db = make_db(cur_);
+  u32* sc = P<u32>(scal_);
+  k_list_sizes(db, P<u32>(pv_), E_, 0, P<u32>(vsz_), st_);
+  scan_u32(P<u32>(vsz_), P<u32>(vbase_), E_, sc + 5, tmp_.p, st_);
+  const u32 O = read_u32(sc + 5);
+  const size_t ob = std::max<size_t>(O, 1);
+  mem_.ensure(snap_ref_, ob * 4, 0, st_);
+  mem_.ensure(snap_flg_, ob, 0, st_);
+  mem_.ensure(snap_size_, ob * 4, 0, st_);
+  mem_.ensure(snap_sig_, ob * 4, 0, st_);
+  mem_.ensure(snap_lbd_, ob * 4, 0, st_);
+  SubSnap snap{P<u32>(vbase_), P<u32>(snap_np_), P<u32>(snap_nn_), P<u32>(snap_ref_),
+               P<u8>(snap_flg_), P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_)};
+  k_sub_snapshot(db, P<u32>(pv_), E_, snap, st_);
+  std::vector<u32> committed;
+  u32 from = 0, passes = 0;
+  const u32 cap = V_ + 1;
+  bool ok = true;
+  while (true) {
+    if (++passes > E_ + 2) throw DeviceFail{"SUB re-runs did not converge"};
+    fill_u32(sc, 0, 1, st_);
+    fill_u32(sc + 1, INF, 1, st_);
+    fill_u32(sc + 2, 0, 1, st_);
+    SubOut out{P<u32>(recs_), sc, cap, sc + 1, sc + 2};
+    k_sub(db, opt, P<u32>(pv_), from, E_, out, st_);
+    u32 h[3];
+    d2h(h, sc, 12, st_);
+    stream_sync(st_);
+    if (h[2]) throw DeviceFail{"a variable derived more units in SUB than the per-variable buffer holds"};
+    if (h[0] > cap) throw DeviceFail{"SUB unit record buffer overflow"};
+    const u32 nrec = h[0], unsat_at = h[1];
+    if (!nrec && unsat_at == INF) break;
+    std::vector<u32> rec(3 * (size_t)nrec);
+    d2h(rec.data(), recs_.p, rec.size() * 4, st_);
+    stream_sync(st_);
+    std::vector<u32> idx(nrec);
+    for (u32 i = 0; i < nrec; i++) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](u32 a, u32 b) {
+      return rec[3 * a] != rec[3 * b] ? rec[3 * a] < rec[3 * b] : rec[3 * a + 1] < rec[3 * b + 1];
+    });
+    u32 first_dirty = INF;
+    if (nrec) {
+      fill_u32(P<u32>(unit_by_), INF, 2 * (size_t)V_ + 2, st_);
+      k_del_by(P<u32>(recs_), nrec, P<u32>(unit_by_), st_);
+      fill_u32(sc + 3, INF, 1, st_);
+      k_sub_dirty(db, from, E_, snap, P<u32>(unit_by_), sc + 3, st_);
+      first_dirty = read_u32(sc + 3);
+    }
+    // units of the exact prefix, in elected order (strengthen,
+    // elimination.cpp:214-226)
+    std::vector<u32> fresh;
+    for (u32 i : idx) {
+      const u32 k = rec[3 * i], u = rec[3 * i + 2];
+      if (k >= first_dirty) break;
+      const u32 a = h_assign_[ABS(u)];
+      if (a) {
+        if (a - 1 != u) ok = false;
+        continue;
+      }
+      h_assign_[ABS(u)] = u + 1;
+      fresh.push_back(u);
+      units_out_.push_back(u);
+      unassigned_--;
+    }
+    if (!ok || unsat_at < first_dirty) {
+      ok = false;
+      break;
+    }
+    commit_units(fresh);
+    committed.insert(committed.end(), fresh.begin(), fresh.end());
+    if (first_dirty == INF) {
+      if (!fresh.empty()) k_oblivion(db, P<u32>(ids_), (u32)fresh.size(), st_);
+      break;
+    }
+    // re-run the suffix after the units of the prefix took effect
+    k_sub_restore(db, P<u32>(pv_), first_dirty, E_, snap, st_);
+    if (!committed.empty()) {
+      mem_.ensure(ids_, committed.size() * 4, 0, st_);
+      h2d(ids_.p, committed.data(), committed.size() * 4, st_);
+      k_oblivion(db, P<u32>(ids_), (u32)committed.size(), st_);
+    }
+    from = first_dirty;
+  }
+  if (ok && !committed.empty()) {
+    // toblivion(ot[unit]) also empties the unit's list
+    mem_.ensure(ids_, committed.size() * 4, 0, st_);
+    h2d(ids_.p, committed.data(), committed.size() * 4, st_);
+    k_zero_sizes(db, P<u32>(ids_), (u32)committed.size(), st_);
+  }
+  stream_sync(st_);
+  rounds_.back().units += (u32)committed.size();
+  rounds_.back().sub_passes = passes;
+  stage_end();
+  return ok;
+}
+
+// ---------------------------------------------------------------- BVE (bve.cpp:127-243)
+bool Simplifier::ve() {
+  if (!E_) return true;
+  if (!opt.ve_en) {
+    fill_u32(P<u32>(type_), 0, E_, st_);
+    return true;
+  }
+  stage_begin(S_VE);
+  DB db = make_db(cur_);
+  u32* sc = P<u32>(scal_);
+  k_list_sizes(db, P<u32>(pv_), E_, 2, P<u32>(vsz_), st_);
+  scan_u32(P<u32>(vsz_), P<u32>(vbase_), E_, sc, tmp_.p, st_);
+  const u32 S = read_u32(sc);
+  mem_.ensure(scratch_, std::max<size_t>(S, 1) * 4, 0, st_);
+  VeDecision d{P<u32>(type_), P<u32>(def_), P<u32>(vncls_), P<u32>(vnlits_), P<u32>(vnstack_),
+               P<u32>(scratch_), P<u32>(vbase_)};
+  k_ve_decide(db, opt, P<u32>(pv_), E_, d, st_);
+  scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, sc, tmp_.p, st_);
+  scan_u32(P<u32>(vnlits_), P<u32>(litb_), E_, sc + 1, tmp_.p, st_);
+  scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc + 2, tmp_.p, st_);
+  u32 tot[3];
+  d2h(tot, sc, 12, st_);
+  stream_sync(st_);
+  ensure_clause_room(tot[0], tot[1]);
+  ensure_stack(stack_n_ + tot[2]);
+  db = make_db(cur_);
+  fill_u32(sc + 3, 0, 1, st_);
+  VeApply ap;
+  ap.cls_base = P<u32>(clsb_);
+  ap.lit_base = P<u32>(litb_);
+  ap.stack_base = P<u32>(stkb_);
+  ap.cls0 = (u32)ncls_;
+  ap.lit0 = (u32)lits_end_;
+  ap.stack0 = (u32)stack_n_;
+  ap.cls_cap = (u32)ccap_;
+  ap.lit_cap = (u32)lcap_;
+  ap.stack_cap = (u32)(stack_.cap / 4);
+  ap.stack = P<u32>(stack_);
+  ap.units = UnitOut{P<u32>(recs_), sc + 3, V_ + 1};
+  ap.error = sc + 6;
+  fill_u32(sc + 6, 0, 1, st_);
+  k_ve_apply(db, P<u32>(pv_), E_, d, ap, st_);
+  if (read_u32(sc + 6)) throw DeviceFail{"BVE resolvent writes disagree with the counted space"};
+  ncls_ += tot[0];
+  lits_end_ += tot[1];
+  stack_n_ += tot[2];
+  const u32 nrec = read_u32(sc + 3);
+  if (nrec > V_ + 1) throw DeviceFail{"BVE unit record buffer overflow"};
+  bool ok = true;
+  std::vector<u32> fresh;
+  if (nrec) {
+    std::vector<u32> rec(3 * (size_t)nrec);
+    d2h(rec.data(), recs_.p, rec.size() * 4, st_);
+    stream_sync(st_);
+    std::vector<u32> idx(nrec);
+    for (u32 i = 0; i < nrec; i++) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](u32 a, u32 b) {
+      return rec[3 * a] != rec[3 * b] ? rec[3 * a] < rec[3 * b] : rec[3 * a + 1] < rec[3 * b + 1];
+    });
+    // newResolvent / substitute_single unit handling (bve.cpp:300-311,
+    // equivalence.h:58-66): assign if free, refute if false
+    for (u32 i : idx) {
+      const u32 u = rec[3 * i + 2];
+      const u32 a = h_assign_[ABS(u)];
+      if (a) {
+        if (a - 1 != u) { ok = false; break; }
+        continue;
+      }
+      h_assign_[ABS(u)] = u + 1;
+      fresh.push_back(u);
+      units_out_.push_back(u);
+      unassigned_--;
+    }
+  }
+  if (ok) commit_units(fresh);
+  // eliminated variables (markEliminated, solver.h:535-541)
+  std::vector<u32> types(E_);
+  d2h(types.data(), type_.p, (size_t)E_ * 4, st_);
+  stream_sync(st_);
+  u32 elim = 0;
+  for (u32 t : types) elim += t ? 1 : 0;
+  unassigned_ -= elim;
+  rounds_.back().eliminated = elim;
+  rounds_.back().resolvents = tot[0];
+  rounds_.back().units += (u32)fresh.size();
+  stage_end();
+  return ok;
+}
+
+// ---------------------------------------------------------------- BCE (elimination.cpp:131-167)
+void Simplifier::bce() {
+  if (!opt.bce_en || !E_) return;
+  stage_begin(S_BCE);
+  DB db = make_db(cur_);
+  u32* sc = P<u32>(scal_);
+  k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, P<u32>(vnstack_), nullptr, nullptr, true, st_);
+  scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc, tmp_.p, st_);
+  const u32 T = read_u32(sc);
+  if (T) {
+    ensure_stack(stack_n_ + T);
+    k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, nullptr, P<u32>(stkb_), P<u32>(stack_) + stack_n_, false,
+          st_);
+    stack_n_ += T;
+  }
+  stream_sync(st_);
+  stage_end();
+}
+
+// ---------------------------------------------------------------- ERE (elimination.cpp:169-197)
+void Simplifier::ere() {
+  if (!opt.ere_en || !E_) return;
+  stage_begin(S_ERE);
+  DB db = make_db(cur_);
+  u32* sc = P<u32>(scal_);
+  const u32 cap = V_ + 1;
+  mem_.ensure(unit_by_, std::max<size_t>(ncls_, 2 * (size_t)V_ + 3) * 4, 0, st_);
+  u32 from = 0, passes = 0;
+  while (true) {
+    if (++passes > E_ + 2) throw DeviceFail{"ERE re-runs did not converge"};
+    fill_u32(sc, 0, 2, st_);
+    EreOut out{P<u32>(recs_), sc, cap, sc + 1};
+    k_ere(db, opt, P<u32>(pv_), from, E_, out, st_);
+    u32 h[2];
+    d2h(h, sc, 8, st_);
+    stream_sync(st_);
+    if (h[1]) throw DeviceFail{"a variable deleted more clauses in ERE than the per-variable buffer holds"};
+    if (h[0] > cap) throw DeviceFail{"ERE record buffer overflow"};
+    const u32 nrec = h[0];
+    if (!nrec) break;
+    std::vector<u32> rec(3 * (size_t)nrec);
+    d2h(rec.data(), recs_.p, rec.size() * 4, st_);
+    stream_sync(st_);
+    // a variable whose target was deleted by an earlier variable must re-run
+    std::unordered_map<u32, u32> first_by;
+    for (u32 i = 0; i < nrec; i++) {
+      const u32 k = rec[3 * i], c = rec[3 * i + 2];
+      auto it = first_by.find(c);
+      if (it == first_by.end() || k < it->second) first_by[c] = k;
+    }
+    u32 first_dirty = INF;
+    for (u32 i = 0; i < nrec; i++) {
+      const u32 k = rec[3 * i], c = rec[3 * i + 2];
+      if (first_by[c] < k) first_dirty = std::min(first_dirty, k);
+    }
+    fill_u32(P<u32>(unit_by_), INF, ncls_, st_);
+    k_del_by(P<u32>(recs_), nrec, P<u32>(unit_by_), st_);
+    fill_u32(sc + 2, INF, 1, st_);
+    k_ere_dirty(db, P<u32>(pv_), from, E_, P<u32>(unit_by_), sc + 2, st_);
+    first_dirty = std::min(first_dirty, read_u32(sc + 2));
+    std::vector<u32> del;
+    for (u32 i = 0; i < nrec; i++)
+      if (rec[3 * i] < first_dirty) del.push_back(rec[3 * i + 2]);
+    if (!del.empty()) {
+      mem_.ensure(ids_, del.size() * 4, 0, st_);
+      h2d(ids_.p, del.data(), del.size() * 4, st_);
+      k_apply_deletions(db, P<u32>(ids_), (u32)del.size(), st_);
+    }
+    if (first_dirty == INF) break;
+    from = first_dirty;
+  }
+  stream_sync(st_);
+  rounds_.back().ere_passes = passes;
+  stage_end();
+}
+
+// ---------------------------------------------------------------- simplify (simplify.cpp:157-236)
+int Simplifier::simplify(uint32_t max_rounds) {
+  rounds_.clear();
+  for (int s = 0; s < S_NSTAGES; s++) stage_ms[s] = 0, stage_calls[s] = 0;
+  kernel_times.clear();
+  prof_enable(st_, profile);
+  const int r = simplify_rounds(max_rounds);
+  if (profile) prof_collect(st_, kernel_times);
+  prof_enable(st_, false);
+  return r;
+}
+
+int Simplifier::simplify_rounds(uint32_t max_rounds) {
+  if (!ncls_) return 2;
+  phase_ = 0;
+  mu_inc_ = 0;
+  uint64_t nC, nL;
+  count(nC, nL);
+  int64_t litsbefore = (int64_t)nL, diff = INT64_MAX;
+  u32 r = 0;
+  while (nC && nL) {
+    if (max_rounds && r >= max_rounds) break;
+    rounds_.emplace_back();
+    // compaction keeps clause order, so when it runs does not change the
+    // result; collect once more than half of the slots are dead
+    if (ncls_ > 2 * nC + 1024) gc();
+    create_ot();
+    if (!prop()) return 0;
+    if (!lcve()) break;
+    sort_ot();
+    if ((phase_ == opt.phases) || (diff <= opt.lits_min && phase_ > 2)) {
+      ere();
+      break;
+    }
+    if (!sub()) return 0;
+    if (!ve()) return 0;
+    bce();
+    count(nC, nL);
+    rounds_.back().clauses_after = nC;
+    rounds_.back().literals_after = nL;
+    diff = litsbefore - (int64_t)nL;
+    litsbefore = (int64_t)nL;
+    phase_++;
+    mu_inc_++;
+    mu_inc_ += phase_ == opt.phases;
+    r++;
+  }
+  if (!prop()) return 0;
+  count(nC, nL);
+  gc();
+  if (!unassigned_ || !nC) return 1;
+  return 2;
+}
+
+// ---------------------------------------------------------------- results
+void Simplifier::download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd) {
+  const size_t n = ncls_;
+  std::vector<u32> off(n), size(n), lb(n);
+  std::vector<u8> fl(n);
+  d2h(off.data(), c_off_[cur_].p, n * 4, st_);
+  d2h(size.data(), c_size_[cur_].p, n * 4, st_);
+  d2h(fl.data(), c_flags_[cur_].p, n, st_);
+  d2h(lb.data(), c_lbd_[cur_].p, n * 4, st_);
+  if (lits) d2h(lits, lits_[cur_].p, lits_end_ * 4, st_);
+  stream_sync(st_);
+  if (offsets) {
+    uint64_t a = 0;
+    for (size_t i = 0; i < n; i++) {
+      offsets[i] = a;
+      a += size[i];
+    }
+    offsets[n] = a;
+  }
+  if (flags)
+    for (size_t i = 0; i < n; i++)
+      flags[i] = (u8)((st_learnt(fl[i]) ? 1 : 0) | ((fl[i] & F_ADDED) ? 2 : 0) | (fl[i] & USAGE_MASK));
+  if (lbd)
+    for (size_t i = 0; i < n; i++) lbd[i] = st_learnt(fl[i]) ? lb[i] : 0;
+}
+void Simplifier::download_vstate(uint8_t* vs) {
+  d2h(vs, vstate_.p, (size_t)V_ + 1, st_);
+  stream_sync(st_);
+}
+void Simplifier::download_stack(uint32_t* words) {
+  d2h(words, stack_.p, stack_n_ * 4, st_);
+  stream_sync(st_);
+}
+
+}  // namespace pfs

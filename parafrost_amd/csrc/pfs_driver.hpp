@@ -1,0 +1,157 @@
+// Round driver of the HIP simplifier: one Simplifier per context owns the
+// device state of one formula (clause database, occurrence table,
+// per-variable state, eliminated-variable stack) and runs the simplify
+// rounds of the reference (src/cpu/simplify.cpp:157-236) on it.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pfs_launch.hpp"
+
+namespace pfs {
+
+// Device memory owned by a context.  Every buffer is accounted against a
+// budget (admission control for contexts that share a device); the two
+// clause-database blocks form the dynamic region that garbage collection
+// compacts into and that grows when resolvents need more room.
+class DeviceMemory {
+ public:
+  explicit DeviceMemory(size_t budget) : budget_(budget) {}
+  struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+  };
+  // make b hold at least 'bytes'; keeps the first 'keep' bytes of content
+  void ensure(Buf& b, size_t bytes, size_t keep, Stream* st);
+  void release(Buf& b);
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+  size_t budget() const { return budget_; }
+  void set_budget(size_t b) { budget_ = b; }
+
+ private:
+  size_t budget_, used_ = 0, peak_ = 0;
+};
+
+struct AllocFail {
+  std::string what;
+};
+struct DeviceFail {
+  std::string what;
+};
+
+struct RoundStats {
+  uint32_t elected = 0, eliminated = 0, units = 0, resolvents = 0;
+  uint64_t clauses_after = 0, literals_after = 0;
+  uint32_t mis_iterations = 0, sub_passes = 0, ere_passes = 0;
+};
+
+// per-stage device time (ms) and launches, when profiling is enabled
+enum Stage { S_GC, S_OT, S_BCP, S_LCVE, S_SORT, S_SUB, S_VE, S_BCE, S_ERE, S_COUNT, S_NSTAGES };
+extern const char* const kStageNames[S_NSTAGES];
+
+class Simplifier {
+ public:
+  Simplifier(int device, size_t budget, bool keep_input);
+  ~Simplifier();
+
+  Options opt;
+  bool profile = false;
+
+  void upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits,
+              const uint8_t* flags, const uint32_t* lbd, const uint8_t* vstate, const uint8_t* assumed,
+              const uint32_t* vorg, const uint32_t* trail, uint32_t ntrail);
+  void rewind();                    // restore the last uploaded input (device copy)
+  int simplify(uint32_t max_rounds); // 0 UNSAT, 1 SAT, 2 UNSOLVED
+
+  // results (valid after simplify)
+  uint64_t out_ncls() const { return ncls_; }
+  uint64_t out_nlits() const { return lits_end_; }
+  void download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd);
+  const std::vector<uint32_t>& new_units() const { return units_out_; }
+  void download_vstate(uint8_t* vstate);
+  uint64_t stack_words() const { return stack_n_; }
+  void download_stack(uint32_t* words);
+  const std::vector<RoundStats>& rounds() const { return rounds_; }
+  double stage_ms[S_NSTAGES] = {0};
+  uint64_t stage_calls[S_NSTAGES] = {0};
+  uint64_t stage_bytes[S_NSTAGES] = {0};  // algorithmic bytes per stage
+  std::vector<KernelTime> kernel_times;  // per kernel, when profiling is enabled
+  size_t mem_used() const { return mem_.used(); }
+  size_t mem_peak() const { return mem_.peak(); }
+  uint32_t nvars() const { return V_; }
+
+ private:
+  typedef DeviceMemory::Buf Buf;
+  template <class T>
+  T* P(Buf& b) { return (T*)b.p; }
+
+  int simplify_rounds(uint32_t max_rounds);
+  void alloc_state();
+  DB make_db(int which);
+  void gc();
+  void count(uint64_t& cls, uint64_t& lits);
+  void create_ot();
+  bool prop();
+  bool lcve();
+  void sort_ot();
+  bool sub();
+  bool ve();
+  void bce();
+  void ere();
+  void ensure_clause_room(uint64_t add_cls, uint64_t add_lits);
+  void ensure_stack(uint64_t words);
+  void commit_units(const std::vector<uint32_t>& lits);
+  uint32_t read_u32(const void* dptr);
+  void stage_begin(Stage s);
+  void stage_end();
+
+  Stream* st_ = nullptr;
+  DeviceMemory mem_;
+  bool keep_input_;
+
+  // sizes
+  uint32_t V_ = 0;
+  uint64_t ncls_ = 0, lits_end_ = 0, ccap_ = 0, lcap_ = 0;
+  int cur_ = 0;  // which clause block is live
+  // clause blocks (dynamic region)
+  Buf c_off_[2], c_size_[2], c_sig_[2], c_flags_[2], c_lbd_[2], lits_[2];
+  // occurrence table
+  Buf ot_off_, ot_size_, ot_refs_, hist_;
+  // per variable
+  Buf assign_, vstate_, assumed_, vorg_, trail_;
+  uint64_t trail_n_ = 0, propagated_ = 0;
+  // LCVE
+  Buf score_, cflag_, flag_, cand_, key_, pos_, state_, pv_;
+  uint32_t E_ = 0;
+  // per elected variable
+  Buf vsz_, vbase_, type_, def_, vncls_, vnlits_, vnstack_, clsb_, litb_, stkb_;
+  Buf snap_ref_, snap_flg_, snap_size_, snap_sig_, snap_lbd_, snap_np_, snap_nn_, scratch_;
+  Buf elits_, unit_by_, recs_, ids_, mark_;
+  // stack
+  Buf stack_;
+  uint64_t stack_n_ = 0;
+  // temporaries
+  Buf tmp_, tmp2_, tmp3_, scal_;
+  // pristine input copy
+  Buf in_c_off_, in_c_size_, in_c_sig_, in_c_flags_, in_c_lbd_, in_lits_, in_assign_, in_vstate_;
+  uint64_t in_ncls_ = 0, in_nlits_ = 0, in_trail_ = 0;
+  uint32_t in_unassigned_ = 0;
+  Buf in_trail_buf_;
+
+  // host mirrors
+  std::vector<uint32_t> h_assign_;  // per variable
+  std::vector<uint32_t> units_out_;  // new root units in trail order
+  std::vector<uint32_t> in_units_;
+  uint32_t unassigned_ = 0;
+  int phase_ = 0;
+  uint32_t mu_inc_ = 0;
+  std::vector<RoundStats> rounds_;
+  // profiling
+  Stage prof_stage_ = S_NSTAGES;
+  Event* prof_ev_ = nullptr;
+};
+
+}  // namespace pfs

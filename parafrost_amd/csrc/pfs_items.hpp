@@ -1,0 +1,1195 @@
+// Per-item work of the simplifier stages: one call handles one clause, one
+// occurrence list or one elected variable.  The HIP kernels in
+// pfs_kernels.hip map work-items onto these functions.
+//
+// The per-variable functions restate the sequential stages of the
+// reference CPU simplifier for one elected variable.  Elected variables
+// share no live clause (LCVE), so each function only writes clauses in its
+// own occurrence lists; the few effects that cross variables (units found
+// by strengthening, ERE deletions, root units) are returned to the driver,
+// which applies them in elected order.
+#pragma once
+
+#include "pfs_common.hpp"
+
+namespace pfs {
+
+// ---------------------------------------------------------------- atomics
+PF_HD u32 at_add(u32* p, u32 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicAdd(p, v);
+#else
+  return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#endif
+}
+PF_HD u32 at_min(u32* p, u32 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicMin(p, v);
+#else
+  u32 o = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (v < o && !__atomic_compare_exchange_n(p, &o, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+  return o;
+#endif
+}
+PF_HD u32 at_cas(u32* p, u32 cmp, u32 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicCAS(p, cmp, v);
+#else
+  u32 e = cmp;
+  __atomic_compare_exchange_n(p, &e, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
+  return e;
+#endif
+}
+PF_HD u32 at_load(const u32* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __atomic_load_n(p, __ATOMIC_RELAXED);
+#else
+  return __atomic_load_n(p, __ATOMIC_RELAXED);
+#endif
+}
+PF_HD u64 at_min64(u64* p, u64 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicMin((unsigned long long*)p, (unsigned long long)v);
+#else
+  u64 o = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (v < o && !__atomic_compare_exchange_n(p, &o, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+  return o;
+#endif
+}
+
+// ---------------------------------------------------------------- clause access
+struct CL {
+  u32* l;
+  u32 n;
+};
+PF_HD CL clause(const DB& db, u32 c) { return CL{db.lits + db.c_off[c], db.c_size[c]}; }
+PF_HD u8 flags(const DB& db, u32 c) { return db.c_flags[c]; }
+PF_HD bool is_original(const DB& db, u32 c) { return st_original(db.c_flags[c]); }
+PF_HD bool is_learnt(const DB& db, u32 c) { return st_learnt(db.c_flags[c]); }
+PF_HD bool is_deleted(const DB& db, u32 c) { return st_deleted(db.c_flags[c]); }
+PF_HD bool is_molten(const DB& db, u32 c) { return (db.c_flags[c] & F_MOLTEN) != 0; }
+PF_HD void set_deleted(const DB& db, u32 c) {
+  const u8 f = db.c_flags[c];
+  if (!st_deleted(f)) db.c_flags[c] = (u8)((f & ~ST_MASK) | ST_DELETED);
+}
+PF_HD void set_status(const DB& db, u32 c, u8 st) { db.c_flags[c] = (u8)((db.c_flags[c] & ~ST_MASK) | st); }
+PF_HD void melt(const DB& db, u32 c) { db.c_flags[c] = (u8)(db.c_flags[c] | F_MOLTEN); }
+PF_HD void freeze(const DB& db, u32 c) { db.c_flags[c] = (u8)(db.c_flags[c] & ~F_MOLTEN); }
+PF_HD u32 calc_sig(const u32* l, u32 n) {
+  u32 s = 0;
+  for (u32 i = 0; i < n; i++) s |= MAPHASH(l[i]);
+  return s;
+}
+// value of a literal under the root assignment: -1 unassigned, 0 false, 1 true
+PF_HD int lit_value(const DB& db, u32 l) {
+  const u32 a = db.assign[ABS(l)];
+  if (!a) return -1;
+  return (a - 1) == l ? 1 : 0;
+}
+
+// an occurrence list as the sequential simplifier sees it
+struct OLV {
+  u32* r;
+  u32 n;
+};
+PF_HD OLV olist(const DB& db, u32 lit) { return OLV{db.ot_refs + db.ot_off[lit], db.ot_size[lit]}; }
+
+// ---------------------------------------------------------------- OT construction
+// histogram of live literals (simplify.cpp:45-54 counts, histogram.cpp:82-95)
+PF_HD void hist_clause(const DB& db, u32 c, u32* hist) {
+  if (is_deleted(db, c)) return;
+  const CL cl = clause(db, c);
+  for (u32 i = 0; i < cl.n; i++) at_add(&hist[cl.l[i]], 1u);
+}
+// scatter of clause ids into the lists (simplify.cpp:45-54); order within a
+// list is fixed afterwards where it matters (elected lists are sorted)
+PF_HD void scatter_clause(const DB& db, u32 c, u32* cursor) {
+  if (is_deleted(db, c)) return;
+  const CL cl = clause(db, c);
+  for (u32 i = 0; i < cl.n; i++) {
+    const u32 l = cl.l[i];
+    const u32 pos = at_add(&cursor[l], 1u);
+    db.ot_refs[db.ot_off[l] + pos] = c;
+  }
+}
+
+// CNF_CMP_KEY (src/cpu/simplify.h:29-42) completed by the clause id
+struct SortKey {
+  u32 size, lit0, back, sig, id;
+};
+PF_HD SortKey make_key(const DB& db, u32 c) {
+  const u32 n = db.c_size[c];
+  const u32* l = db.lits + db.c_off[c];
+  SortKey k;
+  k.size = n;
+  k.lit0 = n ? l[0] : 0;
+  k.back = n ? l[n - 1] : 0;
+  k.sig = db.c_sig[c];
+  k.id = c;
+  return k;
+}
+PF_HD bool key_less(const SortKey& a, const SortKey& b) {
+  if (a.size != b.size) return a.size < b.size;
+  if (a.lit0 != b.lit0) return a.lit0 < b.lit0;
+  if (a.back != b.back) return a.back < b.back;
+  if (a.sig != b.sig) return a.sig < b.sig;
+  return a.id < b.id;
+}
+// insertion sort of one short list (used for lists up to a few dozen)
+PF_HD void sort_list_insertion(const DB& db, u32* r, u32 n) {
+  for (u32 i = 1; i < n; i++) {
+    const u32 t = r[i];
+    const SortKey kt = make_key(db, t);
+    u32 j = i;
+    while (j > 0) {
+      const SortKey kj = make_key(db, r[j - 1]);
+      if (!key_less(kt, kj)) break;
+      r[j] = r[j - 1];
+      j--;
+    }
+    r[j] = t;
+  }
+}
+
+// ---------------------------------------------------------------- BCP
+// Visit clause c after one of its literals became false.  Claims a new unit
+// with a compare-and-swap on its variable and appends it to 'next'.
+PF_HD void bcp_visit(const DB& db, u32 c, u32* next, u32* nnext, u32* conflict) {
+  if (is_deleted(db, c)) return;
+  const CL cl = clause(db, c);
+  u32 nun = 0, last = 0;
+  for (u32 i = 0; i < cl.n; i++) {
+    const u32 l = cl.l[i];
+    const u32 a = at_load(&db.assign[ABS(l)]);
+    if (!a) {
+      nun++;
+      last = l;
+    } else if (a - 1 == l) {
+      return;  // satisfied
+    }
+  }
+  if (nun == 0) {
+    *conflict = 1;
+    return;
+  }
+  if (nun == 1) {
+    const u32 old = at_cas(&db.assign[ABS(last)], 0u, last + 1);
+    if (old == 0) next[at_add(nnext, 1u)] = last;
+    else if (old != last + 1) *conflict = 1;
+  }
+}
+// After the fixpoint: delete satisfied clauses, strip false literals
+// (elimination.cpp:41-62 applied for every assigned literal).
+PF_HD void bcp_clean(const DB& db, u32 c) {
+  if (is_deleted(db, c)) return;
+  u32* l = db.lits + db.c_off[c];
+  const u32 n = db.c_size[c];
+  u32 j = 0, sig = 0;
+  for (u32 i = 0; i < n; i++) {
+    const int v = lit_value(db, l[i]);
+    if (v > 0) {
+      set_deleted(db, c);
+      return;
+    }
+    if (v < 0) {
+      l[j++] = l[i];
+      sig |= MAPHASH(l[i]);
+    }
+  }
+  if (j != n) {
+    db.c_size[c] = j;
+    db.c_sig[c] = sig;
+  }
+}
+
+// ---------------------------------------------------------------- LCVE
+// Greedy election (lcve.cpp:46-99) as the lexicographically-first
+// independent set over the candidate order: a candidate is elected iff no
+// earlier candidate sharing a live clause is elected.  state: 0 undecided,
+// 1 elected, 2 frozen.  Returns 1 if the candidate stays undecided.
+PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* state) {
+  if (at_load(&state[i])) return 0;
+  const u32 v = cand[i];
+  bool blocked = false;
+  for (u32 s = 0; s < 2; s++) {
+    const OLV ol = olist(db, V2L(v) | s);
+    for (u32 k = 0; k < ol.n; k++) {
+      const u32 c = ol.r[k];
+      if (is_deleted(db, c)) continue;
+      const CL cl = clause(db, c);
+      for (u32 q = 0; q < cl.n; q++) {
+        const u32 u = ABS(cl.l[q]);
+        if (u == v) continue;
+        const u32 pu = pos[u];
+        if (pu >= i) continue;
+        const u32 su = at_load(&state[pu]);
+        if (su == 1) {
+          state[i] = 2;
+          return 0;
+        }
+        if (su == 0) blocked = true;
+      }
+    }
+  }
+  if (blocked) return 1;
+  state[i] = 1;
+  return 0;
+}
+
+// ---------------------------------------------------------------- shared helpers
+PF_HD bool sub_sig(u32 A, u32 B) { return !(A & ~B); }  // simplify.h:84
+PF_HD bool selfsub_sig(u32 A, u32 B) {                   // simplify.h:86-90
+  const u32 Bt = B | ((B & 0xAAAAAAAAu) >> 1) | ((B & 0x55555555u) << 1);
+  return !(A & ~Bt);
+}
+// simplify.h:175-202: resolvent length on x, 0 when tautological
+PF_HD u32 merge_len(u32 x, CL a, CL b) {
+  u32 i1 = 0, i2 = 0;
+  u32 len = a.n + b.n - 2;
+  while (i1 < a.n && i2 < b.n) {
+    const u32 l1 = a.l[i1], l2 = b.l[i2];
+    const u32 v1 = ABS(l1), v2 = ABS(l2);
+    if (v1 == x) i1++;
+    else if (v2 == x) i2++;
+    else if ((l1 ^ l2) == 1u) return 0;
+    else if (v1 < v2) i1++;
+    else if (v2 < v1) i2++;
+    else { i1++; i2++; len--; }
+  }
+  return len;
+}
+// simplify.h:139-173 for a resolvent known not to be tautological
+PF_HD u32 merge_out(u32 x, CL a, CL b, u32* out) {
+  u32 i1 = 0, i2 = 0, n = 0;
+  while (i1 < a.n && i2 < b.n) {
+    const u32 l1 = a.l[i1], l2 = b.l[i2];
+    const u32 v1 = ABS(l1), v2 = ABS(l2);
+    if (v1 == x) i1++;
+    else if (v2 == x) i2++;
+    else if (v1 < v2) { i1++; out[n++] = l1; }
+    else if (v2 < v1) { i2++; out[n++] = l2; }
+    else { i1++; i2++; out[n++] = l1; }
+  }
+  while (i1 < a.n) { const u32 l = a.l[i1++]; if (ABS(l) != x) out[n++] = l; }
+  while (i2 < b.n) { const u32 l = b.l[i2++]; if (ABS(l) != x) out[n++] = l; }
+  return n;
+}
+// simplify.h:119-137
+PF_HD bool is_tautology(u32 x, CL a, CL b) {
+  u32 i1 = 0, i2 = 0;
+  while (i1 < a.n && i2 < b.n) {
+    const u32 l1 = a.l[i1], l2 = b.l[i2];
+    const u32 v1 = ABS(l1), v2 = ABS(l2);
+    if (v1 == x) i1++;
+    else if (v2 == x) i2++;
+    else if ((l1 ^ l2) == 1u) return true;
+    else if (v1 < v2) i1++;
+    else if (v2 < v1) i2++;
+    else { i1++; i2++; }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------- SUB (subsume.h)
+constexpr int SUB_MAX_UNITS = 32;
+struct SubOut {
+  u32* rec;       // (elected index, sequence, unit literal) records
+  u32* n;         // number of records
+  u32 cap;
+  u32* unsat_at;  // min elected index whose strengthening refuted the formula
+  u32* overflow;  // set when a variable derives more than SUB_MAX_UNITS units
+};
+
+struct SubLocal {
+  u32 units[SUB_MAX_UNITS];
+  u32 n;
+  bool overflow;
+  bool unsat;
+};
+// value of a literal seen by one variable: committed root assignment plus
+// the units this variable derived so far
+PF_HD int sub_value(const DB& db, const SubLocal& sl, u32 l) {
+  const int v = lit_value(db, l);
+  if (v >= 0) return v;
+  for (u32 i = 0; i < sl.n; i++) {
+    if (sl.units[i] == l) return 1;
+    if (sl.units[i] == FLIP(l)) return 0;
+  }
+  return -1;
+}
+PF_HD bool has_lit(CL c, u32 l) {
+  for (u32 i = 0; i < c.n; i++)
+    if (c.l[i] == l) return true;
+  return false;
+}
+// deletes the clauses of this variable's lists that contain 'unit'
+// (the part of toblivion(ot[unit]) this variable observes)
+PF_HD void sub_local_oblivion(const DB& db, u32 p, u32 unit) {
+  for (u32 s = 0; s < 2; s++) {
+    const OLV ol = olist(db, p | s);
+    for (u32 k = 0; k < ol.n; k++) {
+      const u32 c = ol.r[k];
+      if (!is_deleted(db, c) && has_lit(clause(db, c), unit)) set_deleted(db, c);
+    }
+  }
+}
+PF_HD void bump_shrunken(const DB& db, u32 c, int lbd_tier1) {  // elimination.cpp:28-39
+  const int old = (int)db.c_lbd[c];
+  if (old <= lbd_tier1) return;
+  int nl = (int)db.c_size[c] - 1;
+  if (nl > old) nl = old;
+  if (nl >= old) return;
+  db.c_lbd[c] = (u32)nl;
+  db.c_flags[c] = (u8)((db.c_flags[c] & ~USAGE_MASK) | (1u << USAGE_SHIFT));
+}
+// elimination.cpp:199-234
+PF_HD void strengthen(const DB& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_tier1) {
+  u32* l = db.lits + db.c_off[c];
+  const u32 n0 = db.c_size[c];
+  u32 n = 0, sig = 0;
+  for (u32 k = 0; k < n0; k++) {
+    const u32 x = l[k];
+    if (x != me) { l[n++] = x; sig |= MAPHASH(x); }
+  }
+  db.c_sig[c] = sig;
+  db.c_size[c] = n0 - 1;
+  if (n == 1) {
+    const u32 unit = l[0];
+    const int val = sub_value(db, sl, unit);
+    if (val < 0) {
+      if (sl.n < (u32)SUB_MAX_UNITS) sl.units[sl.n++] = unit;
+      else sl.overflow = true;
+      sub_local_oblivion(db, p, unit);
+    } else if (!val) {
+      sl.unsat = true;
+    }
+  } else if (is_learnt(db, c)) {
+    bump_shrunken(db, c, lbd_tier1);
+  }
+}
+// subsume.h:39-56
+PF_HD bool sub_lits(CL sm, CL lr) {
+  u32 i1 = 0, i2 = 0, s = 0;
+  while (i1 < sm.n && i2 < lr.n) {
+    const u32 a = sm.l[i1], b = lr.l[i2];
+    if (a < b) i1++;
+    else if (b < a) i2++;
+    else { s++; i1++; i2++; }
+  }
+  return s == sm.n;
+}
+// subsume.h:75-103
+PF_HD bool selfsub_lits(u32 x, u32 fx, CL sm, CL lr) {
+  u32 i1 = 0, i2 = 0, s = 0;
+  bool self = false;
+  while (i1 < sm.n && i2 < lr.n) {
+    const u32 a = sm.l[i1], b = lr.l[i2];
+    if (a == fx) i1++;
+    else if (b == x) { self = true; i2++; }
+    else if (a < b) i1++;
+    else if (b < a) i2++;
+    else { s++; i1++; i2++; }
+  }
+  if (s + 1 == sm.n) {
+    if (self) return true;
+    while (i2 < lr.n) {
+      if (lr.l[i2] == x) return true;
+      i2++;
+    }
+  }
+  return false;
+}
+// subsume.h:105-121: is 'cand' subsumed by an earlier entry of the list
+PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand) {
+  const u32 csz = db.c_size[cand];
+  const bool cm = is_molten(db, cand);
+  for (u32 j = 0; j < end; j++) {
+    const u32 s = ol.r[j];
+    if (is_deleted(db, s)) continue;
+    const u32 ssz = db.c_size[s];
+    if (cm && ssz > csz) continue;
+    if (ssz > 1 && sub_sig(db.c_sig[s], db.c_sig[cand]) && sub_lits(clause(db, s), clause(db, cand))) {
+      if (is_learnt(db, s) && is_original(db, cand)) set_status(db, s, ST_ORIGINAL);
+      set_deleted(db, cand);
+      return true;
+    }
+  }
+  return false;
+}
+// subsume.h:123-142: strengthen 'cand' by removing x if a clause of 'ol'
+// self-subsumes it
+PF_HD bool selfsubsume_cand(const DB& db, u32 x, u32 fx, OLV ol, u32 cand, u32 p, SubLocal& sl,
+                            int lbd_tier1) {
+  const u32 csz = db.c_size[cand];
+  const u32 csig = db.c_sig[cand];
+  for (u32 j = 0; j < ol.n; j++) {
+    const u32 s = ol.r[j];
+    const u32 ssz = db.c_size[s];
+    if (ssz > csz) break;
+    if (is_deleted(db, s)) continue;
+    if (ssz > 1 && selfsub_sig(db.c_sig[s], csig) && selfsub_lits(x, fx, clause(db, s), clause(db, cand))) {
+      strengthen(db, cand, x, p, sl, lbd_tier1);
+      melt(db, cand);
+      return true;
+    }
+  }
+  return false;
+}
+// subsume.h:27-37
+PF_HD void update_ol(const DB& db, u32 lit) {
+  u32* r = db.ot_refs + db.ot_off[lit];
+  const u32 n = db.ot_size[lit];
+  u32 j = 0;
+  for (u32 i = 0; i < n; i++) {
+    const u32 c = r[i];
+    if (is_molten(db, c)) freeze(db, c);
+    else if (!is_deleted(db, c)) r[j++] = c;
+  }
+  db.ot_size[lit] = j;
+}
+// subsume.h:144-179 + the list-size gate of elimination.cpp:121-124, for
+// elected variable v (index k in elected order)
+PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& out) {
+  const u32 p = V2L(v), n = NEGL(p);
+  if ((int)db.ot_size[p] > o.sub_limit || (int)db.ot_size[n] > o.sub_limit) return;
+  SubLocal sl;
+  sl.n = 0;
+  sl.overflow = false;
+  sl.unsat = false;
+  {
+    const OLV poss = olist(db, p);
+    for (u32 i = 0; i < poss.n; i++) {
+      const u32 c = poss.r[i];
+      if ((int)db.c_size[c] > HSE_MAX_CL_SIZE) break;
+      if (is_deleted(db, c)) continue;
+      selfsubsume_cand(db, p, n, olist(db, n), c, p, sl, o.lbd_tier1);
+      subsume_cand(db, poss, i, c);
+    }
+    update_ol(db, p);
+  }
+  {
+    const OLV negs = olist(db, n);
+    for (u32 i = 0; i < negs.n; i++) {
+      const u32 c = negs.r[i];
+      if ((int)db.c_size[c] > HSE_MAX_CL_SIZE) break;
+      if (is_deleted(db, c)) continue;
+      selfsubsume_cand(db, n, p, olist(db, p), c, p, sl, o.lbd_tier1);
+      subsume_cand(db, negs, i, c);
+    }
+    update_ol(db, n);
+  }
+  for (u32 i = 0; i < sl.n; i++) {
+    const u32 r = at_add(out.n, 1u);
+    if (r < out.cap) {
+      out.rec[3 * (u64)r] = k;
+      out.rec[3 * (u64)r + 1] = i;
+      out.rec[3 * (u64)r + 2] = sl.units[i];
+    }
+  }
+  if (sl.overflow) at_add(out.overflow, 1u);
+  if (sl.unsat) at_min(out.unsat_at, k);
+}
+
+// snapshot / restore of what sub_var may change (list contents and the
+// clauses in them), so that a variable can be re-run after earlier units
+struct SubSnap {
+  u32* base;   // per elected variable: start in the entry arrays
+  u32* np;     // per elected variable: list sizes before SUB
+  u32* nn;
+  u32* ref;    // per entry
+  u8* flg;
+  u32* size;
+  u32* sig;
+  u32* lbd;
+};
+PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
+  u32 e = s.base[k];
+  for (u32 q = 0; q < 2; q++) {
+    const OLV ol = olist(db, V2L(v) | q);
+    if (q == 0) s.np[k] = ol.n;
+    else s.nn[k] = ol.n;
+    for (u32 j = 0; j < ol.n; j++, e++) {
+      const u32 c = ol.r[j];
+      s.ref[e] = c;
+      s.flg[e] = db.c_flags[c];
+      s.size[e] = db.c_size[c];
+      s.sig[e] = db.c_sig[c];
+      s.lbd[e] = db.c_lbd[c];
+    }
+  }
+}
+PF_HD void sub_restore_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
+  u32 e = s.base[k];
+  for (u32 q = 0; q < 2; q++) {
+    const u32 lit = V2L(v) | q;
+    const u32 n = q == 0 ? s.np[k] : s.nn[k];
+    u32* r = db.ot_refs + db.ot_off[lit];
+    for (u32 j = 0; j < n; j++, e++) {
+      const u32 c = s.ref[e];
+      r[j] = c;
+      db.c_flags[c] = s.flg[e];
+      db.c_sig[c] = s.sig[e];
+      db.c_lbd[c] = s.lbd[e];
+      const u32 cur = db.c_size[c];
+      if (cur < s.size[e]) {
+        // the only literal SUB removes from a clause of this list is 'lit'
+        u32* l = db.lits + db.c_off[c];
+        u32 pos = cur;
+        while (pos > 0 && l[pos - 1] > lit) {
+          l[pos] = l[pos - 1];
+          pos--;
+        }
+        l[pos] = lit;
+        db.c_size[c] = s.size[e];
+      }
+    }
+    db.ot_size[lit] = n;
+  }
+}
+// a variable must be re-run when one of its clauses contains a unit that an
+// earlier elected variable derived (unit_by[lit] = smallest such index)
+PF_HD bool sub_dirty_var(const DB& db, u32 k, const SubSnap& s, const u32* unit_by) {
+  u32 e = s.base[k];
+  const u32 tot = s.np[k] + s.nn[k];
+  for (u32 j = 0; j < tot; j++, e++) {
+    const u32 c = s.ref[e];
+    const u32* l = db.lits + db.c_off[c];
+    const u32 n = s.size[e];  // restored length covers the original literals
+    const u32 cur = db.c_size[c];
+    for (u32 i = 0; i < (cur < n ? cur : n); i++)
+      if (unit_by[l[i]] < k) return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------- BVE (bve.cpp, gates)
+enum : u32 { VT_NONE = 0, VT_PURE = 1, VT_EQU = 2, VT_RES = 3, VT_SUBST = 4 };
+struct VeDecision {
+  u32* type;      // per elected variable
+  u32* def;       // EQU definition literal
+  u32* ncls;      // resolvents of size > 1
+  u32* nlits;     // their literals
+  u32* nstack;    // eliminated-stack words
+  u32* scratch;   // per variable scratch (fan-in literals)
+  u32* scr_base;  // per variable start in scratch
+};
+
+PF_HD u32 count_orgs(const DB& db, OLV ol) {
+  u32 n = 0;
+  for (u32 i = 0; i < ol.n; i++) n += is_original(db, ol.r[i]);
+  return n;
+}
+PF_HD u32 count_lits_before(const DB& db, OLV ol) {
+  u32 n = 0;
+  for (u32 i = 0; i < ol.n; i++)
+    if (is_original(db, ol.r[i])) n += db.c_size[ol.r[i]];
+  return n;
+}
+// simplify.h:237-295: true when the bound is exceeded; nAddedCls / lits
+PF_HD bool count_res(const DB& db, const Options& o, u32 x, int clsbefore, OLV me, OLV other,
+                     int& nAddedCls, u32& nAddedLits, bool substitute) {
+  nAddedLits = 0;
+  const int rlimit = o.ve_clause_limit;
+  for (u32 i = 0; i < me.n; i++) {
+    const u32 ci = me.r[i];
+    if (!is_original(db, ci)) continue;
+    const bool a = is_molten(db, ci);
+    const CL A = clause(db, ci);
+    for (u32 j = 0; j < other.n; j++) {
+      const u32 cj = other.r[j];
+      if (!is_original(db, cj)) continue;
+      if (substitute && a == is_molten(db, cj)) continue;
+      const u32 rsize = merge_len(x, A, clause(db, cj));
+      if (rsize > 1) {
+        if (++nAddedCls > clsbefore || (rlimit && (int)rsize > rlimit)) return true;
+        nAddedLits += rsize;
+      }
+    }
+  }
+  if (o.ve_lbound_en) {
+    const u32 before = count_lits_before(db, me) + count_lits_before(db, other);
+    if (nAddedLits > before) return true;
+  }
+  return false;
+}
+PF_HD void freeze_binaries(const DB& db, OLV ol) {  // simplify.h:204-211
+  for (u32 i = 0; i < ol.n; i++) {
+    const u32 c = ol.r[i];
+    if (is_original(db, c) && db.c_size[c] == 2) freeze(db, c);
+  }
+}
+// equivalence.h:89-131
+PF_HD u32 find_sfanin(const DB& db, u32 gate_out, OLV ol) {
+  u32 imp = 0;
+  int nImps = 0;
+  for (u32 i = 0; i < ol.n; i++) {
+    const u32 c = ol.r[i];
+    if (is_original(db, c) && db.c_size[c] == 2) {
+      const u32* l = db.lits + db.c_off[c];
+      imp = FLIP(l[0] ^ l[1] ^ gate_out);
+      melt(db, c);
+      nImps++;
+    }
+    if (nImps > 1) return 0;
+  }
+  return imp;
+}
+PF_HD u32 find_equ_gate(const DB& db, u32 p) {
+  const u32 n = NEGL(p);
+  const OLV poss = olist(db, p), negs = olist(db, n);
+  u32 first = find_sfanin(db, p, poss);
+  if (first) {
+    u32 second = n, def = first;
+    if (second < first) first = second, second = def;
+    for (u32 i = 0; i < negs.n; i++) {
+      const u32 c = negs.r[i];
+      if (is_original(db, c) && db.c_size[c] == 2) {
+        const u32* l = db.lits + db.c_off[c];
+        if (l[0] == first && l[1] == second) {
+          melt(db, c);
+          return def;
+        }
+      }
+    }
+  }
+  freeze_binaries(db, poss);
+  return 0;
+}
+// and.h:25-86; 'out' is per-variable scratch
+PF_HD bool find_ao_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, u32* out, int& nAddedCls,
+                        u32& nAddedLits) {
+  const u32 x = ABS(dx);
+  const OLV it = olist(db, dx);
+  u32 nout = 0, sig = 0;
+  for (u32 i = 0; i < it.n; i++) {
+    const u32 c = it.r[i];
+    if (is_original(db, c) && db.c_size[c] == 2) {
+      const u32* l = db.lits + db.c_off[c];
+      const u32 imp = FLIP(l[0] ^ l[1] ^ dx);
+      out[nout++] = imp;
+      sig |= MAPHASH(imp);
+      melt(db, c);
+    }
+  }
+  if (nout > 1) {
+    const u32 f = FLIP(dx);
+    out[nout++] = f;
+    sig |= MAPHASH(f);
+    for (u32 i = 1; i < nout; i++) {  // ascending order (distinct literals)
+      const u32 t = out[i];
+      u32 j = i;
+      while (j > 0 && out[j - 1] > t) { out[j] = out[j - 1]; j--; }
+      out[j] = t;
+    }
+    const OLV ot2 = olist(db, f);
+    for (u32 i = 0; i < ot2.n; i++) {
+      const u32 c = ot2.r[i];
+      if (!is_original(db, c) || db.c_size[c] != nout || !sub_sig(db.c_sig[c], sig)) continue;
+      const u32* l = db.lits + db.c_off[c];
+      bool eq = true;
+      for (u32 q = 0; q < nout && eq; q++) eq = (l[q] == out[q]);
+      if (!eq) continue;
+      melt(db, c);
+      nAddedCls = 0;
+      if (count_res(db, o, x, nOrgCls, it, ot2, nAddedCls, nAddedLits, true)) {
+        freeze(db, c);
+        break;
+      }
+      return true;
+    }
+  }
+  freeze_binaries(db, it);
+  return false;
+}
+// ifthenelse.h:25-45: first (lowest id) live non-molten original ternary
+// equal to {x, y, z}, searched in the shortest of the three lists; NOPOS
+// if none
+PF_HD u32 fast_equality_check(const DB& db, u32 x, u32 y, u32 z) {
+  if (db.ot_size[y] > db.ot_size[z]) { const u32 t = y; y = z; z = t; }
+  if (db.ot_size[x] > db.ot_size[y]) { const u32 t = x; x = y; y = t; }
+  const OLV ol = olist(db, x);
+  u32 a = x, b = y, c = z, t;
+  if (b > c) { t = b; b = c; c = t; }
+  if (a > c) { t = a; a = c; c = t; }
+  if (a > b) { t = a; a = b; b = t; }
+  u32 best = NOPOS;
+  for (u32 i = 0; i < ol.n; i++) {
+    const u32 cl = ol.r[i];
+    if (is_molten(db, cl)) continue;
+    if (!is_original(db, cl) || db.c_size[cl] != 3) continue;
+    const u32* l = db.lits + db.c_off[cl];
+    if (l[0] == a && l[1] == b && l[2] == c && cl < best) best = cl;
+  }
+  return best;
+}
+PF_HD bool find_ite_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
+                         u32& nAddedLits) {
+  const u32 fx = FLIP(dx);
+  const OLV it = olist(db, dx);
+  for (u32 i = 0; i < it.n; i++) {
+    const u32 ci = it.r[i];
+    if (!(is_original(db, ci) && db.c_size[ci] == 3)) continue;
+    const u32* li = db.lits + db.c_off[ci];
+    u32 xi = li[0], yi = li[1], zi = li[2], t;
+    if (yi == dx) { t = xi; xi = yi; yi = t; }
+    if (zi == dx) { t = xi; xi = zi; zi = t; }
+    for (u32 j = i + 1; j < it.n; j++) {
+      const u32 cj = it.r[j];
+      if (!(is_original(db, cj) && db.c_size[cj] == 3)) continue;
+      const u32* lj = db.lits + db.c_off[cj];
+      u32 xj = lj[0], yj = lj[1], zj = lj[2];
+      if (yj == dx) { t = xj; xj = yj; yj = t; }
+      if (zj == dx) { t = xj; xj = zj; zj = t; }
+      if (ABS(yi) == ABS(zj)) { t = yj; yj = zj; zj = t; }
+      if (ABS(zi) == ABS(zj)) continue;
+      if (yi != FLIP(yj)) continue;
+      const u32 d1 = fast_equality_check(db, fx, yi, FLIP(zi));
+      if (d1 == NOPOS) continue;
+      const u32 d2 = fast_equality_check(db, fx, yj, FLIP(zj));
+      if (d2 == NOPOS) continue;
+      melt(db, ci); melt(db, cj); melt(db, d1); melt(db, d2);
+      nAddedCls = 0;
+      if (count_res(db, o, ABS(dx), nOrgCls, it, olist(db, fx), nAddedCls, nAddedLits, true)) {
+        freeze(db, ci); freeze(db, cj); freeze(db, d1); freeze(db, d2);
+        return false;
+      }
+      return true;
+    }
+  }
+  return false;
+}
+// xor.h:31-43
+PF_HD void freeze_arities(const DB& db, OLV me, OLV other) {
+  for (u32 i = 0; i < me.n; i++) {
+    const u32 c = me.r[i];
+    if (db.c_size[c] > 2 && is_molten(db, c)) freeze(db, c);
+  }
+  for (u32 i = 0; i < other.n; i++) {
+    const u32 c = other.r[i];
+    if (db.c_size[c] > 2 && is_molten(db, c)) freeze(db, c);
+  }
+}
+// xor.h:70-102: flip the literals for the next parity pattern and look for
+// the matching clause (lowest id) in the shortest list
+PF_HD bool make_arity(const DB& db, u32& parity, u32* ls, u32 size) {
+  const u32 old = parity;
+  do { parity++; } while (__builtin_popcount(parity) & 1);
+  for (u32 k = 0; k < size; k++) {
+    const u32 bit = 1u << k;
+    if ((parity & bit) != (old & bit)) ls[k] = FLIP(ls[k]);
+  }
+  u32 best = ls[0];
+  u32 minsize = db.ot_size[best];
+  for (u32 k = 1; k < size; k++) {
+    const u32 s = db.ot_size[ls[k]];
+    if (s < minsize) { minsize = s; best = ls[k]; }
+  }
+  const OLV ol = olist(db, best);
+  u32 found = NOPOS;
+  for (u32 i = 0; i < ol.n; i++) {
+    const u32 c = ol.r[i];
+    if (!is_original(db, c) || db.c_size[c] != size || c >= found) continue;
+    const u32* l = db.lits + db.c_off[c];
+    bool all = true;
+    for (u32 q = 0; q < size && all; q++) {
+      bool in = false;
+      for (u32 w = 0; w < size; w++) in |= (l[q] == ls[w]);
+      all = in;
+    }
+    if (all) found = c;
+  }
+  if (found == NOPOS) return false;
+  melt(db, found);
+  return true;
+}
+PF_HD bool find_xor_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
+                         u32& nAddedLits) {
+  const u32 fx = FLIP(dx), v = ABS(dx);
+  const OLV it = olist(db, dx), ot2 = olist(db, fx);
+  u32 ls[32];
+  for (u32 i = 0; i < it.n; i++) {
+    const u32 ci = it.r[i];
+    if (!is_original(db, ci)) continue;
+    const u32 size = db.c_size[ci];
+    const int arity = (int)size - 1;
+    if (size < 3 || arity > o.xor_max_arity || size > 32) continue;
+    const u32* l = db.lits + db.c_off[ci];
+    for (u32 q = 0; q < size; q++) ls[q] = l[q];
+    u32 parity = 0;
+    int itargets = 1 << arity;
+    while (--itargets && make_arity(db, parity, ls, size)) {
+    }
+    if (itargets) freeze_arities(db, it, ot2);
+    else {
+      melt(db, ci);
+      nAddedCls = 0;
+      if (count_res(db, o, v, nOrgCls, it, ot2, nAddedCls, nAddedLits, true)) {
+        freeze_arities(db, it, ot2);
+        break;
+      }
+      return true;
+    }
+  }
+  return false;
+}
+// stack words written by toblivion / save_BN_gate (simplify.h:305-332,
+// equivalence.h:133-154): the originals of the side with fewer originals
+PF_HD u32 stack_words(const DB& db, OLV side) {
+  u32 w = 2;
+  for (u32 i = 0; i < side.n; i++) {
+    const u32 c = side.r[i];
+    if (is_original(db, c)) w += db.c_size[c] + 1;
+  }
+  return w;
+}
+// bve.cpp:138-241, decision part, for elected variable v at index k
+PF_HD void ve_decide(const DB& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
+  const u32 p = V2L(v), n = NEGL(p);
+  const OLV poss = olist(db, p), negs = olist(db, n);
+  const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);
+  const OLV side = pOrgs > nOrgs ? negs : poss;
+  u32 type = VT_NONE, def = 0;
+  int nAddedCls = 0;
+  u32 nAddedLits = 0;
+  if (!pOrgs || !nOrgs) {
+    type = VT_PURE;
+  } else if ((def = find_equ_gate(db, p)) != 0) {
+    type = VT_EQU;
+  } else {
+    const int nOrgCls = (int)(pOrgs + nOrgs);
+    u32* out = d.scratch + d.scr_base[k];
+    if (nOrgCls > 2) {
+      if (find_ao_gate(db, o, n, nOrgCls, out, nAddedCls, nAddedLits)) type = VT_SUBST;
+      else if (!nAddedCls && find_ao_gate(db, o, p, nOrgCls, out, nAddedCls, nAddedLits)) type = VT_SUBST;
+    }
+    if (!type && nOrgCls > 3) {
+      if (find_ite_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+      else if (!nAddedCls && find_ite_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+      else if (find_xor_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+      else if (!nAddedCls && find_xor_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+    }
+    if (!type && !nAddedCls && !count_res(db, o, v, nOrgCls, poss, negs, nAddedCls, nAddedLits, false))
+      type = VT_RES;
+  }
+  d.type[k] = type;
+  d.def[k] = def;
+  const bool adds = (type == VT_RES || type == VT_SUBST) && nAddedCls;
+  d.ncls[k] = adds ? (u32)nAddedCls : 0;
+  d.nlits[k] = adds ? nAddedLits : 0;
+  d.nstack[k] = type ? stack_words(db, side) : 0;
+}
+
+// unit records appended by VE / BCP: (elected index, sequence, literal)
+struct UnitOut {
+  u32* rec;   // 3 words per record
+  u32* n;
+  u32 cap;
+};
+PF_HD void push_unit(const UnitOut& u, u32 k, u32 seq, u32 lit) {
+  const u32 i = at_add(u.n, 1u);
+  if (i < u.cap) {
+    u.rec[3 * (u64)i] = k;
+    u.rec[3 * (u64)i + 1] = seq;
+    u.rec[3 * (u64)i + 2] = lit;
+  }
+}
+
+// model.cpp:261-305
+PF_HD u32 org_lit(const DB& db, u32 l) { return V2L(db.vorg[ABS(l)]) | SIGN(l); }
+PF_HD u32 save_clause(const DB& db, u32 c, u32 witlit, u32* st) {
+  const CL cl = clause(db, c);
+  u32 pos = 0;
+  for (u32 i = 0; i < cl.n; i++) {
+    st[i] = org_lit(db, cl.l[i]);
+    if (cl.l[i] == witlit) pos = i;
+  }
+  if (pos) { const u32 t = st[pos]; st[pos] = st[0]; st[0] = t; }
+  st[cl.n] = cl.n;
+  return cl.n + 1;
+}
+PF_HD u32 save_witness(const DB& db, u32 w, u32* st) {
+  st[0] = org_lit(db, w);
+  st[1] = 1;
+  return 2;
+}
+// writes the eliminated-stack records of toblivion / save_BN_gate
+PF_HD void save_side(const DB& db, u32 p, u32 pOrgs, u32 nOrgs, u32* st) {
+  const u32 n = NEGL(p);
+  const bool negside = pOrgs > nOrgs;
+  const OLV side = olist(db, negside ? n : p);
+  const u32 wit = negside ? n : p;
+  u32 w = 0;
+  for (u32 i = 0; i < side.n; i++) {
+    const u32 c = side.r[i];
+    if (is_original(db, c)) w += save_clause(db, c, wit, st + w);
+  }
+  save_witness(db, negside ? p : n, st + w);
+}
+PF_HD void delete_lists(const DB& db, u32 p) {
+  for (u32 q = 0; q < 2; q++) {
+    const OLV ol = olist(db, p | q);
+    for (u32 i = 0; i < ol.n; i++) set_deleted(db, ol.r[i]);
+    db.ot_size[p | q] = 0;
+  }
+}
+// equivalence.h:25-43: substitute def for dx in clause c (kept sorted);
+// returns the unit literal when one literal remains, else 0
+PF_HD u32 substitute_clause(const DB& db, u32 dx, u32 c, u32 def) {
+  u32* l = db.lits + db.c_off[c];
+  const u32 n0 = db.c_size[c];
+  u32 n = 0;
+  for (u32 i = 0; i < n0; i++) {
+    const u32 x = l[i];
+    if (x == dx) l[n++] = def;
+    else if (x != def) l[n++] = x;
+  }
+  db.c_size[c] = n;
+  if (n == 1) return l[0];
+  for (u32 i = 1; i < n; i++) {
+    const u32 t = l[i];
+    u32 j = i;
+    while (j > 0 && l[j - 1] > t) { l[j] = l[j - 1]; j--; }
+    l[j] = t;
+  }
+  db.c_sig[c] = calc_sig(l, n);
+  return 0;
+}
+struct VeApply {
+  const u32* cls_base;   // per elected variable: first new clause id - cls0
+  const u32* lit_base;   // per elected variable: first literal slot - lit0
+  const u32* stack_base; // per elected variable: first stack word - stack0
+  u32 cls0, lit0, stack0;
+  u32 cls_cap, lit_cap, stack_cap;  // capacities: writes past them are refused
+  u32* stack;
+  UnitOut units;
+  u32* error;  // set when a write would leave its buffer
+};
+// bve.cpp:138-241, execution part
+PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApply& a) {
+  const u32 type = d.type[k];
+  if (!type) return;
+  const u32 p = V2L(v), n = NEGL(p);
+  const OLV poss = olist(db, p), negs = olist(db, n);
+  const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);
+  const u32 sbase = a.stack0 + a.stack_base[k];
+  if (sbase + d.nstack[k] > a.stack_cap) {
+    at_add(a.error, 1u);
+    return;
+  }
+  u32* st = a.stack + sbase;
+  u32 seq = 0;
+  if (type == VT_EQU) {
+    save_side(db, p, pOrgs, nOrgs, st);
+    // equivalence.h:45-87
+    const u32 def = d.def[k], def_f = FLIP(def);
+    for (u32 q = 0; q < 2; q++) {
+      const OLV ol = q == 0 ? negs : poss;
+      const u32 dx = q == 0 ? n : p;
+      const u32 sub = q == 0 ? def_f : def;  // replacement literal
+      for (u32 i = 0; i < ol.n; i++) {
+        const u32 c = ol.r[i];
+        if (is_learnt(db, c) || is_molten(db, c) || has_lit(clause(db, c), FLIP(sub))) set_deleted(db, c);
+        else if (is_original(db, c)) {
+          const u32 unit = substitute_clause(db, dx, c, sub);
+          if (unit) push_unit(a.units, k, seq++, unit);
+        }
+      }
+    }
+  } else {
+    if (type == VT_RES || type == VT_SUBST) {
+      const bool substitute = type == VT_SUBST;
+      u32 dx = p, fx = n;
+      if (db.ot_size[dx] > db.ot_size[fx]) { dx = n; fx = p; }
+      const OLV me = olist(db, dx), other = olist(db, fx);
+      u32 cid = a.cls0 + a.cls_base[k], lpos = a.lit0 + a.lit_base[k];
+      for (u32 i = 0; i < me.n; i++) {
+        const u32 ci = me.r[i];
+        if (!is_original(db, ci)) continue;
+        const bool ma = is_molten(db, ci);
+        const CL A = clause(db, ci);
+        for (u32 j = 0; j < other.n; j++) {
+          const u32 cj = other.r[j];
+          if (!is_original(db, cj)) continue;
+          if (substitute && ma == is_molten(db, cj)) continue;
+          const CL B = clause(db, cj);
+          const u32 len = merge_len(v, A, B);
+          if (!len) {
+            // tautology, or a unit resolvent of two literals (len counts
+            // duplicates once; a length of 0 only arises from a tautology)
+            continue;
+          }
+          if (len == 1) {
+            u32 one[2];
+            merge_out(v, A, B, one);
+            push_unit(a.units, k, seq++, one[0]);
+            continue;
+          }
+          if (cid >= a.cls_cap || lpos + len > a.lit_cap) {
+            at_add(a.error, 1u);
+            return;
+          }
+          u32* out = db.lits + lpos;
+          merge_out(v, A, B, out);
+          db.c_off[cid] = lpos;
+          db.c_size[cid] = len;
+          db.c_sig[cid] = calc_sig(out, len);
+          db.c_flags[cid] = F_ADDED;
+          db.c_lbd[cid] = 0;
+          cid++;
+          lpos += len;
+        }
+      }
+    }
+    save_side(db, p, pOrgs, nOrgs, st);
+    delete_lists(db, p);
+  }
+  db.vstate[v] = VS_MELTED;
+}
+
+// ---------------------------------------------------------------- BCE (elimination.cpp:131-167)
+// count == true: returns the stack words; otherwise writes them at 'st'
+PF_HD u32 bce_var(const DB& db, const Options& o, u32 v, bool count, u32* st) {
+  const u32 p = V2L(v), n = NEGL(p);
+  if ((int)db.ot_size[p] > o.bce_limit || (int)db.ot_size[n] > o.bce_limit) return 0;
+  const OLV poss = olist(db, p), negs = olist(db, n);
+  u32 w = 0;
+  for (u32 i = 0; i < negs.n; i++) {
+    const u32 ng = negs.r[i];
+    if (!is_original(db, ng)) continue;
+    const CL A = clause(db, ng);
+    bool all = true;
+    for (u32 j = 0; j < poss.n && all; j++) {
+      const u32 ps = poss.r[j];
+      if (is_original(db, ps) && !is_tautology(v, A, clause(db, ps))) all = false;
+    }
+    if (all) {
+      if (count) w += A.n + 1;
+      else {
+        w += save_clause(db, ng, n, st + w);
+        set_deleted(db, ng);
+      }
+    }
+  }
+  return w;
+}
+
+// ---------------------------------------------------------------- ERE (redundancy.h)
+constexpr int ERE_MAX_DEL = 32;
+struct EreOut {
+  u32* rec;       // (elected index, sequence, clause id)
+  u32* n;
+  u32 cap;
+  u32* overflow;  // set when one variable exceeds ERE_MAX_DEL deletions
+};
+// redundancy.h:27-61
+PF_HD bool merge_ere(u32 x, CL c1, CL c2, CL t) {
+  u32 i1 = 0, i2 = 0, it = 0;
+  while (it < t.n && i1 < c1.n && i2 < c2.n) {
+    const u32 l = t.l[it];
+    if (ABS(l) == x) return false;
+    const u32 l1 = c1.l[i1], l2 = c2.l[i2];
+    const u32 v1 = ABS(l1), v2 = ABS(l2);
+    if (v1 == x) i1++;
+    else if (v2 == x) i2++;
+    else if (v1 < v2) { if (l1 != l) return false; i1++; it++; }
+    else if (v2 < v1) { if (l2 != l) return false; i2++; it++; }
+    else { if (l1 != l) return false; i1++; i2++; it++; }
+  }
+  while (it < t.n && i1 < c1.n) {
+    const u32 l1 = c1.l[i1];
+    if (ABS(l1) != x && l1 != t.l[it++]) return false;
+    i1++;
+  }
+  while (it < t.n && i2 < c2.n) {
+    const u32 l2 = c2.l[i2];
+    if (ABS(l2) != x && l2 != t.l[it++]) return false;
+    i2++;
+  }
+  return true;
+}
+// redundancy.h:67-121 for one resolvent; 'dl' holds this variable's own
+// deletions so far
+PF_HD void forward_equ(const DB& db, u32 x, u32 c1, u32 c2, int maxsize, u32* dl, u32& nd, bool& ovf) {
+  const CL A = clause(db, c1), B = clause(db, c2);
+  u32 len = A.n + B.n - 2;
+  u32 i1 = 0, i2 = 0;
+  u32 minsize = 0xFFFFFFFFu, msig = 0, best = 0;
+  while (i1 < A.n && i2 < B.n) {
+    const u32 l1 = A.l[i1], l2 = B.l[i2];
+    const u32 v1 = ABS(l1), v2 = ABS(l2);
+    u32 lit;
+    if (v1 == x) { i1++; continue; }
+    else if (v2 == x) { i2++; continue; }
+    else if ((l1 ^ l2) == 1u) return;
+    else if (v1 < v2) { i1++; lit = l1; }
+    else if (v2 < v1) { i2++; lit = l2; }
+    else { i1++; i2++; len--; lit = l1; }
+    const u32 s = db.ot_size[lit];
+    if (s < minsize) { minsize = s; best = lit; }
+    msig |= MAPHASH(lit);
+  }
+  if (!(len > 1 && (!maxsize || (int)len <= maxsize))) return;
+  const u8 type = (is_learnt(db, c1) || is_learnt(db, c2)) ? ST_LEARNT : ST_ORIGINAL;
+  const OLV ml = olist(db, best);
+  u32 found = NOPOS;
+  for (u32 i = 0; i < ml.n; i++) {
+    const u32 c = ml.r[i];
+    if (c >= found) continue;
+    const u8 f = db.c_flags[c];
+    if (db.c_size[c] != len) continue;
+    if (!(st_learnt(f) || (f & ST_MASK) == type)) continue;
+    bool mine = false;
+    for (u32 q = 0; q < nd && q < (u32)ERE_MAX_DEL; q++) mine |= (dl[q] == c);
+    if (mine) continue;
+    if (!sub_sig(msig, db.c_sig[c])) continue;
+    if (!merge_ere(x, A, B, clause(db, c))) continue;
+    found = c;
+  }
+  if (found == NOPOS) return;
+  if (nd < (u32)ERE_MAX_DEL) dl[nd] = found;
+  else ovf = true;
+  nd++;
+}
+// elimination.cpp:169-197 for elected variable v at index k: records the
+// clauses it deletes without applying them
+PF_HD void ere_var(const DB& db, const Options& o, u32 v, u32 k, const EreOut& out) {
+  u32 dx = V2L(v), fx = NEGL(dx);
+  if (db.ot_size[dx] > db.ot_size[fx]) { const u32 t = dx; dx = fx; fx = t; }
+  const OLV me = olist(db, dx), other = olist(db, fx);
+  if ((int)me.n > o.ere_limit || (int)other.n > o.ere_limit) return;
+  u32 dl[ERE_MAX_DEL];
+  u32 nd = 0;
+  bool ovf = false;
+  for (u32 i = 0; i < me.n; i++) {
+    const u32 ci = me.r[i];
+    if (is_deleted(db, ci)) continue;
+    for (u32 j = 0; j < other.n; j++) {
+      const u32 cj = other.r[j];
+      if (is_deleted(db, cj)) continue;
+      forward_equ(db, v, ci, cj, o.ere_max_resolvent, dl, nd, ovf);
+    }
+  }
+  for (u32 q = 0; q < nd && q < (u32)ERE_MAX_DEL; q++) {
+    const u32 i = at_add(out.n, 1u);
+    if (i < out.cap) {
+      out.rec[3 * (u64)i] = k;
+      out.rec[3 * (u64)i + 1] = q;
+      out.rec[3 * (u64)i + 2] = dl[q];
+    }
+  }
+  if (ovf) at_add(out.overflow, 1u);
+}
+// a variable's ERE result is exact unless an earlier variable deleted one
+// of the clauses it read: its own list entries or one of its targets
+PF_HD bool ere_dirty_var(const DB& db, u32 v, u32 k, const u32* del_by) {
+  for (u32 q = 0; q < 2; q++) {
+    const OLV ol = olist(db, V2L(v) | q);
+    for (u32 i = 0; i < ol.n; i++)
+      if (del_by[ol.r[i]] < k) return true;
+  }
+  return false;
+}
+
+}  // namespace pfs

@@ -1,0 +1,836 @@
+// HIP kernels of the simplifier for CDNA4 (gfx950, wave64).
+//
+// Work mapping:
+//   * clause-parallel stages (histogram, scatter, BCP clean-up, counting,
+//     compaction) use one thread per clause;
+//   * list-parallel stages (BCP visits, list deletions) use one workgroup
+//     per occurrence list with the threads striding the list;
+//   * per-variable stages (SUB, BVE, BCE, ERE, LCVE election) use one
+//     thread per elected variable in 64-thread workgroups, so that one
+//     wavefront finishes as soon as its own variables are done;
+//   * scans and the stable radix sort use 256-thread workgroups with
+//     wave64 shuffles for the in-wave prefix and LDS for the wave totals.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <new>
+#include <vector>
+#include <stdexcept>
+#include <string>
+
+#include "pfs_launch.hpp"
+
+namespace pfs {
+
+#define PFS_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));                 \
+  } while (0)
+
+struct KRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+struct Stream {
+  hipStream_t s;
+  int device;
+  bool prof = false;        // record events around every kernel launch
+  std::vector<KRec> recs;   // pending event pairs
+};
+struct Event {
+  hipEvent_t e;
+};
+
+static inline unsigned grid_for(size_t n, unsigned block) {
+  return (unsigned)((n + block - 1) / block);
+}
+#define LAUNCH_CHECK() PFS_HIP(hipGetLastError())
+// launch on the context stream; with profiling on, bracket it with events
+#define KLAUNCH(kern, grid, block, st, ...)                   \
+  do {                                                         \
+    hipEvent_t e0_ = nullptr;                                  \
+    if ((st)->prof) {                                          \
+      PFS_HIP(hipEventCreate(&e0_));                           \
+      PFS_HIP(hipEventRecord(e0_, (st)->s));                   \
+    }                                                          \
+    kern<<<(grid), (block), 0, (st)->s>>>(__VA_ARGS__);        \
+    LAUNCH_CHECK();                                            \
+    if ((st)->prof) {                                          \
+      hipEvent_t e1_;                                          \
+      PFS_HIP(hipEventCreate(&e1_));                           \
+      PFS_HIP(hipEventRecord(e1_, (st)->s));                   \
+      (st)->recs.push_back(KRec{#kern, e0_, e1_});             \
+    }                                                          \
+  } while (0)
+
+// ---------------------------------------------------------------- runtime
+Stream* stream_create(int device) {
+  PFS_HIP(hipSetDevice(device));
+  Stream* st = new Stream();
+  st->device = device;
+  PFS_HIP(hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking));
+  return st;
+}
+void prof_enable(Stream* st, bool on) { st->prof = on; }
+void prof_collect(Stream* st, std::vector<KernelTime>& out) {
+  PFS_HIP(hipStreamSynchronize(st->s));
+  std::map<std::string, size_t> idx;
+  for (size_t i = 0; i < out.size(); i++) idx[out[i].name] = i;
+  for (const KRec& r : st->recs) {
+    float ms = 0;
+    PFS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+    auto it = idx.find(r.name);
+    if (it == idx.end()) {
+      idx[r.name] = out.size();
+      out.push_back(KernelTime{r.name, 0.0, 0});
+      it = idx.find(r.name);
+    }
+    out[it->second].ms += ms;
+    out[it->second].launches++;
+  }
+  st->recs.clear();
+}
+void stream_destroy(Stream* st) {
+  if (!st) return;
+  for (const KRec& r : st->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  (void)hipStreamDestroy(st->s);
+  delete st;
+}
+void stream_sync(Stream* st) { PFS_HIP(hipStreamSynchronize(st->s)); }
+void* dev_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 256;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    throw std::bad_alloc();
+  }
+  return p;
+}
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+void h2d(void* dst, const void* src, size_t bytes, Stream* st) {
+  if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->s));
+}
+void d2h(void* dst, const void* src, size_t bytes, Stream* st) {
+  if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st->s));
+}
+void d2d(void* dst, const void* src, size_t bytes, Stream* st) {
+  if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st->s));
+}
+__global__ void fill_u32_k(u32* d, u32 v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+__global__ void fill_u8_k(u8* d, u8 v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+void fill_u32(u32* dst, u32 value, size_t n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(fill_u32_k, grid_for(n, 256), 256, st, dst, value, n);
+}
+void fill_u8(u8* dst, u8 value, size_t n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(fill_u8_k, grid_for(n, 256), 256, st, dst, value, n);
+}
+Event* event_record(Stream* st) {
+  Event* e = new Event();
+  PFS_HIP(hipEventCreate(&e->e));
+  PFS_HIP(hipEventRecord(e->e, st->s));
+  return e;
+}
+float event_ms(Event* a, Event* b) {
+  PFS_HIP(hipEventSynchronize(b->e));
+  float ms = 0;
+  PFS_HIP(hipEventElapsedTime(&ms, a->e, b->e));
+  return ms;
+}
+void event_free(Event* e) {
+  if (!e) return;
+  (void)hipEventDestroy(e->e);
+  delete e;
+}
+const char* device_name(int device) {
+  static hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown";
+  return prop.name;
+}
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ---------------------------------------------------------------- block scan
+constexpr int ST = 256;          // threads per scan workgroup
+constexpr int SIPT = 8;          // items per thread
+constexpr int STILE = ST * SIPT; // items per workgroup
+constexpr int NW = ST / 64;      // wavefronts per workgroup
+
+// exclusive scan of one value per thread over the workgroup; *total gets
+// the workgroup sum.  'lw' holds NW words of LDS.
+__device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  u32 x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const u32 y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) lw[wid] = x;
+  __syncthreads();
+  u32 pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    const u32 s = lw[w];
+    pre += (w < wid) ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+__global__ void scan_reduce_k(const u32* in, size_t n, u32* part) {
+  __shared__ u32 lw[NW];
+  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
+  u32 s = 0;
+#pragma unroll
+  for (int i = 0; i < SIPT; i++)
+    if (base + i < n) s += in[base + i];
+  u32 tot;
+  block_excl_scan(s, lw, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+// exclusive scan of the workgroup partials by one workgroup
+__global__ void scan_top_k(u32* part, size_t nb, u32* total) {
+  __shared__ u32 lw[NW];
+  u32 carry = 0;
+  for (size_t base = 0; base < nb; base += STILE) {
+    const size_t b = base + (size_t)threadIdx.x * SIPT;
+    u32 v[SIPT];
+    u32 s = 0;
+#pragma unroll
+    for (int i = 0; i < SIPT; i++) {
+      v[i] = (b + i < nb) ? part[b + i] : 0u;
+      s += v[i];
+    }
+    u32 tot;
+    u32 pre = block_excl_scan(s, lw, &tot) + carry;
+#pragma unroll
+    for (int i = 0; i < SIPT; i++) {
+      if (b + i < nb) part[b + i] = pre;
+      pre += v[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+__global__ void scan_down_k(const u32* in, u32* out, size_t n, const u32* part) {
+  __shared__ u32 lw[NW];
+  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
+  u32 v[SIPT];
+  u32 s = 0;
+#pragma unroll
+  for (int i = 0; i < SIPT; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0u;
+    s += v[i];
+  }
+  u32 tot;
+  u32 pre = block_excl_scan(s, lw, &tot) + part[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < SIPT; i++) {
+    if (base + i < n) out[base + i] = pre;
+    pre += v[i];
+  }
+}
+size_t scan_tmp_bytes(size_t n) { return (grid_for(n, STILE) + 64) * sizeof(u32); }
+void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream* st) {
+  if (n == 0) {
+    if (total) fill_u32(total, 0, 1, st);
+    return;
+  }
+  const unsigned nb = grid_for(n, STILE);
+  u32* part = (u32*)tmp;
+  KLAUNCH(scan_reduce_k, nb, ST, st, in, n, part);
+  KLAUNCH(scan_top_k, 1, ST, st, part, nb, total);
+  KLAUNCH(scan_down_k, nb, ST, st, in, out, n, part);
+}
+
+// ---------------------------------------------------------------- stable radix sort
+constexpr int RB = 4, RBUCK = 1 << RB;
+
+__global__ void rs_count_k(const u32* keys, size_t n, int shift, u32* counts, u32 nb) {
+  __shared__ u32 h[RBUCK];
+  if (threadIdx.x < RBUCK) h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
+#pragma unroll
+  for (int i = 0; i < SIPT; i++)
+    if (base + i < n) atomicAdd(&h[(keys[base + i] >> shift) & (RBUCK - 1)], 1u);
+  __syncthreads();
+  if (threadIdx.x < RBUCK) counts[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+// each thread owns SIPT consecutive items, so ranks follow input order
+__global__ void rs_scatter_k(const u32* kin, const u32* vin, u32* kout, u32* vout, size_t n, int shift,
+                             const u32* offs, u32 nb) {
+  __shared__ u32 lw[NW];
+  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
+  u32 k[SIPT], v[SIPT];
+  u32 cnt[RBUCK];
+#pragma unroll
+  for (int d = 0; d < RBUCK; d++) cnt[d] = 0;
+#pragma unroll
+  for (int i = 0; i < SIPT; i++) {
+    if (base + i < n) {
+      k[i] = kin[base + i];
+      v[i] = vin[base + i];
+      cnt[(k[i] >> shift) & (RBUCK - 1)]++;
+    }
+  }
+  u32 pre[RBUCK];
+#pragma unroll
+  for (int d = 0; d < RBUCK; d++) {
+    u32 tot;
+    pre[d] = block_excl_scan(cnt[d], lw, &tot) + offs[(size_t)d * nb + blockIdx.x];
+  }
+#pragma unroll
+  for (int i = 0; i < SIPT; i++) {
+    if (base + i < n) {
+      const u32 d = (k[i] >> shift) & (RBUCK - 1);
+      const u32 dst = pre[d]++;
+      kout[dst] = k[i];
+      vout[dst] = v[i];
+    }
+  }
+}
+size_t radix_tmp_bytes(size_t n) {
+  const size_t nb = grid_for(n ? n : 1, STILE);
+  return 2 * n * sizeof(u32) + RBUCK * nb * sizeof(u32) + scan_tmp_bytes(RBUCK * nb) + 256;
+}
+void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Stream* st) {
+  if (n < 2 || bits <= 0) return;
+  const u32 nb = grid_for(n, STILE);
+  u32* k2 = (u32*)tmp;
+  u32* v2 = k2 + n;
+  u32* counts = v2 + n;
+  void* stmp = counts + (size_t)RBUCK * nb;
+  u32 *ka = keys, *va = vals, *kb = k2, *vb = v2;
+  int passes = 0;
+  for (int shift = 0; shift < bits; shift += RB) {
+    KLAUNCH(rs_count_k, nb, ST, st, ka, n, shift, counts, nb);
+    scan_u32(counts, counts, (size_t)RBUCK * nb, nullptr, stmp, st);
+    KLAUNCH(rs_scatter_k, nb, ST, st, ka, va, kb, vb, n, shift, counts, nb);
+    u32* t = ka; ka = kb; kb = t;
+    t = va; va = vb; vb = t;
+    passes++;
+  }
+  if (passes & 1) {
+    d2d(keys, ka, n * sizeof(u32), st);
+    d2d(vals, va, n * sizeof(u32), st);
+  }
+}
+
+// ---------------------------------------------------------------- OT
+__global__ void hist_k(DB db, u32* hist) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < db.ncls) hist_clause(db, c, hist);
+}
+__global__ void scatter_k(DB db, u32* cursor) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < db.ncls) scatter_clause(db, c, cursor);
+}
+void k_hist(const DB& db, u32* hist, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(hist_k, grid_for(db.ncls, 256), 256, st, db, hist);
+}
+void k_scatter(const DB& db, u32* cursor, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(scatter_k, grid_for(db.ncls, 256), 256, st, db, cursor);
+}
+
+constexpr u32 SORT_SMALL = 16;
+constexpr u32 SORT_BLOCK = 2048;
+__global__ void sort_small_k(DB db, const u32* lits, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 lit = lits[i];
+  const u32 len = db.ot_size[lit];
+  if (len < 2 || len > SORT_SMALL) return;
+  sort_list_insertion(db, db.ot_refs + db.ot_off[lit], len);
+}
+// bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK)
+__global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, u32 n) {
+  __shared__ u64 ka[SORT_BLOCK];
+  __shared__ u64 kb[SORT_BLOCK];
+  __shared__ u32 kid[SORT_BLOCK];
+  const u32 lit = lits[blockIdx.x];
+  const u32 len = db.ot_size[lit];
+  if (len <= SORT_SMALL || len > SORT_BLOCK) return;
+  u32* r = db.ot_refs + db.ot_off[lit];
+  u32 N = 1;
+  while (N < len) N <<= 1;
+  for (u32 i = threadIdx.x; i < N; i += blockDim.x) {
+    if (i < len) {
+      const SortKey k = make_key(db, r[i]);
+      ka[i] = ((u64)k.size << 32) | k.lit0;
+      kb[i] = ((u64)k.back << 32) | k.sig;
+      kid[i] = k.id;
+    } else {
+      ka[i] = ~0ull;
+      kb[i] = ~0ull;
+      kid[i] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  for (u32 k = 2; k <= N; k <<= 1) {
+    for (u32 j = k >> 1; j > 0; j >>= 1) {
+      for (u32 i = threadIdx.x; i < N; i += blockDim.x) {
+        const u32 ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const u64 a0 = ka[i], a1 = ka[ixj], b0 = kb[i], b1 = kb[ixj];
+          const u32 i0 = kid[i], i1 = kid[ixj];
+          const bool gt = a0 != a1 ? a0 > a1 : (b0 != b1 ? b0 > b1 : i0 > i1);
+          if (gt == up) {
+            ka[i] = a1; ka[ixj] = a0;
+            kb[i] = b1; kb[ixj] = b0;
+            kid[i] = i1; kid[ixj] = i0;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (u32 i = threadIdx.x; i < len; i += blockDim.x) r[i] = kid[i];
+}
+// bottom-up merge sort of one long list by one thread (rare lists above
+// SORT_BLOCK entries); 'scratch' has one slot per occurrence entry
+__global__ void sort_large_k(DB db, const u32* lits, u32 n, u32* scratch) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 lit = lits[i];
+  const u32 len = db.ot_size[lit];
+  if (len <= SORT_BLOCK) return;
+  u32* a = db.ot_refs + db.ot_off[lit];
+  u32* b = scratch + db.ot_off[lit];
+  u32 *src = a, *dst = b;
+  for (u32 w = 1; w < len; w <<= 1) {
+    for (u32 lo = 0; lo < len; lo += 2 * w) {
+      const u32 mid = lo + w < len ? lo + w : len;
+      const u32 hi = lo + 2 * w < len ? lo + 2 * w : len;
+      u32 p = lo, q = mid, o = lo;
+      while (p < mid && q < hi) {
+        if (key_less(make_key(db, src[q]), make_key(db, src[p]))) dst[o++] = src[q++];
+        else dst[o++] = src[p++];
+      }
+      while (p < mid) dst[o++] = src[p++];
+      while (q < hi) dst[o++] = src[q++];
+    }
+    u32* t = src; src = dst; dst = t;
+  }
+  if (src != a)
+    for (u32 k = 0; k < len; k++) a[k] = src[k];
+}
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32 max_len, u32* scratch, Stream* st) {
+  if (!n) return;
+  KLAUNCH(sort_small_k, grid_for(n, 64), 64, st, db, lits, n);
+  if (max_len > SORT_SMALL) {
+    KLAUNCH(sort_block_k, n, 256, st, db, lits, n);
+  }
+  if (max_len > SORT_BLOCK) {
+    KLAUNCH(sort_large_k, grid_for(n, 64), 64, st, db, lits, n, scratch);
+  }
+}
+
+// ---------------------------------------------------------------- BCP
+__global__ void bcp_visit_k(DB db, const u32* front, u32* next, u32* nnext, u32* conflict) {
+  const u32 t = front[blockIdx.x];
+  const OLV ol = olist(db, FLIP(t));
+  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) bcp_visit(db, ol.r[i], next, nnext, conflict);
+}
+void k_bcp_visit(const DB& db, const u32* front, u32 nfront, u32* next, u32* nnext, u32* conflict,
+                 Stream* st) {
+  if (!nfront) return;
+  KLAUNCH(bcp_visit_k, nfront, 256, st, db, front, next, nnext, conflict);
+}
+__global__ void bcp_mark_k(DB db, const u32* lits, u8* mark) {
+  const u32 t = lits[blockIdx.x];
+  for (u32 q = 0; q < 2; q++) {
+    const OLV ol = olist(db, t ^ q);
+    for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) mark[ol.r[i]] = 1;
+  }
+}
+void k_bcp_mark(const DB& db, const u32* lits, u32 n, u8* mark, Stream* st) {
+  if (!n) return;
+  KLAUNCH(bcp_mark_k, n, 256, st, db, lits, mark);
+}
+__global__ void bcp_clean_k(DB db, u8* mark) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= db.ncls || !mark[c]) return;
+  mark[c] = 0;
+  bcp_clean(db, c);
+}
+void k_bcp_clean(const DB& db, u8* mark, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(bcp_clean_k, grid_for(db.ncls, 256), 256, st, db, mark);
+}
+__global__ void clear_lists_k(DB db, const u32* lits, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 t = lits[i];
+  db.ot_size[t] = 0;
+  db.ot_size[FLIP(t)] = 0;
+}
+void k_clear_lists(const DB& db, const u32* lits, u32 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(clear_lists_k, grid_for(n, 256), 256, st, db, lits, n);
+}
+__global__ void set_vstate_k(DB db, const u32* lits, u32 n, u8 state) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) db.vstate[ABS(lits[i])] = state;
+}
+void k_set_vstate(const DB& db, const u32* lits, u32 n, u8 state, Stream* st) {
+  if (!n) return;
+  KLAUNCH(set_vstate_k, grid_for(n, 256), 256, st, db, lits, n, state);
+}
+
+// ---------------------------------------------------------------- LCVE
+__global__ void lcve_classify_k(DB db, LcveArgs a) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (v > db.nvars) return;
+  const u32 ps = a.hist[V2L(v)], ns = a.hist[V2L(v) | 1];
+  const u32 score = ps * ns;  // 32-bit product as in the reference
+  a.score[v] = score;
+  const bool cand = !db.assumed[v] && !db.vstate[v] && (ps || ns);
+  const bool brk = db.ot_size[V2L(v)] >= a.pt && db.ot_size[V2L(v) | 1] >= a.nt;
+  const bool hard = brk && ps >= a.pt && ns >= a.nt;
+  if (cand && hard) at_min64(a.hard, ((u64)score << 32) | v);
+  a.cflag[v] = cand ? (brk ? 3 : 1) : 0;
+}
+void k_lcve_classify(const DB& db, const LcveArgs& a, Stream* st) {
+  KLAUNCH(lcve_classify_k, grid_for(db.nvars, 256), 256, st, db, a);
+}
+__global__ void lcve_flag_k(DB db, LcveArgs a, u32* flag) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > db.nvars) return;
+  u32 f = 0;
+  if (v && a.cflag[v]) f = (((u64)a.score[v] << 32) | v) < *a.hard;
+  flag[v] = f;
+}
+__global__ void lcve_pack_k(DB db, LcveArgs a, const u32* flag, const u32* pos, u32* cand, u32* key) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > db.nvars || !v) return;
+  if (!a.cflag[v] || !((((u64)a.score[v] << 32) | v) < *a.hard)) return;
+  const u32 i = pos[v];
+  cand[i] = v;
+  key[i] = a.score[v];
+}
+void k_lcve_select(const DB& db, const LcveArgs& a, u32* flag_tmp, u32* cand, u32* key, u32* ncand,
+                   void* tmp, Stream* st) {
+  const size_t n = (size_t)db.nvars + 1;
+  KLAUNCH(lcve_flag_k, grid_for(n, 256), 256, st, db, a, flag_tmp);
+  scan_u32(flag_tmp, flag_tmp, n, ncand, tmp, st);
+  KLAUNCH(lcve_pack_k, grid_for(n, 256), 256, st, db, a, flag_tmp, flag_tmp, cand, key);
+}
+__global__ void lcve_pos_k(const u32* cand, u32 n, u32* pos) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pos[cand[i]] = i;
+}
+void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
+  if (!n) return;
+  KLAUNCH(lcve_pos_k, grid_for(n, 256), 256, st, cand, n, pos);
+}
+__global__ void mis_step_k(DB db, const u32* cand, u32 n, const u32* pos, u32* state, u32* undecided) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mis_step(db, i, cand, pos, state)) atomicAdd(undecided, 1u);
+}
+void k_mis_step(const DB& db, const u32* cand, u32 n, const u32* pos, u32* state, u32* undecided,
+                Stream* st) {
+  if (!n) return;
+  KLAUNCH(mis_step_k, grid_for(n, 64), 64, st, db, cand, n, pos, state, undecided);
+}
+__global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && state[i] == 1 && cflag[cand[i]] == 3) atomicMin(brk, i);
+}
+void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream* st) {
+  if (!n) return;
+  KLAUNCH(mis_break_k, grid_for(n, 256), 256, st, cand, n, cflag, state, brk);
+}
+__global__ void mis_flag_k(const u32* state, u32 n, const u32* brk, u32* flag) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (state[i] == 1 && i < *brk) ? 1u : 0u;
+}
+__global__ void mis_pack_k(const u32* cand, const u32* state, u32 n, const u32* brk, const u32* pos,
+                           u32* out) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && state[i] == 1 && i < *brk) out[pos[i]] = cand[i];
+}
+void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk_unused, u32* flag_tmp, u32* out,
+                   u32* nout, void* tmp, Stream* st) {
+  (void)brk_unused;
+  // brk lives in nout[1] on the device; nout[0] receives the count
+  if (!n) {
+    fill_u32(nout, 0, 1, st);
+    return;
+  }
+  KLAUNCH(mis_flag_k, grid_for(n, 256), 256, st, state, n, nout + 1, flag_tmp);
+  scan_u32(flag_tmp, flag_tmp, n, nout, tmp, st);
+  KLAUNCH(mis_pack_k, grid_for(n, 256), 256, st, cand, state, n, nout + 1, flag_tmp, out);
+}
+
+// ---------------------------------------------------------------- per elected variable
+constexpr int VB = 64;  // one wavefront per workgroup for divergent per-variable work
+
+__global__ void list_sizes_k(DB db, const u32* pv, u32 n, u32 extra, u32* out) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = db.ot_size[V2L(pv[k])] + db.ot_size[V2L(pv[k]) | 1] + extra;
+}
+void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Stream* st) {
+  if (!n) return;
+  KLAUNCH(list_sizes_k, grid_for(n, 256), 256, st, db, pv, n, extra, out);
+}
+__global__ void sub_k(DB db, Options o, const u32* pv, u32 from, u32 n, SubOut out) {
+  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) sub_var(db, o, pv[k], k, out);
+}
+void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, Stream* st) {
+  if (from >= n) return;
+  KLAUNCH(sub_k, grid_for(n - from, VB), VB, st, db, o, pv, from, n, out);
+}
+__global__ void sub_snapshot_k(DB db, const u32* pv, u32 n, SubSnap s) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) sub_snapshot_var(db, pv[k], k, s);
+}
+void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream* st) {
+  if (!n) return;
+  KLAUNCH(sub_snapshot_k, grid_for(n, VB), VB, st, db, pv, n, s);
+}
+__global__ void sub_restore_k(DB db, const u32* pv, u32 from, u32 n, SubSnap s) {
+  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) sub_restore_var(db, pv[k], k, s);
+}
+void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& s, Stream* st) {
+  if (from >= n) return;
+  KLAUNCH(sub_restore_k, grid_for(n - from, VB), VB, st, db, pv, from, n, s);
+}
+__global__ void sub_dirty_k(DB db, u32 from, u32 n, SubSnap s, const u32* unit_by, u32* first) {
+  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && sub_dirty_var(db, k, s, unit_by)) atomicMin(first, k);
+}
+void k_sub_dirty(const DB& db, u32 from, u32 n, const SubSnap& s, const u32* unit_by, u32* first,
+                 Stream* st) {
+  if (from >= n) return;
+  KLAUNCH(sub_dirty_k, grid_for(n - from, VB), VB, st, db, from, n, s, unit_by, first);
+}
+__global__ void oblivion_k(DB db, const u32* lits) {
+  const u32 t = lits[blockIdx.x];
+  const OLV ol = olist(db, t);
+  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) set_deleted(db, ol.r[i]);
+}
+void k_oblivion(const DB& db, const u32* lits, u32 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(oblivion_k, n, 256, st, db, lits);
+}
+__global__ void ve_decide_k(DB db, Options o, const u32* pv, u32 n, VeDecision d) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) ve_decide(db, o, pv[k], k, d);
+}
+void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, Stream* st) {
+  if (!n) return;
+  KLAUNCH(ve_decide_k, grid_for(n, VB), VB, st, db, o, pv, n, d);
+}
+__global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) ve_apply(db, pv[k], k, d, a);
+}
+void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, Stream* st) {
+  if (!n) return;
+  KLAUNCH(ve_apply_k, grid_for(n, VB), VB, st, db, pv, n, d, a);
+}
+__global__ void bce_k(DB db, Options o, const u32* pv, const u32* type, u32 n, u32* words,
+                      const u32* base, u32* stack, bool count) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (type && type[k]) {
+    if (count) words[k] = 0;
+    return;
+  }
+  if (count) words[k] = bce_var(db, o, pv[k], true, nullptr);
+  else bce_var(db, o, pv[k], false, stack + base[k]);
+}
+void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
+           const u32* base, u32* stack, bool count, Stream* st) {
+  if (!n) return;
+  KLAUNCH(bce_k, grid_for(n, VB), VB, st, db, o, pv, type, n, words, base, stack, count);
+}
+__global__ void ere_k(DB db, Options o, const u32* pv, u32 from, u32 n, EreOut out) {
+  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) ere_var(db, o, pv[k], k, out);
+}
+void k_ere(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const EreOut& out, Stream* st) {
+  if (from >= n) return;
+  KLAUNCH(ere_k, grid_for(n - from, VB), VB, st, db, o, pv, from, n, out);
+}
+__global__ void ere_dirty_k(DB db, const u32* pv, u32 from, u32 n, const u32* del_by, u32* first) {
+  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && ere_dirty_var(db, pv[k], k, del_by)) atomicMin(first, k);
+}
+void k_ere_dirty(const DB& db, const u32* pv, u32 from, u32 n, const u32* del_by, u32* first,
+                 Stream* st) {
+  if (from >= n) return;
+  KLAUNCH(ere_dirty_k, grid_for(n - from, VB), VB, st, db, pv, from, n, del_by, first);
+}
+__global__ void del_by_k(const u32* rec, u32 nrec, u32* del_by) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nrec) atomicMin(&del_by[rec[3 * (u64)i + 2]], rec[3 * (u64)i]);
+}
+void k_del_by(const u32* rec, u32 nrec, u32* del_by, Stream* st) {
+  if (!nrec) return;
+  KLAUNCH(del_by_k, grid_for(nrec, 256), 256, st, rec, nrec, del_by);
+}
+__global__ void apply_del_k(DB db, const u32* ids, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) set_deleted(db, ids[i]);
+}
+void k_apply_deletions(const DB& db, const u32* ids, u32 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(apply_del_k, grid_for(n, 256), 256, st, db, ids, n);
+}
+
+// ---------------------------------------------------------------- small helpers
+__global__ void assign_units_k(DB db, const u32* lits, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 l = lits[i];
+  db.assign[ABS(l)] = l + 1;
+  db.vstate[ABS(l)] = VS_FROZEN;
+}
+void k_assign_units(const DB& db, const u32* lits, u32 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(assign_units_k, grid_for(n, 256), 256, st, db, lits, n);
+}
+__global__ void pv_lits_k(const u32* pv, u32 n, u32* lits) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  lits[2 * k] = V2L(pv[k]);
+  lits[2 * k + 1] = V2L(pv[k]) | 1;
+}
+void k_pv_lits(const u32* pv, u32 n, u32* lits, Stream* st) {
+  if (!n) return;
+  KLAUNCH(pv_lits_k, grid_for(n, 256), 256, st, pv, n, lits);
+}
+__global__ void max_list_k(DB db, const u32* lits, u32 n, u32* out) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(out, db.ot_size[lits[i]]);
+}
+void k_max_list(const DB& db, const u32* lits, u32 n, u32* out, Stream* st) {
+  fill_u32(out, 0, 1, st);
+  if (!n) return;
+  KLAUNCH(max_list_k, grid_for(n, 256), 256, st, db, lits, n, out);
+}
+__global__ void reduce_lists_k(DB db) {
+  const u32 l = blockIdx.x * blockDim.x + threadIdx.x + 2;
+  if (l >= 2 * db.nvars + 2) return;
+  u32* r = db.ot_refs + db.ot_off[l];
+  const u32 n = db.ot_size[l];
+  u32 j = 0;
+  for (u32 i = 0; i < n; i++)
+    if (!is_deleted(db, r[i])) r[j++] = r[i];
+  db.ot_size[l] = j;
+}
+void k_reduce_lists(const DB& db, Stream* st) {
+  KLAUNCH(reduce_lists_k, grid_for(2 * (size_t)db.nvars, 256), 256, st, db);
+}
+__global__ void prep_k(DB db) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= db.ncls) return;
+  u32* l = db.lits + db.c_off[c];
+  const u32 n = db.c_size[c];
+  for (u32 i = 1; i < n; i++) {
+    const u32 t = l[i];
+    u32 j = i;
+    while (j > 0 && l[j - 1] > t) { l[j] = l[j - 1]; j--; }
+    l[j] = t;
+  }
+  db.c_sig[c] = calc_sig(l, n);
+}
+void k_prep(const DB& db, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(prep_k, grid_for(db.ncls, 256), 256, st, db);
+}
+__global__ void zero_sizes_k(DB db, const u32* lits, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) db.ot_size[lits[i]] = 0;
+}
+void k_zero_sizes(const DB& db, const u32* lits, u32 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(zero_sizes_k, grid_for(n, 256), 256, st, db, lits, n);
+}
+
+// ---------------------------------------------------------------- count / compact
+__global__ void count_k(DB db, unsigned long long* out) {
+  __shared__ u32 lw[NW];
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 live = 0, lits = 0;
+  if (c < db.ncls && !is_deleted(db, c)) {
+    live = 1;
+    lits = db.c_size[c];
+  }
+  u32 t1, t2;
+  block_excl_scan(live, lw, &t1);
+  block_excl_scan(lits, lw, &t2);
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], (unsigned long long)t1);
+    atomicAdd(&out[1], (unsigned long long)t2);
+  }
+}
+void k_count(const DB& db, u64* out2, Stream* st) {
+  fill_u32((u32*)out2, 0, 4, st);
+  if (!db.ncls) return;
+  KLAUNCH(count_k, grid_for(db.ncls, ST), ST, st, db, (unsigned long long*)out2);
+}
+__global__ void live_k(DB db, u32* tc, u32* tl) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= db.ncls) return;
+  const bool live = !is_deleted(db, c);
+  tc[c] = live ? 1u : 0u;
+  tl[c] = live ? db.c_size[c] : 0u;
+}
+__global__ void compact_k(DB src, DB dst, const u32* tc, const u32* tl) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= src.ncls || is_deleted(src, c)) return;
+  const u32 nid = tc[c], noff = tl[c], n = src.c_size[c];
+  dst.c_off[nid] = noff;
+  dst.c_size[nid] = n;
+  dst.c_sig[nid] = src.c_sig[c];
+  dst.c_flags[nid] = (u8)(src.c_flags[c] & ~F_MOLTEN);
+  dst.c_lbd[nid] = src.c_lbd[c];
+  const u32* a = src.lits + src.c_off[c];
+  u32* b = dst.lits + noff;
+  for (u32 i = 0; i < n; i++) b[i] = a[i];
+}
+void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals, Stream* st) {
+  if (!src.ncls) {
+    fill_u32(totals, 0, 2, st);
+    return;
+  }
+  KLAUNCH(live_k, grid_for(src.ncls, 256), 256, st, src, tmp_cls, tmp_lit);
+  scan_u32(tmp_cls, tmp_cls, src.ncls, totals, tmp, st);
+  scan_u32(tmp_lit, tmp_lit, src.ncls, totals + 1, tmp, st);
+  KLAUNCH(compact_k, grid_for(src.ncls, 256), 256, st, src, dst, tmp_cls, tmp_lit);
+}
+
+}  // namespace pfs

@@ -1,0 +1,119 @@
+// Host-side launch interface of the simplifier kernels and the small
+// device runtime they need.  The round driver (pfs_driver.cpp) only talks
+// to the device through these functions.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pfs_items.hpp"
+
+namespace pfs {
+
+struct Stream;  // a HIP stream plus reusable pinned staging memory
+
+// ---- runtime
+Stream* stream_create(int device);
+void stream_destroy(Stream*);
+void stream_sync(Stream*);
+void* dev_alloc(size_t bytes);  // throws std::bad_alloc on failure
+void dev_free(void*);
+void h2d(void* dst, const void* src, size_t bytes, Stream*);
+void d2h(void* dst, const void* src, size_t bytes, Stream*);
+void d2d(void* dst, const void* src, size_t bytes, Stream*);
+void fill_u32(u32* dst, u32 value, size_t n, Stream*);  // fill kernel, no memset nodes
+void fill_u8(u8* dst, u8 value, size_t n, Stream*);
+// stage timing: events recorded on the stream
+struct Event;
+Event* event_record(Stream*);
+float event_ms(Event* a, Event* b);  // synchronises on b
+void event_free(Event*);
+const char* device_name(int device);
+// per-kernel profiling: events around every launch while enabled
+struct KernelTime {
+  std::string name;
+  double ms;
+  uint64_t launches;
+};
+void prof_enable(Stream*, bool on);
+void prof_collect(Stream*, std::vector<KernelTime>& out);  // adds to out, synchronises
+int device_count();
+
+// ---- primitives
+size_t scan_tmp_bytes(size_t n);
+// exclusive prefix sum of n u32 values; out may alias in; *total (device)
+// receives the sum when non-null
+void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream*);
+size_t radix_tmp_bytes(size_t n);
+// stable sort of (key, val) pairs by the low 'bits' bits of key
+void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Stream*);
+
+// ---- occurrence table
+void k_hist(const DB& db, u32* hist, Stream*);
+void k_scatter(const DB& db, u32* cursor, Stream*);
+// sort the lists of the given literals (lits[0..n)) by the total order
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32 max_len, u32* scratch, Stream*);
+
+// ---- BCP
+void k_bcp_visit(const DB& db, const u32* front, u32 nfront, u32* next, u32* nnext, u32* conflict,
+                 Stream*);
+void k_bcp_mark(const DB& db, const u32* lits, u32 n, u8* mark, Stream*);
+void k_bcp_clean(const DB& db, u8* mark, Stream*);
+void k_clear_lists(const DB& db, const u32* lits, u32 n, Stream*);
+void k_set_vstate(const DB& db, const u32* lits, u32 n, u8 state, Stream*);
+
+// ---- LCVE
+struct LcveArgs {
+  const u32* hist;  // per literal
+  u32* score;       // per variable
+  u64* hard;        // min (score, var) over hard breakers
+  u8* cflag;        // per variable: 1 candidate, 3 soft breaker candidate, 0 none
+  u32 pt, nt;
+};
+void k_lcve_classify(const DB& db, const LcveArgs& a, Stream*);
+// compact candidates with (score, var) < hard into cand/key (var order)
+void k_lcve_select(const DB& db, const LcveArgs& a, u32* flag_tmp, u32* cand, u32* key, u32* ncand,
+                   void* tmp, Stream*);
+void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream*);
+void k_mis_step(const DB& db, const u32* cand, u32 n, const u32* pos, u32* state, u32* undecided,
+                Stream*);
+void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
+void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_tmp, u32* out,
+                   u32* nout, void* tmp, Stream*);
+
+// ---- per elected variable stages
+void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Stream*);  // |poss|+|negs|+extra
+void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, Stream*);
+void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
+void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& s, Stream*);
+void k_sub_dirty(const DB& db, u32 from, u32 n, const SubSnap& s, const u32* unit_by, u32* first,
+                 Stream*);
+void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
+void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, Stream*);
+void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, Stream*);
+void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
+           const u32* base, u32* stack, bool count, Stream*);
+void k_ere(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const EreOut& out,
+           Stream*);
+void k_ere_dirty(const DB& db, const u32* pv, u32 from, u32 n, const u32* del_by, u32* first,
+                 Stream*);
+void k_del_by(const u32* rec, u32 nrec, u32* del_by, Stream*);
+void k_apply_deletions(const DB& db, const u32* ids, u32 n, Stream*);
+
+// ---- small helpers
+void k_assign_units(const DB& db, const u32* lits, u32 n, Stream*);  // assign + FROZEN
+void k_pv_lits(const u32* pv, u32 n, u32* lits, Stream*);            // 2 literals per variable
+void k_max_list(const DB& db, const u32* lits, u32 n, u32* out, Stream*);
+void k_reduce_lists(const DB& db, Stream*);  // drop deleted entries from every list
+void k_prep(const DB& db, Stream*);          // sort literals, signature (extraction)
+void k_zero_sizes(const DB& db, const u32* lits, u32 n, Stream*);
+
+// ---- counting and garbage collection
+void k_count(const DB& db, u64* out2, Stream*);  // live clauses, live literals
+// stable compaction of the live clauses into dst (ids and pool renumbered)
+void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
+               Stream*);
+
+}  // namespace pfs

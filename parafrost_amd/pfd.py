@@ -11,6 +11,7 @@ Clause flags: bit0 learnt, bit1 added (resolvent), bits 4-5 usage.
 """
 from __future__ import annotations
 
+import gzip
 from dataclasses import dataclass
 
 import numpy as np
@@ -40,7 +41,10 @@ class State:
 
 
 def read(path: str) -> State:
-    b = open(path, "rb").read()
+    """Reads a PFD1 file (gzip-compressed when the name ends in .gz)."""
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rb") as f:
+        b = f.read()
     if b[:4] != b"PFD1":
         raise ValueError(f"{path}: not a PFD1 file")
     pos = 4
@@ -62,7 +66,8 @@ def read(path: str) -> State:
 
 
 def write(path: str, s: State) -> None:
-    with open(path, "wb") as f:
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "wb") as f:
         f.write(b"PFD1")
         f.write(np.array([s.status, s.maxvar], np.uint32).tobytes())
         f.write(np.array([s.sizes.shape[0]], np.uint64).tobytes())

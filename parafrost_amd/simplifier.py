@@ -1,0 +1,230 @@
+"""Python binding of libpfsimp.so (the C ABI in include/pfsimp.h).
+
+The binding is thin: it marshals numpy arrays into the C entry points and
+back.  There is no fallback: if the HIP library is missing or no GPU is
+visible, constructing a ``Simplifier`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_LIB = None
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpfsimp.so")
+
+PFS_OK = 0
+STATUS_NAMES = {0: "OK", 1: "ALLOC_FAIL", 2: "BAD_INPUT", 3: "DEVICE_ERROR", 4: "NO_DEVICE", 5: "BAD_STATE",
+                6: "UNKNOWN_OPTION"}
+RESULT_UNSAT, RESULT_SAT, RESULT_UNSOLVED = 0, 1, 2
+
+# exported symbols declared in include/pfsimp.h
+SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rewind", "pfs_simplify",
+           "pfs_result_sizes", "pfs_download_cnf", "pfs_download_units", "pfs_download_vstate",
+           "pfs_download_stack", "pfs_round_stats_get", "pfs_stage_times", "pfs_stage_name", "pfs_stage_count",
+           "pfs_kernel_count", "pfs_kernel_stats", "pfs_memory", "pfs_last_error", "pfs_version",
+           "pfs_device_count")
+
+
+class PfsError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("mem_budget", ctypes.c_uint64), ("keep_input", ctypes.c_int32)]
+
+
+class _RoundStats(ctypes.Structure):
+    _fields_ = [("elected", ctypes.c_uint32), ("eliminated", ctypes.c_uint32), ("units", ctypes.c_uint32),
+                ("resolvents", ctypes.c_uint32), ("clauses_after", ctypes.c_uint64),
+                ("literals_after", ctypes.c_uint64), ("mis_iterations", ctypes.c_uint32),
+                ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def load_library():
+    """Loads libpfsimp.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(_LIB_PATH):
+        raise PfsError(f"HIP simplifier library not built: {_LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(_LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32
+    L.pfs_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(vp)]
+    L.pfs_destroy.argtypes = [vp]
+    L.pfs_destroy.restype = None
+    L.pfs_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
+    L.pfs_upload.argtypes = [vp, u32, u64, vp, vp, vp, vp, vp, vp, vp, vp, u32]
+    L.pfs_rewind.argtypes = [vp]
+    L.pfs_simplify.argtypes = [vp, ctypes.POINTER(i32)]
+    L.pfs_result_sizes.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                   ctypes.POINTER(u64), ctypes.POINTER(u32)]
+    L.pfs_download_cnf.argtypes = [vp, vp, vp, vp, vp]
+    L.pfs_download_units.argtypes = [vp, vp]
+    L.pfs_download_vstate.argtypes = [vp, vp]
+    L.pfs_download_stack.argtypes = [vp, vp]
+    L.pfs_round_stats_get.argtypes = [vp, ctypes.POINTER(_RoundStats), u32, ctypes.POINTER(u32)]
+    L.pfs_stage_times.argtypes = [vp, vp, vp, u32]
+    L.pfs_stage_name.argtypes = [u32]
+    L.pfs_stage_name.restype = ctypes.c_char_p
+    L.pfs_stage_count.restype = u32
+    L.pfs_memory.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.pfs_kernel_count.argtypes = [vp]
+    L.pfs_kernel_count.restype = u32
+    L.pfs_kernel_stats.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(u64)]
+    L.pfs_last_error.argtypes = [vp]
+    L.pfs_last_error.restype = ctypes.c_char_p
+    L.pfs_version.restype = ctypes.c_char_p
+    L.pfs_device_count.restype = i32
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class Result:
+    status: int  # 0 UNSAT, 1 SAT, 2 UNSOLVED
+    maxvar: int
+    sizes: np.ndarray
+    flags: np.ndarray
+    lbd: np.ndarray
+    lits: np.ndarray
+    units: np.ndarray
+    resolved: np.ndarray
+    vstate: np.ndarray
+    rounds: list
+
+
+class Simplifier:
+    """One simplifier context (device state of one formula)."""
+
+    def __init__(self, device: int = 0, mem_budget: int = 0, keep_input: bool = False, **options):
+        L = load_library()
+        self._L = L
+        self._h = ctypes.c_void_p()
+        cfg = _Config(device, mem_budget, 1 if keep_input else 0)
+        st = L.pfs_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if st != PFS_OK:
+            raise PfsError(f"pfs_create: {STATUS_NAMES.get(st, st)}: {L.pfs_last_error(None).decode()}")
+        self.maxvar = 0
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def _check(self, st, what):
+        if st != PFS_OK:
+            msg = self._L.pfs_last_error(self._h).decode()
+            raise PfsError(f"{what}: {STATUS_NAMES.get(st, st)}: {msg}")
+
+    def close(self):
+        if self._h:
+            self._L.pfs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, name: str, value) -> None:
+        if isinstance(value, bool):
+            value = "1" if value else "0"
+        self._check(self._L.pfs_set_option(self._h, name.encode(), str(value).encode()), f"option {name}")
+
+    def upload(self, maxvar, offsets, lits, flags=None, lbd=None, vstate=None, assumed=None, vorg=None,
+               trail=None) -> None:
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        lits = np.ascontiguousarray(lits, np.uint32)
+        flags = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        lbd = None if lbd is None else np.ascontiguousarray(lbd, np.uint32)
+        vstate = None if vstate is None else np.ascontiguousarray(vstate, np.uint8)
+        assumed = None if assumed is None else np.ascontiguousarray(assumed, np.uint8)
+        vorg = None if vorg is None else np.ascontiguousarray(vorg, np.uint32)
+        trail = np.zeros(0, np.uint32) if trail is None else np.ascontiguousarray(trail, np.uint32)
+        self.maxvar = int(maxvar)
+        st = self._L.pfs_upload(self._h, maxvar, offsets.shape[0] - 1, _ptr(offsets), _ptr(lits), _ptr(flags),
+                                _ptr(lbd), _ptr(vstate), _ptr(assumed), _ptr(vorg), _ptr(trail), trail.shape[0])
+        self._check(st, "pfs_upload")
+
+    def upload_state(self, st) -> None:
+        """Uploads a parafrost_amd.pfd.State input dump."""
+        off = np.zeros(st.sizes.shape[0] + 1, np.uint64)
+        np.cumsum(st.sizes, out=off[1:])
+        self.upload(st.maxvar, off, st.lits, st.flags, st.lbd, st.vstate, None, None, st.units)
+
+    def rewind(self) -> None:
+        self._check(self._L.pfs_rewind(self._h), "pfs_rewind")
+
+    def simplify(self) -> int:
+        r = ctypes.c_int32(0)
+        self._check(self._L.pfs_simplify(self._h, ctypes.byref(r)), "pfs_simplify")
+        return int(r.value)
+
+    def result(self, status: int) -> Result:
+        L = self._L
+        nc, nl, nu, ns = (ctypes.c_uint64() for _ in range(4))
+        nr = ctypes.c_uint32()
+        self._check(L.pfs_result_sizes(self._h, ctypes.byref(nc), ctypes.byref(nl), ctypes.byref(nu),
+                                       ctypes.byref(ns), ctypes.byref(nr)), "pfs_result_sizes")
+        off = np.zeros(nc.value + 1, np.uint64)
+        lits = np.zeros(max(nl.value, 1), np.uint32)
+        flags = np.zeros(max(nc.value, 1), np.uint8)
+        lbd = np.zeros(max(nc.value, 1), np.uint32)
+        self._check(L.pfs_download_cnf(self._h, _ptr(off), _ptr(lits), _ptr(flags), _ptr(lbd)), "pfs_download_cnf")
+        units = np.zeros(max(nu.value, 1), np.uint32)
+        self._check(L.pfs_download_units(self._h, _ptr(units)), "pfs_download_units")
+        stack = np.zeros(max(ns.value, 1), np.uint32)
+        self._check(L.pfs_download_stack(self._h, _ptr(stack)), "pfs_download_stack")
+        vst = np.zeros(self.maxvar + 1, np.uint8)
+        self._check(L.pfs_download_vstate(self._h, _ptr(vst)), "pfs_download_vstate")
+        rs = (_RoundStats * max(nr.value, 1))()
+        n = ctypes.c_uint32()
+        L.pfs_round_stats_get(self._h, rs, nr.value, ctypes.byref(n))
+        rounds = [{k: getattr(rs[i], k) for k, _ in _RoundStats._fields_ if k != "reserved"} for i in range(n.value)]
+        sizes = np.diff(off).astype(np.uint32)
+        return Result(status, self.maxvar, sizes, flags[: nc.value], lbd[: nc.value], lits[: nl.value],
+                      units[: nu.value], stack[: ns.value], vst, rounds)
+
+    def stage_times(self) -> dict:
+        L = self._L
+        n = L.pfs_stage_count()
+        ms = np.zeros(n, np.float64)
+        calls = np.zeros(n, np.uint64)
+        L.pfs_stage_times(self._h, _ptr(ms), _ptr(calls), n)
+        return {L.pfs_stage_name(i).decode(): (float(ms[i]), int(calls[i])) for i in range(n)}
+
+    def kernel_times(self) -> dict:
+        """{kernel: (total ms, launches)} of the last simplify (profilegpu on)."""
+        L = self._L
+        out = {}
+        for i in range(L.pfs_kernel_count(self._h)):
+            name, ms, n = ctypes.c_char_p(), ctypes.c_double(), ctypes.c_uint64()
+            L.pfs_kernel_stats(self._h, i, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(n))
+            out[name.value.decode()] = (ms.value, n.value)
+        return out
+
+    def memory(self) -> tuple[int, int]:
+        used, peak = ctypes.c_uint64(), ctypes.c_uint64()
+        self._L.pfs_memory(self._h, ctypes.byref(used), ctypes.byref(peak))
+        return used.value, peak.value
+
+
+def simplify_state(st, device: int = 0, **options) -> Result:
+    """Simplifies one PFD input state on the GPU and returns the result."""
+    s = Simplifier(device, **options)
+    try:
+        s.upload_state(st)
+        status = s.simplify()
+        return s.result(status)
+    finally:
+        s.close()

@@ -1,0 +1,73 @@
+"""Helpers shared by the tests: golden fixtures, option names, comparisons."""
+import json
+import os
+
+import numpy as np
+
+from parafrost_amd import pfd
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# reference option name -> oracle keyword (oracle/oracle.py OracleOpts)
+ORACLE_KW = {"phases": "phases", "mupos": "mu_pos", "muneg": "mu_neg", "bce": "bce_en", "all": "all_en",
+             "ve": "ve_en", "ve+": "ve_plus_en", "sub": "sub_en", "ere": "ere_en", "velbound": "ve_lbound_en",
+             "veclausemax": "ve_clause_limit", "xormaxarity": "xor_max_arity", "submax": "sub_limit",
+             "bcemax": "bce_limit", "eremax": "ere_limit", "ereclausemax": "ere_max_resolvent",
+             "lcvemin": "lcve_min", "gcfreq": "shrink_rate", "vephasemin": "lits_min"}
+
+
+def oracle_kw(opts):
+    kw = {ORACLE_KW[k]: int(v) for k, v in opts.items()}
+    # ve is "ve || ve+" in the reference (src/cpu/options.cpp:282)
+    if "ve_en" in kw or "ve_plus_en" in kw:
+        kw["ve_en"] = int(bool(kw.get("ve_en", 1)) or bool(kw.get("ve_plus_en", 1)))
+    return kw
+
+
+def load_manifest():
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        return json.load(f)
+
+
+def read_golden(entry):
+    return pfd.read(os.path.join(GOLDEN, entry["input"])), pfd.read(os.path.join(GOLDEN, entry["output"]))
+
+
+def diff(a, b, units_as_set=True):
+    """Differences between two simplifier results (objects with status, sizes,
+    flags, lbd, lits, units, resolved, vstate)."""
+    d = []
+    if a.status != b.status:
+        return [f"status {a.status} != {b.status}"]
+    if a.status == 0:
+        return d
+    for k in ("sizes", "flags", "lbd", "lits", "resolved", "vstate"):
+        x, y = np.asarray(getattr(a, k)), np.asarray(getattr(b, k))
+        if x.shape != y.shape:
+            d.append(f"{k}: shape {x.shape} != {y.shape}")
+        elif not np.array_equal(x, y):
+            i = int(np.nonzero(x != y)[0][0])
+            d.append(f"{k}: first difference at {i}: {x[i]} != {y[i]}")
+    ua, ub = np.asarray(a.units), np.asarray(b.units)
+    if units_as_set:
+        ua, ub = np.sort(ua), np.sort(ub)
+    if ua.shape != ub.shape or not np.array_equal(ua, ub):
+        d.append(f"units differ ({ua.shape[0]} vs {ub.shape[0]})")
+    return d
+
+
+def extend_model(values, resolved):
+    """Model reconstruction over the eliminated-variable stack: records are
+    [lits (witness first)..., size], replayed from the end; a record with no
+    satisfied literal sets its witness true (src/cpu/model.cpp:101-162).
+    values: dict var -> bool, updated in place."""
+    i = len(resolved) - 1
+    while i > 0:
+        k = int(resolved[i])
+        lits = [int(x) for x in resolved[i - k:i]]
+        i -= k + 1
+        sat = any(values.get(l >> 1, False) == (not (l & 1)) for l in lits)
+        if not sat:
+            w = lits[0]
+            values[w >> 1] = not (w & 1)
+    return values
