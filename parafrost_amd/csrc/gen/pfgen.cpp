@@ -127,16 +127,21 @@ int pfg_uniform_ksat(uint64_t seed, uint32_t nvars, uint64_t nclauses, uint32_t 
 //   AND o = a & b      : (-o a) (-o b) (o -a -b)
 //   XOR o = a ^ b      : (-o a b) (-o -a -b) (o -a b) (o a -b)
 //   ITE o = c ? a : b  : (-o -c a) (-o c b) (o -c -a) (o c -b)
-// Finally noutputs distinct gate outputs are asserted as unit clauses with
-// random signs (appended after all gate clauses).
+// Finally noutputs distinct gate outputs are asserted as unit clauses
+// (appended after all gate clauses).  With sat_outputs = 0 each output gets
+// a random sign; with sat_outputs = 1 the inputs get random values, the
+// circuit is simulated, and each output is asserted with its simulated
+// value, so the formula is satisfiable by construction.
 int pfg_tseitin(uint64_t seed, uint32_t ninputs, uint32_t ngates, uint32_t noutputs,
-                double pand, double pxor, pfg_cnf* out) {
+                double pand, double pxor, int sat_outputs, pfg_cnf* out) {
   if (!out || ninputs < 3 || noutputs > ngates) return -2;
   Rng r(seed);
   Cnf c;
   c.maxvar = ninputs + ngates;
   c.lits.reserve((uint64_t)ngates * 10 + noutputs);
   c.off.reserve((uint64_t)ngates * 4 + noutputs + 1);
+  std::vector<uint8_t> gtype(sat_outputs ? ngates + 1 : 0);
+  std::vector<uint32_t> gin(sat_outputs ? 3 * ((size_t)ngates + 1) : 0);
   for (uint32_t g = 1; g <= ngates; g++) {
     const uint32_t o = ninputs + g;
     const uint32_t earlier = o - 1;
@@ -155,6 +160,10 @@ int pfg_tseitin(uint64_t seed, uint32_t ninputs, uint32_t ngates, uint32_t noutp
       fv[j] = v;
       fl[j] = mk(v, r.bit());
     }
+    if (sat_outputs) {
+      gtype[g] = (uint8_t)type;
+      for (int j = 0; j < nin; j++) gin[3 * (size_t)g + j] = fl[j];
+    }
     const uint32_t po = mk(o, 0), no = mk(o, 1);
     if (type == 0) {
       const uint32_t a = fl[0], b = fl[1];
@@ -172,13 +181,28 @@ int pfg_tseitin(uint64_t seed, uint32_t ninputs, uint32_t ngates, uint32_t noutp
       c.add(c1, 3); c.add(c2, 3); c.add(c3, 3); c.add(c4, 3);
     }
   }
+  // simulated values (1 = true) of every node under random inputs
+  std::vector<uint8_t> val(sat_outputs ? (size_t)ninputs + ngates + 1 : 0);
+  if (sat_outputs) {
+    for (uint32_t v = 1; v <= ninputs; v++) val[v] = (uint8_t)r.bit();
+    auto lv = [&](uint32_t l) { return (uint8_t)(val[l >> 1] ^ (l & 1)); };
+    for (uint32_t g = 1; g <= ngates; g++) {
+      const uint32_t* f = &gin[3 * (size_t)g];
+      uint8_t x;
+      if (gtype[g] == 0) x = lv(f[0]) & lv(f[1]);
+      else if (gtype[g] == 1) x = lv(f[0]) ^ lv(f[1]);
+      else x = lv(f[0]) ? lv(f[1]) : lv(f[2]);
+      val[ninputs + g] = x;
+    }
+  }
   // distinct outputs by partial Fisher-Yates over the gate range
   std::vector<uint32_t> pick(ngates);
   for (uint32_t i = 0; i < ngates; i++) pick[i] = ninputs + 1 + i;
   for (uint32_t i = 0; i < noutputs; i++) {
     const uint32_t j = i + (uint32_t)r.below(ngates - i);
     std::swap(pick[i], pick[j]);
-    uint32_t u = mk(pick[i], r.bit());
+    const uint32_t sign = sat_outputs ? (uint32_t)(val[pick[i]] ^ 1u) : r.bit();
+    uint32_t u = mk(pick[i], sign);
     c.add(&u, 1);
   }
   return export_cnf(c, out);

@@ -60,7 +60,7 @@ Simplifier::~Simplifier() {
                 &score_, &cflag_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_,
                 &def_, &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_ref_, &snap_flg_,
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
-                &recs_, &ids_, &mark_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_, &in_c_size_,
+                &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_, &in_c_size_,
                 &in_c_sig_, &in_c_flags_, &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
@@ -129,7 +129,7 @@ void Simplifier::alloc_state() {
   mem_.ensure(vorg_, V1 * 4, 0, st_);
   mem_.ensure(trail_, (V1 + 1) * 4, 0, st_);
   for (Buf* b : {&score_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_, &def_,
-                 &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_np_, &snap_nn_})
+                 &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_np_, &snap_nn_, &order_})
     mem_.ensure(*b, (V1 + 1) * 4, 0, st_);
   mem_.ensure(cflag_, V1, 0, st_);
   mem_.ensure(elits_, 2 * (V1 + 1) * 4, 0, st_);
@@ -459,11 +459,10 @@ void Simplifier::sort_ot() {
   stage_begin(S_SORT);
   DB db = make_db(cur_);
   k_pv_lits(P<u32>(pv_), E_, P<u32>(elits_), st_);
-  u32* sc = P<u32>(scal_);
-  k_max_list(db, P<u32>(elits_), 2 * E_, sc, st_);
-  const u32 mx = read_u32(sc);
-  if (mx > 2048) mem_.ensure(scratch_, std::max<size_t>(lits_end_, 1) * 4, 0, st_);
-  k_sort_lists(db, P<u32>(elits_), 2 * E_, mx, P<u32>(scratch_), st_);
+  mem_.ensure(scratch_, std::max<size_t>(lits_end_, 6 * (size_t)E_ + 3) * 4, 0, st_);
+  mem_.ensure(tmp2_, (6 * (size_t)E_ + 3) * 4, 0, st_);
+  k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), st_);
+  k_work_order(db, P<u32>(pv_), E_, P<u32>(tmp2_), P<u32>(order_), tmp_.p, st_);
   stage_end();
 }
 
@@ -495,7 +494,7 @@ bool Simplifier::sub() {
     fill_u32(sc + 1, INF, 1, st_);
     fill_u32(sc + 2, 0, 1, st_);
     SubOut out{P<u32>(recs_), sc, cap, sc + 1, sc + 2};
-    k_sub(db, opt, P<u32>(pv_), from, E_, out, st_);
+    k_sub(db, opt, P<u32>(pv_), from, E_, out, P<u32>(order_), st_);
     u32 h[3];
     d2h(h, sc, 12, st_);
     stream_sync(st_);
@@ -583,7 +582,7 @@ bool Simplifier::ve() {
   mem_.ensure(scratch_, std::max<size_t>(S, 1) * 4, 0, st_);
   VeDecision d{P<u32>(type_), P<u32>(def_), P<u32>(vncls_), P<u32>(vnlits_), P<u32>(vnstack_),
                P<u32>(scratch_), P<u32>(vbase_)};
-  k_ve_decide(db, opt, P<u32>(pv_), E_, d, st_);
+  k_ve_decide(db, opt, P<u32>(pv_), E_, d, P<u32>(order_), st_);
   scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, sc, tmp_.p, st_);
   scan_u32(P<u32>(vnlits_), P<u32>(litb_), E_, sc + 1, tmp_.p, st_);
   scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc + 2, tmp_.p, st_);
@@ -608,7 +607,7 @@ bool Simplifier::ve() {
   ap.units = UnitOut{P<u32>(recs_), sc + 3, V_ + 1};
   ap.error = sc + 6;
   fill_u32(sc + 6, 0, 1, st_);
-  k_ve_apply(db, P<u32>(pv_), E_, d, ap, st_);
+  k_ve_apply(db, P<u32>(pv_), E_, d, ap, P<u32>(order_), st_);
   if (read_u32(sc + 6)) throw DeviceFail{"BVE resolvent writes disagree with the counted space"};
   ncls_ += tot[0];
   lits_end_ += tot[1];
@@ -662,13 +661,13 @@ void Simplifier::bce() {
   stage_begin(S_BCE);
   DB db = make_db(cur_);
   u32* sc = P<u32>(scal_);
-  k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, P<u32>(vnstack_), nullptr, nullptr, true, st_);
+  k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, P<u32>(vnstack_), nullptr, nullptr, true, P<u32>(order_), st_);
   scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc, tmp_.p, st_);
   const u32 T = read_u32(sc);
   if (T) {
     ensure_stack(stack_n_ + T);
     k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, nullptr, P<u32>(stkb_), P<u32>(stack_) + stack_n_, false,
-          st_);
+          P<u32>(order_), st_);
     stack_n_ += T;
   }
   stream_sync(st_);
@@ -677,6 +676,92 @@ void Simplifier::bce() {
 
 // ---------------------------------------------------------------- ERE (elimination.cpp:169-197)
 void Simplifier::ere() {
+  if (!opt.ere_en || !E_) return;
+  if (!ere_pairs()) ere_per_var();
+}
+
+// Pair-parallel ERE: every resolvent of every elected variable is looked up
+// in a hash table of the live clauses; the pairs that found equal clauses are
+// then replayed in elected-variable and pair order, which decides exactly
+// which clauses the sequential ERE (elimination.cpp:169-197) deletes: a pair
+// whose antecedent was deleted by an earlier pair is skipped, and a pair
+// deletes its lowest-id match that is still live.
+bool Simplifier::ere_pairs() {
+  stage_begin(S_ERE);
+  DB db = make_db(cur_);
+  u32* sc = P<u32>(scal_);
+  u64* tot64 = (u64*)(sc + 32);
+  k_ere_pair_counts(db, opt, P<u32>(pv_), E_, P<u32>(vncls_), tot64, st_);
+  u64 total = 0;
+  d2h(&total, tot64, 8, st_);
+  stream_sync(st_);
+  if (total >= (1ull << 31)) {
+    stage_end();
+    return false;
+  }
+  const u32 npairs = (u32)total;
+  bool ok = true;
+  if (npairs) {
+    scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, nullptr, tmp_.p, st_);
+    size_t cap = 1024;
+    while (cap < 2 * (size_t)ncls_) cap <<= 1;
+    mem_.ensure(ht_, cap * 8, 0, st_);
+    HashTable ht{(u64*)ht_.p, (u32)(cap - 1)};
+    k_ht_build(db, ht, opt.ere_max_resolvent, st_);
+    const u32 rcap = std::max<u32>(1u << 16, V_ + 1);
+    mem_.ensure(erecs_, (size_t)rcap * ERE_REC_WORDS * 4, 0, st_);
+    fill_u32(sc, 0, 2, st_);
+    ErePairs a{P<u32>(pv_), P<u32>(clsb_), E_, P<u32>(erecs_), sc, rcap, sc + 1, ht, opt.ere_max_resolvent};
+    k_ere_pairs(db, a, npairs, st_);
+    u32 h[2];
+    d2h(h, sc, 8, st_);
+    stream_sync(st_);
+    if (h[1] || h[0] > rcap) {
+      ok = false;  // more equal clauses than kept per pair: replay per variable
+    } else if (h[0]) {
+      const u32 nrec = h[0];
+      std::vector<u32> rec((size_t)nrec * ERE_REC_WORDS);
+      d2h(rec.data(), erecs_.p, rec.size() * 4, st_);
+      stream_sync(st_);
+      std::vector<u32> idx(nrec);
+      for (u32 i = 0; i < nrec; i++) idx[i] = i;
+      auto R = [&](u32 i, int w) { return rec[(size_t)i * ERE_REC_WORDS + w]; };
+      std::sort(idx.begin(), idx.end(), [&](u32 x, u32 y) {
+        return R(x, 0) != R(y, 0) ? R(x, 0) < R(y, 0) : R(x, 1) < R(y, 1);
+      });
+      std::unordered_map<u32, bool> dead;
+      std::vector<u32> del;
+      for (u32 i : idx) {
+        if (dead.count(R(i, 2)) || dead.count(R(i, 3))) continue;
+        const u32 nm = R(i, 4);
+        u32 target = INF;
+        for (u32 q = 0; q < nm && q < (u32)ERE_K; q++) {
+          if (!dead.count(R(i, 5 + q))) {
+            target = R(i, 5 + q);
+            break;
+          }
+        }
+        if (target == INF) {
+          if (nm > (u32)ERE_K) { ok = false; break; }
+          continue;
+        }
+        dead[target] = true;
+        del.push_back(target);
+      }
+      if (ok && !del.empty()) {
+        mem_.ensure(ids_, del.size() * 4, 0, st_);
+        h2d(ids_.p, del.data(), del.size() * 4, st_);
+        k_apply_deletions(db, P<u32>(ids_), (u32)del.size(), st_);
+        stream_sync(st_);
+      }
+    }
+  }
+  if (ok) rounds_.back().ere_passes = 1;
+  stage_end();
+  return ok;
+}
+
+void Simplifier::ere_per_var() {
   if (!opt.ere_en || !E_) return;
   stage_begin(S_ERE);
   DB db = make_db(cur_);

@@ -101,6 +101,8 @@ class Simplifier {
   bool ve();
   void bce();
   void ere();
+  bool ere_pairs();   // pair-parallel ERE; false when it must fall back
+  void ere_per_var(); // per-variable ERE with speculative re-runs
   void ensure_clause_room(uint64_t add_cls, uint64_t add_lits);
   void ensure_stack(uint64_t words);
   void commit_units(const std::vector<uint32_t>& lits);
@@ -129,7 +131,7 @@ class Simplifier {
   // per elected variable
   Buf vsz_, vbase_, type_, def_, vncls_, vnlits_, vnstack_, clsb_, litb_, stkb_;
   Buf snap_ref_, snap_flg_, snap_size_, snap_sig_, snap_lbd_, snap_np_, snap_nn_, scratch_;
-  Buf elits_, unit_by_, recs_, ids_, mark_;
+  Buf elits_, unit_by_, recs_, ids_, mark_, ht_, erecs_, order_;
   // stack
   Buf stack_;
   uint64_t stack_n_ = 0;

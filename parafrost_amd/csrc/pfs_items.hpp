@@ -998,7 +998,11 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
       }
     }
   } else {
-    if (type == VT_RES || type == VT_SUBST) {
+    // resolvents are produced only when some resolvent is longer than one
+    // literal ("if (nAddedCls) xresolve(...)", bve.cpp:226-235): a variable
+    // whose resolvents are all units or tautologies is eliminated without
+    // emitting those units
+    if ((type == VT_RES || type == VT_SUBST) && d.ncls[k]) {
       const bool substitute = type == VT_SUBST;
       u32 dx = p, fx = n;
       if (db.ot_size[dx] > db.ot_size[fx]) { dx = n; fx = p; }
@@ -1190,6 +1194,134 @@ PF_HD bool ere_dirty_var(const DB& db, u32 v, u32 k, const u32* del_by) {
       if (del_by[ol.r[i]] < k) return true;
   }
   return false;
+}
+
+// ---------------------------------------------------------------- ERE, pair-parallel
+// ERE deletes, for each resolvent, the first live clause equal to it in the
+// shortest occurrence list of its literals (redundancy.h:67-121).  A clause
+// equal to the resolvent contains every resolvent literal, so that list
+// holds exactly the live clauses equal to the resolvent, and "first" is the
+// lowest clause id (lists of non-elected literals keep clause order).  The
+// pair-parallel form therefore looks every resolvent up in a hash table of
+// the live clauses, keyed by the literal sequence, and returns all matches;
+// the order-dependent part (which match is still live) is resolved
+// afterwards over the few pairs that matched, in elected-then-pair order.
+PF_HD u32 hash_step(u32 h, u32 lit) { return (h ^ lit) * 0x01000193u; }
+PF_HD u32 hash_final(u32 h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+constexpr u32 HASH_SEED = 0x811c9dc5u;
+constexpr u64 HT_EMPTY = ~0ull;
+struct HashTable {
+  u64* slot;  // (hash << 32) | clause id, HT_EMPTY when free
+  u32 mask;   // capacity - 1 (capacity a power of two, load <= 1/2)
+};
+PF_HD u64 at_cas64(u64* p, u64 cmp, u64 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicCAS((unsigned long long*)p, (unsigned long long)cmp, (unsigned long long)v);
+#else
+  u64 e = cmp;
+  __atomic_compare_exchange_n(p, &e, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
+  return e;
+#endif
+}
+PF_HD void ht_insert_clause(const DB& db, const HashTable& ht, u32 c, int maxsize) {
+  if (is_deleted(db, c)) return;
+  const CL cl = clause(db, c);
+  if (cl.n < 2 || (maxsize && (int)cl.n > maxsize)) return;
+  u32 h = HASH_SEED;
+  for (u32 i = 0; i < cl.n; i++) h = hash_step(h, cl.l[i]);
+  h = hash_final(h);
+  const u64 e = ((u64)h << 32) | c;
+  u32 s = h & ht.mask;
+  while (at_cas64(&ht.slot[s], HT_EMPTY, e) != HT_EMPTY) s = (s + 1) & ht.mask;
+}
+constexpr int ERE_K = 4;        // matches kept per resolvent
+constexpr int ERE_REC_WORDS = 5 + ERE_K;  // k, pair, ci, cj, nmatch, ids
+struct ErePairs {
+  const u32* pv;
+  const u32* pbase;  // per elected variable: first pair index (exclusive scan)
+  u32 nvars;
+  u32* rec;
+  u32* nrec;
+  u32 cap;
+  u32* overflow;
+  HashTable ht;
+  int maxsize;
+};
+// the lists of elected variable v in ERE order (elimination.cpp:177-182)
+PF_HD void ere_lists(const DB& db, u32 v, OLV& me, OLV& other) {
+  u32 dx = V2L(v), fx = NEGL(dx);
+  if (db.ot_size[dx] > db.ot_size[fx]) { const u32 t = dx; dx = fx; fx = t; }
+  me = olist(db, dx);
+  other = olist(db, fx);
+}
+PF_HD u32 ere_pair_count(const DB& db, const Options& o, u32 v) {
+  OLV me, other;
+  ere_lists(db, v, me, other);
+  if ((int)me.n > o.ere_limit || (int)other.n > o.ere_limit) return 0;
+  return me.n * other.n;
+}
+// one resolvent (pair 'local' of elected variable index k)
+PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
+  const u32 x = a.pv[k];
+  OLV me, other;
+  ere_lists(db, x, me, other);
+  const u32 i = local / other.n, j = local % other.n;
+  const u32 c1 = me.r[i], c2 = other.r[j];
+  if (is_deleted(db, c1) || is_deleted(db, c2)) return;
+  const CL A = clause(db, c1), B = clause(db, c2);
+  u32 i1 = 0, i2 = 0, len = 0, h = HASH_SEED;
+  while (i1 < A.n && i2 < B.n) {
+    const u32 l1 = A.l[i1], l2 = B.l[i2];
+    const u32 v1 = ABS(l1), v2 = ABS(l2);
+    if (v1 == x) i1++;
+    else if (v2 == x) i2++;
+    else if ((l1 ^ l2) == 1u) return;
+    else if (v1 < v2) { i1++; h = hash_step(h, l1); len++; }
+    else if (v2 < v1) { i2++; h = hash_step(h, l2); len++; }
+    else { i1++; i2++; h = hash_step(h, l1); len++; }
+  }
+  while (i1 < A.n) { const u32 l = A.l[i1++]; if (ABS(l) != x) { h = hash_step(h, l); len++; } }
+  while (i2 < B.n) { const u32 l = B.l[i2++]; if (ABS(l) != x) { h = hash_step(h, l); len++; } }
+  if (!(len > 1 && (!a.maxsize || (int)len <= a.maxsize))) return;
+  h = hash_final(h);
+  const u8 type = (is_learnt(db, c1) || is_learnt(db, c2)) ? ST_LEARNT : ST_ORIGINAL;
+  u32 ids[ERE_K];
+  u32 nm = 0;
+  for (u32 s = h & a.ht.mask;; s = (s + 1) & a.ht.mask) {
+    const u64 e = a.ht.slot[s];
+    if (e == HT_EMPTY) break;
+    if ((u32)(e >> 32) != h) continue;
+    const u32 c = (u32)e;
+    const u8 f = db.c_flags[c];
+    if (st_deleted(f) || db.c_size[c] != len) continue;
+    if (!(st_learnt(f) || (f & ST_MASK) == type)) continue;
+    if (!merge_ere(x, A, B, clause(db, c))) continue;
+    // keep the ERE_K lowest ids, ascending
+    u32 ins = c;
+    for (u32 q = 0; q < nm && q < (u32)ERE_K; q++) {
+      if (ins < ids[q]) { const u32 t = ids[q]; ids[q] = ins; ins = t; }
+    }
+    if (nm < (u32)ERE_K) ids[nm] = ins;
+    nm++;
+  }
+  if (!nm) return;
+  const u32 r = at_add(a.nrec, 1u);
+  if (r >= a.cap) return;
+  u32* w = a.rec + (u64)r * ERE_REC_WORDS;
+  w[0] = k;
+  w[1] = local;
+  w[2] = c1;
+  w[3] = c2;
+  w[4] = nm;
+  for (u32 q = 0; q < (u32)ERE_K; q++) w[5 + q] = q < nm ? ids[q] : 0xFFFFFFFFu;
+  if (nm > (u32)ERE_K) at_add(a.overflow, 1u);
 }
 
 }  // namespace pfs

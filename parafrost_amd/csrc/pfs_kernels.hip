@@ -201,6 +201,20 @@ __device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) {
   return pre + x - v;
 }
 
+// grid-stride reductions: a bounded number of workgroups, one atomic per
+// wavefront, so a single result word never sees more than a few thousand
+// atomics
+constexpr unsigned RED_BLOCKS = 1024;
+__device__ inline u32 wave_max(u32 x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x = max(x, (u32)__shfl_xor(x, d, 64));
+  return x;
+}
+__device__ inline unsigned long long wave_sum64(unsigned long long x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
 __global__ void scan_reduce_k(const u32* in, size_t n, u32* part) {
   __shared__ u32 lw[NW];
   const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
@@ -361,13 +375,59 @@ void k_scatter(const DB& db, u32* cursor, Stream* st) {
 
 constexpr u32 SORT_SMALL = 16;
 constexpr u32 SORT_BLOCK = 2048;
-__global__ void sort_small_k(DB db, const u32* lits, u32 n) {
+// wave-aggregated append: one atomic per wavefront per counter
+__device__ inline void wave_append(bool pred, u32 val, u32* arr, u32* cnt) {
+  const u64 mask = __ballot(pred);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  u32 base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (u32)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) arr[base + __popcll(mask & ((1ull << lane) - 1))] = val;
+}
+// splits the lists to sort into the three size classes
+__global__ void sort_classify_k(DB db, const u32* lits, u32 n, u32* small, u32* mid, u32* large, u32* cnt) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u32 lit = lits[i];
-  const u32 len = db.ot_size[lit];
-  if (len < 2 || len > SORT_SMALL) return;
-  sort_list_insertion(db, db.ot_refs + db.ot_off[lit], len);
+  const u32 lit = i < n ? lits[i] : 0;
+  const u32 len = i < n ? db.ot_size[lit] : 0;
+  wave_append(len >= 2 && len <= SORT_SMALL, lit, small, cnt);
+  wave_append(len > SORT_SMALL && len <= SORT_BLOCK, lit, mid, cnt + 1);
+  wave_append(len > SORT_BLOCK, lit, large, cnt + 2);
+}
+// 16 lanes per list: bitonic network over wave64 shuffles, keys in registers
+__global__ void sort_small_k(DB db, const u32* lits, u32 n) {
+  const u32 g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const u32 j = threadIdx.x & 15;
+  const bool valid = g < n;
+  const u32 lit = valid ? lits[g] : 0;
+  const u32 len = valid ? db.ot_size[lit] : 0;
+  u32* r = db.ot_refs + (valid ? db.ot_off[lit] : 0);
+  u64 a = ~0ull, b = ~0ull;
+  u32 id = 0xFFFFFFFFu;
+  if (j < len) {
+    const SortKey k = make_key(db, r[j]);
+    a = ((u64)k.size << 32) | k.lit0;
+    b = ((u64)k.back << 32) | k.sig;
+    id = k.id;
+  }
+#pragma unroll
+  for (u32 k = 2; k <= 16; k <<= 1) {
+#pragma unroll
+    for (u32 d = k >> 1; d > 0; d >>= 1) {
+      const u64 pa = __shfl_xor(a, d, 64), pb = __shfl_xor(b, d, 64);
+      const u32 pid = __shfl_xor(id, d, 64);
+      const bool mine_less = a != pa ? a < pa : (b != pb ? b < pb : id < pid);
+      const bool up = (j & k) == 0, lower = (j & d) == 0;
+      const bool keep_min = lower == up;
+      if (keep_min != mine_less) {
+        a = pa;
+        b = pb;
+        id = pid;
+      }
+    }
+  }
+  if (j < len) r[j] = id;
 }
 // bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK)
 __global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, u32 n) {
@@ -442,15 +502,19 @@ __global__ void sort_large_k(DB db, const u32* lits, u32 n, u32* scratch) {
   if (src != a)
     for (u32 k = 0; k < len; k++) a[k] = src[k];
 }
-void k_sort_lists(const DB& db, const u32* lits, u32 n, u32 max_len, u32* scratch, Stream* st) {
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, Stream* st) {
   if (!n) return;
-  KLAUNCH(sort_small_k, grid_for(n, 64), 64, st, db, lits, n);
-  if (max_len > SORT_SMALL) {
-    KLAUNCH(sort_block_k, n, 256, st, db, lits, n);
-  }
-  if (max_len > SORT_BLOCK) {
-    KLAUNCH(sort_large_k, grid_for(n, 64), 64, st, db, lits, n, scratch);
-  }
+  u32* small = classes;
+  u32* mid = classes + n;
+  u32* large = classes + 2 * (size_t)n;
+  fill_u32(cnt, 0, 3, st);
+  KLAUNCH(sort_classify_k, grid_for(n, 256), 256, st, db, lits, n, small, mid, large, cnt);
+  u32 h[3];
+  PFS_HIP(hipMemcpyAsync(h, cnt, 12, hipMemcpyDeviceToHost, st->s));
+  PFS_HIP(hipStreamSynchronize(st->s));
+  if (h[0]) KLAUNCH(sort_small_k, grid_for((size_t)h[0] * 16, 256), 256, st, db, small, h[0]);
+  if (h[1]) KLAUNCH(sort_block_k, h[1], 256, st, db, mid, h[1]);
+  if (h[2]) KLAUNCH(sort_large_k, grid_for(h[2], 64), 64, st, db, large, h[2], scratch);
 }
 
 // ---------------------------------------------------------------- BCP
@@ -602,13 +666,18 @@ void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Strea
   if (!n) return;
   KLAUNCH(list_sizes_k, grid_for(n, 256), 256, st, db, pv, n, extra, out);
 }
-__global__ void sub_k(DB db, Options o, const u32* pv, u32 from, u32 n, SubOut out) {
-  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) sub_var(db, o, pv[k], k, out);
+// 'order' (nullable) permutes the elected indices so that the heaviest
+// variables start first and share wavefronts with variables of similar work
+__global__ void sub_k(DB db, Options o, const u32* pv, u32 from, u32 n, SubOut out, const u32* order) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (from + t >= n) return;
+  const u32 k = order ? order[t] : from + t;
+  sub_var(db, o, pv[k], k, out);
 }
-void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, Stream* st) {
+void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, const u32* order,
+           Stream* st) {
   if (from >= n) return;
-  KLAUNCH(sub_k, grid_for(n - from, VB), VB, st, db, o, pv, from, n, out);
+  KLAUNCH(sub_k, grid_for(n - from, VB), VB, st, db, o, pv, from, n, out, from ? nullptr : order);
 }
 __global__ void sub_snapshot_k(DB db, const u32* pv, u32 n, SubSnap s) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -644,26 +713,33 @@ void k_oblivion(const DB& db, const u32* lits, u32 n, Stream* st) {
   if (!n) return;
   KLAUNCH(oblivion_k, n, 256, st, db, lits);
 }
-__global__ void ve_decide_k(DB db, Options o, const u32* pv, u32 n, VeDecision d) {
-  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) ve_decide(db, o, pv[k], k, d);
+__global__ void ve_decide_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const u32 k = order ? order[t] : t;
+  ve_decide(db, o, pv[k], k, d);
 }
-void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, Stream* st) {
+void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
+                 Stream* st) {
   if (!n) return;
-  KLAUNCH(ve_decide_k, grid_for(n, VB), VB, st, db, o, pv, n, d);
+  KLAUNCH(ve_decide_k, grid_for(n, VB), VB, st, db, o, pv, n, d, order);
 }
-__global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a) {
-  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) ve_apply(db, pv[k], k, d, a);
+__global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a, const u32* order) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const u32 k = order ? order[t] : t;
+  ve_apply(db, pv[k], k, d, a);
 }
-void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, Stream* st) {
+void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, const u32* order,
+                Stream* st) {
   if (!n) return;
-  KLAUNCH(ve_apply_k, grid_for(n, VB), VB, st, db, pv, n, d, a);
+  KLAUNCH(ve_apply_k, grid_for(n, VB), VB, st, db, pv, n, d, a, order);
 }
 __global__ void bce_k(DB db, Options o, const u32* pv, const u32* type, u32 n, u32* words,
-                      const u32* base, u32* stack, bool count) {
-  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+                      const u32* base, u32* stack, bool count, const u32* order) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const u32 k = order ? order[t] : t;
   if (type && type[k]) {
     if (count) words[k] = 0;
     return;
@@ -672,9 +748,23 @@ __global__ void bce_k(DB db, Options o, const u32* pv, const u32* type, u32 n, u
   else bce_var(db, o, pv[k], false, stack + base[k]);
 }
 void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
-           const u32* base, u32* stack, bool count, Stream* st) {
+           const u32* base, u32* stack, bool count, const u32* order, Stream* st) {
   if (!n) return;
-  KLAUNCH(bce_k, grid_for(n, VB), VB, st, db, o, pv, type, n, words, base, stack, count);
+  KLAUNCH(bce_k, grid_for(n, VB), VB, st, db, o, pv, type, n, words, base, stack, count, order);
+}
+// heaviest-first order of the elected variables: key = 0xFFFF - min(|poss| +
+// |negs|, 0xFFFF), stable radix sort on 16 bits
+__global__ void work_keys_k(DB db, const u32* pv, u32 n, u32* key, u32* idx) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 w = db.ot_size[V2L(pv[k])] + db.ot_size[V2L(pv[k]) | 1];
+  key[k] = 0xFFFFu - (w < 0xFFFFu ? w : 0xFFFFu);
+  idx[k] = k;
+}
+void k_work_order(const DB& db, const u32* pv, u32 n, u32* key, u32* order, void* tmp, Stream* st) {
+  if (!n) return;
+  KLAUNCH(work_keys_k, grid_for(n, 256), 256, st, db, pv, n, key, order);
+  radix_sort_pairs(key, order, n, 16, tmp, st);
 }
 __global__ void ere_k(DB db, Options o, const u32* pv, u32 from, u32 n, EreOut out) {
   const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
@@ -709,6 +799,48 @@ void k_apply_deletions(const DB& db, const u32* ids, u32 n, Stream* st) {
   if (!n) return;
   KLAUNCH(apply_del_k, grid_for(n, 256), 256, st, db, ids, n);
 }
+__global__ void ht_build_k(DB db, HashTable ht, int maxsize) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < db.ncls) ht_insert_clause(db, ht, c, maxsize);
+}
+void k_ht_build(const DB& db, const HashTable& ht, int maxsize, Stream* st) {
+  fill_u32((u32*)ht.slot, 0xFFFFFFFFu, 2 * ((size_t)ht.mask + 1), st);
+  if (!db.ncls) return;
+  KLAUNCH(ht_build_k, grid_for(db.ncls, 256), 256, st, db, ht, maxsize);
+}
+__global__ void ere_pair_counts_k(DB db, Options o, const u32* pv, u32 n, u32* np, unsigned long long* total) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  if (k < n) {
+    c = ere_pair_count(db, o, pv[k]);
+    np[k] = (u32)c;
+  }
+  c = wave_sum64(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+}
+void k_ere_pair_counts(const DB& db, const Options& o, const u32* pv, u32 n, u32* np, u64* total, Stream* st) {
+  fill_u32((u32*)total, 0, 2, st);
+  if (!n) return;
+  KLAUNCH(ere_pair_counts_k, grid_for(n, 256), 256, st, db, o, pv, n, np, (unsigned long long*)total);
+}
+// one thread per resolvent pair; the owning variable is found by a binary
+// search over the per-variable pair offsets
+__global__ void ere_pairs_k(DB db, ErePairs a, u32 npairs) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  u32 lo = 0, hi = a.nvars;  // first k with pbase[k] > t
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (a.pbase[mid] <= t) lo = mid + 1;
+    else hi = mid;
+  }
+  const u32 k = lo - 1;
+  ere_pair(db, a, k, t - a.pbase[k]);
+}
+void k_ere_pairs(const DB& db, const ErePairs& a, u32 npairs, Stream* st) {
+  if (!npairs) return;
+  KLAUNCH(ere_pairs_k, grid_for(npairs, 256), 256, st, db, a, npairs);
+}
 
 // ---------------------------------------------------------------- small helpers
 __global__ void assign_units_k(DB db, const u32* lits, u32 n) {
@@ -733,13 +865,16 @@ void k_pv_lits(const u32* pv, u32 n, u32* lits, Stream* st) {
   KLAUNCH(pv_lits_k, grid_for(n, 256), 256, st, pv, n, lits);
 }
 __global__ void max_list_k(DB db, const u32* lits, u32 n, u32* out) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicMax(out, db.ot_size[lits[i]]);
+  u32 m = 0;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = max(m, db.ot_size[lits[i]]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 void k_max_list(const DB& db, const u32* lits, u32 n, u32* out, Stream* st) {
   fill_u32(out, 0, 1, st);
   if (!n) return;
-  KLAUNCH(max_list_k, grid_for(n, 256), 256, st, db, lits, n, out);
+  KLAUNCH(max_list_k, std::min(grid_for(n, 256), RED_BLOCKS), 256, st, db, lits, n, out);
 }
 __global__ void reduce_lists_k(DB db) {
   const u32 l = blockIdx.x * blockDim.x + threadIdx.x + 2;
@@ -782,25 +917,24 @@ void k_zero_sizes(const DB& db, const u32* lits, u32 n, Stream* st) {
 
 // ---------------------------------------------------------------- count / compact
 __global__ void count_k(DB db, unsigned long long* out) {
-  __shared__ u32 lw[NW];
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  u32 live = 0, lits = 0;
-  if (c < db.ncls && !is_deleted(db, c)) {
-    live = 1;
-    lits = db.c_size[c];
+  unsigned long long live = 0, lits = 0;
+  for (u32 c = blockIdx.x * blockDim.x + threadIdx.x; c < db.ncls; c += gridDim.x * blockDim.x) {
+    if (!is_deleted(db, c)) {
+      live++;
+      lits += db.c_size[c];
+    }
   }
-  u32 t1, t2;
-  block_excl_scan(live, lw, &t1);
-  block_excl_scan(lits, lw, &t2);
-  if (threadIdx.x == 0) {
-    atomicAdd(&out[0], (unsigned long long)t1);
-    atomicAdd(&out[1], (unsigned long long)t2);
+  live = wave_sum64(live);
+  lits = wave_sum64(lits);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], live);
+    atomicAdd(&out[1], lits);
   }
 }
 void k_count(const DB& db, u64* out2, Stream* st) {
   fill_u32((u32*)out2, 0, 4, st);
   if (!db.ncls) return;
-  KLAUNCH(count_k, grid_for(db.ncls, ST), ST, st, db, (unsigned long long*)out2);
+  KLAUNCH(count_k, std::min(grid_for(db.ncls, ST), RED_BLOCKS), ST, st, db, (unsigned long long*)out2);
 }
 __global__ void live_k(DB db, u32* tc, u32* tl) {
   const u32 c = blockIdx.x * blockDim.x + threadIdx.x;

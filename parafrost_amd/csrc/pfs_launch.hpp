@@ -54,7 +54,8 @@ void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Strea
 void k_hist(const DB& db, u32* hist, Stream*);
 void k_scatter(const DB& db, u32* cursor, Stream*);
 // sort the lists of the given literals (lits[0..n)) by the total order
-void k_sort_lists(const DB& db, const u32* lits, u32 n, u32 max_len, u32* scratch, Stream*);
+// classes: 3n words of scratch for the size classes; cnt: 3 device words
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, Stream*);
 
 // ---- BCP
 void k_bcp_visit(const DB& db, const u32* front, u32 nfront, u32* next, u32* nnext, u32* conflict,
@@ -85,22 +86,31 @@ void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_
 
 // ---- per elected variable stages
 void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Stream*);  // |poss|+|negs|+extra
-void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, Stream*);
+void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, const u32* order,
+           Stream*);
 void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
 void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& s, Stream*);
 void k_sub_dirty(const DB& db, u32 from, u32 n, const SubSnap& s, const u32* unit_by, u32* first,
                  Stream*);
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
-void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, Stream*);
-void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, Stream*);
+void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
+                 Stream*);
+void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, const u32* order,
+                Stream*);
 void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
-           const u32* base, u32* stack, bool count, Stream*);
+           const u32* base, u32* stack, bool count, const u32* order, Stream*);
+// heaviest-first permutation of the elected indices (key: scratch of n words)
+void k_work_order(const DB& db, const u32* pv, u32 n, u32* key, u32* order, void* tmp, Stream*);
 void k_ere(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const EreOut& out,
            Stream*);
 void k_ere_dirty(const DB& db, const u32* pv, u32 from, u32 n, const u32* del_by, u32* first,
                  Stream*);
 void k_del_by(const u32* rec, u32 nrec, u32* del_by, Stream*);
 void k_apply_deletions(const DB& db, const u32* ids, u32 n, Stream*);
+// pair-parallel ERE
+void k_ht_build(const DB& db, const HashTable& ht, int maxsize, Stream*);
+void k_ere_pair_counts(const DB& db, const Options& o, const u32* pv, u32 n, u32* np, u64* total, Stream*);
+void k_ere_pairs(const DB& db, const ErePairs& a, u32 npairs, Stream*);
 
 // ---- small helpers
 void k_assign_units(const DB& db, const u32* lits, u32 n, Stream*);  // assign + FROZEN

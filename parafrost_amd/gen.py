@@ -38,7 +38,7 @@ def _lib():
         P = ctypes.POINTER(_PfgCnf)
         lib.pfg_uniform_ksat.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, P]
         lib.pfg_tseitin.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                    ctypes.c_double, ctypes.c_double, P]
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
         lib.pfg_heavytail.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double,
                                       ctypes.c_uint32, P]
         lib.pfg_prepare.argtypes = [P, P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -92,9 +92,13 @@ def uniform_ksat(nvars: int, nclauses: int, k: int = 3, seed: int = BASE_SEED + 
 
 
 def tseitin(ngates: int, ninputs: int = 5000, noutputs: int = 1000, seed: int = BASE_SEED + 2,
-            pand: float = 0.6, pxor: float = 0.3) -> Cnf:
+            pand: float = 0.6, pxor: float = 0.3, sat_outputs: bool = False) -> Cnf:
+    """Tseitin-encoded random circuit; sat_outputs asserts each output with its
+    value under a random simulation (satisfiable by construction) instead of
+    a random sign."""
     c = _PfgCnf()
-    _check(_lib().pfg_tseitin(seed, ninputs, ngates, noutputs, pand, pxor, ctypes.byref(c)), "tseitin")
+    _check(_lib().pfg_tseitin(seed, ninputs, ngates, noutputs, pand, pxor, int(sat_outputs), ctypes.byref(c)),
+           "tseitin")
     return _take(c)
 
 
@@ -145,7 +149,8 @@ def workload(name: str, scale: float = 1.0, seed: int | None = None) -> Cnf:
 
     uniform3  : uniform random 3-SAT, 20,000 vars, 85,200 clauses
     tseitin   : 2M-gate random circuit (60% AND, 30% XOR, 10% ITE), 5,000
-                inputs, 1,000 unit outputs (root-propagated)
+                inputs, 1,000 outputs asserted with their values under a random
+                simulation (satisfiable), root-propagated
     heavytail : 10M vars, 40M clauses, length 2+Geom(1/2) capped at 30,
                 variable weights rank^(-2/3)
     """
@@ -156,7 +161,7 @@ def workload(name: str, scale: float = 1.0, seed: int | None = None) -> Cnf:
         g = max(100, int(2_000_000 * scale))
         ni = max(10, int(5000 * min(1.0, scale * 10)))
         no = max(1, min(g, int(1000 * scale)))
-        raw = tseitin(g, ni, no, seed if seed is not None else BASE_SEED + 2)
+        raw = tseitin(g, ni, no, seed if seed is not None else BASE_SEED + 2, sat_outputs=True)
         cnf, unsat = prepare(raw)
         if unsat:
             raise RuntimeError("tseitin workload refuted at the root")

@@ -124,3 +124,21 @@ def test_memory_budget_admission():
             s.upload(cnf.maxvar, cnf.offsets, cnf.lits)
     finally:
         s.close()
+
+
+from tests.fuzz_cases import case as fuzz_case  # noqa: E402
+
+
+@pytest.mark.parametrize("i", range(150))
+def test_fuzz_dense_formulas(i):
+    """Small dense formulas: units from strengthening (re-run path),
+    cross-variable ERE deletions, refutations, learnt clauses."""
+    cnf, flags, lbd, ro, oo = fuzz_case(i)
+    prep, refuted = (cnf, False) if flags is not None else gen.prepare(cnf)
+    if refuted:
+        pytest.skip("refuted at the root")
+    g = run_gpu(prep.maxvar, prep.offsets, prep.lits, prep.trail, flags, lbd, **ro)
+    r = oracle.simplify(prep.maxvar, prep.offsets, prep.lits, flags, lbd, vstate=_vstate(prep),
+                        nassigned=int(prep.trail.shape[0]), det=1, **oo)
+    d = diff(g, r)
+    assert not d, (i, d)

@@ -1,0 +1,45 @@
+"""The oracle against the reference CPU simplifier run live on seeded
+random formulas (needs oracle/_ref/pfref, built from /root/reference by
+oracle/build_ref.sh; skipped where it is absent).  REF mode must match
+every output array bit for bit."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+from oracle import oracle
+from parafrost_amd import gen, pfd
+from tests.common import diff, oracle_kw
+from tests.fuzz_cases import case
+
+REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "pfref")
+pytestmark = pytest.mark.skipif(not os.path.exists(REF), reason="reference harness not built")
+
+
+def ref_args(opts):
+    out = []
+    for k, v in opts.items():
+        if k in ("phases", "mupos", "muneg"):
+            out.append(f"--{k}={v}")
+        else:
+            out.append(f"-{k}" if v else f"-no-{k}")
+    return out
+
+
+@pytest.mark.parametrize("i", range(60))
+def test_random_formulas(i):
+    cnf, flags, lbd, ro, oo = case(i)
+    if flags is not None:
+        pytest.skip("the reference parser reads original clauses only")
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "f.cnf")
+        gen.write_dimacs(cnf, p)
+        subprocess.run([REF, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd")] + ref_args(ro),
+                       capture_output=True, timeout=120)
+        if not os.path.exists(os.path.join(d, "i.pfd")):
+            pytest.skip("refuted by the reference parser")
+        inp, out = pfd.read(os.path.join(d, "i.pfd")), pfd.read(os.path.join(d, "o.pfd"))
+    r = oracle.simplify_state(inp, det=0, **oo)
+    d = diff(r, out, units_as_set=False)
+    assert not d, d
