@@ -341,6 +341,7 @@ void Simplifier::create_ot() {
   stage_begin(S_OT);
   const size_t n2 = 2 * (size_t)V_ + 2;
   DB db = make_db(cur_);
+  prop_changed_ = false;
   u32* hist = P<u32>(hist_);
   fill_u32(hist, 0, n2, st_);
   k_hist(db, hist, st_);
@@ -355,6 +356,7 @@ void Simplifier::create_ot() {
 bool Simplifier::prop() {
   if (trail_n_ == propagated_) return true;
   stage_begin(S_BCP);
+  prop_changed_ = true;
   DB db = make_db(cur_);
   u32* trail = P<u32>(trail_);
   u32* sc = P<u32>(scal_);
@@ -404,8 +406,13 @@ bool Simplifier::lcve() {
   const size_t n2 = 2 * (size_t)V_ + 2;
   DB db = make_db(cur_);
   u32* hist = P<u32>(hist_);
-  fill_u32(hist, 0, n2, st_);
-  k_hist(db, hist, st_);
+  // histSimp (lcve.cpp:32): the occurrence counts of the live clauses.  The
+  // occurrence-table scatter leaves exactly these counts in hist_ unless the
+  // round's BCP changed clauses since.
+  if (prop_changed_) {
+    fill_u32(hist, 0, n2, st_);
+    k_hist(db, hist, st_);
+  }
   u32* sc = P<u32>(scal_);
   u64* hard = (u64*)(sc + 16);
   fill_u32((u32*)hard, INF, 2, st_);
@@ -481,8 +488,10 @@ bool Simplifier::sub() {
   mem_.ensure(snap_size_, ob * 4, 0, st_);
   mem_.ensure(snap_sig_, ob * 4, 0, st_);
   mem_.ensure(snap_lbd_, ob * 4, 0, st_);
-  SubSnap snap{P<u32>(vbase_), P<u32>(snap_np_), P<u32>(snap_nn_), P<u32>(snap_ref_),
-               P<u8>(snap_flg_), P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_)};
+  mem_.ensure(unit_by_, std::max<size_t>(ncls_, 2 * (size_t)V_ + 3) * 4, 0, st_);
+  fill_u32(P<u32>(unit_by_), INF, ncls_, st_);  // clause owners
+  SubSnap snap{P<u32>(vbase_),    P<u32>(snap_np_),  P<u32>(snap_nn_),  P<u32>(snap_ref_), P<u8>(snap_flg_),
+               P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_), P<u32>(unit_by_)};
   k_sub_snapshot(db, P<u32>(pv_), E_, snap, st_);
   std::vector<u32> committed;
   u32 from = 0, passes = 0;
@@ -512,10 +521,8 @@ bool Simplifier::sub() {
     });
     u32 first_dirty = INF;
     if (nrec) {
-      fill_u32(P<u32>(unit_by_), INF, 2 * (size_t)V_ + 2, st_);
-      k_del_by(P<u32>(recs_), nrec, P<u32>(unit_by_), st_);
       fill_u32(sc + 3, INF, 1, st_);
-      k_sub_dirty(db, from, E_, snap, P<u32>(unit_by_), sc + 3, st_);
+      k_sub_dirty(db, snap, P<u32>(recs_), nrec, sc + 3, st_);
       first_dirty = read_u32(sc + 3);
     }
     // units of the exact prefix, in elected order (strengthen,

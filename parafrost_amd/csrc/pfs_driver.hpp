@@ -149,6 +149,7 @@ class Simplifier {
   std::vector<uint32_t> in_units_;
   uint32_t unassigned_ = 0;
   int phase_ = 0;
+  bool prop_changed_ = true;  // BCP touched clauses since the occurrence table was built
   uint32_t mu_inc_ = 0;
   std::vector<RoundStats> rounds_;
   // profiling

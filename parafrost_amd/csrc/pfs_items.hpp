@@ -503,6 +503,7 @@ struct SubSnap {
   u32* size;
   u32* sig;
   u32* lbd;
+  u32* owner;  // per clause: elected index of the variable whose lists hold it live
 };
 PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
   u32 e = s.base[k];
@@ -512,6 +513,7 @@ PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
     else s.nn[k] = ol.n;
     for (u32 j = 0; j < ol.n; j++, e++) {
       const u32 c = ol.r[j];
+      if (!is_deleted(db, c)) s.owner[c] = k;  // live clauses have one elected owner
       s.ref[e] = c;
       s.flg[e] = db.c_flags[c];
       s.size[e] = db.c_size[c];
@@ -548,20 +550,13 @@ PF_HD void sub_restore_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
     db.ot_size[lit] = n;
   }
 }
-// a variable must be re-run when one of its clauses contains a unit that an
-// earlier elected variable derived (unit_by[lit] = smallest such index)
-PF_HD bool sub_dirty_var(const DB& db, u32 k, const SubSnap& s, const u32* unit_by) {
-  u32 e = s.base[k];
-  const u32 tot = s.np[k] + s.nn[k];
-  for (u32 j = 0; j < tot; j++, e++) {
-    const u32 c = s.ref[e];
-    const u32* l = db.lits + db.c_off[c];
-    const u32 n = s.size[e];  // restored length covers the original literals
-    const u32 cur = db.c_size[c];
-    for (u32 i = 0; i < (cur < n ? cur : n); i++)
-      if (unit_by[l[i]] < k) return true;
-  }
-  return false;
+// A variable must be re-run when one of its clauses contains a unit that an
+// earlier elected variable derived: toblivion(ot[unit]) would have deleted
+// that clause first.  The clauses containing the unit are its occurrence
+// list; their owners later than the deriving variable are the ones affected.
+PF_HD u32 sub_dirty_owner(const SubSnap& s, u32 c, u32 k_unit) {
+  const u32 o = s.owner[c];
+  return (o != NOPOS && o > k_unit) ? o : NOPOS;
 }
 
 // ---------------------------------------------------------------- BVE (bve.cpp, gates)

@@ -386,22 +386,22 @@ __device__ inline void wave_append(bool pred, u32 val, u32* arr, u32* cnt) {
   base = __shfl(base, leader, 64);
   if (pred) arr[base + __popcll(mask & ((1ull << lane) - 1))] = val;
 }
-// splits the lists to sort into the three size classes
-__global__ void sort_classify_k(DB db, const u32* lits, u32 n, u32* small, u32* mid, u32* large, u32* cnt) {
+// collects the lists longer than SORT_SMALL (few: elected variables have
+// at least one short list) for the workgroup and long-list sorts
+__global__ void sort_classify_k(DB db, const u32* lits, u32 n, u32* big, u32* cnt) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 lit = i < n ? lits[i] : 0;
   const u32 len = i < n ? db.ot_size[lit] : 0;
-  wave_append(len >= 2 && len <= SORT_SMALL, lit, small, cnt);
-  wave_append(len > SORT_SMALL && len <= SORT_BLOCK, lit, mid, cnt + 1);
-  wave_append(len > SORT_BLOCK, lit, large, cnt + 2);
+  wave_append(len > SORT_SMALL, lit, big, cnt);
 }
 // 16 lanes per list: bitonic network over wave64 shuffles, keys in registers
 __global__ void sort_small_k(DB db, const u32* lits, u32 n) {
   const u32 g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const u32 j = threadIdx.x & 15;
-  const bool valid = g < n;
-  const u32 lit = valid ? lits[g] : 0;
-  const u32 len = valid ? db.ot_size[lit] : 0;
+  const u32 lit = g < n ? lits[g] : 0;
+  const u32 len0 = g < n ? db.ot_size[lit] : 0;
+  const bool valid = len0 >= 2 && len0 <= SORT_SMALL;  // longer lists go to sort_block_k
+  const u32 len = valid ? len0 : 0;
   u32* r = db.ot_refs + (valid ? db.ot_off[lit] : 0);
   u64 a = ~0ull, b = ~0ull;
   u32 id = 0xFFFFFFFFu;
@@ -504,17 +504,17 @@ __global__ void sort_large_k(DB db, const u32* lits, u32 n, u32* scratch) {
 }
 void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, Stream* st) {
   if (!n) return;
-  u32* small = classes;
-  u32* mid = classes + n;
-  u32* large = classes + 2 * (size_t)n;
-  fill_u32(cnt, 0, 3, st);
-  KLAUNCH(sort_classify_k, grid_for(n, 256), 256, st, db, lits, n, small, mid, large, cnt);
-  u32 h[3];
-  PFS_HIP(hipMemcpyAsync(h, cnt, 12, hipMemcpyDeviceToHost, st->s));
+  u32* big = classes;
+  fill_u32(cnt, 0, 1, st);
+  // short lists (the bulk) are sorted in place by 16-lane groups, longer
+  // ones are collected and sorted per workgroup (or by the long-list merge)
+  KLAUNCH(sort_small_k, grid_for((size_t)n * 16, 256), 256, st, db, lits, n);
+  KLAUNCH(sort_classify_k, grid_for(n, 256), 256, st, db, lits, n, big, cnt);
+  u32 nb = 0;
+  PFS_HIP(hipMemcpyAsync(&nb, cnt, 4, hipMemcpyDeviceToHost, st->s));
   PFS_HIP(hipStreamSynchronize(st->s));
-  if (h[0]) KLAUNCH(sort_small_k, grid_for((size_t)h[0] * 16, 256), 256, st, db, small, h[0]);
-  if (h[1]) KLAUNCH(sort_block_k, h[1], 256, st, db, mid, h[1]);
-  if (h[2]) KLAUNCH(sort_large_k, grid_for(h[2], 64), 64, st, db, large, h[2], scratch);
+  if (nb) KLAUNCH(sort_block_k, nb, 256, st, db, big, nb);
+  if (nb) KLAUNCH(sort_large_k, grid_for(nb, 64), 64, st, db, big, nb, scratch);
 }
 
 // ---------------------------------------------------------------- BCP
@@ -695,14 +695,18 @@ void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& 
   if (from >= n) return;
   KLAUNCH(sub_restore_k, grid_for(n - from, VB), VB, st, db, pv, from, n, s);
 }
-__global__ void sub_dirty_k(DB db, u32 from, u32 n, SubSnap s, const u32* unit_by, u32* first) {
-  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n && sub_dirty_var(db, k, s, unit_by)) atomicMin(first, k);
+// one workgroup per derived unit: the owners of the clauses in its list that
+// come after the deriving variable
+__global__ void sub_dirty_k(DB db, SubSnap s, const u32* rec, u32* first) {
+  const u32 k_unit = rec[3 * (u64)blockIdx.x], u = rec[3 * (u64)blockIdx.x + 2];
+  const OLV ol = olist(db, u);
+  u32 m = NOPOS;
+  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) m = min(m, sub_dirty_owner(s, ol.r[i], k_unit));
+  if (m != NOPOS) atomicMin(first, m);
 }
-void k_sub_dirty(const DB& db, u32 from, u32 n, const SubSnap& s, const u32* unit_by, u32* first,
-                 Stream* st) {
-  if (from >= n) return;
-  KLAUNCH(sub_dirty_k, grid_for(n - from, VB), VB, st, db, from, n, s, unit_by, first);
+void k_sub_dirty(const DB& db, const SubSnap& s, const u32* rec, u32 nrec, u32* first, Stream* st) {
+  if (!nrec) return;
+  KLAUNCH(sub_dirty_k, nrec, 256, st, db, s, rec, first);
 }
 __global__ void oblivion_k(DB db, const u32* lits) {
   const u32 t = lits[blockIdx.x];

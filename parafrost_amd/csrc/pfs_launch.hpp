@@ -90,8 +90,9 @@ void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const
            Stream*);
 void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
 void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& s, Stream*);
-void k_sub_dirty(const DB& db, u32 from, u32 n, const SubSnap& s, const u32* unit_by, u32* first,
-                 Stream*);
+// first elected index (> the deriving one) owning a clause that contains a
+// unit of the records (elected index, sequence, literal)
+void k_sub_dirty(const DB& db, const SubSnap& s, const u32* rec, u32 nrec, u32* first, Stream*);
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
                  Stream*);
