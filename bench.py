@@ -29,26 +29,54 @@ REF_OPTIONS = ["--phases=1", "-bce"]
 
 
 def kernel_bytes(name, rounds, cnf):
-    """Algorithmic bytes per launch of the clause-parallel kernels: each reads
-    the live clause headers (offset, size: 8 B; status: 1 B per slot) and
-    literals (4 B) once and writes its output once.  Counts per launch are the
-    live clauses/literals at the start of the round the launch belongs to."""
-    c0, l0 = cnf.nclauses, cnf.nlits
-    starts = [(c0, l0)] + [(r["clauses_after"], r["literals_after"]) for r in rounds[:-1]]
+    """Algorithmic bytes per launch of a kernel, from the work measures of
+    the profiled step.  A clause costs a 12-byte header (offset, size,
+    status/signature) plus 4 bytes per literal; a list entry costs 4 bytes.
+
+    clause-parallel kernels, per round (C, L, V at round start):
+      hist_k     D + 8V            (read every clause, write the 2V histogram)
+      scatter_k  D + 4L + 8V       (read every clause and the list offsets, write one ref per literal)
+      count_k    D                 (read every clause)
+    per-elected-variable kernels (O_E list entries, D_E clause bytes of the
+    elected variables' lists; elected variables share no clause):
+      sub_k, ve_decide_k, bce_k   4 O_E + D_E
+      ve_apply_k                  4 O_E + D_E + 4 E (decisions)
+    """
     V = cnf.maxvar
     per = []
-    for c, l in starts:
-        if name == "hist_k":      # read headers + literals, write the 2V histogram
-            per.append(9 * c + 4 * l + 8 * V)
-        elif name == "scatter_k":  # read headers + literals + list offsets, write refs
-            per.append(9 * c + 4 * l + 8 * V + 4 * l)
+    for r in rounds:
+        c, l = r["clauses_before"], r["literals_before"]
+        d = 12 * c + 4 * l
+        oe, de = r["elected_occurs"], r["elected_bytes"]
+        if name == "hist_k":
+            per.append(d + 8 * V)
+        elif name == "scatter_k":
+            per.append(d + 4 * l + 8 * V)
         elif name == "count_k":
-            per.append(9 * c)
-        elif name == "bcp_clean_k":
-            per.append(c)
+            per.append(d)
+        elif name in ("sub_k", "ve_decide_k", "bce_k"):
+            if oe and r["clauses_after"]:  # rounds that ran SUB/VE/BCE (not the stopping round)
+                per.append(4 * oe + de)
+        elif name == "ve_apply_k":
+            if oe and r["clauses_after"]:
+                per.append(4 * oe + de + 4 * r["elected"])
         else:
             return None
+    if not per or not all(per):
+        return None
     return sum(per) / len(per)
+
+
+def calibrate_copy(nbytes=256 << 20, reps=3):
+    """Plain device-to-device copies of a known size, so that hardware
+    counter runs over this script can scale FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"bench: calibration copy {nbytes} bytes x {reps}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(cnf, name, budget_clauses=7_500_000):
@@ -99,6 +127,10 @@ def main():
     ap.add_argument("--workload", default="tseitin", choices=["tseitin", "uniform3", "heavytail"])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="extra simplifier option (launch shaping such as workorder=0)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run 256 MiB device copies first (reference kernel for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -114,7 +146,13 @@ def main():
     print(f"bench: workload {args.workload} scale {args.scale}: {cnf.maxvar} vars, {cnf.nclauses} clauses, "
           f"{cnf.nlits} literals ({gen_s:.1f}s to generate)", file=sys.stderr, flush=True)
 
-    s = Simplifier(0, keep_input=True, **OPTIONS)
+    if args.calibrate_copy:
+        calibrate_copy()
+    opts = dict(OPTIONS)
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        opts[k] = int(v)
+    s = Simplifier(0, keep_input=True, **opts)
     s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
     try:
         import torch

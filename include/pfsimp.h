@@ -52,6 +52,10 @@ typedef struct {
   uint32_t elected, eliminated, units, resolvents;
   uint64_t clauses_after, literals_after;
   uint32_t mis_iterations, sub_passes, ere_passes, reserved;
+  /* work measures, filled when "profilegpu" is on: live clauses and
+   * literals at round start, live occurrences in the elected variables'
+   * lists and their clause bytes (12-byte header + 4 bytes per literal) */
+  uint64_t clauses_before, literals_before, elected_occurs, elected_bytes;
 } pfs_round_stats;
 
 /* Creates a context on a device.  Replaces the allocation part of the
@@ -65,7 +69,11 @@ void pfs_destroy(pfs_ctx* ctx);
  * "vephasemin", "veclausemax", "xormaxarity", "submax", "bcemax", "eremax",
  * "ereclausemax", "lcvemin", "gcfreq", "lbdtier1".  GPU-build aliases
  * ("eliminatedlitsmin", "resolventmax", "submaxoccurs", "bcemaxoccurs",
- * "eremaxoccurs", "electionsmin", "collectfreq", "velitsbound") are accepted. */
+ * "eremaxoccurs", "electionsmin", "collectfreq", "velitsbound") are accepted.
+ * "proof" (boolean, default off) records the binary DRAT lines of the run
+ * (pfs_download_proof).
+ * "workorder" (boolean, default on; no reference counterpart) launches the
+ * per-variable kernels heaviest lists first; results do not depend on it. */
 pfs_status pfs_set_option(pfs_ctx* ctx, const char* name, const char* value);
 
 /* Hands over the root-level clause database, as the host solver has it after
@@ -107,6 +115,15 @@ pfs_status pfs_download_vstate(pfs_ctx* ctx, uint8_t* vstate);
 /* Eliminated-variable stack words to append to the model's stack, in
  * original numbering (model.resolved; cacheResolved, src/gpu/transfer.cu:83-96). */
 pfs_status pfs_download_stack(pfs_ctx* ctx, uint32_t* words);
+/* Binary DRAT proof of the last pfs_simplify when option "proof" is on:
+ * the lines the reference simplifier writes through its PROOF object
+ * (src/cpu/proof.cpp:94-229, called from bve.cpp:214-215,
+ * elimination.cpp:229-230, equivalence.h:65-83, subsume.h:114,
+ * redundancy.h:137, solver.h:251-254 and 475-479; the GPU build's cuPROOF,
+ * src/gpu/proof.cu:161-254): 'a'/'d', literals 2*vorg(v)+sign as 7-bit
+ * groups, 0.  Lines appended by the host solver's own proof go around it. */
+pfs_status pfs_proof_size(pfs_ctx* ctx, uint64_t* nbytes);
+pfs_status pfs_download_proof(pfs_ctx* ctx, uint8_t* bytes);
 pfs_status pfs_round_stats_get(pfs_ctx* ctx, pfs_round_stats* out, uint32_t max, uint32_t* n);
 /* Per-stage device milliseconds (when "profilegpu" is on) and stage calls. */
 pfs_status pfs_stage_times(pfs_ctx* ctx, double* ms, uint64_t* calls, uint32_t n);

@@ -19,14 +19,14 @@ class OracleOpts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "phases", "mu_pos", "mu_neg", "lits_min", "ve_clause_limit", "xor_max_arity",
         "sub_limit", "bce_limit", "ere_limit", "ere_max_resolvent", "lcve_min", "shrink_rate", "lbd_tier1",
-        "ve_en", "ve_plus_en", "sub_en", "bce_en", "ere_en", "all_en", "ve_lbound_en", "det")]
+        "ve_en", "ve_plus_en", "sub_en", "bce_en", "ere_en", "all_en", "ve_lbound_en", "det", "proof_en")]
 
 
 # Defaults of the reference CPU simplifier options (src/cpu/options.cpp:24-46).
 DEFAULTS = dict(phases=5, mu_pos=32, mu_neg=32, lits_min=500, ve_clause_limit=100, xor_max_arity=10,
                 sub_limit=3000, bce_limit=3000, ere_limit=3000, ere_max_resolvent=1000, lcve_min=1,
                 shrink_rate=2, lbd_tier1=2, ve_en=1, ve_plus_en=1, sub_en=1, bce_en=0, ere_en=1, all_en=0,
-                ve_lbound_en=0, det=1)
+                ve_lbound_en=0, det=1, proof_en=0)
 
 
 def lib():
@@ -44,6 +44,9 @@ def lib():
         L.oracle_sizes.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_get.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 8
         L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_proof_size.argtypes = [ctypes.c_void_p]
+        L.oracle_proof_size.restype = ctypes.c_uint64
+        L.oracle_get_proof.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -61,6 +64,7 @@ class Result:
     vstate: np.ndarray
     elected: np.ndarray
     seconds: float
+    proof: bytes = b""  # binary DRAT lines of the run (when proof_en)
 
 
 def _p(a):
@@ -92,6 +96,11 @@ def simplify(maxvar, offsets, lits, flags=None, lbd=None, vstate=None, assumed=N
                    L.oracle_seconds(h))
         L.oracle_get(h, _p(r.sizes), _p(r.flags), _p(r.lbd), _p(r.lits), _p(r.units), _p(r.resolved),
                      _p(r.vstate), _p(r.elected))
+        npf = int(L.oracle_proof_size(h))
+        if npf:
+            pf = np.zeros(npf, np.uint8)
+            L.oracle_get_proof(h, pf.ctypes.data)
+            r.proof = pf.tobytes()
         return r
     finally:
         L.oracle_free(h)

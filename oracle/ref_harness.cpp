@@ -30,6 +30,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <sys/stat.h>
 #include <iostream>
 #include <fstream>
 #include <sstream>
@@ -106,7 +107,7 @@ double now_s() {
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    fprintf(stderr, "usage: pfref <in.cnf> <input.pfd> <output.pfd> [options]\n");
+    fprintf(stderr, "usage: pfref <in.cnf> <input.pfd> <output.pfd> [options, e.g. -proof --proofout=p.drat]\n");
     return 2;
   }
   std::string cnf = argv[1], in_path = argv[2];
@@ -149,6 +150,18 @@ int main(int argc, char** argv) {
     write_pfd(in_path, 2u, inf.maxVar, cd, units, none, vst);
   }
   const uint32_t trail_start = s->trail.size();
+  // with -proof: the simplifier's proof lines start here; the position is
+  // written next to the output dump as <output.pfd>.proofpos
+  if (s->opts.proof_en) {
+    fflush(NULL);  // the proof FILE is private to PROOF: flush every stream
+    struct stat sb;
+    const long pos = stat(s->opts.proof_path, &sb) == 0 ? (long)sb.st_size : -1L;
+    FILE* pp = fopen((g_out_path + ".proofpos").c_str(), "w");
+    if (pp) {
+      fprintf(pp, "%ld\n", pos);
+      fclose(pp);
+    }
+  }
 
   std::map<std::string, double> t;
   auto T = [&](const char* k, double d) { t[k] += d; };
@@ -225,5 +238,6 @@ int main(int argc, char** argv) {
   for (auto& kv : t) printf(", \"%s_s\": %.6f", kv.first.c_str(), kv.second);
   printf("}\n");
   fflush(stdout);
+  if (s->opts.proof_en) fflush(NULL);
   _exit(0);  // skip the reference destructors' logging
 }

@@ -56,6 +56,7 @@ struct Opts {
   int32_t sub_limit, bce_limit, ere_limit, ere_max_resolvent, lcve_min, shrink_rate, lbd_tier1;
   int32_t ve_en, ve_plus_en, sub_en, bce_en, ere_en, all_en, ve_lbound_en;
   int32_t det;
+  int32_t proof_en;
 };
 
 // Clause record restating SCLAUSE (src/cpu/sclause.h:29-160); literals live
@@ -111,8 +112,39 @@ struct Oracle {
   }
 
   // ------------------------------------------------------------ trail / model
-  void learnEmpty() { throw Unsat(); }
+  // ------------------------------------------------------------ DRAT proof
+  // Binary DRAT as src/cpu/proof.cpp:94-175 writes it: 'a' or 'd', each
+  // literal as 2*vorg(v)+sign in 7-bit groups (low first, bit 7 set on all
+  // but the last byte), then a 0 byte.
+  std::vector<uint8_t> proof;
+  void proofLits(const u32* l, int n) {
+    for (int i = 0; i < n; i++) {
+      u32 r = (vorg[ABS(l[i])] << 1) | SIGN(l[i]);
+      while (r > 127) { proof.push_back((uint8_t)((r & 127) | 128)); r >>= 7; }
+      proof.push_back((uint8_t)r);
+    }
+    proof.push_back(0);
+  }
+  void proofAdd(const u32* l, int n) { proof.push_back('a'); proofLits(l, n); }
+  void proofDelete(u32 c) {  // removeClause -> PROOF::deleteClause (sclause.cpp:77-84)
+    if (!opts.proof_en) return;
+    proof.push_back('d');
+    proofLits(L(c), cls[c].size);
+  }
+  void proofResolvent(const u32* l, int n) {  // PROOF::addResolvent (proof.cpp:214-229)
+    if (!opts.proof_en) return;
+    for (int i = 0; i < n; i++)
+      if (value[l[i]] > 0) return;  // satisfied: no line
+    proofAdd(l, n);
+  }
+
+  // ------------------------------------------------------------ trail / model
+  void learnEmpty() {  // solver.h:251-254
+    if (opts.proof_en) { proof.push_back('a'); proof.push_back(0); }
+    throw Unsat();
+  }
   void enqueueUnit(u32 l) {  // solver.h:430-441 (learnUnit marks FROZEN, solver.h:475-479)
+    if (opts.proof_en) proofAdd(&l, 1);
     vstate[ABS(l)] = FROZEN_M;
     value[l] = 1;
     value[FLIP(l)] = 0;
@@ -364,11 +396,11 @@ struct Oracle {
         if (propClause(f, c)) cls[c].st = DELETED;
         else {
           const int size = cls[c].size;
-          if (!size) return false;
+          if (!size) learnEmpty();  // elimination.cpp:80 (learnEmpty; prop fails)
           if (size == 1) {
             const u32 unit = lit(c, 0);
             if (value[unit] < 0) enqueueUnit(unit);
-            else return false;
+            else learnEmpty();
           }
         }
       }
@@ -507,8 +539,9 @@ struct Oracle {
       } else if (!val) {
         learnEmpty();
       }
-    } else if (learnt(c)) {
-      bumpShrunken(c);
+    } else {
+      proofResolvent(d, n);
+      if (learnt(c)) bumpShrunken(c);
     }
   }
   bool subsume(OL& l, size_t end, u32 cand) {  // subsume.h:105-121
@@ -519,6 +552,7 @@ struct Oracle {
       if (cls[cand].molten && cls[s].size > csz) continue;
       if (cls[s].size > 1 && subSig(cls[s].sig, cls[cand].sig) && subLits(s, cand)) {
         if (learnt(s) && original(cand)) cls[s].st = ORIGINAL;
+        proofDelete(cand);
         cls[cand].st = DELETED;  // removeClause (sclause.cpp:77-84)
         return true;
       }
@@ -600,6 +634,8 @@ struct Oracle {
           const int8_t val = value[unit];
           if (val < 0) enqueueUnit(unit);
           else if (!val) return true;
+        } else {
+          proofResolvent(L(c), cls[c].size);
         }
       }
     }
@@ -611,6 +647,8 @@ struct Oracle {
           const int8_t val = value[unit];
           if (val < 0) enqueueUnit(unit);
           else if (!val) return true;
+        } else {
+          proofResolvent(L(c), cls[c].size);
         }
       }
     }
@@ -824,6 +862,7 @@ struct Oracle {
       else if (!val) learnEmpty();
       return;
     }
+    proofResolvent(r.data(), size);
     SC s;
     s.st = ORIGINAL;
     s.size = size;
@@ -991,6 +1030,7 @@ struct Oracle {
         const u32 c = ml[i];
         if (len == cls[c].size && (learnt(c) || cls[c].st == type) && subSig(msig, cls[c].sig) &&
             merge_ere(x, c1, c2, c)) {
+          proofDelete(c);
           cls[c].st = DELETED;
           break;
         }
@@ -1079,6 +1119,7 @@ struct Result {
   std::vector<u32> sizes, lbds, lits, units, resolved;
   std::vector<uint8_t> flags, vstate;
   std::vector<u32> elected;
+  std::vector<uint8_t> proof;
   double seconds = 0;
 };
 
@@ -1091,6 +1132,7 @@ typedef struct {
   int32_t sub_limit, bce_limit, ere_limit, ere_max_resolvent, lcve_min, shrink_rate, lbd_tier1;
   int32_t ve_en, ve_plus_en, sub_en, bce_en, ere_en, all_en, ve_lbound_en;
   int32_t det;
+  int32_t proof_en;
 } oracle_opts;
 
 // Runs the restated simplifier.  Input: maxvar, clauses as offsets/lits
@@ -1157,6 +1199,7 @@ void* oracle_run(const oracle_opts* o, uint32_t maxvar, uint64_t ncls, const uin
   r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   r->maxvar = maxvar;
   r->elected = s->elected_per_round;
+  r->proof = s->proof;  // kept for UNSAT too (ends with the empty clause)
   if (r->status) {
     for (u32 c : s->scnf) {
       const SC& x = s->cls[c];
@@ -1196,6 +1239,11 @@ void oracle_get(void* h, uint32_t* sizes, uint8_t* flags, uint32_t* lbds, uint32
   if (resolved) std::copy(r->resolved.begin(), r->resolved.end(), resolved);
   if (vstate) std::copy(r->vstate.begin(), r->vstate.end(), vstate);
   if (elected) std::copy(r->elected.begin(), r->elected.end(), elected);
+}
+uint64_t oracle_proof_size(void* h) { return ((Result*)h)->proof.size(); }
+void oracle_get_proof(void* h, uint8_t* out) {
+  Result* r = (Result*)h;
+  std::copy(r->proof.begin(), r->proof.end(), out);
 }
 void oracle_free(void* h) { delete (Result*)h; }
 

@@ -21,7 +21,7 @@ struct pfs_ctx {
           xor_max_arity = 10, sub_limit = 3000, bce_limit = 3000, ere_limit = 3000,
           ere_max_resolvent = 1000, lcve_min = 1, shrink_rate = 2, lbd_tier1 = 2;
   bool ve = true, ve_plus = true, sub = true, bce = false, ere = true, all = false, velbound = false,
-       profile = false;
+       profile = false, work_order = true, proof = false;
 };
 
 namespace {
@@ -51,6 +51,8 @@ Options normalise(const pfs_ctx* c) {
   o.phases = c->phases;
   o.shrink_rate = c->shrink_rate;
   o.lbd_tier1 = c->lbd_tier1;
+  o.work_order = c->work_order;
+  o.proof_en = c->proof;
   if (o.all_en) o.ve_en = 1, o.ve_plus_en = 1, o.bce_en = 1, o.ere_en = 1;
   if (!o.phases && (o.ve_en || o.sub_en || o.bce_en)) o.phases = 1;
   if (o.phases && !(o.ve_en || o.sub_en || o.bce_en)) o.phases = 0;
@@ -170,6 +172,8 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"ere", nullptr, &pfs_ctx::ere}, {"all", nullptr, &pfs_ctx::all},
       {"velbound", "velitsbound", &pfs_ctx::velbound},
       {"profilegpu", "profilesimp", &pfs_ctx::profile},
+      {"workorder", nullptr, &pfs_ctx::work_order},
+      {"proof", nullptr, &pfs_ctx::proof},
   };
   for (const BoolOpt& o : bools) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -287,6 +291,22 @@ pfs_status pfs_download_stack(pfs_ctx* c, uint32_t* words) {
   });
 }
 
+pfs_status pfs_proof_size(pfs_ctx* c, uint64_t* nbytes) {
+  if (!c || !nbytes) return PFS_BAD_INPUT;
+  if (!c->simplified) return PFS_BAD_STATE;
+  *nbytes = c->s->proof_bytes();
+  return PFS_OK;
+}
+
+pfs_status pfs_download_proof(pfs_ctx* c, uint8_t* bytes) {
+  if (!c || (!bytes && c->simplified && c->s->proof_bytes())) return PFS_BAD_INPUT;
+  if (!c->simplified) return PFS_BAD_STATE;
+  return guard(c, [&] {
+    c->s->download_proof(bytes);
+    return PFS_OK;
+  });
+}
+
 pfs_status pfs_round_stats_get(pfs_ctx* c, pfs_round_stats* out, uint32_t max, uint32_t* n) {
   if (!c) return PFS_BAD_INPUT;
   const auto& r = c->s->rounds();
@@ -302,6 +322,10 @@ pfs_status pfs_round_stats_get(pfs_ctx* c, pfs_round_stats* out, uint32_t max, u
     out[i].sub_passes = r[i].sub_passes;
     out[i].ere_passes = r[i].ere_passes;
     out[i].reserved = 0;
+    out[i].clauses_before = r[i].clauses_before;
+    out[i].literals_before = r[i].literals_before;
+    out[i].elected_occurs = r[i].elected_occurs;
+    out[i].elected_bytes = r[i].elected_bytes;
   }
   if (n) *n = (uint32_t)r.size();
   return PFS_OK;

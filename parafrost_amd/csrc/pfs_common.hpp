@@ -63,18 +63,27 @@ struct Options {
           ere_max_resolvent = 1000, lcve_min = 1, shrink_rate = 2, lbd_tier1 = 2;
   int32_t ve_en = 1, ve_plus_en = 1, sub_en = 1, bce_en = 0, ere_en = 1, all_en = 0,
           ve_lbound_en = 0;
+  // launch shaping only (results do not depend on it): process the elected
+  // variables heaviest-first
+  int32_t work_order = 1;
+  int32_t proof_en = 0;  // write binary DRAT lines (src/cpu/proof.cpp)
 };
 
 // Device view of the clause database and per-variable state.  Root
 // assignments are kept per variable (assign[v] = true literal + 1) so that
 // parallel unit propagation can claim a variable with one compare-and-swap.
+//
+// Clause c lives in the word pool 'lits' at c_off[c]: a two-word header
+// followed by its literals, so one clause visit reads one contiguous run:
+//   word 0: size (bits 0-23) | flags (bits 24-31: status, molten, added, usage)
+//   word 1: signature (OR of 1 << (lit & 31))
+//   word 2..: literals, sorted ascending
+// The flags byte and the size bytes are written with separate byte/short
+// stores, so a flag update never rewrites the size and vice versa.
 struct DB {
-  u32* lits;
-  u32* c_off;
-  u32* c_size;
-  u32* c_sig;
-  u8* c_flags;
-  u32* c_lbd;
+  u32* lits;      // word pool: headers and literals
+  u32* c_off;     // per clause: pool offset of its header
+  u32* c_lbd;     // per clause: LBD of learnt clauses
   u32* ot_off;
   u32* ot_size;
   u32* ot_refs;
@@ -85,5 +94,23 @@ struct DB {
   u32 nvars;
   u32 ncls;
 };
+
+constexpr u32 CL_HDR = 2;                 // header words per clause
+constexpr u32 CL_MAX_SIZE = 0xFFFFFFu;     // largest clause the header holds
+PF_HD u32 make_hdr(u32 size, u8 flags) { return size | ((u32)flags << 24); }
+PF_HD u32 hdr_size(u32 h) { return h & CL_MAX_SIZE; }
+PF_HD u8 hdr_flags(u32 h) { return (u8)(h >> 24); }
+PF_HD u32* chead(const DB& db, u32 c) { return db.lits + db.c_off[c]; }
+PF_HD u32 csize(const DB& db, u32 c) { return hdr_size(chead(db, c)[0]); }
+PF_HD u8 cflags(const DB& db, u32 c) { return hdr_flags(chead(db, c)[0]); }
+PF_HD u32 csig(const DB& db, u32 c) { return chead(db, c)[1]; }
+PF_HD u32* clits(const DB& db, u32 c) { return chead(db, c) + CL_HDR; }
+PF_HD void set_cflags(const DB& db, u32 c, u8 f) { ((u8*)chead(db, c))[3] = f; }
+PF_HD void set_csize(const DB& db, u32 c, u32 n) {
+  u32* h = chead(db, c);
+  ((uint16_t*)h)[0] = (uint16_t)n;
+  ((u8*)h)[2] = (u8)(n >> 16);
+}
+PF_HD void set_csig(const DB& db, u32 c, u32 sig) { chead(db, c)[1] = sig; }
 
 }  // namespace pfs

@@ -54,14 +54,14 @@ Simplifier::Simplifier(int device, size_t budget, bool keep_input)
 }
 
 Simplifier::~Simplifier() {
-  Buf* all[] = {&c_off_[0], &c_size_[0], &c_sig_[0], &c_flags_[0], &c_lbd_[0], &lits_[0],
-                &c_off_[1], &c_size_[1], &c_sig_[1], &c_flags_[1], &c_lbd_[1], &lits_[1],
+  Buf* all[] = {&c_off_[0], &c_lbd_[0], &lits_[0], &c_off_[1], &c_lbd_[1], &lits_[1],
                 &ot_off_, &ot_size_, &ot_refs_, &hist_, &assign_, &vstate_, &assumed_, &vorg_, &trail_,
                 &score_, &cflag_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_,
                 &def_, &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_ref_, &snap_flg_,
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
-                &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_, &in_c_size_,
-                &in_c_sig_, &in_c_flags_, &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_};
+                &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
+                &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
+                &pstamp_, &plines_, &pbytes_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
@@ -71,9 +71,6 @@ DB Simplifier::make_db(int w) {
   DB db;
   db.lits = P<u32>(lits_[w]);
   db.c_off = P<u32>(c_off_[w]);
-  db.c_size = P<u32>(c_size_[w]);
-  db.c_sig = P<u32>(c_sig_[w]);
-  db.c_flags = P<u8>(c_flags_[w]);
   db.c_lbd = P<u32>(c_lbd_[w]);
   db.ot_off = P<u32>(ot_off_);
   db.ot_size = P<u32>(ot_size_);
@@ -113,9 +110,6 @@ void Simplifier::alloc_state() {
   const size_t V1 = (size_t)V_ + 1, L2 = 2 * (size_t)V_ + 3;
   for (int w = 0; w < 2; w++) {
     mem_.ensure(c_off_[w], ccap_ * 4, 0, st_);
-    mem_.ensure(c_size_[w], ccap_ * 4, 0, st_);
-    mem_.ensure(c_sig_[w], ccap_ * 4, 0, st_);
-    mem_.ensure(c_flags_[w], ccap_, 0, st_);
     mem_.ensure(c_lbd_[w], ccap_ * 4, 0, st_);
     mem_.ensure(lits_[w], lcap_ * 4, 0, st_);
   }
@@ -156,26 +150,27 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     if (!(flags && (flags[i] & 1))) order.push_back(i);
   for (uint64_t i = 0; i < ncls; i++)
     if (flags && (flags[i] & 1)) order.push_back(i);
-  const uint64_t L = offsets[ncls] - offsets[0];
+  // pool words: a header per clause, then its literals
+  const uint64_t L = offsets[ncls] - offsets[0] + CL_HDR * ncls;
   if (ncls >= INF / 2 || L >= INF / 4) throw AllocFail{"formula too large for 32-bit clause ids"};
-  std::vector<u32> h_off(ncls), h_size(ncls), h_lbd(ncls), h_lits(L);
-  std::vector<u8> h_flags(ncls);
+  std::vector<u32> h_off(ncls), h_lbd(ncls), h_lits(L);
   uint64_t w = 0;
   for (uint64_t k = 0; k < ncls; k++) {
     const uint64_t i = order[k];
     const uint64_t n = offsets[i + 1] - offsets[i];
+    if (n > CL_MAX_SIZE) throw std::invalid_argument("clause longer than 2^24-1 literals");
     h_off[k] = (u32)w;
-    h_size[k] = (u32)n;
     const u8 f = flags ? flags[i] : 0;
     const bool learnt = f & 1;
-    h_flags[k] = learnt ? (u8)(ST_LEARNT | (f & USAGE_MASK)) : (u8)ST_ORIGINAL;
+    h_lits[w] = make_hdr((u32)n, learnt ? (u8)(ST_LEARNT | (f & USAGE_MASK)) : (u8)ST_ORIGINAL);
+    h_lits[w + 1] = 0;  // signature: set by the literal sort on the device
     h_lbd[k] = learnt && lbd ? (lbd[i] & 0x03FFFFFFu) : 0u;
     for (uint64_t j = 0; j < n; j++) {
       const u32 l = lits[offsets[i] + j];
       if (ABS(l) == 0 || ABS(l) > maxvar) throw std::invalid_argument("literal out of range");
-      h_lits[w + j] = l;
+      h_lits[w + CL_HDR + j] = l;
     }
-    w += n;
+    w += CL_HDR + n;
   }
   ccap_ = std::max<uint64_t>(2 * ncls + 1024, 4096);
   lcap_ = std::max<uint64_t>(2 * L + 4096, 16384);
@@ -184,8 +179,6 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   ncls_ = ncls;
   lits_end_ = L;
   h2d(c_off_[0].p, h_off.data(), ncls * 4, st_);
-  h2d(c_size_[0].p, h_size.data(), ncls * 4, st_);
-  h2d(c_flags_[0].p, h_flags.data(), ncls, st_);
   h2d(c_lbd_[0].p, h_lbd.data(), ncls * 4, st_);
   h2d(lits_[0].p, h_lits.data(), L * 4, st_);
   // per-variable state
@@ -220,18 +213,12 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     in_trail_ = ntrail;
     in_unassigned_ = unassigned_;
     mem_.ensure(in_c_off_, ncls * 4, 0, st_);
-    mem_.ensure(in_c_size_, ncls * 4, 0, st_);
-    mem_.ensure(in_c_sig_, ncls * 4, 0, st_);
-    mem_.ensure(in_c_flags_, ncls, 0, st_);
     mem_.ensure(in_c_lbd_, ncls * 4, 0, st_);
     mem_.ensure(in_lits_, L * 4, 0, st_);
     mem_.ensure(in_assign_, ((size_t)V_ + 1) * 4, 0, st_);
     mem_.ensure(in_vstate_, (size_t)V_ + 1, 0, st_);
     mem_.ensure(in_trail_buf_, ((size_t)ntrail + 1) * 4, 0, st_);
     d2d(in_c_off_.p, c_off_[0].p, ncls * 4, st_);
-    d2d(in_c_size_.p, c_size_[0].p, ncls * 4, st_);
-    d2d(in_c_sig_.p, c_sig_[0].p, ncls * 4, st_);
-    d2d(in_c_flags_.p, c_flags_[0].p, ncls, st_);
     d2d(in_c_lbd_.p, c_lbd_[0].p, ncls * 4, st_);
     d2d(in_lits_.p, lits_[0].p, L * 4, st_);
     d2d(in_assign_.p, assign_.p, ((size_t)V_ + 1) * 4, st_);
@@ -248,9 +235,6 @@ void Simplifier::rewind() {
   lits_end_ = in_nlits_;
   ensure_clause_room(0, 0);
   d2d(c_off_[0].p, in_c_off_.p, in_ncls_ * 4, st_);
-  d2d(c_size_[0].p, in_c_size_.p, in_ncls_ * 4, st_);
-  d2d(c_sig_[0].p, in_c_sig_.p, in_ncls_ * 4, st_);
-  d2d(c_flags_[0].p, in_c_flags_.p, in_ncls_, st_);
   d2d(c_lbd_[0].p, in_c_lbd_.p, in_ncls_ * 4, st_);
   d2d(lits_[0].p, in_lits_.p, in_nlits_ * 4, st_);
   d2d(assign_.p, in_assign_.p, ((size_t)V_ + 1) * 4, st_);
@@ -273,9 +257,6 @@ void Simplifier::ensure_clause_room(uint64_t add_cls, uint64_t add_lits) {
   for (int w = 0; w < 2; w++) {
     const bool keep = (w == cur_);
     mem_.ensure(c_off_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
-    mem_.ensure(c_size_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
-    mem_.ensure(c_sig_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
-    mem_.ensure(c_flags_[w], nc, keep ? ncls_ : 0, st_);
     mem_.ensure(c_lbd_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
     mem_.ensure(lits_[w], nl * 4, keep ? lits_end_ * 4 : 0, st_);
   }
@@ -369,6 +350,8 @@ bool Simplifier::prop() {
     d2h(h, sc, 8, st_);
     stream_sync(st_);
     if (h[1]) {
+      proof_line(PR_EMPTY, 0);  // learnEmpty (elimination.cpp:80-86)
+      proof_flush();
       stage_end();
       return false;
     }
@@ -385,7 +368,9 @@ bool Simplifier::prop() {
     for (u32 u : fresh) {
       h_assign_[ABS(u)] = u + 1;
       units_out_.push_back(u);
+      proof_line(PR_UNIT, u);  // enqueueUnit -> learnUnit (solver.h:430-441, 475-479)
     }
+    proof_flush();
     unassigned_ -= (u32)nnew;
     if (!rounds_.empty()) rounds_.back().units += (u32)nnew;
   }
@@ -469,7 +454,15 @@ void Simplifier::sort_ot() {
   mem_.ensure(scratch_, std::max<size_t>(lits_end_, 6 * (size_t)E_ + 3) * 4, 0, st_);
   mem_.ensure(tmp2_, (6 * (size_t)E_ + 3) * 4, 0, st_);
   k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), st_);
-  k_work_order(db, P<u32>(pv_), E_, P<u32>(tmp2_), P<u32>(order_), tmp_.p, st_);
+  if (opt.work_order) k_work_order(db, P<u32>(pv_), E_, P<u32>(tmp2_), P<u32>(order_), tmp_.p, st_);
+  if (profile) {
+    uint64_t w[2];
+    k_elected_work(db, P<u32>(pv_), E_, (u64*)(P<u32>(scal_) + 40), st_);
+    d2h(w, P<u32>(scal_) + 40, sizeof(w), st_);
+    stream_sync(st_);
+    rounds_.back().elected_occurs = w[0];
+    rounds_.back().elected_bytes = w[1];
+  }
   stage_end();
 }
 
@@ -494,6 +487,7 @@ bool Simplifier::sub() {
                P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_), P<u32>(unit_by_)};
   k_sub_snapshot(db, P<u32>(pv_), E_, snap, st_);
   std::vector<u32> committed;
+  std::vector<ProofEv> pev;
   u32 from = 0, passes = 0;
   const u32 cap = V_ + 1;
   bool ok = true;
@@ -502,15 +496,18 @@ bool Simplifier::sub() {
     fill_u32(sc, 0, 1, st_);
     fill_u32(sc + 1, INF, 1, st_);
     fill_u32(sc + 2, 0, 1, st_);
-    SubOut out{P<u32>(recs_), sc, cap, sc + 1, sc + 2};
-    k_sub(db, opt, P<u32>(pv_), from, E_, out, P<u32>(order_), st_);
+    SubOut out{P<u32>(recs_), sc, cap, sc + 1, sc + 2, proof_out(2 * O + E_ + 64)};
+    k_sub(db, opt, P<u32>(pv_), from, E_, out, work_order(), st_);
     u32 h[3];
     d2h(h, sc, 12, st_);
     stream_sync(st_);
     if (h[2]) throw DeviceFail{"a variable derived more units in SUB than the per-variable buffer holds"};
     if (h[0] > cap) throw DeviceFail{"SUB unit record buffer overflow"};
     const u32 nrec = h[0], unsat_at = h[1];
-    if (!nrec && unsat_at == INF) break;
+    if (!nrec && unsat_at == INF) {
+      proof_collect(pev, INF);
+      break;
+    }
     std::vector<u32> rec(3 * (size_t)nrec);
     d2h(rec.data(), recs_.p, rec.size() * 4, st_);
     stream_sync(st_);
@@ -525,6 +522,7 @@ bool Simplifier::sub() {
       k_sub_dirty(db, snap, P<u32>(recs_), nrec, sc + 3, st_);
       first_dirty = read_u32(sc + 3);
     }
+    proof_collect(pev, first_dirty);
     // units of the exact prefix, in elected order (strengthen,
     // elimination.cpp:214-226)
     std::vector<u32> fresh;
@@ -533,9 +531,13 @@ bool Simplifier::sub() {
       if (k >= first_dirty) break;
       const u32 a = h_assign_[ABS(u)];
       if (a) {
-        if (a - 1 != u) ok = false;
+        if (a - 1 != u) {
+          ok = false;
+          pev.push_back({k, rec[3 * i + 1], PR_EMPTY, 0});
+        }
         continue;
       }
+      pev.push_back({k, rec[3 * i + 1], PR_UNIT, u});
       h_assign_[ABS(u)] = u + 1;
       fresh.push_back(u);
       units_out_.push_back(u);
@@ -566,6 +568,8 @@ bool Simplifier::sub() {
     h2d(ids_.p, committed.data(), committed.size() * 4, st_);
     k_zero_sizes(db, P<u32>(ids_), (u32)committed.size(), st_);
   }
+  proof_events(pev);
+  proof_flush();
   stream_sync(st_);
   rounds_.back().units += (u32)committed.size();
   rounds_.back().sub_passes = passes;
@@ -589,7 +593,7 @@ bool Simplifier::ve() {
   mem_.ensure(scratch_, std::max<size_t>(S, 1) * 4, 0, st_);
   VeDecision d{P<u32>(type_), P<u32>(def_), P<u32>(vncls_), P<u32>(vnlits_), P<u32>(vnstack_),
                P<u32>(scratch_), P<u32>(vbase_)};
-  k_ve_decide(db, opt, P<u32>(pv_), E_, d, P<u32>(order_), st_);
+  k_ve_decide(db, opt, P<u32>(pv_), E_, d, work_order(), st_);
   scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, sc, tmp_.p, st_);
   scan_u32(P<u32>(vnlits_), P<u32>(litb_), E_, sc + 1, tmp_.p, st_);
   scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc + 2, tmp_.p, st_);
@@ -613,8 +617,9 @@ bool Simplifier::ve() {
   ap.stack = P<u32>(stack_);
   ap.units = UnitOut{P<u32>(recs_), sc + 3, V_ + 1};
   ap.error = sc + 6;
+  ap.proof = proof_out(tot[0] + S + V_ + 64);
   fill_u32(sc + 6, 0, 1, st_);
-  k_ve_apply(db, P<u32>(pv_), E_, d, ap, P<u32>(order_), st_);
+  k_ve_apply(db, P<u32>(pv_), E_, d, ap, work_order(), st_);
   if (read_u32(sc + 6)) throw DeviceFail{"BVE resolvent writes disagree with the counted space"};
   ncls_ += tot[0];
   lits_end_ += tot[1];
@@ -623,6 +628,8 @@ bool Simplifier::ve() {
   if (nrec > V_ + 1) throw DeviceFail{"BVE unit record buffer overflow"};
   bool ok = true;
   std::vector<u32> fresh;
+  std::vector<ProofEv> pev;
+  proof_collect(pev, INF);
   if (nrec) {
     std::vector<u32> rec(3 * (size_t)nrec);
     d2h(rec.data(), recs_.p, rec.size() * 4, st_);
@@ -638,9 +645,14 @@ bool Simplifier::ve() {
       const u32 u = rec[3 * i + 2];
       const u32 a = h_assign_[ABS(u)];
       if (a) {
-        if (a - 1 != u) { ok = false; break; }
+        if (a - 1 != u) {
+          ok = false;
+          pev.push_back({rec[3 * i], rec[3 * i + 1], PR_EMPTY, 0});
+          break;
+        }
         continue;
       }
+      pev.push_back({rec[3 * i], rec[3 * i + 1], PR_UNIT, u});
       h_assign_[ABS(u)] = u + 1;
       fresh.push_back(u);
       units_out_.push_back(u);
@@ -648,6 +660,8 @@ bool Simplifier::ve() {
     }
   }
   if (ok) commit_units(fresh);
+  proof_events(pev);
+  proof_flush();
   // eliminated variables (markEliminated, solver.h:535-541)
   std::vector<u32> types(E_);
   d2h(types.data(), type_.p, (size_t)E_ * 4, st_);
@@ -668,13 +682,13 @@ void Simplifier::bce() {
   stage_begin(S_BCE);
   DB db = make_db(cur_);
   u32* sc = P<u32>(scal_);
-  k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, P<u32>(vnstack_), nullptr, nullptr, true, P<u32>(order_), st_);
+  k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, P<u32>(vnstack_), nullptr, nullptr, true, work_order(), st_);
   scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc, tmp_.p, st_);
   const u32 T = read_u32(sc);
   if (T) {
     ensure_stack(stack_n_ + T);
     k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, nullptr, P<u32>(stkb_), P<u32>(stack_) + stack_n_, false,
-          P<u32>(order_), st_);
+          work_order(), st_);
     stack_n_ += T;
   }
   stream_sync(st_);
@@ -755,6 +769,10 @@ bool Simplifier::ere_pairs() {
         dead[target] = true;
         del.push_back(target);
       }
+      if (ok) {
+        for (u32 c : del) proof_line(PR_DEL, c);  // removeClause (redundancy.h:137)
+        proof_flush();
+      }
       if (ok && !del.empty()) {
         mem_.ensure(ids_, del.size() * 4, 0, st_);
         h2d(ids_.p, del.data(), del.size() * 4, st_);
@@ -776,6 +794,7 @@ void Simplifier::ere_per_var() {
   const u32 cap = V_ + 1;
   mem_.ensure(unit_by_, std::max<size_t>(ncls_, 2 * (size_t)V_ + 3) * 4, 0, st_);
   u32 from = 0, passes = 0;
+  std::vector<ProofEv> pev;
   while (true) {
     if (++passes > E_ + 2) throw DeviceFail{"ERE re-runs did not converge"};
     fill_u32(sc, 0, 2, st_);
@@ -810,7 +829,10 @@ void Simplifier::ere_per_var() {
     first_dirty = std::min(first_dirty, read_u32(sc + 2));
     std::vector<u32> del;
     for (u32 i = 0; i < nrec; i++)
-      if (rec[3 * i] < first_dirty) del.push_back(rec[3 * i + 2]);
+      if (rec[3 * i] < first_dirty) {
+        del.push_back(rec[3 * i + 2]);
+        pev.push_back({rec[3 * i], rec[3 * i + 1], PR_DEL, rec[3 * i + 2]});
+      }
     if (!del.empty()) {
       mem_.ensure(ids_, del.size() * 4, 0, st_);
       h2d(ids_.p, del.data(), del.size() * 4, st_);
@@ -819,15 +841,107 @@ void Simplifier::ere_per_var() {
     if (first_dirty == INF) break;
     from = first_dirty;
   }
+  proof_events(pev);
+  proof_flush();
   stream_sync(st_);
   rounds_.back().ere_passes = passes;
   stage_end();
+}
+
+// ---------------------------------------------------------------- DRAT proof
+// The sequential simplifier writes its proof as it goes (src/cpu/proof.cpp):
+// units (learnUnit), strengthened clauses, resolvents and substituted
+// clauses (addResolvent: dropped when already satisfied), SUB and ERE
+// deletions (removeClause), the empty clause (learnEmpty).  Here every stage
+// queues its lines in that order and encodes them on the device before the
+// next stage can renumber or change clauses.
+void Simplifier::proof_line(u32 kind, u32 payload) {
+  if (!opt.proof_en) return;
+  if (kind == PR_UNIT) pst_.push_back({ABS(payload), (u32)(pl_.size() / 2 + 1)});
+  pl_.push_back(kind);
+  pl_.push_back(payload);
+}
+void Simplifier::proof_events(std::vector<ProofEv>& ev) {
+  if (!opt.proof_en) return;
+  std::stable_sort(ev.begin(), ev.end(), [](const ProofEv& a, const ProofEv& b) {
+    return a.k != b.k ? a.k < b.k : a.seq < b.seq;
+  });
+  for (const ProofEv& e : ev) {
+    proof_line(e.kind, e.payload);
+    if (e.kind == PR_EMPTY) break;  // the simplifier stops at a refutation
+  }
+  ev.clear();
+}
+ProofOut Simplifier::proof_out(u32 cap) {
+  if (!opt.proof_en) return ProofOut{nullptr, nullptr, 0, nullptr};
+  mem_.ensure(precs_, (size_t)cap * 16, 0, st_);
+  u32* sc = P<u32>(scal_) + 48;
+  fill_u32(sc, 0, 2, st_);
+  return ProofOut{P<u32>(precs_), sc, cap, sc + 1};
+}
+void Simplifier::proof_collect(std::vector<ProofEv>& ev, u32 below_k) {
+  if (!opt.proof_en) return;
+  u32 h[2];
+  d2h(h, P<u32>(scal_) + 48, 8, st_);
+  stream_sync(st_);
+  if (h[1]) throw DeviceFail{"proof record buffer overflow"};
+  if (!h[0]) return;
+  std::vector<u32> r((size_t)h[0] * 4);
+  d2h(r.data(), precs_.p, r.size() * 4, st_);
+  stream_sync(st_);
+  for (u32 i = 0; i < h[0]; i++)
+    if (r[4 * i] < below_k) ev.push_back({r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]});
+}
+void Simplifier::proof_flush() {
+  if (!opt.proof_en || pl_.empty()) {
+    pl_.clear();
+    pst_.clear();
+    return;
+  }
+  const u32 n = (u32)(pl_.size() / 2), np = (u32)pst_.size();
+  for (const auto& q : pst_) {
+    pl_.push_back(q.first);
+    pl_.push_back(q.second);
+  }
+  mem_.ensure(plines_, pl_.size() * 4, 0, st_);
+  h2d(plines_.p, pl_.data(), pl_.size() * 4, st_);
+  const u32* lines = P<u32>(plines_);
+  const u32* pairs = lines + 2 * (size_t)n;
+  if (pstamp_.cap < ((size_t)V_ + 1) * 4) {
+    mem_.ensure(pstamp_, ((size_t)V_ + 1) * 4, 0, st_);
+    fill_u32(P<u32>(pstamp_), 0, (size_t)V_ + 1, st_);
+  }
+  mem_.ensure(pbytes_, 2 * (size_t)n * 4, 0, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(n)), 0, st_);
+  u32* bytes = P<u32>(pbytes_);
+  u32* off = bytes + n;
+  u32* total = P<u32>(scal_) + 50;
+  DB db = make_db(cur_);
+  k_set_stamps(P<u32>(pstamp_), pairs, np, false, st_);
+  k_proof_sizes(db, lines, n, P<u32>(pstamp_), bytes, st_);
+  scan_u32(bytes, off, n, total, tmp_.p, st_);
+  const u32 add = read_u32(total);
+  if (proof_.cap < proof_n_ + add) mem_.ensure(proof_, std::max<size_t>((proof_n_ + add) * 3 / 2, 4096), proof_n_, st_);
+  k_proof_write(db, lines, n, bytes, off, P<u8>(proof_) + proof_n_, st_);
+  k_set_stamps(P<u32>(pstamp_), pairs, np, true, st_);
+  stream_sync(st_);
+  proof_n_ += add;
+  pl_.clear();
+  pst_.clear();
+}
+void Simplifier::download_proof(uint8_t* bytes) {
+  if (!proof_n_) return;
+  d2h(bytes, proof_.p, proof_n_, st_);
+  stream_sync(st_);
 }
 
 // ---------------------------------------------------------------- simplify (simplify.cpp:157-236)
 int Simplifier::simplify(uint32_t max_rounds) {
   rounds_.clear();
   for (int s = 0; s < S_NSTAGES; s++) stage_ms[s] = 0, stage_calls[s] = 0;
+  proof_n_ = 0;
+  pl_.clear();
+  pst_.clear();
   kernel_times.clear();
   prof_enable(st_, profile);
   const int r = simplify_rounds(max_rounds);
@@ -847,6 +961,8 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
   while (nC && nL) {
     if (max_rounds && r >= max_rounds) break;
     rounds_.emplace_back();
+    rounds_.back().clauses_before = nC;
+    rounds_.back().literals_before = nL;
     // compaction keeps clause order, so when it runs does not change the
     // result; collect once more than half of the slots are dead
     if (ncls_ > 2 * nC + 1024) gc();
@@ -880,28 +996,26 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
 
 // ---------------------------------------------------------------- results
 void Simplifier::download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd) {
+  // after the final compaction the pool holds every clause in id order:
+  // header words, then literals
   const size_t n = ncls_;
-  std::vector<u32> off(n), size(n), lb(n);
-  std::vector<u8> fl(n);
+  std::vector<u32> off(n), lb(n), pool(lits_end_);
   d2h(off.data(), c_off_[cur_].p, n * 4, st_);
-  d2h(size.data(), c_size_[cur_].p, n * 4, st_);
-  d2h(fl.data(), c_flags_[cur_].p, n, st_);
   d2h(lb.data(), c_lbd_[cur_].p, n * 4, st_);
-  if (lits) d2h(lits, lits_[cur_].p, lits_end_ * 4, st_);
+  d2h(pool.data(), lits_[cur_].p, lits_end_ * 4, st_);
   stream_sync(st_);
-  if (offsets) {
-    uint64_t a = 0;
-    for (size_t i = 0; i < n; i++) {
-      offsets[i] = a;
-      a += size[i];
-    }
-    offsets[n] = a;
+  uint64_t a = 0;
+  for (size_t i = 0; i < n; i++) {
+    const u32* h = pool.data() + off[i];
+    const u32 sz = hdr_size(h[0]);
+    const u8 fl = hdr_flags(h[0]);
+    if (offsets) offsets[i] = a;
+    if (lits) std::copy(h + CL_HDR, h + CL_HDR + sz, lits + a);
+    if (flags) flags[i] = (u8)((st_learnt(fl) ? 1 : 0) | ((fl & F_ADDED) ? 2 : 0) | (fl & USAGE_MASK));
+    if (lbd) lbd[i] = st_learnt(fl) ? lb[i] : 0;
+    a += sz;
   }
-  if (flags)
-    for (size_t i = 0; i < n; i++)
-      flags[i] = (u8)((st_learnt(fl[i]) ? 1 : 0) | ((fl[i] & F_ADDED) ? 2 : 0) | (fl[i] & USAGE_MASK));
-  if (lbd)
-    for (size_t i = 0; i < n; i++) lbd[i] = st_learnt(fl[i]) ? lb[i] : 0;
+  if (offsets) offsets[n] = a;
 }
 void Simplifier::download_vstate(uint8_t* vs) {
   d2h(vs, vstate_.p, (size_t)V_ + 1, st_);

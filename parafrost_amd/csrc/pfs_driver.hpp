@@ -46,6 +46,9 @@ struct RoundStats {
   uint32_t elected = 0, eliminated = 0, units = 0, resolvents = 0;
   uint64_t clauses_after = 0, literals_after = 0;
   uint32_t mis_iterations = 0, sub_passes = 0, ere_passes = 0;
+  // work measures (filled when profiling): live clauses/literals at round
+  // start, occurrences and clause bytes in the elected variables' lists
+  uint64_t clauses_before = 0, literals_before = 0, elected_occurs = 0, elected_bytes = 0;
 };
 
 // per-stage device time (ms) and launches, when profiling is enabled
@@ -68,16 +71,19 @@ class Simplifier {
 
   // results (valid after simplify)
   uint64_t out_ncls() const { return ncls_; }
-  uint64_t out_nlits() const { return lits_end_; }
+  // literals of the result (after the final compaction every clause is live)
+  uint64_t out_nlits() const { return lits_end_ - CL_HDR * ncls_; }
   void download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd);
   const std::vector<uint32_t>& new_units() const { return units_out_; }
   void download_vstate(uint8_t* vstate);
   uint64_t stack_words() const { return stack_n_; }
   void download_stack(uint32_t* words);
   const std::vector<RoundStats>& rounds() const { return rounds_; }
+  // binary DRAT proof of the last simplify call (when opt.proof_en)
+  uint64_t proof_bytes() const { return proof_n_; }
+  void download_proof(uint8_t* bytes);
   double stage_ms[S_NSTAGES] = {0};
   uint64_t stage_calls[S_NSTAGES] = {0};
-  uint64_t stage_bytes[S_NSTAGES] = {0};  // algorithmic bytes per stage
   std::vector<KernelTime> kernel_times;  // per kernel, when profiling is enabled
   size_t mem_used() const { return mem_.used(); }
   size_t mem_peak() const { return mem_.peak(); }
@@ -106,7 +112,18 @@ class Simplifier {
   void ensure_clause_room(uint64_t add_cls, uint64_t add_lits);
   void ensure_stack(uint64_t words);
   void commit_units(const std::vector<uint32_t>& lits);
+  // proof lines: queued per stage on the host as (kind, payload), encoded
+  // on the device by proof_flush() before clause ids can change
+  struct ProofEv {
+    u32 k, seq, kind, payload;
+  };
+  void proof_line(u32 kind, u32 payload);
+  void proof_events(std::vector<ProofEv>& ev);  // sorts by (k, seq), queues up to the first empty clause
+  void proof_flush();
+  ProofOut proof_out(u32 cap);                  // device record buffer for a per-variable kernel
+  void proof_collect(std::vector<ProofEv>& ev, u32 below_k);  // records of the last kernel with k < below_k
   uint32_t read_u32(const void* dptr);
+  const u32* work_order() { return opt.work_order ? P<u32>(order_) : nullptr; }
   void stage_begin(Stage s);
   void stage_end();
 
@@ -116,10 +133,10 @@ class Simplifier {
 
   // sizes
   uint32_t V_ = 0;
-  uint64_t ncls_ = 0, lits_end_ = 0, ccap_ = 0, lcap_ = 0;
+  uint64_t ncls_ = 0, lits_end_ = 0, ccap_ = 0, lcap_ = 0;  // lits_end_, lcap_: pool words
   int cur_ = 0;  // which clause block is live
   // clause blocks (dynamic region)
-  Buf c_off_[2], c_size_[2], c_sig_[2], c_flags_[2], c_lbd_[2], lits_[2];
+  Buf c_off_[2], c_lbd_[2], lits_[2];  // lits_: word pool of headers + literals
   // occurrence table
   Buf ot_off_, ot_size_, ot_refs_, hist_;
   // per variable
@@ -132,13 +149,18 @@ class Simplifier {
   Buf vsz_, vbase_, type_, def_, vncls_, vnlits_, vnstack_, clsb_, litb_, stkb_;
   Buf snap_ref_, snap_flg_, snap_size_, snap_sig_, snap_lbd_, snap_np_, snap_nn_, scratch_;
   Buf elits_, unit_by_, recs_, ids_, mark_, ht_, erecs_, order_;
+  // proof
+  Buf proof_, precs_, pstamp_, plines_, pbytes_;
+  uint64_t proof_n_ = 0;
+  std::vector<u32> pl_;                          // queued lines: kind, payload
+  std::vector<std::pair<u32, u32>> pst_;         // (variable, line number + 1) of queued units
   // stack
   Buf stack_;
   uint64_t stack_n_ = 0;
   // temporaries
   Buf tmp_, tmp2_, tmp3_, scal_;
   // pristine input copy
-  Buf in_c_off_, in_c_size_, in_c_sig_, in_c_flags_, in_c_lbd_, in_lits_, in_assign_, in_vstate_;
+  Buf in_c_off_, in_c_lbd_, in_lits_, in_assign_, in_vstate_;
   uint64_t in_ncls_ = 0, in_nlits_ = 0, in_trail_ = 0;
   uint32_t in_unassigned_ = 0;
   Buf in_trail_buf_;

@@ -64,19 +64,19 @@ struct CL {
   u32* l;
   u32 n;
 };
-PF_HD CL clause(const DB& db, u32 c) { return CL{db.lits + db.c_off[c], db.c_size[c]}; }
-PF_HD u8 flags(const DB& db, u32 c) { return db.c_flags[c]; }
-PF_HD bool is_original(const DB& db, u32 c) { return st_original(db.c_flags[c]); }
-PF_HD bool is_learnt(const DB& db, u32 c) { return st_learnt(db.c_flags[c]); }
-PF_HD bool is_deleted(const DB& db, u32 c) { return st_deleted(db.c_flags[c]); }
-PF_HD bool is_molten(const DB& db, u32 c) { return (db.c_flags[c] & F_MOLTEN) != 0; }
+PF_HD CL clause(const DB& db, u32 c) { return CL{clits(db, c), csize(db, c)}; }
+PF_HD u8 flags(const DB& db, u32 c) { return cflags(db, c); }
+PF_HD bool is_original(const DB& db, u32 c) { return st_original(cflags(db, c)); }
+PF_HD bool is_learnt(const DB& db, u32 c) { return st_learnt(cflags(db, c)); }
+PF_HD bool is_deleted(const DB& db, u32 c) { return st_deleted(cflags(db, c)); }
+PF_HD bool is_molten(const DB& db, u32 c) { return (cflags(db, c) & F_MOLTEN) != 0; }
 PF_HD void set_deleted(const DB& db, u32 c) {
-  const u8 f = db.c_flags[c];
-  if (!st_deleted(f)) db.c_flags[c] = (u8)((f & ~ST_MASK) | ST_DELETED);
+  const u8 f = cflags(db, c);
+  if (!st_deleted(f)) set_cflags(db, c, (u8)((f & ~ST_MASK) | ST_DELETED));
 }
-PF_HD void set_status(const DB& db, u32 c, u8 st) { db.c_flags[c] = (u8)((db.c_flags[c] & ~ST_MASK) | st); }
-PF_HD void melt(const DB& db, u32 c) { db.c_flags[c] = (u8)(db.c_flags[c] | F_MOLTEN); }
-PF_HD void freeze(const DB& db, u32 c) { db.c_flags[c] = (u8)(db.c_flags[c] & ~F_MOLTEN); }
+PF_HD void set_status(const DB& db, u32 c, u8 st) { set_cflags(db, c, (u8)((cflags(db, c) & ~ST_MASK) | st)); }
+PF_HD void melt(const DB& db, u32 c) { set_cflags(db, c, (u8)(cflags(db, c) | F_MOLTEN)); }
+PF_HD void freeze(const DB& db, u32 c) { set_cflags(db, c, (u8)(cflags(db, c) & ~F_MOLTEN)); }
 PF_HD u32 calc_sig(const u32* l, u32 n) {
   u32 s = 0;
   for (u32 i = 0; i < n; i++) s |= MAPHASH(l[i]);
@@ -120,13 +120,13 @@ struct SortKey {
   u32 size, lit0, back, sig, id;
 };
 PF_HD SortKey make_key(const DB& db, u32 c) {
-  const u32 n = db.c_size[c];
-  const u32* l = db.lits + db.c_off[c];
+  const u32 n = csize(db, c);
+  const u32* l = clits(db, c);
   SortKey k;
   k.size = n;
   k.lit0 = n ? l[0] : 0;
   k.back = n ? l[n - 1] : 0;
-  k.sig = db.c_sig[c];
+  k.sig = csig(db, c);
   k.id = c;
   return k;
 }
@@ -184,8 +184,8 @@ PF_HD void bcp_visit(const DB& db, u32 c, u32* next, u32* nnext, u32* conflict) 
 // (elimination.cpp:41-62 applied for every assigned literal).
 PF_HD void bcp_clean(const DB& db, u32 c) {
   if (is_deleted(db, c)) return;
-  u32* l = db.lits + db.c_off[c];
-  const u32 n = db.c_size[c];
+  u32* l = clits(db, c);
+  const u32 n = csize(db, c);
   u32 j = 0, sig = 0;
   for (u32 i = 0; i < n; i++) {
     const int v = lit_value(db, l[i]);
@@ -199,8 +199,8 @@ PF_HD void bcp_clean(const DB& db, u32 c) {
     }
   }
   if (j != n) {
-    db.c_size[c] = j;
-    db.c_sig[c] = sig;
+    set_csize(db, c, j);
+    set_csig(db, c, sig);
   }
 }
 
@@ -294,19 +294,108 @@ PF_HD bool is_tautology(u32 x, CL a, CL b) {
 
 // ---------------------------------------------------------------- SUB (subsume.h)
 constexpr int SUB_MAX_UNITS = 32;
+// DRAT proof events of the per-variable stages: records (elected index,
+// event sequence, kind, payload).  The sequence numbers of one variable are
+// shared with its unit records, so sorting everything by (index, sequence)
+// gives the order in which the sequential simplifier writes its proof.
+constexpr u32 PR_ADD = 1;     // 'a' + clause payload, unless satisfied when written (addResolvent)
+constexpr u32 PR_DEL = 2;     // 'd' + clause payload (removeClause)
+constexpr u32 PR_UNIT = 3;    // 'a' + literal payload (enqueueUnit)
+constexpr u32 PR_EMPTY = 4;   // 'a' + empty clause (learnEmpty)
+struct ProofOut {
+  u32* rec;       // 4 words per record; null when proof output is off
+  u32* n;
+  u32 cap;
+  u32* overflow;
+};
+PF_HD void push_proof(const ProofOut& po, u32 k, u32 seq, u32 kind, u32 payload) {
+  if (!po.rec) return;
+  const u32 r = at_add(po.n, 1u);
+  if (r >= po.cap) {
+    at_add(po.overflow, 1u);
+    return;
+  }
+  po.rec[4 * (u64)r] = k;
+  po.rec[4 * (u64)r + 1] = seq;
+  po.rec[4 * (u64)r + 2] = kind;
+  po.rec[4 * (u64)r + 3] = payload;
+}
+
+// encoding of one queued proof line (kind, payload) as the binary DRAT
+// writer does it (src/cpu/proof.cpp:94-135): literal 2*vorg(v)+sign in
+// 7-bit groups, low group first, continuation bit 7
+PF_HD u32 proof_lit_value(const DB& db, u32 l) { return (db.vorg[ABS(l)] << 1) | SIGN(l); }
+PF_HD u32 varint_bytes(u32 r) {
+  u32 b = 1;
+  while (r > 127) {
+    r >>= 7;
+    b++;
+  }
+  return b;
+}
+// PROOF::addResolvent (proof.cpp:214-229) drops a clause that holds a
+// literal already true: assigned before this batch (stamp 0) or by a unit
+// line earlier in it
+PF_HD bool proof_line_dropped(const DB& db, const u32* lines, u32 i, const u32* stamp) {
+  if (lines[2 * i] != PR_ADD) return false;
+  const CL c = clause(db, lines[2 * i + 1]);
+  for (u32 j = 0; j < c.n; j++) {
+    const u32 l = c.l[j];
+    if (lit_value(db, l) > 0) {
+      const u32 s = stamp[ABS(l)];
+      if (s == 0 || s < i + 1) return true;
+    }
+  }
+  return false;
+}
+PF_HD u32 proof_line_size(const DB& db, const u32* lines, u32 i, const u32* stamp) {
+  const u32 kind = lines[2 * i], pay = lines[2 * i + 1];
+  if (kind == PR_EMPTY) return 2;
+  if (kind == PR_UNIT) return 2 + varint_bytes(proof_lit_value(db, pay));
+  if (proof_line_dropped(db, lines, i, stamp)) return 0;
+  const CL c = clause(db, pay);
+  u32 b = 2;
+  for (u32 j = 0; j < c.n; j++) b += varint_bytes(proof_lit_value(db, c.l[j]));
+  return b;
+}
+PF_HD u8* put_varint(u8* o, u32 r) {
+  while (r > 127) {
+    *o++ = (u8)((r & 127) | 128);
+    r >>= 7;
+  }
+  *o++ = (u8)r;
+  return o;
+}
+PF_HD void proof_line_write(const DB& db, const u32* lines, u32 i, u8* o) {
+  const u32 kind = lines[2 * i], pay = lines[2 * i + 1];
+  *o++ = kind == PR_DEL ? 'd' : 'a';
+  if (kind == PR_UNIT) {
+    o = put_varint(o, proof_lit_value(db, pay));
+  } else if (kind != PR_EMPTY) {
+    const CL c = clause(db, pay);
+    for (u32 j = 0; j < c.n; j++) o = put_varint(o, proof_lit_value(db, c.l[j]));
+  }
+  *o = 0;
+}
+
 struct SubOut {
   u32* rec;       // (elected index, sequence, unit literal) records
   u32* n;         // number of records
   u32 cap;
   u32* unsat_at;  // min elected index whose strengthening refuted the formula
   u32* overflow;  // set when a variable derives more than SUB_MAX_UNITS units
+  ProofOut proof;
 };
 
 struct SubLocal {
   u32 units[SUB_MAX_UNITS];
+  u32 useq[SUB_MAX_UNITS];  // event sequence of each unit
   u32 n;
+  u32 ev;                   // event counter of this variable
   bool overflow;
   bool unsat;
+  u32 k;                    // elected index
+  ProofOut proof;
 };
 // value of a literal seen by one variable: committed root assignment plus
 // the units this variable derived so far
@@ -338,35 +427,41 @@ PF_HD void sub_local_oblivion(const DB& db, u32 p, u32 unit) {
 PF_HD void bump_shrunken(const DB& db, u32 c, int lbd_tier1) {  // elimination.cpp:28-39
   const int old = (int)db.c_lbd[c];
   if (old <= lbd_tier1) return;
-  int nl = (int)db.c_size[c] - 1;
+  int nl = (int)csize(db, c) - 1;
   if (nl > old) nl = old;
   if (nl >= old) return;
   db.c_lbd[c] = (u32)nl;
-  db.c_flags[c] = (u8)((db.c_flags[c] & ~USAGE_MASK) | (1u << USAGE_SHIFT));
+  set_cflags(db, c, (u8)((cflags(db, c) & ~USAGE_MASK) | (1u << USAGE_SHIFT)));
 }
 // elimination.cpp:199-234
 PF_HD void strengthen(const DB& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_tier1) {
-  u32* l = db.lits + db.c_off[c];
-  const u32 n0 = db.c_size[c];
+  u32* l = clits(db, c);
+  const u32 n0 = csize(db, c);
   u32 n = 0, sig = 0;
   for (u32 k = 0; k < n0; k++) {
     const u32 x = l[k];
     if (x != me) { l[n++] = x; sig |= MAPHASH(x); }
   }
-  db.c_sig[c] = sig;
-  db.c_size[c] = n0 - 1;
+  set_csig(db, c, sig);
+  set_csize(db, c, n0 - 1);
   if (n == 1) {
     const u32 unit = l[0];
     const int val = sub_value(db, sl, unit);
     if (val < 0) {
-      if (sl.n < (u32)SUB_MAX_UNITS) sl.units[sl.n++] = unit;
-      else sl.overflow = true;
+      if (sl.n < (u32)SUB_MAX_UNITS) {
+        sl.useq[sl.n] = sl.ev++;
+        sl.units[sl.n++] = unit;
+      } else {
+        sl.overflow = true;
+      }
       sub_local_oblivion(db, p, unit);
     } else if (!val) {
+      if (!sl.unsat) push_proof(sl.proof, sl.k, sl.ev++, PR_EMPTY, 0);
       sl.unsat = true;
     }
-  } else if (is_learnt(db, c)) {
-    bump_shrunken(db, c, lbd_tier1);
+  } else {
+    push_proof(sl.proof, sl.k, sl.ev++, PR_ADD, c);
+    if (is_learnt(db, c)) bump_shrunken(db, c, lbd_tier1);
   }
 }
 // subsume.h:39-56
@@ -402,16 +497,17 @@ PF_HD bool selfsub_lits(u32 x, u32 fx, CL sm, CL lr) {
   return false;
 }
 // subsume.h:105-121: is 'cand' subsumed by an earlier entry of the list
-PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand) {
-  const u32 csz = db.c_size[cand];
+PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand, SubLocal& sl) {
+  const u32 csz = csize(db, cand);
   const bool cm = is_molten(db, cand);
   for (u32 j = 0; j < end; j++) {
     const u32 s = ol.r[j];
     if (is_deleted(db, s)) continue;
-    const u32 ssz = db.c_size[s];
+    const u32 ssz = csize(db, s);
     if (cm && ssz > csz) continue;
-    if (ssz > 1 && sub_sig(db.c_sig[s], db.c_sig[cand]) && sub_lits(clause(db, s), clause(db, cand))) {
+    if (ssz > 1 && sub_sig(csig(db, s), csig(db, cand)) && sub_lits(clause(db, s), clause(db, cand))) {
       if (is_learnt(db, s) && is_original(db, cand)) set_status(db, s, ST_ORIGINAL);
+      push_proof(sl.proof, sl.k, sl.ev++, PR_DEL, cand);
       set_deleted(db, cand);
       return true;
     }
@@ -422,14 +518,14 @@ PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand) {
 // self-subsumes it
 PF_HD bool selfsubsume_cand(const DB& db, u32 x, u32 fx, OLV ol, u32 cand, u32 p, SubLocal& sl,
                             int lbd_tier1) {
-  const u32 csz = db.c_size[cand];
-  const u32 csig = db.c_sig[cand];
+  const u32 csz = csize(db, cand);
+  const u32 cand_sig = csig(db, cand);
   for (u32 j = 0; j < ol.n; j++) {
     const u32 s = ol.r[j];
-    const u32 ssz = db.c_size[s];
+    const u32 ssz = csize(db, s);
     if (ssz > csz) break;
     if (is_deleted(db, s)) continue;
-    if (ssz > 1 && selfsub_sig(db.c_sig[s], csig) && selfsub_lits(x, fx, clause(db, s), clause(db, cand))) {
+    if (ssz > 1 && selfsub_sig(csig(db, s), cand_sig) && selfsub_lits(x, fx, clause(db, s), clause(db, cand))) {
       strengthen(db, cand, x, p, sl, lbd_tier1);
       melt(db, cand);
       return true;
@@ -456,16 +552,19 @@ PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& o
   if ((int)db.ot_size[p] > o.sub_limit || (int)db.ot_size[n] > o.sub_limit) return;
   SubLocal sl;
   sl.n = 0;
+  sl.ev = 0;
   sl.overflow = false;
   sl.unsat = false;
+  sl.k = k;
+  sl.proof = out.proof;
   {
     const OLV poss = olist(db, p);
     for (u32 i = 0; i < poss.n; i++) {
       const u32 c = poss.r[i];
-      if ((int)db.c_size[c] > HSE_MAX_CL_SIZE) break;
+      if ((int)csize(db, c) > HSE_MAX_CL_SIZE) break;
       if (is_deleted(db, c)) continue;
       selfsubsume_cand(db, p, n, olist(db, n), c, p, sl, o.lbd_tier1);
-      subsume_cand(db, poss, i, c);
+      subsume_cand(db, poss, i, c, sl);
     }
     update_ol(db, p);
   }
@@ -473,10 +572,10 @@ PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& o
     const OLV negs = olist(db, n);
     for (u32 i = 0; i < negs.n; i++) {
       const u32 c = negs.r[i];
-      if ((int)db.c_size[c] > HSE_MAX_CL_SIZE) break;
+      if ((int)csize(db, c) > HSE_MAX_CL_SIZE) break;
       if (is_deleted(db, c)) continue;
       selfsubsume_cand(db, n, p, olist(db, p), c, p, sl, o.lbd_tier1);
-      subsume_cand(db, negs, i, c);
+      subsume_cand(db, negs, i, c, sl);
     }
     update_ol(db, n);
   }
@@ -484,7 +583,7 @@ PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& o
     const u32 r = at_add(out.n, 1u);
     if (r < out.cap) {
       out.rec[3 * (u64)r] = k;
-      out.rec[3 * (u64)r + 1] = i;
+      out.rec[3 * (u64)r + 1] = sl.useq[i];
       out.rec[3 * (u64)r + 2] = sl.units[i];
     }
   }
@@ -515,9 +614,9 @@ PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
       const u32 c = ol.r[j];
       if (!is_deleted(db, c)) s.owner[c] = k;  // live clauses have one elected owner
       s.ref[e] = c;
-      s.flg[e] = db.c_flags[c];
-      s.size[e] = db.c_size[c];
-      s.sig[e] = db.c_sig[c];
+      s.flg[e] = cflags(db, c);
+      s.size[e] = csize(db, c);
+      s.sig[e] = csig(db, c);
       s.lbd[e] = db.c_lbd[c];
     }
   }
@@ -531,20 +630,20 @@ PF_HD void sub_restore_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
     for (u32 j = 0; j < n; j++, e++) {
       const u32 c = s.ref[e];
       r[j] = c;
-      db.c_flags[c] = s.flg[e];
-      db.c_sig[c] = s.sig[e];
+      set_cflags(db, c, s.flg[e]);
+      set_csig(db, c, s.sig[e]);
       db.c_lbd[c] = s.lbd[e];
-      const u32 cur = db.c_size[c];
+      const u32 cur = csize(db, c);
       if (cur < s.size[e]) {
         // the only literal SUB removes from a clause of this list is 'lit'
-        u32* l = db.lits + db.c_off[c];
+        u32* l = clits(db, c);
         u32 pos = cur;
         while (pos > 0 && l[pos - 1] > lit) {
           l[pos] = l[pos - 1];
           pos--;
         }
         l[pos] = lit;
-        db.c_size[c] = s.size[e];
+        set_csize(db, c, s.size[e]);
       }
     }
     db.ot_size[lit] = n;
@@ -565,7 +664,7 @@ struct VeDecision {
   u32* type;      // per elected variable
   u32* def;       // EQU definition literal
   u32* ncls;      // resolvents of size > 1
-  u32* nlits;     // their literals
+  u32* nlits;     // their pool words (headers + literals)
   u32* nstack;    // eliminated-stack words
   u32* scratch;   // per variable scratch (fan-in literals)
   u32* scr_base;  // per variable start in scratch
@@ -579,7 +678,7 @@ PF_HD u32 count_orgs(const DB& db, OLV ol) {
 PF_HD u32 count_lits_before(const DB& db, OLV ol) {
   u32 n = 0;
   for (u32 i = 0; i < ol.n; i++)
-    if (is_original(db, ol.r[i])) n += db.c_size[ol.r[i]];
+    if (is_original(db, ol.r[i])) n += csize(db, ol.r[i]);
   return n;
 }
 // simplify.h:237-295: true when the bound is exceeded; nAddedCls / lits
@@ -612,7 +711,7 @@ PF_HD bool count_res(const DB& db, const Options& o, u32 x, int clsbefore, OLV m
 PF_HD void freeze_binaries(const DB& db, OLV ol) {  // simplify.h:204-211
   for (u32 i = 0; i < ol.n; i++) {
     const u32 c = ol.r[i];
-    if (is_original(db, c) && db.c_size[c] == 2) freeze(db, c);
+    if (is_original(db, c) && csize(db, c) == 2) freeze(db, c);
   }
 }
 // equivalence.h:89-131
@@ -621,8 +720,8 @@ PF_HD u32 find_sfanin(const DB& db, u32 gate_out, OLV ol) {
   int nImps = 0;
   for (u32 i = 0; i < ol.n; i++) {
     const u32 c = ol.r[i];
-    if (is_original(db, c) && db.c_size[c] == 2) {
-      const u32* l = db.lits + db.c_off[c];
+    if (is_original(db, c) && csize(db, c) == 2) {
+      const u32* l = clits(db, c);
       imp = FLIP(l[0] ^ l[1] ^ gate_out);
       melt(db, c);
       nImps++;
@@ -640,8 +739,8 @@ PF_HD u32 find_equ_gate(const DB& db, u32 p) {
     if (second < first) first = second, second = def;
     for (u32 i = 0; i < negs.n; i++) {
       const u32 c = negs.r[i];
-      if (is_original(db, c) && db.c_size[c] == 2) {
-        const u32* l = db.lits + db.c_off[c];
+      if (is_original(db, c) && csize(db, c) == 2) {
+        const u32* l = clits(db, c);
         if (l[0] == first && l[1] == second) {
           melt(db, c);
           return def;
@@ -660,8 +759,8 @@ PF_HD bool find_ao_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, u32
   u32 nout = 0, sig = 0;
   for (u32 i = 0; i < it.n; i++) {
     const u32 c = it.r[i];
-    if (is_original(db, c) && db.c_size[c] == 2) {
-      const u32* l = db.lits + db.c_off[c];
+    if (is_original(db, c) && csize(db, c) == 2) {
+      const u32* l = clits(db, c);
       const u32 imp = FLIP(l[0] ^ l[1] ^ dx);
       out[nout++] = imp;
       sig |= MAPHASH(imp);
@@ -681,8 +780,8 @@ PF_HD bool find_ao_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, u32
     const OLV ot2 = olist(db, f);
     for (u32 i = 0; i < ot2.n; i++) {
       const u32 c = ot2.r[i];
-      if (!is_original(db, c) || db.c_size[c] != nout || !sub_sig(db.c_sig[c], sig)) continue;
-      const u32* l = db.lits + db.c_off[c];
+      if (!is_original(db, c) || csize(db, c) != nout || !sub_sig(csig(db, c), sig)) continue;
+      const u32* l = clits(db, c);
       bool eq = true;
       for (u32 q = 0; q < nout && eq; q++) eq = (l[q] == out[q]);
       if (!eq) continue;
@@ -713,8 +812,8 @@ PF_HD u32 fast_equality_check(const DB& db, u32 x, u32 y, u32 z) {
   for (u32 i = 0; i < ol.n; i++) {
     const u32 cl = ol.r[i];
     if (is_molten(db, cl)) continue;
-    if (!is_original(db, cl) || db.c_size[cl] != 3) continue;
-    const u32* l = db.lits + db.c_off[cl];
+    if (!is_original(db, cl) || csize(db, cl) != 3) continue;
+    const u32* l = clits(db, cl);
     if (l[0] == a && l[1] == b && l[2] == c && cl < best) best = cl;
   }
   return best;
@@ -725,15 +824,15 @@ PF_HD bool find_ite_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
   const OLV it = olist(db, dx);
   for (u32 i = 0; i < it.n; i++) {
     const u32 ci = it.r[i];
-    if (!(is_original(db, ci) && db.c_size[ci] == 3)) continue;
-    const u32* li = db.lits + db.c_off[ci];
+    if (!(is_original(db, ci) && csize(db, ci) == 3)) continue;
+    const u32* li = clits(db, ci);
     u32 xi = li[0], yi = li[1], zi = li[2], t;
     if (yi == dx) { t = xi; xi = yi; yi = t; }
     if (zi == dx) { t = xi; xi = zi; zi = t; }
     for (u32 j = i + 1; j < it.n; j++) {
       const u32 cj = it.r[j];
-      if (!(is_original(db, cj) && db.c_size[cj] == 3)) continue;
-      const u32* lj = db.lits + db.c_off[cj];
+      if (!(is_original(db, cj) && csize(db, cj) == 3)) continue;
+      const u32* lj = clits(db, cj);
       u32 xj = lj[0], yj = lj[1], zj = lj[2];
       if (yj == dx) { t = xj; xj = yj; yj = t; }
       if (zj == dx) { t = xj; xj = zj; zj = t; }
@@ -759,11 +858,11 @@ PF_HD bool find_ite_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
 PF_HD void freeze_arities(const DB& db, OLV me, OLV other) {
   for (u32 i = 0; i < me.n; i++) {
     const u32 c = me.r[i];
-    if (db.c_size[c] > 2 && is_molten(db, c)) freeze(db, c);
+    if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
   }
   for (u32 i = 0; i < other.n; i++) {
     const u32 c = other.r[i];
-    if (db.c_size[c] > 2 && is_molten(db, c)) freeze(db, c);
+    if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
   }
 }
 // xor.h:70-102: flip the literals for the next parity pattern and look for
@@ -785,8 +884,8 @@ PF_HD bool make_arity(const DB& db, u32& parity, u32* ls, u32 size) {
   u32 found = NOPOS;
   for (u32 i = 0; i < ol.n; i++) {
     const u32 c = ol.r[i];
-    if (!is_original(db, c) || db.c_size[c] != size || c >= found) continue;
-    const u32* l = db.lits + db.c_off[c];
+    if (!is_original(db, c) || csize(db, c) != size || c >= found) continue;
+    const u32* l = clits(db, c);
     bool all = true;
     for (u32 q = 0; q < size && all; q++) {
       bool in = false;
@@ -807,10 +906,10 @@ PF_HD bool find_xor_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
   for (u32 i = 0; i < it.n; i++) {
     const u32 ci = it.r[i];
     if (!is_original(db, ci)) continue;
-    const u32 size = db.c_size[ci];
+    const u32 size = csize(db, ci);
     const int arity = (int)size - 1;
     if (size < 3 || arity > o.xor_max_arity || size > 32) continue;
-    const u32* l = db.lits + db.c_off[ci];
+    const u32* l = clits(db, ci);
     for (u32 q = 0; q < size; q++) ls[q] = l[q];
     u32 parity = 0;
     int itargets = 1 << arity;
@@ -835,7 +934,7 @@ PF_HD u32 stack_words(const DB& db, OLV side) {
   u32 w = 2;
   for (u32 i = 0; i < side.n; i++) {
     const u32 c = side.r[i];
-    if (is_original(db, c)) w += db.c_size[c] + 1;
+    if (is_original(db, c)) w += csize(db, c) + 1;
   }
   return w;
 }
@@ -872,7 +971,7 @@ PF_HD void ve_decide(const DB& db, const Options& o, u32 v, u32 k, const VeDecis
   d.def[k] = def;
   const bool adds = (type == VT_RES || type == VT_SUBST) && nAddedCls;
   d.ncls[k] = adds ? (u32)nAddedCls : 0;
-  d.nlits[k] = adds ? nAddedLits : 0;
+  d.nlits[k] = adds ? nAddedLits + CL_HDR * (u32)nAddedCls : 0;  // pool words
   d.nstack[k] = type ? stack_words(db, side) : 0;
 }
 
@@ -932,15 +1031,15 @@ PF_HD void delete_lists(const DB& db, u32 p) {
 // equivalence.h:25-43: substitute def for dx in clause c (kept sorted);
 // returns the unit literal when one literal remains, else 0
 PF_HD u32 substitute_clause(const DB& db, u32 dx, u32 c, u32 def) {
-  u32* l = db.lits + db.c_off[c];
-  const u32 n0 = db.c_size[c];
+  u32* l = clits(db, c);
+  const u32 n0 = csize(db, c);
   u32 n = 0;
   for (u32 i = 0; i < n0; i++) {
     const u32 x = l[i];
     if (x == dx) l[n++] = def;
     else if (x != def) l[n++] = x;
   }
-  db.c_size[c] = n;
+  set_csize(db, c, n);
   if (n == 1) return l[0];
   for (u32 i = 1; i < n; i++) {
     const u32 t = l[i];
@@ -948,7 +1047,7 @@ PF_HD u32 substitute_clause(const DB& db, u32 dx, u32 c, u32 def) {
     while (j > 0 && l[j - 1] > t) { l[j] = l[j - 1]; j--; }
     l[j] = t;
   }
-  db.c_sig[c] = calc_sig(l, n);
+  set_csig(db, c, calc_sig(l, n));
   return 0;
 }
 struct VeApply {
@@ -960,6 +1059,7 @@ struct VeApply {
   u32* stack;
   UnitOut units;
   u32* error;  // set when a write would leave its buffer
+  ProofOut proof;
 };
 // bve.cpp:138-241, execution part
 PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApply& a) {
@@ -989,6 +1089,7 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
         else if (is_original(db, c)) {
           const u32 unit = substitute_clause(db, dx, c, sub);
           if (unit) push_unit(a.units, k, seq++, unit);
+          else push_proof(a.proof, k, seq++, PR_ADD, c);
         }
       }
     }
@@ -1025,19 +1126,19 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
             push_unit(a.units, k, seq++, one[0]);
             continue;
           }
-          if (cid >= a.cls_cap || lpos + len > a.lit_cap) {
+          if (cid >= a.cls_cap || lpos + CL_HDR + len > a.lit_cap) {
             at_add(a.error, 1u);
             return;
           }
-          u32* out = db.lits + lpos;
-          merge_out(v, A, B, out);
+          u32* h = db.lits + lpos;
+          merge_out(v, A, B, h + CL_HDR);
+          h[0] = make_hdr(len, F_ADDED);
+          h[1] = calc_sig(h + CL_HDR, len);
           db.c_off[cid] = lpos;
-          db.c_size[cid] = len;
-          db.c_sig[cid] = calc_sig(out, len);
-          db.c_flags[cid] = F_ADDED;
           db.c_lbd[cid] = 0;
+          push_proof(a.proof, k, seq++, PR_ADD, cid);
           cid++;
-          lpos += len;
+          lpos += CL_HDR + len;
         }
       }
     }
@@ -1136,13 +1237,13 @@ PF_HD void forward_equ(const DB& db, u32 x, u32 c1, u32 c2, int maxsize, u32* dl
   for (u32 i = 0; i < ml.n; i++) {
     const u32 c = ml.r[i];
     if (c >= found) continue;
-    const u8 f = db.c_flags[c];
-    if (db.c_size[c] != len) continue;
+    const u8 f = cflags(db, c);
+    if (csize(db, c) != len) continue;
     if (!(st_learnt(f) || (f & ST_MASK) == type)) continue;
     bool mine = false;
     for (u32 q = 0; q < nd && q < (u32)ERE_MAX_DEL; q++) mine |= (dl[q] == c);
     if (mine) continue;
-    if (!sub_sig(msig, db.c_sig[c])) continue;
+    if (!sub_sig(msig, csig(db, c))) continue;
     if (!merge_ere(x, A, B, clause(db, c))) continue;
     found = c;
   }
@@ -1294,8 +1395,8 @@ PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
     if (e == HT_EMPTY) break;
     if ((u32)(e >> 32) != h) continue;
     const u32 c = (u32)e;
-    const u8 f = db.c_flags[c];
-    if (st_deleted(f) || db.c_size[c] != len) continue;
+    const u8 f = cflags(db, c);
+    if (st_deleted(f) || csize(db, c) != len) continue;
     if (!(st_learnt(f) || (f & ST_MASK) == type)) continue;
     if (!merge_ere(x, A, B, clause(db, c))) continue;
     // keep the ERE_K lowest ids, ascending

@@ -896,15 +896,16 @@ void k_reduce_lists(const DB& db, Stream* st) {
 __global__ void prep_k(DB db) {
   const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= db.ncls) return;
-  u32* l = db.lits + db.c_off[c];
-  const u32 n = db.c_size[c];
+  u32* h = chead(db, c);
+  u32* l = h + CL_HDR;
+  const u32 n = hdr_size(h[0]);
   for (u32 i = 1; i < n; i++) {
     const u32 t = l[i];
     u32 j = i;
     while (j > 0 && l[j - 1] > t) { l[j] = l[j - 1]; j--; }
     l[j] = t;
   }
-  db.c_sig[c] = calc_sig(l, n);
+  h[1] = calc_sig(l, n);
 }
 void k_prep(const DB& db, Stream* st) {
   if (!db.ncls) return;
@@ -919,13 +920,41 @@ void k_zero_sizes(const DB& db, const u32* lits, u32 n, Stream* st) {
   KLAUNCH(zero_sizes_k, grid_for(n, 256), 256, st, db, lits, n);
 }
 
+// ---------------------------------------------------------------- DRAT proof
+__global__ void proof_sizes_k(DB db, const u32* lines, u32 n, const u32* stamp, u32* bytes) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bytes[i] = proof_line_size(db, lines, i, stamp);
+}
+__global__ void proof_write_k(DB db, const u32* lines, u32 n, const u32* bytes, const u32* off, u8* out) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && bytes[i]) proof_line_write(db, lines, i, out + off[i]);  // 0 bytes: dropped line
+}
+void k_proof_sizes(const DB& db, const u32* lines, u32 n, const u32* stamp, u32* bytes, Stream* st) {
+  if (!n) return;
+  KLAUNCH(proof_sizes_k, grid_for(n, 256), 256, st, db, lines, n, stamp, bytes);
+}
+void k_proof_write(const DB& db, const u32* lines, u32 n, const u32* bytes, const u32* off, u8* out,
+                   Stream* st) {
+  if (!n) return;
+  KLAUNCH(proof_write_k, grid_for(n, 256), 256, st, db, lines, n, bytes, off, out);
+}
+__global__ void set_stamps_k(u32* stamp, const u32* pairs, u32 n, int clear) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) stamp[pairs[2 * i]] = clear ? 0u : pairs[2 * i + 1];
+}
+void k_set_stamps(u32* stamp, const u32* pairs, u32 n, bool clear, Stream* st) {
+  if (!n) return;
+  KLAUNCH(set_stamps_k, grid_for(n, 256), 256, st, stamp, pairs, n, clear ? 1 : 0);
+}
+
 // ---------------------------------------------------------------- count / compact
 __global__ void count_k(DB db, unsigned long long* out) {
   unsigned long long live = 0, lits = 0;
   for (u32 c = blockIdx.x * blockDim.x + threadIdx.x; c < db.ncls; c += gridDim.x * blockDim.x) {
-    if (!is_deleted(db, c)) {
+    const u32 h = chead(db, c)[0];
+    if (!st_deleted(hdr_flags(h))) {
       live++;
-      lits += db.c_size[c];
+      lits += hdr_size(h);
     }
   }
   live = wave_sum64(live);
@@ -935,6 +964,34 @@ __global__ void count_k(DB db, unsigned long long* out) {
     atomicAdd(&out[1], lits);
   }
 }
+// occurrences (O_E) and clause bytes (D_E: 12-byte header + 4 bytes per
+// literal) of the live clauses in the elected variables' lists; elected
+// variables share no clause, so every clause is counted once
+__global__ void elected_work_k(DB db, const u32* pv, u32 n, unsigned long long* out) {
+  unsigned long long occ = 0, bytes = 0;
+  for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (u32 s = 0; s < 2; s++) {
+      const OLV ol = olist(db, V2L(pv[k]) | s);
+      for (u32 i = 0; i < ol.n; i++) {
+        const u32 h = chead(db, ol.r[i])[0];
+        if (st_deleted(hdr_flags(h))) continue;
+        occ++;
+        bytes += 12 + 4ull * hdr_size(h);
+      }
+    }
+  }
+  occ = wave_sum64(occ);
+  bytes = wave_sum64(bytes);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], occ);
+    atomicAdd(&out[1], bytes);
+  }
+}
+void k_elected_work(const DB& db, const u32* pv, u32 n, u64* out2, Stream* st) {
+  fill_u32((u32*)out2, 0, 4, st);
+  if (!n) return;
+  KLAUNCH(elected_work_k, std::min(grid_for(n, ST), RED_BLOCKS), ST, st, db, pv, n, (unsigned long long*)out2);
+}
 void k_count(const DB& db, u64* out2, Stream* st) {
   fill_u32((u32*)out2, 0, 4, st);
   if (!db.ncls) return;
@@ -943,22 +1000,23 @@ void k_count(const DB& db, u64* out2, Stream* st) {
 __global__ void live_k(DB db, u32* tc, u32* tl) {
   const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= db.ncls) return;
-  const bool live = !is_deleted(db, c);
+  const u32 h = chead(db, c)[0];
+  const bool live = !st_deleted(hdr_flags(h));
   tc[c] = live ? 1u : 0u;
-  tl[c] = live ? db.c_size[c] : 0u;
+  tl[c] = live ? CL_HDR + hdr_size(h) : 0u;  // pool words
 }
 __global__ void compact_k(DB src, DB dst, const u32* tc, const u32* tl) {
   const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= src.ncls || is_deleted(src, c)) return;
-  const u32 nid = tc[c], noff = tl[c], n = src.c_size[c];
+  if (c >= src.ncls) return;
+  const u32* a = chead(src, c);
+  const u32 h = a[0];
+  if (st_deleted(hdr_flags(h))) return;
+  const u32 nid = tc[c], noff = tl[c], n = hdr_size(h);
   dst.c_off[nid] = noff;
-  dst.c_size[nid] = n;
-  dst.c_sig[nid] = src.c_sig[c];
-  dst.c_flags[nid] = (u8)(src.c_flags[c] & ~F_MOLTEN);
   dst.c_lbd[nid] = src.c_lbd[c];
-  const u32* a = src.lits + src.c_off[c];
   u32* b = dst.lits + noff;
-  for (u32 i = 0; i < n; i++) b[i] = a[i];
+  b[0] = make_hdr(n, (u8)(hdr_flags(h) & ~F_MOLTEN));
+  for (u32 i = 1; i < CL_HDR + n; i++) b[i] = a[i];
 }
 void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals, Stream* st) {
   if (!src.ncls) {

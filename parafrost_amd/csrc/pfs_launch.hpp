@@ -121,8 +121,20 @@ void k_reduce_lists(const DB& db, Stream*);  // drop deleted entries from every 
 void k_prep(const DB& db, Stream*);          // sort literals, signature (extraction)
 void k_zero_sizes(const DB& db, const u32* lits, u32 n, Stream*);
 
+// ---- DRAT proof (binary format of src/cpu/proof.cpp:94-175)
+// lines: (kind, payload) pairs, kinds PR_*; stamp: per variable, the line
+// number + 1 of the unit line that assigned it in this batch (0 = assigned
+// before).  A PR_ADD line whose clause holds a literal made true before it
+// is dropped (PROOF::addResolvent).  bytes[i] = encoded size of line i.
+void k_proof_sizes(const DB& db, const u32* lines, u32 n, const u32* stamp, u32* bytes, Stream*);
+void k_proof_write(const DB& db, const u32* lines, u32 n, const u32* bytes, const u32* off, u8* out, Stream*);
+// stamp[pairs[2i]] = pairs[2i+1], or 0 when clear
+void k_set_stamps(u32* stamp, const u32* pairs, u32 n, bool clear, Stream*);
+
 // ---- counting and garbage collection
 void k_count(const DB& db, u64* out2, Stream*);  // live clauses, live literals
+// live occurrences and clause bytes in the elected variables' lists
+void k_elected_work(const DB& db, const u32* pv, u32 n, u64* out2, Stream*);
 // stable compaction of the live clauses into dst (ids and pool renumbered)
 void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
                Stream*);

@@ -23,7 +23,7 @@ RESULT_UNSAT, RESULT_SAT, RESULT_UNSOLVED = 0, 1, 2
 # exported symbols declared in include/pfsimp.h
 SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rewind", "pfs_simplify",
            "pfs_result_sizes", "pfs_download_cnf", "pfs_download_units", "pfs_download_vstate",
-           "pfs_download_stack", "pfs_round_stats_get", "pfs_stage_times", "pfs_stage_name", "pfs_stage_count",
+           "pfs_download_stack", "pfs_proof_size", "pfs_download_proof", "pfs_round_stats_get", "pfs_stage_times", "pfs_stage_name", "pfs_stage_count",
            "pfs_kernel_count", "pfs_kernel_stats", "pfs_memory", "pfs_last_error", "pfs_version",
            "pfs_device_count")
 
@@ -40,7 +40,9 @@ class _RoundStats(ctypes.Structure):
     _fields_ = [("elected", ctypes.c_uint32), ("eliminated", ctypes.c_uint32), ("units", ctypes.c_uint32),
                 ("resolvents", ctypes.c_uint32), ("clauses_after", ctypes.c_uint64),
                 ("literals_after", ctypes.c_uint64), ("mis_iterations", ctypes.c_uint32),
-                ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("clauses_before", ctypes.c_uint64), ("literals_before", ctypes.c_uint64),
+                ("elected_occurs", ctypes.c_uint64), ("elected_bytes", ctypes.c_uint64)]
 
 
 def lib_path() -> str:
@@ -69,6 +71,8 @@ def load_library():
     L.pfs_download_units.argtypes = [vp, vp]
     L.pfs_download_vstate.argtypes = [vp, vp]
     L.pfs_download_stack.argtypes = [vp, vp]
+    L.pfs_proof_size.argtypes = [vp, ctypes.POINTER(u64)]
+    L.pfs_download_proof.argtypes = [vp, vp]
     L.pfs_round_stats_get.argtypes = [vp, ctypes.POINTER(_RoundStats), u32, ctypes.POINTER(u32)]
     L.pfs_stage_times.argtypes = [vp, vp, vp, u32]
     L.pfs_stage_name.argtypes = [u32]
@@ -103,6 +107,7 @@ class Result:
     resolved: np.ndarray
     vstate: np.ndarray
     rounds: list
+    proof: bytes = b""  # binary DRAT lines (option "proof")
 
 
 class Simplifier:
@@ -192,8 +197,15 @@ class Simplifier:
         L.pfs_round_stats_get(self._h, rs, nr.value, ctypes.byref(n))
         rounds = [{k: getattr(rs[i], k) for k, _ in _RoundStats._fields_ if k != "reserved"} for i in range(n.value)]
         sizes = np.diff(off).astype(np.uint32)
+        npf = ctypes.c_uint64()
+        self._check(L.pfs_proof_size(self._h, ctypes.byref(npf)), "pfs_proof_size")
+        proof = b""
+        if npf.value:
+            pf = np.zeros(npf.value, np.uint8)
+            self._check(L.pfs_download_proof(self._h, _ptr(pf)), "pfs_download_proof")
+            proof = pf.tobytes()
         return Result(status, self.maxvar, sizes, flags[: nc.value], lbd[: nc.value], lits[: nl.value],
-                      units[: nu.value], stack[: ns.value], vst, rounds)
+                      units[: nu.value], stack[: ns.value], vst, rounds, proof)
 
     def stage_times(self) -> dict:
         L = self._L

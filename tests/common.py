@@ -71,3 +71,32 @@ def extend_model(values, resolved):
             w = lits[0]
             values[w >> 1] = not (w & 1)
     return values
+
+
+def read_golden_proof(entry):
+    import gzip
+    with gzip.open(os.path.join(GOLDEN, entry["proof"]), "rb") as f:
+        return f.read()
+
+
+def clause_list(st):
+    """Clauses of a PFD input state as lists of literals."""
+    off = [0] + np.cumsum(st.sizes.astype(np.int64)).tolist()
+    return [st.lits[off[i]:off[i + 1]].tolist() for i in range(st.sizes.shape[0])]
+
+
+def proof_lines_canonical(proof: bytes):
+    """Proof lines with each run of consecutive unit lines sorted: unit
+    propagation finds the same units in any order, so its lines are compared
+    as a set (the parallel propagation claims units level by level)."""
+    from tests.drat import parse
+    out, run = [], []
+    for line in parse(proof):
+        if line[0] == "a" and len(line[1]) == 1:
+            run.append(line)
+            continue
+        out.extend(sorted(run))
+        run = []
+        out.append(line)
+    out.extend(sorted(run))
+    return out

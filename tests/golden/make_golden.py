@@ -2,18 +2,19 @@
 simplifier (ParaFROST src/cpu, compiled from its own sources by
 oracle/build_ref.sh into oracle/_ref/pfref).
 
-Each fixture is a pair of PFD1 dumps (parafrost_amd/pfd.py):
+Each fixture is a pair of PFD1 dumps (parafrost_amd/pfd.py) and a proof:
   <case>.in.pfd.gz        -- the simplifier input the reference extracted (after
                           its parser, root BCP and rootify/shrinkTop)
   <case>.<opts>.out.pfd.gz -- the reference's output: clauses in simplifier order,
                     new root units, eliminated-variable stack, variable states
+  <case>.<opts>.proof.gz  -- the binary DRAT lines the reference simplifier
+                    wrote (run with -proof; the part after rootify/shrinkTop)
 manifest.json lists every fixture with the reference options it was run
 with.  Run from the repository root:  python tests/golden/make_golden.py
 """
 import gzip
 import json
 import os
-import shutil
 import subprocess
 import sys
 
@@ -49,9 +50,18 @@ def cases():
 
 
 def gz(path):
-    with open(path, "rb") as a, gzip.open(path + ".gz", "wb", compresslevel=9) as b:
-        shutil.copyfileobj(a, b)
+    """Compresses path to path.gz (no timestamp in the header) unless an
+    existing path.gz already holds the same bytes."""
+    with open(path, "rb") as a:
+        data = a.read()
     os.remove(path)
+    if os.path.exists(path + ".gz"):
+        with gzip.open(path + ".gz", "rb") as old:
+            if old.read() == data:
+                return
+    with open(path + ".gz", "wb") as f, gzip.GzipFile(filename="", mode="wb", fileobj=f, compresslevel=9,
+                                                       mtime=0) as b:
+        b.write(data)
 
 
 def main():
@@ -65,13 +75,24 @@ def main():
         for oname, (ropts, kw) in OPTSETS.items():
             tag = f"{name}.{oname}"
             out = os.path.join(HERE, tag + ".out.pfd")
-            r = subprocess.run([REF, cnf_path, inp, out] + ropts, capture_output=True, text=True, timeout=600)
+            drat = os.path.join(HERE, tag + ".drat")
+            r = subprocess.run([REF, cnf_path, inp, out] + ropts + ["-proof", f"--proofout={drat}"],
+                               capture_output=True, text=True, timeout=600)
             if not os.path.exists(out):
                 print("skip (refuted at parse time)", tag)
                 continue
             gz(out)
+            with open(out + ".proofpos") as f:
+                pos = int(f.read())
+            os.remove(out + ".proofpos")
+            with open(drat, "rb") as f:
+                proof = f.read()[pos:]
+            os.remove(drat)
+            with open(os.path.join(HERE, tag + ".proof"), "wb") as f:
+                f.write(proof)
+            gz(os.path.join(HERE, tag + ".proof"))
             manifest.append({"name": tag, "input": name + ".in.pfd.gz", "output": tag + ".out.pfd.gz",
-                             "ref_options": ropts, "options": kw})
+                             "proof": tag + ".proof.gz", "ref_options": ropts, "options": kw})
             print(tag, r.stdout.strip().splitlines()[-1][:100])
         os.remove(cnf_path)
         if os.path.exists(inp):

@@ -43,3 +43,27 @@ def test_random_formulas(i):
     r = oracle.simplify_state(inp, det=0, **oo)
     d = diff(r, out, units_as_set=False)
     assert not d, d
+
+
+@pytest.mark.parametrize("i", range(60))
+def test_random_formulas_proof(i):
+    """The oracle's DRAT lines equal the reference simplifier's (run with
+    -proof; the lines written after rootify/shrinkTop)."""
+    cnf, flags, lbd, ro, oo = case(i)
+    if flags is not None:
+        pytest.skip("the reference parser reads original clauses only")
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "f.cnf")
+        gen.write_dimacs(cnf, p)
+        drat = os.path.join(d, "p.drat")
+        subprocess.run([REF, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd"), "-proof",
+                        f"--proofout={drat}"] + ref_args(ro), capture_output=True, timeout=120)
+        if not os.path.exists(os.path.join(d, "i.pfd")):
+            pytest.skip("refuted by the reference parser")
+        with open(os.path.join(d, "o.pfd.proofpos")) as f:
+            pos = int(f.read())
+        with open(drat, "rb") as f:
+            ref_proof = f.read()[pos:]
+        inp = pfd.read(os.path.join(d, "i.pfd"))
+    r = oracle.simplify_state(inp, det=0, proof_en=1, **oo)
+    assert r.proof == ref_proof
