@@ -8,6 +8,15 @@ simplified inside the timed region.  The default workload is the
 Tseitin-encoded random circuit with 2M gates (60% AND, 30% XOR, 10% ITE,
 5,000 inputs, 1,000 unit outputs, root-propagated: about 6.8M clauses).
 
+Two more legs follow (skip with --no-batched / --no-serving):
+* batched: 512 formulas (log-uniform 1k-500k clauses, Tseitin or
+  heavy-tailed) resident in HBM, simplified as a loop of single-formula
+  calls and by the native batch executor; formulas/s of both, per-formula
+  p50/p99;
+* serving: incremental BMC sessions under Poisson arrivals at 10-500
+  requests/s through the native request server; p50/p99 latency and
+  throughput per rate.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tseitin|uniform3|heavytail]
 Prints one JSON line.
 """
@@ -119,6 +128,112 @@ def cpu_baseline(cnf, name, budget_clauses=7_500_000):
             "sample": note + f"; oracle restatement round time {r.seconds:.3f} s", "seconds": r.seconds}
 
 
+def bench_batched(n=512, workers=16, check=16):
+    """Batched workload: n formulas (clause counts log-uniform in 1k-500k,
+    Tseitin or heavy-tailed) resident in HBM, simplified once as a loop of
+    plain single-formula calls and once by pfs_batch_simplify with 'workers'
+    native threads (one stream per formula).  formulas/s of both, per-formula
+    latency percentiles of the batched run, and a bit-exact comparison of
+    the first 'check' outputs."""
+    import numpy as np
+    from parafrost_amd import gen
+    from parafrost_amd.simplifier import Simplifier, batch_simplify
+    t0 = time.time()
+    cnfs = gen.batch(n)
+    ncl = sum(c.nclauses for c in cnfs)
+    ctx = []
+    for c in cnfs:
+        s = Simplifier(0, keep_input=True, **OPTIONS)
+        s.upload(c.maxvar, c.offsets, c.lits, trail=c.trail)
+        ctx.append(s)
+    print(f"bench: batch of {n} formulas, {ncl} clauses ({time.time() - t0:.1f}s to generate and upload)",
+          file=sys.stderr, flush=True)
+    # warm-up (kernel loading, buffer growth), then the timed runs
+    batch_simplify(ctx[:workers], workers)
+    for s in ctx[:workers]:
+        s.rewind()
+    t = time.perf_counter()
+    loop_res = [s.simplify() for s in ctx]
+    loop_s = time.perf_counter() - t
+    ref = [ctx[i].result(loop_res[i]) for i in range(min(check, n))]
+    for s in ctx:
+        s.rewind()
+    t = time.perf_counter()
+    res, ms = batch_simplify(ctx, workers)
+    batch_s = time.perf_counter() - t
+    same = 0
+    for i in range(min(check, n)):
+        b = ctx[i].result(res[i])
+        same += int(b.status == ref[i].status and np.array_equal(b.lits, ref[i].lits) and
+                    np.array_equal(b.sizes, ref[i].sizes) and np.array_equal(b.resolved, ref[i].resolved) and
+                    np.array_equal(b.flags, ref[i].flags) and np.array_equal(b.vstate, ref[i].vstate))
+    for s in ctx:
+        s.close()
+    out = {"formulas": n, "clauses": ncl, "workers": workers,
+           "formulas_per_s": round(n / batch_s, 1), "loop_formulas_per_s": round(n / loop_s, 1),
+           "speedup_vs_loop": round(loop_s / batch_s, 2), "clauses_per_s": round(ncl / batch_s, 1),
+           "p50_ms": round(float(np.percentile(ms, 50)), 3), "p99_ms": round(float(np.percentile(ms, 99)), 3),
+           "batch_s": round(batch_s, 3), "loop_s": round(loop_s, 3), "identical_checked": f"{same}/{min(check, n)}"}
+    print("bench: batched " + json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
+def bench_serving(rates=(10, 20, 50, 100, 200, 500), nsessions=16, nreq=64, workers=8):
+    """Serving workload: BMC sessions (64 latches, 2,000 gates, depth
+    uniform in 10-200 frames, 10 frames per request, 1-32 assumptions) whose
+    formulas stay resident; requests arrive as a Poisson stream at each rate
+    and are served by the request server (continuous batching over
+    'workers' native threads).  Latency = completion - arrival."""
+    import numpy as np
+    from parafrost_amd import bmc
+    from parafrost_amd.simplifier import Server, Simplifier
+    t0 = time.time()
+    sessions = bmc.workload(nsessions)
+    print(f"bench: serving {nsessions} sessions, {sum(len(x) for x in sessions)} requests available "
+          f"({time.time() - t0:.1f}s to generate)", file=sys.stderr, flush=True)
+    sweep = []
+    for rate in rates:
+        rng = np.random.default_rng(rate)
+        ctx = []
+        for _ in sessions:
+            s = Simplifier(0, **OPTIONS)
+            s.upload(0, np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+            ctx.append(s)
+        pending = [0] * nsessions
+        order = []
+        while len(order) < nreq:
+            live = [i for i in range(nsessions) if pending[i] < len(sessions[i])]
+            if not live:
+                break
+            si = int(rng.choice(live))
+            order.append((si, pending[si]))
+            pending[si] += 1
+        times = bmc.poisson_schedule(len(order), rate, rate + 1)
+        srv = Server(workers)
+        tickets = []
+        for (si, q), rel in zip(order, times):
+            rq = sessions[si][q]
+            tickets.append(srv.submit(ctx[si], float(rel), rq.maxvar, rq.offsets, rq.lits, rq.assumed))
+        srv.start()
+        lat, ends = [], []
+        for tk in tickets:
+            st, res, rel, start, end = srv.wait(tk)
+            if st != 0:
+                raise RuntimeError(f"serving request failed with status {st}")
+            lat.append(end - rel)
+            ends.append(end)
+        srv.close()
+        for s in ctx:
+            s.close()
+        span = max(ends) - float(times[0])
+        row = {"rate_rps": rate, "requests": len(order), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+               "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "throughput_rps": round(len(order) / (span / 1e3), 1)}
+        sweep.append(row)
+        print("bench: serving " + json.dumps(row), file=sys.stderr, flush=True)
+    return {"sessions": nsessions, "workers": workers, "frames_per_request": 10, "sweep": sweep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="extra simplifier option (launch shaping such as workorder=0)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 256 MiB device copies first (reference kernel for counter runs)")
     args = ap.parse_args()
@@ -204,6 +321,9 @@ def main():
                     "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
     s.close()
 
+    batched = None if args.no_batched else bench_batched()
+    serving = None if args.no_serving else bench_serving()
+
     base = None
     if not args.no_cpu_baseline:
         try:
@@ -229,6 +349,8 @@ def main():
                    "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": base,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -91,6 +91,20 @@ pfs_status pfs_upload(pfs_ctx* ctx, uint32_t maxvar, uint64_t ncls, const uint64
                       const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
                       const uint32_t* trail, uint32_t ntrail);
 
+/* Incremental use (the reference's incremental front end iadd/itoClause,
+ * src/cpu/addinc.cpp:44-160, and assumptions, src/cpu/assume.cpp): the
+ * formula stays resident in device memory between calls.
+ * pfs_append adds original clauses over variables 1..maxvar (maxvar may grow)
+ * after the current clauses: literals false at the root are dropped,
+ * root-satisfied clauses and tautologies skipped, unit clauses assigned
+ * (propagated by the next pfs_simplify), an empty clause makes the next
+ * pfs_simplify report UNSAT.  The results and stack of later calls refer to
+ * the whole resident formula; stack words and units accumulate.
+ * pfs_assume replaces the set of assumed variables (never eliminated). */
+pfs_status pfs_append(pfs_ctx* ctx, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
+                      const uint32_t* lits);
+pfs_status pfs_assume(pfs_ctx* ctx, const uint32_t* vars, uint32_t n);
+
 /* Restores the last uploaded input from its device copy (keep_input). */
 pfs_status pfs_rewind(pfs_ctx* ctx);
 
@@ -137,6 +151,40 @@ uint32_t pfs_kernel_count(pfs_ctx* ctx);
 pfs_status pfs_kernel_stats(pfs_ctx* ctx, uint32_t i, const char** name, double* total_ms,
                             uint64_t* launches);
 pfs_status pfs_memory(pfs_ctx* ctx, uint64_t* used, uint64_t* peak);
+
+/* Batched simplification (no reference counterpart: the reference
+ * simplifies one formula per process, src/gpu/simplify.cu:43-55).  Runs
+ * pfs_simplify on n contexts that already hold their formulas, with
+ * 'workers' host threads; each context has its own stream, so the rounds of
+ * different formulas overlap on the device and their host-side
+ * synchronisation overlaps in time.  A context is used by one worker at a
+ * time, so every result equals the context's own pfs_simplify.
+ * results[i] receives context i's pfs_result, statuses[i] (nullable) its
+ * status, ms[i] (nullable) its wall time.  Returns the first non-OK status. */
+pfs_status pfs_batch_simplify(pfs_ctx* const* ctxs, uint32_t n, uint32_t workers, int32_t* results,
+                              pfs_status* statuses, double* ms);
+/* Request server for incremental sessions (continuous batching; no
+ * reference counterpart).  A session is a context whose formula stays
+ * resident; a request appends clauses (pfs_append, maxvar may grow),
+ * replaces the assumed variables (pfs_assume) and simplifies (pfs_simplify).
+ * Requests carry a release time in ms after pfs_server_start.  'workers'
+ * host threads take the earliest released request whose session is idle
+ * and, when mem_budget > 0, whose session's device memory fits next to the
+ * sessions running; requests of one session run one at a time in submission
+ * order, so each result equals the session's result alone.  The request's
+ * data are copied at submission.  pfs_server_wait blocks until a request
+ * is done and returns its status, result and release/start/end times (ms).
+ * pfs_server_destroy waits for every submitted request. */
+typedef struct pfs_server pfs_server;
+pfs_status pfs_server_create(uint32_t workers, uint64_t mem_budget, pfs_server** out);
+pfs_status pfs_server_submit(pfs_server* srv, pfs_ctx* session, double release_ms, uint32_t maxvar,
+                             uint64_t ncls, const uint64_t* offsets, const uint32_t* lits,
+                             const uint32_t* assumed, uint32_t nassumed, uint32_t* ticket);
+pfs_status pfs_server_start(pfs_server* srv);
+pfs_status pfs_server_wait(pfs_server* srv, uint32_t ticket, int32_t* result, double* release_ms,
+                           double* start_ms, double* end_ms);
+void pfs_server_destroy(pfs_server* srv);
+
 const char* pfs_last_error(const pfs_ctx* ctx);
 const char* pfs_version(void);
 int32_t pfs_device_count(void);

@@ -38,6 +38,9 @@ def lib():
         L = ctypes.CDLL(path)
         L.oracle_run.restype = ctypes.c_void_p
         L.oracle_run.argtypes = [ctypes.POINTER(OracleOpts), ctypes.c_uint32, ctypes.c_uint64] + [ctypes.c_void_p] * 7 + [ctypes.c_uint32]
+        L.oracle_run_trail.restype = ctypes.c_void_p
+        L.oracle_run_trail.argtypes = ([ctypes.POINTER(OracleOpts), ctypes.c_uint32, ctypes.c_uint64]
+                                       + [ctypes.c_void_p] * 8 + [ctypes.c_uint32, ctypes.c_uint32])
         L.oracle_status.argtypes = [ctypes.c_void_p]
         L.oracle_seconds.argtypes = [ctypes.c_void_p]
         L.oracle_seconds.restype = ctypes.c_double
@@ -72,7 +75,11 @@ def _p(a):
 
 
 def simplify(maxvar, offsets, lits, flags=None, lbd=None, vstate=None, assumed=None, vorg=None,
-             nassigned=0, **opts) -> Result:
+             nassigned=0, trail=None, npropagated=None, **opts) -> Result:
+    """Runs the oracle.  Root assignments: either only their number
+    (nassigned; the formula holds none of their variables) or the literals
+    themselves (trail), of which the first npropagated are propagated and the
+    rest still have to be.  The result's units are the new ones."""
     o = dict(DEFAULTS)
     o.update(opts)
     oo = OracleOpts(**o)
@@ -84,8 +91,16 @@ def simplify(maxvar, offsets, lits, flags=None, lbd=None, vstate=None, assumed=N
     assumed = None if assumed is None else np.ascontiguousarray(assumed, np.uint8)
     vorg = None if vorg is None else np.ascontiguousarray(vorg, np.uint32)
     L = lib()
-    h = L.oracle_run(ctypes.byref(oo), maxvar, offsets.shape[0] - 1, _p(offsets), _p(lits), _p(flags), _p(lbd),
-                     _p(vstate), _p(assumed), _p(vorg), nassigned)
+    if trail is None:
+        h = L.oracle_run(ctypes.byref(oo), maxvar, offsets.shape[0] - 1, _p(offsets), _p(lits), _p(flags),
+                         _p(lbd), _p(vstate), _p(assumed), _p(vorg), nassigned)
+        skip = 0  # the oracle's trail starts empty: every unit is new
+    else:
+        trail = np.ascontiguousarray(trail, np.uint32)
+        npr = trail.shape[0] if npropagated is None else int(npropagated)
+        h = L.oracle_run_trail(ctypes.byref(oo), maxvar, offsets.shape[0] - 1, _p(offsets), _p(lits), _p(flags),
+                               _p(lbd), _p(vstate), _p(assumed), _p(vorg), _p(trail), trail.shape[0], npr)
+        skip = trail.shape[0]  # the given literals lead the oracle's trail
     try:
         sz = np.zeros(6, np.uint64)
         L.oracle_sizes(h, sz.ctypes.data)
@@ -96,6 +111,7 @@ def simplify(maxvar, offsets, lits, flags=None, lbd=None, vstate=None, assumed=N
                    L.oracle_seconds(h))
         L.oracle_get(h, _p(r.sizes), _p(r.flags), _p(r.lbd), _p(r.lits), _p(r.units), _p(r.resolved),
                      _p(r.vstate), _p(r.elected))
+        r.units = r.units[skip:]
         npf = int(L.oracle_proof_size(h))
         if npf:
             pf = np.zeros(npf, np.uint8)

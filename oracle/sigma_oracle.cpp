@@ -1140,10 +1140,28 @@ typedef struct {
 // and lbd (nullable), per-var vstate (nullable), assumed flags (nullable),
 // vorg (nullable = identity), the number of root-assigned variables.
 // Returns an opaque handle; read it with oracle_get_* and free it.
+// Incremental form: 'trail' holds the root-assigned literals, of which the
+// first 'npropagated' are propagated already (their clauses are gone) and the
+// rest still have to be (the next prop() does it, as after the reference's
+// incremental clause addition, src/cpu/addinc.cpp).
+void* oracle_run_trail(const oracle_opts* o, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
+                       const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
+                       const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
+                       const uint32_t* trail, uint32_t ntrail, uint32_t npropagated);
+
 void* oracle_run(const oracle_opts* o, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
                  const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
                  const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
                  uint32_t nassigned) {
+  return oracle_run_trail(o, maxvar, ncls, offsets, lits, flags, lbd, vstate, assumed, vorg, nullptr, nassigned,
+                          nassigned);
+}
+
+void* oracle_run_trail(const oracle_opts* o, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
+                       const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
+                       const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
+                       const uint32_t* trail, uint32_t ntrail, uint32_t npropagated) {
+  const uint32_t nassigned = ntrail;
   Oracle* s = new Oracle();
   Result* r = new Result();
   s->opts = *reinterpret_cast<const Opts*>(o);
@@ -1166,6 +1184,14 @@ void* oracle_run(const oracle_opts* o, uint32_t maxvar, uint64_t ncls, const uin
     if (assumed) s->assumed[v] = assumed[v];
   }
   s->unassigned = maxvar - nassigned - melted;
+  if (trail) {
+    for (uint32_t i = 0; i < ntrail; i++) {
+      s->value[trail[i]] = 1;
+      s->value[FLIP(trail[i])] = 0;
+      s->trail.push_back(trail[i]);
+    }
+    s->propagated = npropagated;
+  }
   s->cls.resize(ncls);
   s->pool.resize(offsets[ncls]);
   s->scnf.resize(ncls);

@@ -1,10 +1,15 @@
 // extern "C" entry points of libpfsimp.so (declared in include/pfsimp.h).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <chrono>
+#include <atomic>
 #include <exception>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/pfsimp.h"
 #include "pfs_driver.hpp"
@@ -77,6 +82,7 @@ bool parse_int(const char* v, int32_t lo, int32_t hi, int32_t& out) {
 template <class F>
 pfs_status guard(pfs_ctx* c, F&& f) {
   try {
+    if (c && c->s) c->s->bind();  // the calling thread may be a worker of pfs_batch_simplify
     return f();
   } catch (AllocFail& e) {
     if (c) c->err = e.what;
@@ -194,7 +200,7 @@ pfs_status pfs_upload(pfs_ctx* c, uint32_t maxvar, uint64_t ncls, const uint64_t
                       const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
                       const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
                       const uint32_t* trail, uint32_t ntrail) {
-  if (!c || !offsets || (!lits && offsets[ncls]) || (!trail && ntrail) || !maxvar) return PFS_BAD_INPUT;
+  if (!c || !offsets || (!lits && offsets[ncls]) || (!trail && ntrail) || (!maxvar && ncls)) return PFS_BAD_INPUT;
   return guard(c, [&] {
     for (uint64_t i = 0; i < ncls; i++)
       if (offsets[i + 1] < offsets[i] + 1) throw std::invalid_argument("empty clause or bad offsets");
@@ -216,6 +222,31 @@ pfs_status pfs_rewind(pfs_ctx* c) {
   return guard(c, [&] {
     c->s->rewind();
     c->simplified = false;
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_append(pfs_ctx* c, uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits) {
+  if (!c || (ncls && (!offsets || !lits))) return PFS_BAD_INPUT;
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    c->s->append(maxvar, ncls, offsets, lits);
+    c->simplified = false;
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_assume(pfs_ctx* c, const uint32_t* vars, uint32_t n) {
+  if (!c || (n && !vars)) return PFS_BAD_INPUT;
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    c->s->assume(vars, n);
     return PFS_OK;
   });
 }
@@ -362,6 +393,33 @@ pfs_status pfs_memory(pfs_ctx* c, uint64_t* used, uint64_t* peak) {
 }
 
 const char* pfs_last_error(const pfs_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+pfs_status pfs_batch_simplify(pfs_ctx* const* ctxs, uint32_t n, uint32_t workers, int32_t* results,
+                              pfs_status* statuses, double* ms) {
+  if (!ctxs || (n && !results)) return PFS_BAD_INPUT;
+  for (uint32_t i = 0; i < n; i++)
+    if (!ctxs[i]) return PFS_BAD_INPUT;
+  const uint32_t nw = std::max<uint32_t>(1, std::min(workers ? workers : 1, n));
+  std::atomic<uint32_t> next{0};
+  std::vector<pfs_status> st(n, PFS_OK);
+  auto work = [&] {
+    for (uint32_t i; (i = next.fetch_add(1)) < n;) {
+      const auto t0 = std::chrono::steady_clock::now();
+      st[i] = pfs_simplify(ctxs[i], &results[i]);
+      if (ms) ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (uint32_t w = 1; w < nw; w++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  pfs_status first = PFS_OK;
+  for (uint32_t i = 0; i < n; i++) {
+    if (statuses) statuses[i] = st[i];
+    if (first == PFS_OK && st[i] != PFS_OK) first = st[i];
+  }
+  return first;
+}
+
 const char* pfs_version(void) { return "pfsimp 0.1 (gfx950)"; }
 int32_t pfs_device_count(void) { return pfs::device_count(); }
 

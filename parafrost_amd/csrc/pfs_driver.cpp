@@ -61,7 +61,7 @@ Simplifier::~Simplifier() {
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
                 &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
                 &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
-                &pstamp_, &plines_, &pbytes_};
+                &pstamp_, &plines_, &pbytes_, &subu_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
@@ -204,9 +204,11 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   unassigned_ = V_ - ntrail - melted;
   stack_n_ = 0;
   units_out_.clear();
+  refuted_ = false;
+  resident_ = false;
   fill_u8(P<u8>(mark_), 0, ccap_, st_);
   // extract: sort literals, signatures
-  k_prep(make_db(0), st_);
+  k_prep(make_db(0), 0, st_);
   if (keep_input_) {
     in_ncls_ = ncls;
     in_nlits_ = L;
@@ -244,8 +246,123 @@ void Simplifier::rewind() {
   unassigned_ = in_unassigned_;
   stack_n_ = 0;
   units_out_.clear();
+  refuted_ = false;
+  resident_ = false;
   std::fill(h_assign_.begin(), h_assign_.end(), 0u);
   for (u32 u : in_units_) h_assign_[ABS(u)] = u + 1;
+}
+
+// ---------------------------------------------------------------- incremental use
+// Adds original clauses over variables 1..maxvar (maxvar >= the current
+// count) to the resident formula, as the reference's incremental front end
+// does (iadd / itoClause, src/cpu/addinc.cpp:44-160): literals false at the
+// root are dropped, root-satisfied clauses skipped, units assigned (they are
+// propagated by the next simplify), an empty clause refutes.
+void Simplifier::append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits) {
+  if (maxvar < V_) throw std::invalid_argument("append cannot reduce the number of variables");
+  if (maxvar > V_) grow_vars(maxvar);
+  std::vector<u32> words, off;
+  std::vector<u32> cl;
+  uint64_t w = lits_end_;
+  for (uint64_t i = 0; i < ncls; i++) {
+    cl.clear();
+    bool sat = false;
+    for (uint64_t j = offsets[i]; j < offsets[i + 1]; j++) {
+      const u32 l = lits[j];
+      if (ABS(l) == 0 || ABS(l) > maxvar) throw std::invalid_argument("literal out of range");
+      const u32 a = h_assign_[ABS(l)];
+      if (a) {
+        if (a - 1 == l) sat = true;
+        continue;
+      }
+      bool dup = false;
+      for (u32 x : cl) {
+        if (x == l) dup = true;
+        if (x == FLIP(l)) sat = true;  // tautology
+      }
+      if (!dup) cl.push_back(l);
+    }
+    if (sat) continue;
+    if (cl.empty()) {
+      refuted_ = true;
+      continue;
+    }
+    if (cl.size() == 1) {
+      const u32 u = cl[0];
+      if (h_assign_[ABS(u)]) continue;  // assigned by an earlier clause of this batch
+      h_assign_[ABS(u)] = u + 1;
+      pending_units_.push_back(u);
+      continue;
+    }
+    if (cl.size() > CL_MAX_SIZE) throw std::invalid_argument("clause longer than 2^24-1 literals");
+    off.push_back((u32)w);
+    words.push_back(make_hdr((u32)cl.size(), ST_ORIGINAL));
+    words.push_back(0);
+    words.insert(words.end(), cl.begin(), cl.end());
+    w += CL_HDR + cl.size();
+  }
+  const uint64_t nadd = off.size();
+  if (nadd) {
+    ensure_clause_room(nadd, words.size());
+    DB db0 = make_db(cur_);
+    h2d(db0.c_off + ncls_, off.data(), nadd * 4, st_);
+    fill_u32(db0.c_lbd + ncls_, 0, nadd, st_);
+    h2d(db0.lits + lits_end_, words.data(), words.size() * 4, st_);
+    const u32 first = (u32)ncls_;
+    ncls_ += nadd;
+    lits_end_ += words.size();
+    k_prep(make_db(cur_), first, st_);
+  }
+  if (!pending_units_.empty()) {
+    // new root units: assigned and frozen like enqueueUnit, appended to the
+    // trail unpropagated
+    mem_.ensure(ids_, pending_units_.size() * 4, 0, st_);
+    h2d(ids_.p, pending_units_.data(), pending_units_.size() * 4, st_);
+    k_assign_units(make_db(cur_), P<u32>(ids_), (u32)pending_units_.size(), st_);
+    h2d(P<u32>(trail_) + trail_n_, pending_units_.data(), pending_units_.size() * 4, st_);
+    trail_n_ += pending_units_.size();
+    for (u32 u : pending_units_) units_out_.push_back(u);
+    unassigned_ -= (u32)pending_units_.size();
+    pending_units_.clear();
+  }
+  stream_sync(st_);
+}
+
+// Replaces the set of assumed variables (never eliminated; the reference's
+// incremental assumptions, src/cpu/assume.cpp, and lcve.cpp's 'iassumed').
+void Simplifier::assume(const uint32_t* vars, uint32_t n) {
+  std::vector<u8> as((size_t)V_ + 1, 0);
+  for (uint32_t i = 0; i < n; i++) {
+    if (vars[i] == 0 || vars[i] > V_) throw std::invalid_argument("assumed variable out of range");
+    as[vars[i]] = 1;
+  }
+  h2d(assumed_.p, as.data(), as.size(), st_);
+  stream_sync(st_);
+}
+
+// grows the per-variable state to maxvar variables, keeping its content
+void Simplifier::grow_vars(uint32_t maxvar) {
+  const size_t o1 = (size_t)V_ + 1, n1 = (size_t)maxvar + 1;
+  mem_.ensure(assign_, n1 * 4, o1 * 4, st_);
+  mem_.ensure(vstate_, n1, o1, st_);
+  mem_.ensure(assumed_, n1, o1, st_);
+  mem_.ensure(vorg_, n1 * 4, o1 * 4, st_);
+  mem_.ensure(trail_, (n1 + 1) * 4, trail_n_ * 4, st_);
+  fill_u32(P<u32>(assign_) + o1, 0, n1 - o1, st_);
+  fill_u8(P<u8>(vstate_) + o1, 0, n1 - o1, st_);
+  fill_u8(P<u8>(assumed_) + o1, 0, n1 - o1, st_);
+  std::vector<u32> vo(n1 - o1);
+  for (size_t v = o1; v < n1; v++) vo[v - o1] = (u32)v;
+  h2d(P<u32>(vorg_) + o1, vo.data(), vo.size() * 4, st_);
+  if (pstamp_.p) {
+    mem_.ensure(pstamp_, n1 * 4, o1 * 4, st_);
+    fill_u32(P<u32>(pstamp_) + o1, 0, n1 - o1, st_);
+  }
+  unassigned_ += maxvar - V_;
+  V_ = maxvar;
+  h_assign_.resize(n1, 0);
+  alloc_state();  // the per-round buffers (their content is rebuilt every round)
+  stream_sync(st_);
 }
 
 // ---------------------------------------------------------------- helpers
@@ -481,6 +598,7 @@ bool Simplifier::sub() {
   mem_.ensure(snap_size_, ob * 4, 0, st_);
   mem_.ensure(snap_sig_, ob * 4, 0, st_);
   mem_.ensure(snap_lbd_, ob * 4, 0, st_);
+  mem_.ensure(subu_, ob * 8, 0, st_);
   mem_.ensure(unit_by_, std::max<size_t>(ncls_, 2 * (size_t)V_ + 3) * 4, 0, st_);
   fill_u32(P<u32>(unit_by_), INF, ncls_, st_);  // clause owners
   SubSnap snap{P<u32>(vbase_),    P<u32>(snap_np_),  P<u32>(snap_nn_),  P<u32>(snap_ref_), P<u8>(snap_flg_),
@@ -496,12 +614,11 @@ bool Simplifier::sub() {
     fill_u32(sc, 0, 1, st_);
     fill_u32(sc + 1, INF, 1, st_);
     fill_u32(sc + 2, 0, 1, st_);
-    SubOut out{P<u32>(recs_), sc, cap, sc + 1, sc + 2, proof_out(2 * O + E_ + 64)};
+    SubOut out{P<u32>(recs_), sc, cap, sc + 1, P<u32>(subu_), P<u32>(vbase_), proof_out(2 * O + E_ + 64)};
     k_sub(db, opt, P<u32>(pv_), from, E_, out, work_order(), st_);
     u32 h[3];
     d2h(h, sc, 12, st_);
     stream_sync(st_);
-    if (h[2]) throw DeviceFail{"a variable derived more units in SUB than the per-variable buffer holds"};
     if (h[0] > cap) throw DeviceFail{"SUB unit record buffer overflow"};
     const u32 nrec = h[0], unsat_at = h[1];
     if (!nrec && unsat_at == INF) {
@@ -942,6 +1059,9 @@ int Simplifier::simplify(uint32_t max_rounds) {
   proof_n_ = 0;
   pl_.clear();
   pst_.clear();
+  // a resident formula simplified before: its resolvents are no longer new
+  if (resident_) k_clear_added(make_db(cur_), st_);
+  resident_ = true;
   kernel_times.clear();
   prof_enable(st_, profile);
   const int r = simplify_rounds(max_rounds);
@@ -951,6 +1071,11 @@ int Simplifier::simplify(uint32_t max_rounds) {
 }
 
 int Simplifier::simplify_rounds(uint32_t max_rounds) {
+  if (refuted_) {
+    proof_line(PR_EMPTY, 0);
+    proof_flush();
+    return 0;
+  }
   if (!ncls_) return 2;
   phase_ = 0;
   mu_inc_ = 0;

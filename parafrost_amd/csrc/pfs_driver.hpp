@@ -67,6 +67,11 @@ class Simplifier {
               const uint8_t* flags, const uint32_t* lbd, const uint8_t* vstate, const uint8_t* assumed,
               const uint32_t* vorg, const uint32_t* trail, uint32_t ntrail);
   void rewind();                    // restore the last uploaded input (device copy)
+  void bind() { stream_bind(st_); }  // select this context's device on the calling thread
+  // incremental use: add clauses / replace the assumed variables of the
+  // resident formula (the state after the last simplify)
+  void append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits);
+  void assume(const uint32_t* vars, uint32_t n);
   int simplify(uint32_t max_rounds); // 0 UNSAT, 1 SAT, 2 UNSOLVED
 
   // results (valid after simplify)
@@ -111,6 +116,7 @@ class Simplifier {
   void ere_per_var(); // per-variable ERE with speculative re-runs
   void ensure_clause_room(uint64_t add_cls, uint64_t add_lits);
   void ensure_stack(uint64_t words);
+  void grow_vars(uint32_t maxvar);
   void commit_units(const std::vector<uint32_t>& lits);
   // proof lines: queued per stage on the host as (kind, payload), encoded
   // on the device by proof_flush() before clause ids can change
@@ -148,7 +154,7 @@ class Simplifier {
   // per elected variable
   Buf vsz_, vbase_, type_, def_, vncls_, vnlits_, vnstack_, clsb_, litb_, stkb_;
   Buf snap_ref_, snap_flg_, snap_size_, snap_sig_, snap_lbd_, snap_np_, snap_nn_, scratch_;
-  Buf elits_, unit_by_, recs_, ids_, mark_, ht_, erecs_, order_;
+  Buf elits_, unit_by_, recs_, ids_, mark_, ht_, erecs_, order_, subu_;
   // proof
   Buf proof_, precs_, pstamp_, plines_, pbytes_;
   uint64_t proof_n_ = 0;
@@ -171,6 +177,9 @@ class Simplifier {
   std::vector<uint32_t> in_units_;
   uint32_t unassigned_ = 0;
   int phase_ = 0;
+  bool refuted_ = false;           // an empty clause was appended
+  bool resident_ = false;          // the clauses come from an earlier simplify call
+  std::vector<u32> pending_units_;  // units of the clauses being appended
   bool prop_changed_ = true;  // BCP touched clauses since the occurrence table was built
   uint32_t mu_inc_ = 0;
   std::vector<RoundStats> rounds_;

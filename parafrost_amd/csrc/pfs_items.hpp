@@ -293,7 +293,6 @@ PF_HD bool is_tautology(u32 x, CL a, CL b) {
 }
 
 // ---------------------------------------------------------------- SUB (subsume.h)
-constexpr int SUB_MAX_UNITS = 32;
 // DRAT proof events of the per-variable stages: records (elected index,
 // event sequence, kind, payload).  The sequence numbers of one variable are
 // shared with its unit records, so sorting everything by (index, sequence)
@@ -383,18 +382,22 @@ struct SubOut {
   u32* n;         // number of records
   u32 cap;
   u32* unsat_at;  // min elected index whose strengthening refuted the formula
-  u32* overflow;  // set when a variable derives more than SUB_MAX_UNITS units
+  // per elected variable k, 2 words (literal, sequence) per unit it derives
+  // at ubuf + 2 * ubase[k]; ubase is the scan of |poss| + |negs| and every
+  // unit consumes one clause of those lists, so a slice never overflows
+  u32* ubuf;
+  const u32* ubase;
   ProofOut proof;
 };
 
+// state of one variable's SUB run; units live in the variable's slice of
+// SubOut::ubuf rather than in a private array (no scratch memory)
 struct SubLocal {
-  u32 units[SUB_MAX_UNITS];
-  u32 useq[SUB_MAX_UNITS];  // event sequence of each unit
-  u32 n;
-  u32 ev;                   // event counter of this variable
-  bool overflow;
+  u32* ub;   // this variable's unit slice
+  u32 n;     // units derived so far
+  u32 ev;    // event counter of this variable
   bool unsat;
-  u32 k;                    // elected index
+  u32 k;     // elected index
   ProofOut proof;
 };
 // value of a literal seen by one variable: committed root assignment plus
@@ -403,8 +406,8 @@ PF_HD int sub_value(const DB& db, const SubLocal& sl, u32 l) {
   const int v = lit_value(db, l);
   if (v >= 0) return v;
   for (u32 i = 0; i < sl.n; i++) {
-    if (sl.units[i] == l) return 1;
-    if (sl.units[i] == FLIP(l)) return 0;
+    if (sl.ub[2 * i] == l) return 1;
+    if (sl.ub[2 * i] == FLIP(l)) return 0;
   }
   return -1;
 }
@@ -448,12 +451,9 @@ PF_HD void strengthen(const DB& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_
     const u32 unit = l[0];
     const int val = sub_value(db, sl, unit);
     if (val < 0) {
-      if (sl.n < (u32)SUB_MAX_UNITS) {
-        sl.useq[sl.n] = sl.ev++;
-        sl.units[sl.n++] = unit;
-      } else {
-        sl.overflow = true;
-      }
+      sl.ub[2 * sl.n] = unit;
+      sl.ub[2 * sl.n + 1] = sl.ev++;
+      sl.n++;
       sub_local_oblivion(db, p, unit);
     } else if (!val) {
       if (!sl.unsat) push_proof(sl.proof, sl.k, sl.ev++, PR_EMPTY, 0);
@@ -551,9 +551,9 @@ PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& o
   const u32 p = V2L(v), n = NEGL(p);
   if ((int)db.ot_size[p] > o.sub_limit || (int)db.ot_size[n] > o.sub_limit) return;
   SubLocal sl;
+  sl.ub = out.ubuf + 2 * (u64)out.ubase[k];
   sl.n = 0;
   sl.ev = 0;
-  sl.overflow = false;
   sl.unsat = false;
   sl.k = k;
   sl.proof = out.proof;
@@ -583,11 +583,10 @@ PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& o
     const u32 r = at_add(out.n, 1u);
     if (r < out.cap) {
       out.rec[3 * (u64)r] = k;
-      out.rec[3 * (u64)r + 1] = sl.useq[i];
-      out.rec[3 * (u64)r + 2] = sl.units[i];
+      out.rec[3 * (u64)r + 1] = sl.ub[2 * i + 1];
+      out.rec[3 * (u64)r + 2] = sl.ub[2 * i];
     }
   }
-  if (sl.overflow) at_add(out.overflow, 1u);
   if (sl.unsat) at_min(out.unsat_at, k);
 }
 
@@ -866,32 +865,31 @@ PF_HD void freeze_arities(const DB& db, OLV me, OLV other) {
   }
 }
 // xor.h:70-102: flip the literals for the next parity pattern and look for
-// the matching clause (lowest id) in the shortest list
-PF_HD bool make_arity(const DB& db, u32& parity, u32* ls, u32 size) {
-  const u32 old = parity;
+// the matching clause (lowest id) in the shortest list.  The pattern is the
+// base clause with literal k flipped where bit k of 'parity' is set; clauses
+// hold distinct variables in ascending order, so a clause of the same size
+// holds exactly the pattern's literals iff it equals the pattern position by
+// position.
+PF_HD u32 xor_lit(const u32* base, u32 parity, u32 k) { return base[k] ^ ((parity >> k) & 1u); }
+PF_HD bool make_arity(const DB& db, u32& parity, const u32* base, u32 size) {
   do { parity++; } while (__builtin_popcount(parity) & 1);
-  for (u32 k = 0; k < size; k++) {
-    const u32 bit = 1u << k;
-    if ((parity & bit) != (old & bit)) ls[k] = FLIP(ls[k]);
-  }
-  u32 best = ls[0];
+  u32 best = xor_lit(base, parity, 0);
   u32 minsize = db.ot_size[best];
   for (u32 k = 1; k < size; k++) {
-    const u32 s = db.ot_size[ls[k]];
-    if (s < minsize) { minsize = s; best = ls[k]; }
+    const u32 lk = xor_lit(base, parity, k);
+    const u32 s = db.ot_size[lk];
+    if (s < minsize) { minsize = s; best = lk; }
   }
   const OLV ol = olist(db, best);
   u32 found = NOPOS;
   for (u32 i = 0; i < ol.n; i++) {
     const u32 c = ol.r[i];
-    if (!is_original(db, c) || csize(db, c) != size || c >= found) continue;
-    const u32* l = clits(db, c);
+    if (c >= found) continue;
+    const u32* h = chead(db, c);
+    if (!st_original(hdr_flags(h[0])) || hdr_size(h[0]) != size) continue;
+    const u32* l = h + CL_HDR;
     bool all = true;
-    for (u32 q = 0; q < size && all; q++) {
-      bool in = false;
-      for (u32 w = 0; w < size; w++) in |= (l[q] == ls[w]);
-      all = in;
-    }
+    for (u32 q = 0; q < size && all; q++) all = (l[q] == xor_lit(base, parity, q));
     if (all) found = c;
   }
   if (found == NOPOS) return false;
@@ -902,18 +900,16 @@ PF_HD bool find_xor_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
                          u32& nAddedLits) {
   const u32 fx = FLIP(dx), v = ABS(dx);
   const OLV it = olist(db, dx), ot2 = olist(db, fx);
-  u32 ls[32];
   for (u32 i = 0; i < it.n; i++) {
     const u32 ci = it.r[i];
     if (!is_original(db, ci)) continue;
     const u32 size = csize(db, ci);
     const int arity = (int)size - 1;
     if (size < 3 || arity > o.xor_max_arity || size > 32) continue;
-    const u32* l = clits(db, ci);
-    for (u32 q = 0; q < size; q++) ls[q] = l[q];
+    const u32* base = clits(db, ci);
     u32 parity = 0;
     int itargets = 1 << arity;
-    while (--itargets && make_arity(db, parity, ls, size)) {
+    while (--itargets && make_arity(db, parity, base, size)) {
     }
     if (itargets) freeze_arities(db, it, ot2);
     else {

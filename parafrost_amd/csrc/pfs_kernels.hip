@@ -76,6 +76,7 @@ Stream* stream_create(int device) {
   return st;
 }
 void prof_enable(Stream* st, bool on) { st->prof = on; }
+void stream_bind(Stream* st) { PFS_HIP(hipSetDevice(st->device)); }
 void prof_collect(Stream* st, std::vector<KernelTime>& out) {
   PFS_HIP(hipStreamSynchronize(st->s));
   std::map<std::string, size_t> idx;
@@ -893,8 +894,8 @@ __global__ void reduce_lists_k(DB db) {
 void k_reduce_lists(const DB& db, Stream* st) {
   KLAUNCH(reduce_lists_k, grid_for(2 * (size_t)db.nvars, 256), 256, st, db);
 }
-__global__ void prep_k(DB db) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void prep_k(DB db, u32 first) {
+  const u32 c = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= db.ncls) return;
   u32* h = chead(db, c);
   u32* l = h + CL_HDR;
@@ -907,9 +908,19 @@ __global__ void prep_k(DB db) {
   }
   h[1] = calc_sig(l, n);
 }
-void k_prep(const DB& db, Stream* st) {
+void k_prep(const DB& db, u32 first, Stream* st) {
+  if (first >= db.ncls) return;
+  KLAUNCH(prep_k, grid_for(db.ncls - first, 256), 256, st, db, first);
+}
+__global__ void clear_added_k(DB db) {
+  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= db.ncls) return;
+  const u8 f = cflags(db, c);
+  if (f & F_ADDED) set_cflags(db, c, (u8)(f & ~F_ADDED));
+}
+void k_clear_added(const DB& db, Stream* st) {
   if (!db.ncls) return;
-  KLAUNCH(prep_k, grid_for(db.ncls, 256), 256, st, db);
+  KLAUNCH(clear_added_k, grid_for(db.ncls, 256), 256, st, db);
 }
 __global__ void zero_sizes_k(DB db, const u32* lits, u32 n) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;

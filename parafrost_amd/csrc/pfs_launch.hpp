@@ -18,6 +18,7 @@ struct Stream;  // a HIP stream plus reusable pinned staging memory
 Stream* stream_create(int device);
 void stream_destroy(Stream*);
 void stream_sync(Stream*);
+void stream_bind(Stream*);  // make the stream's device current for the calling thread
 void* dev_alloc(size_t bytes);  // throws std::bad_alloc on failure
 void dev_free(void*);
 void h2d(void* dst, const void* src, size_t bytes, Stream*);
@@ -118,8 +119,9 @@ void k_assign_units(const DB& db, const u32* lits, u32 n, Stream*);  // assign +
 void k_pv_lits(const u32* pv, u32 n, u32* lits, Stream*);            // 2 literals per variable
 void k_max_list(const DB& db, const u32* lits, u32 n, u32* out, Stream*);
 void k_reduce_lists(const DB& db, Stream*);  // drop deleted entries from every list
-void k_prep(const DB& db, Stream*);          // sort literals, signature (extraction)
+void k_prep(const DB& db, u32 first, Stream*);  // clauses [first, ncls): sort literals, signature
 void k_zero_sizes(const DB& db, const u32* lits, u32 n, Stream*);
+void k_clear_added(const DB& db, Stream*);  // resolvent marks of an earlier simplify call
 
 // ---- DRAT proof (binary format of src/cpu/proof.cpp:94-175)
 // lines: (kind, payload) pairs, kinds PR_*; stamp: per variable, the line

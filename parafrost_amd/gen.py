@@ -1,10 +1,10 @@
 """Seeded synthetic CNF workloads (host C++ generators in libpfgen.so).
 
 Formulas are returned as ``Cnf`` objects holding numpy arrays in the
-simplifier encoding (literal = 2*v + sign, variables 1..maxvar).  The five
-workload families are the uniform random k-SAT, the Tseitin-encoded random
-circuit, the heavy-tailed k-SAT, batches of the latter two, and BMC-style
-unrollings (see ``parafrost_amd.bmc``).
+simplifier encoding (literal = 2*v + sign, variables 1..maxvar).  The workload
+families are the uniform random k-SAT, the Tseitin-encoded random circuit,
+the heavy-tailed k-SAT, batches of the latter two (``batch``), and BMC-style
+unrollings (``parafrost_amd.bmc``).
 """
 from __future__ import annotations
 
@@ -141,6 +141,31 @@ def from_clauses(maxvar: int, clauses) -> Cnf:
             lits.append(2 * abs(x) + (1 if x < 0 else 0))
         off.append(len(lits))
     return Cnf(maxvar, np.array(off, np.uint64), np.array(lits, np.uint32))
+
+
+def batch(n: int = 512, seed: int = BASE_SEED + 4, cmin: float = 1e3, cmax: float = 5e5) -> list[Cnf]:
+    """The batched workload: n formulas with clause counts log-uniform in
+    [cmin, cmax]; each is, with probability 1/2, a Tseitin circuit scaled to
+    that many clauses (60/30/10% AND/XOR/ITE, satisfiable outputs,
+    root-propagated) or a heavy-tailed k-SAT formula with C/4 variables.
+    Formula i uses seed ^ i."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(cmin), np.log(cmax), n))).astype(np.int64)
+    kinds = rng.integers(0, 2, n)
+    out = []
+    for i in range(n):
+        c = int(sizes[i])
+        s = (seed ^ i) & 0xFFFFFFFFFFFFFFFF
+        if kinds[i] == 0:
+            g = max(100, int(c / 3.4))
+            raw = tseitin(g, max(10, g // 400), max(1, g // 2000), s, sat_outputs=True)
+            cnf, unsat = prepare(raw)
+            if unsat:
+                raise RuntimeError(f"batch formula {i} refuted at the root")
+            out.append(cnf)
+        else:
+            out.append(heavytail(max(100, c // 4), c, seed=s))
+    return out
 
 
 # workload definitions -------------------------------------------------------

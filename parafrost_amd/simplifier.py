@@ -25,7 +25,8 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_result_sizes", "pfs_download_cnf", "pfs_download_units", "pfs_download_vstate",
            "pfs_download_stack", "pfs_proof_size", "pfs_download_proof", "pfs_round_stats_get", "pfs_stage_times", "pfs_stage_name", "pfs_stage_count",
            "pfs_kernel_count", "pfs_kernel_stats", "pfs_memory", "pfs_last_error", "pfs_version",
-           "pfs_device_count")
+           "pfs_device_count", "pfs_batch_simplify", "pfs_append", "pfs_assume", "pfs_server_create",
+           "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy")
 
 
 class PfsError(RuntimeError):
@@ -86,6 +87,15 @@ def load_library():
     L.pfs_last_error.argtypes = [vp]
     L.pfs_last_error.restype = ctypes.c_char_p
     L.pfs_version.restype = ctypes.c_char_p
+    L.pfs_batch_simplify.argtypes = [vp, u32, u32, vp, vp, vp]
+    L.pfs_append.argtypes = [vp, u32, u64, vp, vp]
+    L.pfs_assume.argtypes = [vp, vp, u32]
+    L.pfs_server_create.argtypes = [u32, u64, ctypes.POINTER(vp)]
+    L.pfs_server_submit.argtypes = [vp, vp, ctypes.c_double, u32, u64, vp, vp, vp, u32, ctypes.POINTER(u32)]
+    L.pfs_server_start.argtypes = [vp]
+    L.pfs_server_wait.argtypes = [vp, u32, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.pfs_server_destroy.argtypes = [vp]
     L.pfs_device_count.restype = i32
     _LIB = L
     return L
@@ -167,6 +177,19 @@ class Simplifier:
         np.cumsum(st.sizes, out=off[1:])
         self.upload(st.maxvar, off, st.lits, st.flags, st.lbd, st.vstate, None, None, st.units)
 
+    def append(self, maxvar, offsets, lits) -> None:
+        """Adds original clauses to the resident formula (pfs_append)."""
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        lits = np.ascontiguousarray(lits, np.uint32)
+        self.maxvar = max(self.maxvar, int(maxvar))
+        self._check(self._L.pfs_append(self._h, maxvar, offsets.shape[0] - 1, _ptr(offsets), _ptr(lits)),
+                    "pfs_append")
+
+    def assume(self, variables) -> None:
+        """Replaces the assumed (never eliminated) variables (pfs_assume)."""
+        v = np.ascontiguousarray(variables, np.uint32)
+        self._check(self._L.pfs_assume(self._h, _ptr(v), v.shape[0]), "pfs_assume")
+
     def rewind(self) -> None:
         self._check(self._L.pfs_rewind(self._h), "pfs_rewind")
 
@@ -240,3 +263,72 @@ def simplify_state(st, device: int = 0, **options) -> Result:
         return s.result(status)
     finally:
         s.close()
+
+
+def batch_simplify(simplifiers, workers: int = 8):
+    """Runs simplify() on every Simplifier of the list (each already holding
+    its formula) with 'workers' native threads, one stream per context
+    (pfs_batch_simplify).  Returns (results, per-formula wall ms)."""
+    n = len(simplifiers)
+    if not n:
+        return [], []
+    L = simplifiers[0]._L
+    handles = (ctypes.c_void_p * n)(*[s._h.value for s in simplifiers])
+    res = np.zeros(n, np.int32)
+    sts = np.zeros(n, np.int32)
+    ms = np.zeros(n, np.float64)
+    st = L.pfs_batch_simplify(handles, n, workers, _ptr(res), _ptr(sts), _ptr(ms))
+    if st != PFS_OK:
+        bad = int(np.nonzero(sts != PFS_OK)[0][0])
+        simplifiers[bad]._check(int(sts[bad]), f"pfs_batch_simplify (formula {bad})")
+    return [int(x) for x in res], ms.tolist()
+
+
+class Server:
+    """Request server over incremental sessions (pfs_server_*): each session
+    is a Simplifier whose formula stays resident; a request appends clauses,
+    replaces the assumed variables and simplifies, released at a given time
+    (ms after start()) and served by 'workers' native threads."""
+
+    def __init__(self, workers: int = 8, mem_budget: int = 0):
+        self._L = load_library()
+        self._h = ctypes.c_void_p()
+        st = self._L.pfs_server_create(workers, mem_budget, ctypes.byref(self._h))
+        if st != PFS_OK:
+            raise PfsError(f"pfs_server_create: {STATUS_NAMES.get(st, st)}")
+
+    def submit(self, session: "Simplifier", release_ms: float, maxvar: int, offsets, lits, assumed) -> int:
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        lits = np.ascontiguousarray(lits, np.uint32)
+        assumed = np.ascontiguousarray(assumed, np.uint32)
+        t = ctypes.c_uint32()
+        session.maxvar = max(session.maxvar, int(maxvar))
+        st = self._L.pfs_server_submit(self._h, session._h, float(release_ms), maxvar, offsets.shape[0] - 1,
+                                       _ptr(offsets), _ptr(lits), _ptr(assumed), assumed.shape[0], ctypes.byref(t))
+        if st != PFS_OK:
+            raise PfsError(f"pfs_server_submit: {STATUS_NAMES.get(st, st)}")
+        return t.value
+
+    def start(self) -> None:
+        st = self._L.pfs_server_start(self._h)
+        if st != PFS_OK:
+            raise PfsError(f"pfs_server_start: {STATUS_NAMES.get(st, st)}")
+
+    def wait(self, ticket: int):
+        """-> (status, result, release_ms, start_ms, end_ms)"""
+        r = ctypes.c_int32()
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = self._L.pfs_server_wait(self._h, ticket, ctypes.byref(r), ctypes.byref(a), ctypes.byref(b),
+                                     ctypes.byref(c))
+        return st, r.value, a.value, b.value, c.value
+
+    def close(self) -> None:
+        if self._h:
+            self._L.pfs_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

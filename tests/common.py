@@ -100,3 +100,35 @@ def proof_lines_canonical(proof: bytes):
         out.append(line)
     out.extend(sorted(run))
     return out
+
+
+def append_clean(assign, offsets, lits):
+    """The clause clean-up of pfs_append (and of the reference's incremental
+    itoClause, src/cpu/addinc.cpp:85-160) on the host: literals false under
+    'assign' (dict variable -> true literal) are dropped, satisfied clauses
+    and tautologies skipped, unit clauses assigned in order.  Returns (kept
+    clauses, new unit literals, refuted); 'assign' is updated."""
+    kept, units, refuted = [], [], False
+    off = offsets.tolist()
+    for i in range(len(off) - 1):
+        cl, sat = [], False
+        for x in lits[off[i]:off[i + 1]].tolist():
+            a = assign.get(x >> 1)
+            if a is not None:
+                sat |= a == x
+                continue
+            if x ^ 1 in cl:
+                sat = True
+            if x not in cl:
+                cl.append(x)
+        if sat:
+            continue
+        if not cl:
+            refuted = True
+        elif len(cl) == 1:
+            if (cl[0] >> 1) not in assign:
+                assign[cl[0] >> 1] = cl[0]
+                units.append(cl[0])
+        else:
+            kept.append(cl)
+    return kept, units, refuted
