@@ -543,10 +543,11 @@ bool Simplifier::lcve() {
   fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
   k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
   fill_u32(P<u32>(state_), 0, ncand, st_);
+  fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
   u32 iters = 0;
   while (ncand) {
     fill_u32(sc + 4, 0, 1, st_);
-    k_mis_step(db, P<u32>(cand_), ncand, P<u32>(pos_), P<u32>(state_), sc + 4, st_);
+    k_mis_step(db, P<u32>(cand_), ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), sc + 4, st_);
     iters++;
     if (iters > ncand + 2) throw DeviceFail{"LCVE election did not converge"};
     if (!read_u32(sc + 4)) break;

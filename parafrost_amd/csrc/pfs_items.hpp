@@ -209,10 +209,23 @@ PF_HD void bcp_clean(const DB& db, u32 c) {
 // independent set over the candidate order: a candidate is elected iff no
 // earlier candidate sharing a live clause is elected.  state: 0 undecided,
 // 1 elected, 2 frozen.  Returns 1 if the candidate stays undecided.
-PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* state) {
+// blk[i] remembers an undecided earlier neighbour found by the last scan;
+// while it stays undecided the candidate stays blocked without a rescan.
+// The elected set does not depend on the schedule (the lexicographically
+// first independent set is unique), only the number of iterations does.
+PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* state, u32* blk) {
   if (at_load(&state[i])) return 0;
+  const u32 b = blk[i];
+  if (b != NOPOS) {
+    const u32 sb = at_load(&state[b]);
+    if (sb == 0) return 1;
+    if (sb == 1) {
+      state[i] = 2;
+      return 0;
+    }
+  }
   const u32 v = cand[i];
-  bool blocked = false;
+  u32 blocker = NOPOS;
   for (u32 s = 0; s < 2; s++) {
     const OLV ol = olist(db, V2L(v) | s);
     for (u32 k = 0; k < ol.n; k++) {
@@ -229,11 +242,14 @@ PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* st
           state[i] = 2;
           return 0;
         }
-        if (su == 0) blocked = true;
+        if (su == 0 && blocker == NOPOS) blocker = pu;
       }
     }
   }
-  if (blocked) return 1;
+  if (blocker != NOPOS) {
+    blk[i] = blocker;
+    return 1;
+  }
   state[i] = 1;
   return 0;
 }

@@ -616,15 +616,16 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
   if (!n) return;
   KLAUNCH(lcve_pos_k, grid_for(n, 256), 256, st, cand, n, pos);
 }
-__global__ void mis_step_k(DB db, const u32* cand, u32 n, const u32* pos, u32* state, u32* undecided) {
+__global__ void mis_step_k(DB db, const u32* cand, u32 n, const u32* pos, u32* state, u32* blk,
+                           u32* undecided) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (mis_step(db, i, cand, pos, state)) atomicAdd(undecided, 1u);
+  if (mis_step(db, i, cand, pos, state, blk)) atomicAdd(undecided, 1u);
 }
-void k_mis_step(const DB& db, const u32* cand, u32 n, const u32* pos, u32* state, u32* undecided,
+void k_mis_step(const DB& db, const u32* cand, u32 n, const u32* pos, u32* state, u32* blk, u32* undecided,
                 Stream* st) {
   if (!n) return;
-  KLAUNCH(mis_step_k, grid_for(n, 64), 64, st, db, cand, n, pos, state, undecided);
+  KLAUNCH(mis_step_k, grid_for(n, 64), 64, st, db, cand, n, pos, state, blk, undecided);
 }
 __global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
