@@ -76,6 +76,38 @@ def kernel_bytes(name, rounds, cnf):
     return sum(per) / len(per)
 
 
+def counter_traffic(fetch_csv, write_csv, kernel, copy_bytes=256 << 20):
+    """HBM bytes per launch of 'kernel' from two rocprofv3 --pmc runs of this
+    script with --calibrate-copy (FETCH_SIZE, then WRITE_SIZE; KiB per
+    dispatch).  Each counter is scaled by what it reads for the 256 MiB
+    device copy of --calibrate-copy (the three largest copy dispatches).
+    Returns (bytes per launch, {"fetch_scale": f, "write_scale": w})."""
+    import csv
+    import statistics
+
+    def per_kernel(path, counter):
+        vals, copies = [], []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"]
+                v = float(r["Counter_Value"]) * 1024.0
+                if "copyBuffer" in name or "copy" in name.lower() and "pfs::" not in name:
+                    copies.append(v)
+                if name.startswith(f"pfs::{kernel}("):
+                    vals.append(v)
+        copies.sort()
+        ref = statistics.median(copies[-3:]) if copies else 0.0
+        return (sum(vals) / len(vals) if vals else None), (copy_bytes / ref if ref else None)
+
+    fv, fs = per_kernel(fetch_csv, "FETCH_SIZE")
+    wv, ws = per_kernel(write_csv, "WRITE_SIZE")
+    if fv is None or wv is None or not fs or not ws:
+        return None, None
+    return fv * fs + wv * ws, {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}
+
+
 def calibrate_copy(nbytes=256 << 20, reps=3):
     """Plain device-to-device copies of a known size, so that hardware
     counter runs over this script can scale FETCH_SIZE/WRITE_SIZE."""
@@ -244,6 +276,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="extra simplifier option (launch shaping such as workorder=0)")
+    ap.add_argument("--traffic-csv", default=None, metavar="FETCH.csv,WRITE.csv",
+                    help="rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files of this workload "
+                         "(runs with --calibrate-copy) for roofline.traffic")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -315,8 +350,13 @@ def main():
         avg_s = ms / n / 1e3
         b = kernel_bytes(name, rounds, cnf)
         ach = b / avg_s / 1e9
+        traffic, scales = None, None
+        if args.traffic_csv:
+            fcsv, wcsv = args.traffic_csv.split(",")
+            traffic, scales = counter_traffic(fcsv, wcsv, name)
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": int(traffic) if traffic else None, "traffic_scales": scales,
                     "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(b),
                     "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
     s.close()

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <new>
 #include <stdexcept>
 #include <unordered_map>
@@ -61,7 +63,7 @@ Simplifier::~Simplifier() {
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
                 &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
                 &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
-                &pstamp_, &plines_, &pbytes_, &subu_};
+                &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
@@ -123,12 +125,14 @@ void Simplifier::alloc_state() {
   mem_.ensure(vorg_, V1 * 4, 0, st_);
   mem_.ensure(trail_, (V1 + 1) * 4, 0, st_);
   for (Buf* b : {&score_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_, &def_,
-                 &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_np_, &snap_nn_, &order_})
+                 &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_np_, &snap_nn_, &order_, &sublast_,
+                 &ust_, &ulit_})
     mem_.ensure(*b, (V1 + 1) * 4, 0, st_);
   mem_.ensure(cflag_, V1, 0, st_);
   mem_.ensure(elits_, 2 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(unit_by_, L2 * 4, 0, st_);
   mem_.ensure(recs_, 3 * (V1 + 1) * 4, 0, st_);
+  mem_.ensure(subaff_, 3 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(ids_, (V1 + 1) * 4, 0, st_);
   mem_.ensure(mark_, ccap_, 0, st_);
   const size_t big = std::max<size_t>({V1 + 2, L2 + 1, ccap_});
@@ -585,6 +589,16 @@ void Simplifier::sort_ot() {
 }
 
 // ---------------------------------------------------------------- SUB (elimination.cpp:110-129)
+// Every elected variable runs SUB speculatively on the state before the
+// stage.  Elected variables own disjoint clauses, so a variable's run can
+// only be wrong when an earlier variable derived a unit u while one of its
+// clauses holds u (toblivion(ot[u]) deletes that clause first) or when it
+// met such a unit as a value.  Those variables (found through the unit's
+// list and the clause owners) are restored and run again, seeing the units
+// of earlier variables; their new units can affect later variables in turn,
+// until no run is stale.  The smallest affected variable sees final units
+// only, so every pass finalises at least one variable.  The units are then
+// committed in elected order exactly as the sequential loop enqueues them.
 bool Simplifier::sub() {
   if (!(opt.sub_en || opt.ve_plus_en) || !E_) return true;
   stage_begin(S_SUB);
@@ -605,92 +619,161 @@ bool Simplifier::sub() {
   SubSnap snap{P<u32>(vbase_),    P<u32>(snap_np_),  P<u32>(snap_nn_),  P<u32>(snap_ref_), P<u8>(snap_flg_),
                P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_), P<u32>(unit_by_)};
   k_sub_snapshot(db, P<u32>(pv_), E_, snap, st_);
-  std::vector<u32> committed;
-  std::vector<ProofEv> pev;
-  u32 from = 0, passes = 0;
+
+  // per variable (elected index): its latest units (sequence, literal;
+  // literal 0 = refutation) and proof events
+  std::map<u32, std::vector<std::pair<u32, u32>>> units;
+  std::map<u32, std::vector<ProofEv>> pmap;
+  std::vector<u32> changes;  // (elected index, literal, pass)
   const u32 cap = V_ + 1;
-  bool ok = true;
-  while (true) {
-    if (++passes > E_ + 2) throw DeviceFail{"SUB re-runs did not converge"};
+  auto run = [&](const u32* idx, u32 n, const std::vector<u32>& ran, u32 pass, bool vis) {
+    // vis == false: the first pass (every variable, nothing visible yet)
     fill_u32(sc, 0, 1, st_);
-    fill_u32(sc + 1, INF, 1, st_);
-    fill_u32(sc + 2, 0, 1, st_);
-    SubOut out{P<u32>(recs_), sc, cap, sc + 1, P<u32>(subu_), P<u32>(vbase_), proof_out(2 * O + E_ + 64)};
-    k_sub(db, opt, P<u32>(pv_), from, E_, out, work_order(), st_);
-    u32 h[3];
-    d2h(h, sc, 12, st_);
-    stream_sync(st_);
-    if (h[0] > cap) throw DeviceFail{"SUB unit record buffer overflow"};
-    const u32 nrec = h[0], unsat_at = h[1];
-    if (!nrec && unsat_at == INF) {
-      proof_collect(pev, INF);
-      break;
-    }
+    SubOut out{P<u32>(recs_), sc, cap, P<u32>(subu_), P<u32>(vbase_), proof_out(2 * O + E_ + 64),
+               SubVis{vis ? P<u32>(ust_) : nullptr, vis ? P<u32>(ulit_) : nullptr}};
+    k_sub(db, opt, P<u32>(pv_), idx, n, out, st_);
+    const u32 nrec = read_u32(sc);
+    if (nrec > cap) throw DeviceFail{"SUB unit record buffer overflow"};
     std::vector<u32> rec(3 * (size_t)nrec);
-    d2h(rec.data(), recs_.p, rec.size() * 4, st_);
-    stream_sync(st_);
-    std::vector<u32> idx(nrec);
-    for (u32 i = 0; i < nrec; i++) idx[i] = i;
-    std::sort(idx.begin(), idx.end(), [&](u32 a, u32 b) {
-      return rec[3 * a] != rec[3 * b] ? rec[3 * a] < rec[3 * b] : rec[3 * a + 1] < rec[3 * b + 1];
-    });
-    u32 first_dirty = INF;
     if (nrec) {
-      fill_u32(sc + 3, INF, 1, st_);
-      k_sub_dirty(db, snap, P<u32>(recs_), nrec, sc + 3, st_);
-      first_dirty = read_u32(sc + 3);
+      d2h(rec.data(), recs_.p, rec.size() * 4, st_);
+      stream_sync(st_);
     }
-    proof_collect(pev, first_dirty);
-    // units of the exact prefix, in elected order (strengthen,
-    // elimination.cpp:214-226)
-    std::vector<u32> fresh;
-    for (u32 i : idx) {
-      const u32 k = rec[3 * i], u = rec[3 * i + 2];
-      if (k >= first_dirty) break;
+    std::map<u32, std::vector<std::pair<u32, u32>>> fresh;
+    for (u32 i = 0; i < nrec; i++) fresh[rec[3 * i]].push_back({rec[3 * i + 1], rec[3 * i + 2]});
+    if (!vis) {
+      units = std::move(fresh);
+      for (auto& kv : units)
+        for (auto& e : kv.second)
+          if (e.second) changes.insert(changes.end(), {kv.first, e.second, pass});
+      std::vector<ProofEv> pev;
+      proof_collect(pev, INF);
+      for (const ProofEv& e : pev) pmap[e.k].push_back(e);
+      return;
+    }
+    // unit changes of the variables that ran
+    for (u32 k : ran) {
+      std::vector<u32> before, after;
+      auto it = units.find(k);
+      if (it != units.end())
+        for (auto& e : it->second)
+          if (e.second) before.push_back(e.second);
+      auto jt = fresh.find(k);
+      if (jt != fresh.end())
+        for (auto& e : jt->second)
+          if (e.second) after.push_back(e.second);
+      std::sort(before.begin(), before.end());
+      std::sort(after.begin(), after.end());
+      std::vector<u32> diff;
+      std::set_symmetric_difference(before.begin(), before.end(), after.begin(), after.end(),
+                                    std::back_inserter(diff));
+      for (u32 l : diff) changes.insert(changes.end(), {k, l, pass});
+      if (jt != fresh.end()) units[k] = jt->second;
+      else if (it != units.end()) units.erase(it);
+      pmap.erase(k);
+    }
+    std::vector<ProofEv> pev;
+    proof_collect(pev, INF);
+    for (const ProofEv& e : pev) pmap[e.k].push_back(e);
+  };
+
+  // pass 1: every variable, nothing of this stage visible
+  u32 pass = 1;
+  fill_u32(P<u32>(sublast_), 1, E_, st_);
+  run(work_order(), E_, {}, pass, false);
+  std::vector<u32> vis_set;  // variables with a visible stage unit
+  bool vis_ready = false;
+  while (!changes.empty()) {
+    if (pass > E_ + 2) throw DeviceFail{"SUB re-runs did not converge"};
+    const u32 nch = (u32)(changes.size() / 3);
+    mem_.ensure(ids_, changes.size() * 4, 0, st_);
+    h2d(ids_.p, changes.data(), changes.size() * 4, st_);
+    changes.clear();
+    u32* aff = P<u32>(subaff_);
+    u32* pos = aff + E_;
+    u32* idx = aff + 2 * (size_t)E_;
+    fill_u32(aff, 0, E_, st_);
+    k_sub_affected(db, snap, P<u32>(ids_), nch, P<u32>(sublast_), aff, st_);
+    k_pack_flags(aff, E_, pos, idx, sc + 6, tmp_.p, st_);
+    const u32 m = read_u32(sc + 6);
+    if (!m) break;
+    std::vector<u32> ran(m);
+    d2h(ran.data(), idx, (size_t)m * 4, st_);
+    stream_sync(st_);
+    pass++;
+    // visible stage units: per variable, the earliest deriving index
+    if (!vis_ready) {
+      fill_u32(P<u32>(ust_), INF, (size_t)V_ + 1, st_);
+      vis_ready = true;
+    } else if (!vis_set.empty()) {
+      std::vector<u32> clr;
+      for (u32 x : vis_set) clr.insert(clr.end(), {x, INF, 0});
+      mem_.ensure(tmp3_, clr.size() * 4, 0, st_);
+      h2d(tmp3_.p, clr.data(), clr.size() * 4, st_);
+      k_set_vis(P<u32>(ust_), P<u32>(ulit_), P<u32>(tmp3_), (u32)(clr.size() / 3), st_);
+    }
+    std::map<u32, std::pair<u32, u32>> vis;  // variable -> (index, literal)
+    for (auto& kv : units)
+      for (auto& e : kv.second)
+        if (e.second && !vis.count(ABS(e.second))) vis[ABS(e.second)] = {kv.first, e.second};
+    std::vector<u32> ent;
+    vis_set.clear();
+    for (auto& kv : vis) {
+      ent.insert(ent.end(), {kv.first, kv.second.first, kv.second.second});
+      vis_set.push_back(kv.first);
+    }
+    if (!ent.empty()) {
+      mem_.ensure(tmp3_, ent.size() * 4, 0, st_);
+      h2d(tmp3_.p, ent.data(), ent.size() * 4, st_);
+      k_set_vis(P<u32>(ust_), P<u32>(ulit_), P<u32>(tmp3_), (u32)(ent.size() / 3), st_);
+    }
+    k_sub_restore(db, P<u32>(pv_), idx, m, snap, st_);
+    k_set_at(P<u32>(sublast_), idx, m, pass, st_);
+    run(idx, m, ran, pass, true);
+  }
+
+  // commit the units in elected order (strengthen, elimination.cpp:214-226)
+  std::vector<ProofEv> pev;
+  for (auto& kv : pmap) pev.insert(pev.end(), kv.second.begin(), kv.second.end());
+  std::vector<u32> committed;
+  bool ok = true;
+  for (auto& kv : units) {
+    std::vector<std::pair<u32, u32>> ev = kv.second;
+    std::sort(ev.begin(), ev.end());
+    for (auto& e : ev) {
+      const u32 u = e.second;
+      if (!u) {  // refuted by this variable's own strengthening
+        ok = false;
+        break;
+      }
       const u32 a = h_assign_[ABS(u)];
       if (a) {
         if (a - 1 != u) {
           ok = false;
-          pev.push_back({k, rec[3 * i + 1], PR_EMPTY, 0});
+          pev.push_back({kv.first, e.first, PR_EMPTY, 0});
+          break;
         }
         continue;
       }
-      pev.push_back({k, rec[3 * i + 1], PR_UNIT, u});
+      pev.push_back({kv.first, e.first, PR_UNIT, u});
       h_assign_[ABS(u)] = u + 1;
-      fresh.push_back(u);
+      committed.push_back(u);
       units_out_.push_back(u);
       unassigned_--;
     }
-    if (!ok || unsat_at < first_dirty) {
-      ok = false;
-      break;
-    }
-    commit_units(fresh);
-    committed.insert(committed.end(), fresh.begin(), fresh.end());
-    if (first_dirty == INF) {
-      if (!fresh.empty()) k_oblivion(db, P<u32>(ids_), (u32)fresh.size(), st_);
-      break;
-    }
-    // re-run the suffix after the units of the prefix took effect
-    k_sub_restore(db, P<u32>(pv_), first_dirty, E_, snap, st_);
-    if (!committed.empty()) {
-      mem_.ensure(ids_, committed.size() * 4, 0, st_);
-      h2d(ids_.p, committed.data(), committed.size() * 4, st_);
-      k_oblivion(db, P<u32>(ids_), (u32)committed.size(), st_);
-    }
-    from = first_dirty;
+    if (!ok) break;
   }
   if (ok && !committed.empty()) {
-    // toblivion(ot[unit]) also empties the unit's list
-    mem_.ensure(ids_, committed.size() * 4, 0, st_);
-    h2d(ids_.p, committed.data(), committed.size() * 4, st_);
+    commit_units(committed);
+    // toblivion(ot[unit]) deletes the unit's clauses and empties its list
+    k_oblivion(db, P<u32>(ids_), (u32)committed.size(), st_);
     k_zero_sizes(db, P<u32>(ids_), (u32)committed.size(), st_);
   }
   proof_events(pev);
   proof_flush();
   stream_sync(st_);
   rounds_.back().units += (u32)committed.size();
-  rounds_.back().sub_passes = passes;
+  rounds_.back().sub_passes = pass;
   stage_end();
   return ok;
 }

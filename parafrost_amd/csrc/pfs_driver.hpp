@@ -155,6 +155,7 @@ class Simplifier {
   Buf vsz_, vbase_, type_, def_, vncls_, vnlits_, vnstack_, clsb_, litb_, stkb_;
   Buf snap_ref_, snap_flg_, snap_size_, snap_sig_, snap_lbd_, snap_np_, snap_nn_, scratch_;
   Buf elits_, unit_by_, recs_, ids_, mark_, ht_, erecs_, order_, subu_;
+  Buf subaff_, sublast_, ust_, ulit_;  // SUB re-runs: affected flags, last pass, visible stage units
   // proof
   Buf proof_, precs_, pstamp_, plines_, pbytes_;
   uint64_t proof_n_ = 0;

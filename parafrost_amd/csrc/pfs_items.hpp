@@ -393,17 +393,27 @@ PF_HD void proof_line_write(const DB& db, const u32* lines, u32 i, u8* o) {
   *o = 0;
 }
 
+// Units derived by other elected variables of the same SUB stage, as a
+// variable must see them when it runs after them: ust[x] = elected index of
+// the variable whose unit assigned x (NOPOS: none), ulit[x] = that literal.
+struct SubVis {
+  const u32* ust;  // null: no stage units are visible (the first pass)
+  const u32* ulit;
+};
+
 struct SubOut {
-  u32* rec;       // (elected index, sequence, unit literal) records
+  // (elected index, sequence, literal) records of the units a variable
+  // derives; literal 0 records a refutation at that sequence
+  u32* rec;
   u32* n;         // number of records
   u32 cap;
-  u32* unsat_at;  // min elected index whose strengthening refuted the formula
   // per elected variable k, 2 words (literal, sequence) per unit it derives
   // at ubuf + 2 * ubase[k]; ubase is the scan of |poss| + |negs| and every
   // unit consumes one clause of those lists, so a slice never overflows
   u32* ubuf;
   const u32* ubase;
   ProofOut proof;
+  SubVis vis;
 };
 
 // state of one variable's SUB run; units live in the variable's slice of
@@ -413,14 +423,17 @@ struct SubLocal {
   u32 n;     // units derived so far
   u32 ev;    // event counter of this variable
   bool unsat;
+  u32 unsat_seq;
   u32 k;     // elected index
   ProofOut proof;
+  SubVis vis;
 };
 // value of a literal seen by one variable: committed root assignment plus
 // the units this variable derived so far
 PF_HD int sub_value(const DB& db, const SubLocal& sl, u32 l) {
   const int v = lit_value(db, l);
   if (v >= 0) return v;
+  if (sl.vis.ust && sl.vis.ust[ABS(l)] < sl.k) return sl.vis.ulit[ABS(l)] == l ? 1 : 0;
   for (u32 i = 0; i < sl.n; i++) {
     if (sl.ub[2 * i] == l) return 1;
     if (sl.ub[2 * i] == FLIP(l)) return 0;
@@ -472,7 +485,10 @@ PF_HD void strengthen(const DB& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_
       sl.n++;
       sub_local_oblivion(db, p, unit);
     } else if (!val) {
-      if (!sl.unsat) push_proof(sl.proof, sl.k, sl.ev++, PR_EMPTY, 0);
+      if (!sl.unsat) {
+        sl.unsat_seq = sl.ev++;
+        push_proof(sl.proof, sl.k, sl.unsat_seq, PR_EMPTY, 0);
+      }
       sl.unsat = true;
     }
   } else {
@@ -563,16 +579,42 @@ PF_HD void update_ol(const DB& db, u32 lit) {
 }
 // subsume.h:144-179 + the list-size gate of elimination.cpp:121-124, for
 // elected variable v (index k in elected order)
+PF_HD bool sub_visible_unit(const SubVis& vis, u32 k, CL c) {
+  for (u32 i = 0; i < c.n; i++) {
+    const u32 x = ABS(c.l[i]);
+    if (vis.ust[x] < k && vis.ulit[x] == c.l[i]) return true;
+  }
+  return false;
+}
 PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& out) {
   const u32 p = V2L(v), n = NEGL(p);
-  if ((int)db.ot_size[p] > o.sub_limit || (int)db.ot_size[n] > o.sub_limit) return;
+  u32 np = db.ot_size[p], nn = db.ot_size[n];
+  if (out.vis.ust) {
+    // toblivion(ot[unit]) of the units earlier variables derived: their
+    // clauses are deleted and the unit's own list emptied before this
+    // variable runs (elimination.cpp:214-220, simplify.h:297-303)
+    if (out.vis.ust[v] < k) {
+      if (out.vis.ulit[v] == p) np = 0;
+      else nn = 0;
+    }
+    for (u32 q = 0; q < 2; q++) {
+      const OLV ol = olist(db, p | q);
+      for (u32 i = 0; i < ol.n; i++) {
+        const u32 c = ol.r[i];
+        if (!is_deleted(db, c) && sub_visible_unit(out.vis, k, clause(db, c))) set_deleted(db, c);
+      }
+    }
+  }
+  if ((int)np > o.sub_limit || (int)nn > o.sub_limit) return;
   SubLocal sl;
   sl.ub = out.ubuf + 2 * (u64)out.ubase[k];
   sl.n = 0;
   sl.ev = 0;
   sl.unsat = false;
+  sl.unsat_seq = 0;
   sl.k = k;
   sl.proof = out.proof;
+  sl.vis = out.vis;
   {
     const OLV poss = olist(db, p);
     for (u32 i = 0; i < poss.n; i++) {
@@ -603,7 +645,14 @@ PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& o
       out.rec[3 * (u64)r + 2] = sl.ub[2 * i];
     }
   }
-  if (sl.unsat) at_min(out.unsat_at, k);
+  if (sl.unsat) {
+    const u32 r = at_add(out.n, 1u);
+    if (r < out.cap) {
+      out.rec[3 * (u64)r] = k;
+      out.rec[3 * (u64)r + 1] = sl.unsat_seq;
+      out.rec[3 * (u64)r + 2] = 0;
+    }
+  }
 }
 
 // snapshot / restore of what sub_var may change (list contents and the
@@ -664,13 +713,14 @@ PF_HD void sub_restore_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
     db.ot_size[lit] = n;
   }
 }
-// A variable must be re-run when one of its clauses contains a unit that an
-// earlier elected variable derived: toblivion(ot[unit]) would have deleted
-// that clause first.  The clauses containing the unit are its occurrence
-// list; their owners later than the deriving variable are the ones affected.
-PF_HD u32 sub_dirty_owner(const SubSnap& s, u32 c, u32 k_unit) {
+// A variable must run again when one of its clauses contains a unit that an
+// earlier elected variable derived (toblivion(ot[unit]) deletes that clause
+// before the variable runs) and its last run did not see that unit, or saw
+// one that is gone.  The clauses containing the unit are the unit's list;
+// their owners after the deriving variable are the ones affected.
+PF_HD u32 sub_affected_owner(const SubSnap& s, u32 c, u32 k_unit, const u32* lastrun, u32 pass) {
   const u32 o = s.owner[c];
-  return (o != NOPOS && o > k_unit) ? o : NOPOS;
+  return (o != NOPOS && o > k_unit && lastrun[o] <= pass) ? o : NOPOS;
 }
 
 // ---------------------------------------------------------------- BVE (bve.cpp, gates)

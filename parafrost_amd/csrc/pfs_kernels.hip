@@ -670,16 +670,15 @@ void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Strea
 }
 // 'order' (nullable) permutes the elected indices so that the heaviest
 // variables start first and share wavefronts with variables of similar work
-__global__ void sub_k(DB db, Options o, const u32* pv, u32 from, u32 n, SubOut out, const u32* order) {
+__global__ void sub_k(DB db, Options o, const u32* pv, const u32* idx, u32 n, SubOut out) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (from + t >= n) return;
-  const u32 k = order ? order[t] : from + t;
+  if (t >= n) return;
+  const u32 k = idx ? idx[t] : t;
   sub_var(db, o, pv[k], k, out);
 }
-void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, const u32* order,
-           Stream* st) {
-  if (from >= n) return;
-  KLAUNCH(sub_k, grid_for(n - from, VB), VB, st, db, o, pv, from, n, out, from ? nullptr : order);
+void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, Stream* st) {
+  if (!n) return;
+  KLAUNCH(sub_k, grid_for(n, VB), VB, st, db, o, pv, idx, n, out);
 }
 __global__ void sub_snapshot_k(DB db, const u32* pv, u32 n, SubSnap s) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -689,26 +688,60 @@ void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream
   if (!n) return;
   KLAUNCH(sub_snapshot_k, grid_for(n, VB), VB, st, db, pv, n, s);
 }
-__global__ void sub_restore_k(DB db, const u32* pv, u32 from, u32 n, SubSnap s) {
-  const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) sub_restore_var(db, pv[k], k, s);
+__global__ void sub_restore_k(DB db, const u32* pv, const u32* idx, u32 n, SubSnap s) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) sub_restore_var(db, pv[idx[t]], idx[t], s);
 }
-void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& s, Stream* st) {
-  if (from >= n) return;
-  KLAUNCH(sub_restore_k, grid_for(n - from, VB), VB, st, db, pv, from, n, s);
+void k_sub_restore(const DB& db, const u32* pv, const u32* idx, u32 n, const SubSnap& s, Stream* st) {
+  if (!n) return;
+  KLAUNCH(sub_restore_k, grid_for(n, VB), VB, st, db, pv, idx, n, s);
 }
-// one workgroup per derived unit: the owners of the clauses in its list that
-// come after the deriving variable
-__global__ void sub_dirty_k(DB db, SubSnap s, const u32* rec, u32* first) {
-  const u32 k_unit = rec[3 * (u64)blockIdx.x], u = rec[3 * (u64)blockIdx.x + 2];
+// one workgroup per unit change (elected index, literal, pass): flag the
+// owners of the clauses in the literal's list that come after the deriving
+// variable and last ran no later than that pass
+__global__ void sub_affected_k(DB db, SubSnap s, const u32* ch, const u32* lastrun, u32* aff) {
+  const u32 k_unit = ch[3 * (u64)blockIdx.x], u = ch[3 * (u64)blockIdx.x + 1], pass = ch[3 * (u64)blockIdx.x + 2];
   const OLV ol = olist(db, u);
-  u32 m = NOPOS;
-  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) m = min(m, sub_dirty_owner(s, ol.r[i], k_unit));
-  if (m != NOPOS) atomicMin(first, m);
+  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) {
+    const u32 o = sub_affected_owner(s, ol.r[i], k_unit, lastrun, pass);
+    if (o != NOPOS) aff[o] = 1;
+  }
 }
-void k_sub_dirty(const DB& db, const SubSnap& s, const u32* rec, u32 nrec, u32* first, Stream* st) {
-  if (!nrec) return;
-  KLAUNCH(sub_dirty_k, nrec, 256, st, db, s, rec, first);
+void k_sub_affected(const DB& db, const SubSnap& s, const u32* changes, u32 nch, const u32* lastrun, u32* aff,
+                    Stream* st) {
+  if (!nch) return;
+  KLAUNCH(sub_affected_k, nch, 256, st, db, s, changes, lastrun, aff);
+}
+__global__ void set_at_k(u32* dst, const u32* idx, u32 n, u32 value) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[idx[t]] = value;
+}
+void k_set_at(u32* dst, const u32* idx, u32 n, u32 value, Stream* st) {
+  if (!n) return;
+  KLAUNCH(set_at_k, grid_for(n, 256), 256, st, dst, idx, n, value);
+}
+// triples (variable, elected index, literal): ust[variable] / ulit[variable]
+__global__ void set_vis_k(u32* ust, u32* ulit, const u32* e, u32 n) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  ust[e[3 * t]] = e[3 * t + 1];
+  ulit[e[3 * t]] = e[3 * t + 2];
+}
+void k_set_vis(u32* ust, u32* ulit, const u32* entries, u32 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(set_vis_k, grid_for(n, 256), 256, st, ust, ulit, entries, n);
+}
+__global__ void pack_flags_k(const u32* flags, const u32* pos, u32 n, u32* out) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && flags[t]) out[pos[t]] = t;
+}
+void k_pack_flags(const u32* flags, u32 n, u32* pos, u32* out, u32* count, void* tmp, Stream* st) {
+  if (!n) {
+    fill_u32(count, 0, 1, st);
+    return;
+  }
+  scan_u32(flags, pos, n, count, tmp, st);
+  KLAUNCH(pack_flags_k, grid_for(n, 256), 256, st, flags, pos, n, out);
 }
 __global__ void oblivion_k(DB db, const u32* lits) {
   const u32 t = lits[blockIdx.x];

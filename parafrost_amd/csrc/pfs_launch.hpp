@@ -88,13 +88,19 @@ void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_
 
 // ---- per elected variable stages
 void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Stream*);  // |poss|+|negs|+extra
-void k_sub(const DB& db, const Options& o, const u32* pv, u32 from, u32 n, const SubOut& out, const u32* order,
-           Stream*);
+// SUB over the elected indices idx[0..n) (all of 0..n when idx is null)
+void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, Stream*);
 void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
-void k_sub_restore(const DB& db, const u32* pv, u32 from, u32 n, const SubSnap& s, Stream*);
-// first elected index (> the deriving one) owning a clause that contains a
-// unit of the records (elected index, sequence, literal)
-void k_sub_dirty(const DB& db, const SubSnap& s, const u32* rec, u32 nrec, u32* first, Stream*);
+void k_sub_restore(const DB& db, const u32* pv, const u32* idx, u32 n, const SubSnap& s, Stream*);
+// aff[o] = 1 for every owner o of a clause holding a changed unit (changes:
+// triples elected index, literal, pass) that comes after the deriving
+// variable and last ran (lastrun[o]) no later than the change's pass
+void k_sub_affected(const DB& db, const SubSnap& s, const u32* changes, u32 nch, const u32* lastrun, u32* aff,
+                    Stream*);
+void k_set_at(u32* dst, const u32* idx, u32 n, u32 value, Stream*);  // dst[idx[i]] = value
+void k_set_vis(u32* ust, u32* ulit, const u32* entries, u32 n, Stream*);  // triples variable, index, literal
+// indices of the nonzero flags, in order; pos: n words of scratch
+void k_pack_flags(const u32* flags, u32 n, u32* pos, u32* out, u32* count, void* tmp, Stream*);
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
                  Stream*);
