@@ -88,6 +88,9 @@ void write_pfd(const std::string& path, uint32_t status, uint32_t maxvar,
   fclose(f);
 }
 
+double g_t0 = 0;
+double now_s();
+
 void on_exit_unsat() {
   // The reference kills the process (learnEmpty + killSolver) when a
   // simplification stage proves the formula UNSAT.  Record that outcome.
@@ -96,6 +99,11 @@ void on_exit_unsat() {
   std::vector<uint32_t> none;
   std::vector<uint8_t> nov;
   write_pfd(g_out_path, 0u, g_maxvar, empty, none, none, nov);
+  if (g_t0 > 0) {
+    // the refutation ends the run inside sigmify: report its time too
+    printf("{\"total_s\": %.6f, \"status\": \"unsat\"}\n", now_s() - g_t0);
+    fflush(stdout);
+  }
 }
 
 double now_s() {
@@ -166,6 +174,7 @@ int main(int argc, char** argv) {
   std::map<std::string, double> t;
   auto T = [&](const char* k, double d) { t[k] += d; };
   double t0 = now_s(), a;
+  g_t0 = t0;
   uint32_t status = 2;  // UNSOLVED
   int rounds = 0;
   std::vector<uint32_t> elected_per_round;

@@ -549,12 +549,18 @@ bool Simplifier::lcve() {
   fill_u32(P<u32>(state_), 0, ncand, st_);
   fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
   u32 iters = 0;
-  while (ncand) {
+  // worklists of undecided candidates (the selection buffers are free now)
+  u32* wl[2] = {P<u32>(flag_), P<u32>(score_)};
+  const u32* idx = nullptr;
+  u32 nundec = ncand;
+  while (nundec) {
+    u32* next = wl[iters & 1];
     fill_u32(sc + 4, 0, 1, st_);
-    k_mis_step(db, P<u32>(cand_), ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), sc + 4, st_);
+    k_mis_step(db, P<u32>(cand_), idx, nundec, P<u32>(pos_), P<u32>(state_), P<u32>(key_), next, sc + 4, st_);
     iters++;
     if (iters > ncand + 2) throw DeviceFail{"LCVE election did not converge"};
-    if (!read_u32(sc + 4)) break;
+    nundec = read_u32(sc + 4);
+    idx = next;
   }
   fill_u32(sc, 0, 1, st_);
   fill_u32(sc + 1, ncand, 1, st_);

@@ -616,16 +616,20 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
   if (!n) return;
   KLAUNCH(lcve_pos_k, grid_for(n, 256), 256, st, cand, n, pos);
 }
-__global__ void mis_step_k(DB db, const u32* cand, u32 n, const u32* pos, u32* state, u32* blk,
-                           u32* undecided) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (mis_step(db, i, cand, pos, state, blk)) atomicAdd(undecided, 1u);
+// one election step over the undecided candidates idx[0..n) (all when idx
+// is null); the ones still undecided are appended to 'next'
+__global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
+                           u32* next, u32* nnext) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const u32 i = idx ? idx[t] : t;
+  const bool undecided = mis_step(db, i, cand, pos, state, blk) != 0;
+  wave_append(undecided, i, next, nnext);
 }
-void k_mis_step(const DB& db, const u32* cand, u32 n, const u32* pos, u32* state, u32* blk, u32* undecided,
-                Stream* st) {
+void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
+                u32* next, u32* nnext, Stream* st) {
   if (!n) return;
-  KLAUNCH(mis_step_k, grid_for(n, 64), 64, st, db, cand, n, pos, state, blk, undecided);
+  KLAUNCH(mis_step_k, grid_for(n, 64), 64, st, db, cand, idx, n, pos, state, blk, next, nnext);
 }
 __global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -79,9 +79,11 @@ void k_lcve_classify(const DB& db, const LcveArgs& a, Stream*);
 void k_lcve_select(const DB& db, const LcveArgs& a, u32* flag_tmp, u32* cand, u32* key, u32* ncand,
                    void* tmp, Stream*);
 void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream*);
-// blk: per candidate, an undecided earlier neighbour (NOPOS before the first step)
-void k_mis_step(const DB& db, const u32* cand, u32 n, const u32* pos, u32* state, u32* blk, u32* undecided,
-                Stream*);
+// one election step over the candidates idx[0..n) (all when null); blk: per
+// candidate, an undecided earlier neighbour (NOPOS before the first step);
+// the still undecided candidates are appended to next (count in *nnext)
+void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
+                u32* next, u32* nnext, Stream*);
 void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
 void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_tmp, u32* out,
                    u32* nout, void* tmp, Stream*);
