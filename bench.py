@@ -8,7 +8,9 @@ simplified inside the timed region.  The default workload is the
 Tseitin-encoded random circuit with 2M gates (60% AND, 30% XOR, 10% ITE,
 5,000 inputs, 1,000 unit outputs, root-propagated: about 6.8M clauses).
 
-Two more legs follow (skip with --no-batched / --no-serving):
+Three more legs follow (skip with --no-heavytail / --no-batched / --no-serving):
+* heavytail: the 10M-variable heavy-tailed formula (40M clauses), time per
+  simplify call;
 * batched: 512 formulas (log-uniform 1k-500k clauses, Tseitin or
   heavy-tailed) resident in HBM, simplified as a loop of single-formula
   calls and by the native batch executor; formulas/s of both, per-formula
@@ -160,6 +162,36 @@ def cpu_baseline(cnf, name, budget_clauses=7_500_000):
             "sample": note + f"; oracle restatement round time {r.seconds:.3f} s", "seconds": r.seconds}
 
 
+def bench_single(name, steps=3, warmup=1):
+    """Another single-formula workload (default: the 10M-variable heavy-tailed
+    formula, 40M clauses): ms per simplify call with inputs resident."""
+    from parafrost_amd import gen
+    from parafrost_amd.simplifier import Simplifier
+    t0 = time.time()
+    cnf = gen.workload(name)
+    s = Simplifier(0, keep_input=True, **OPTIONS)
+    s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
+    print(f"bench: {name}: {cnf.maxvar} vars, {cnf.nclauses} clauses ({time.time() - t0:.1f}s to generate "
+          "and upload)", file=sys.stderr, flush=True)
+    status = None
+    for _ in range(warmup):
+        s.rewind()
+        status = s.simplify()
+    t = time.perf_counter()
+    for _ in range(steps):
+        s.rewind()
+        status = s.simplify()
+    dt = (time.perf_counter() - t) / steps
+    r = s.result(status).rounds
+    s.close()
+    out = {"workload": name, "variables": cnf.maxvar, "clauses": cnf.nclauses, "literals": cnf.nlits,
+           "ms_per_step": round(dt * 1e3, 3), "clauses_per_s": round(cnf.nclauses / dt, 1),
+           "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "rounds": len(r),
+           "sub_passes": [x["sub_passes"] for x in r]}
+    print("bench: " + json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
 def bench_batched(n=512, workers=16, check=16):
     """Batched workload: n formulas (clause counts log-uniform in 1k-500k,
     Tseitin or heavy-tailed) resident in HBM, simplified once as a loop of
@@ -279,6 +311,7 @@ def main():
     ap.add_argument("--traffic-csv", default=None, metavar="FETCH.csv,WRITE.csv",
                     help="rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files of this workload "
                          "(runs with --calibrate-copy) for roofline.traffic")
+    ap.add_argument("--no-heavytail", action="store_true", help="skip the heavy-tailed single-formula leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -361,6 +394,7 @@ def main():
                     "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
     s.close()
 
+    heavy = None if (args.no_heavytail or args.workload == "heavytail") else bench_single("heavytail")
     batched = None if args.no_batched else bench_batched()
     serving = None if args.no_serving else bench_serving()
 
@@ -389,6 +423,7 @@ def main():
                    "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": base,
+        "heavytail": heavy,
         "batched": batched,
         "serving": serving,
     }

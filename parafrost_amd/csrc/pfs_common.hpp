@@ -86,76 +86,24 @@ struct DB {
   u32* vorg;      // per variable: original numbering
   u32 nvars;
   u32 ncls;
-  // Per-thread clause cache (null: none).  A kernel that re-reads the same
-  // few clauses many times (the gate searches and resolvent counts of one
-  // elected variable) points this at its own slice of LDS: CACHE_SLOTS
-  // slots of CACHE_W words (clause id, header, literals), filled on first
-  // use and never evicted.  Flag and signature writes go to memory and to
-  // the cached copy; literal writes are not allowed while it is in use.
-  u32* cache;
 };
-constexpr u32 CACHE_W = 8;  // id + 2 header words + up to 5 literals
 
 constexpr u32 CL_HDR = 2;                 // header words per clause
 constexpr u32 CL_MAX_SIZE = 0xFFFFFFu;     // largest clause the header holds
 PF_HD u32 make_hdr(u32 size, u8 flags) { return size | ((u32)flags << 24); }
 PF_HD u32 hdr_size(u32 h) { return h & CL_MAX_SIZE; }
 PF_HD u8 hdr_flags(u32 h) { return (u8)(h >> 24); }
-PF_HD u32* chead_mem(const DB& db, u32 c) { return db.lits + db.c_off[c]; }
-#ifndef CACHE_SLOTS
-#define CACHE_SLOTS 8u
-#endif
-constexpr u32 CACHE_PROBES = 4;
-// Open addressing without eviction: a clause keeps its slot for the whole
-// use of the cache, so pointers into it stay valid while other clauses are
-// looked up.  Returns the slot holding c, or the first free slot of its
-// probe sequence (its id word is NOPOS), or null.
-PF_HD u32* cache_find(const DB& db, u32 c) {
-  u32 s = (c * 0x9E3779B1u) >> 16;
-  for (u32 p = 0; p < CACHE_PROBES; p++, s = (s + 1) & (CACHE_SLOTS - 1)) {
-    u32* e = db.cache + (s & (CACHE_SLOTS - 1)) * CACHE_W;
-    if (e[0] == c || e[0] == NOPOS) return e;
-  }
-  return nullptr;
-}
-PF_HD u32* chead(const DB& db, u32 c) {
-  if (!db.cache) return chead_mem(db, c);
-  u32* e = cache_find(db, c);
-  if (e && e[0] == c) return e + 1;
-  const u32* g = chead_mem(db, c);
-  if (!e) return (u32*)g;  // probe sequence full: served from memory
-  const u32 h = g[0];
-  const u32 n = hdr_size(h);
-  if (n + 3 > CACHE_W) return (u32*)g;  // too long: served from memory
-  e[0] = c;
-  e[1] = h;
-  for (u32 i = 1; i < n + 2; i++) e[1 + i] = g[i];
-  return e + 1;
-}
+PF_HD u32* chead(const DB& db, u32 c) { return db.lits + db.c_off[c]; }
 PF_HD u32 csize(const DB& db, u32 c) { return hdr_size(chead(db, c)[0]); }
 PF_HD u8 cflags(const DB& db, u32 c) { return hdr_flags(chead(db, c)[0]); }
 PF_HD u32 csig(const DB& db, u32 c) { return chead(db, c)[1]; }
 PF_HD u32* clits(const DB& db, u32 c) { return chead(db, c) + CL_HDR; }
-// writes go to memory; a cached copy is updated as well
-PF_HD void set_cflags(const DB& db, u32 c, u8 f) {
-  ((u8*)chead_mem(db, c))[3] = f;
-  if (db.cache) {
-    u32* e = cache_find(db, c);
-    if (e && e[0] == c) ((u8*)(e + 1))[3] = f;
-  }
-}
-// size changes rewrite the literals too: not allowed while a cache is in use
+PF_HD void set_cflags(const DB& db, u32 c, u8 f) { ((u8*)chead(db, c))[3] = f; }
 PF_HD void set_csize(const DB& db, u32 c, u32 n) {
-  u32* h = chead_mem(db, c);
+  u32* h = chead(db, c);
   ((uint16_t*)h)[0] = (uint16_t)n;
   ((u8*)h)[2] = (u8)(n >> 16);
 }
-PF_HD void set_csig(const DB& db, u32 c, u32 sig) {
-  chead_mem(db, c)[1] = sig;
-  if (db.cache) {
-    u32* e = cache_find(db, c);
-    if (e && e[0] == c) e[2] = sig;
-  }
-}
+PF_HD void set_csig(const DB& db, u32 c, u32 sig) { chead(db, c)[1] = sig; }
 
 }  // namespace pfs

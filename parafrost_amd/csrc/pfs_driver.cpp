@@ -81,7 +81,6 @@ DB Simplifier::make_db(int w) {
   db.vstate = P<u8>(vstate_);
   db.assumed = P<u8>(assumed_);
   db.vorg = P<u32>(vorg_);
-  db.cache = nullptr;
   db.nvars = V_;
   db.ncls = (u32)ncls_;
   return db;

@@ -756,29 +756,11 @@ void k_oblivion(const DB& db, const u32* lits, u32 n, Stream* st) {
   if (!n) return;
   KLAUNCH(oblivion_k, n, 256, st, db, lits);
 }
-// The gate searches and resolvent counts of one variable re-read the same
-// few clauses many times; each thread keeps them in its own LDS clause cache
-// (without it they are fetched from HBM again and again: the per-CU caches
-// cannot hold the clauses of all resident threads).
-#ifndef VE_CACHE
-#define VE_CACHE 1
-#endif
-__global__ void __launch_bounds__(VB) ve_decide_k(DB db, Options o, const u32* pv, u32 n, VeDecision d,
-                                                  const u32* order) {
-#if VE_CACHE
-  __shared__ u32 cache[VB * CACHE_SLOTS * CACHE_W];
-#endif
+__global__ void ve_decide_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const u32 k = order ? order[t] : t;
-#if VE_CACHE
-  DB dbl = db;
-  dbl.cache = cache + threadIdx.x * CACHE_SLOTS * CACHE_W;
-  for (u32 s = 0; s < CACHE_SLOTS; s++) dbl.cache[s * CACHE_W] = NOPOS;
-  ve_decide(dbl, o, pv[k], k, d);
-#else
   ve_decide(db, o, pv[k], k, d);
-#endif
 }
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
                  Stream* st) {
