@@ -66,20 +66,26 @@ struct Options {
 // assignments are kept per variable (assign[v] = true literal + 1) so that
 // parallel unit propagation can claim a variable with one compare-and-swap.
 //
-// Clause c lives in the word pool 'lits' at c_off[c]: a two-word header
-// followed by its literals, so one clause visit reads one contiguous run:
+// A clause lives in the word pool 'lits' as a three-word header followed by
+// its literals; one clause visit reads one contiguous run:
 //   word 0: size (bits 0-23) | flags (bits 24-31: status, molten, added, usage)
 //   word 1: signature (OR of 1 << (lit & 31))
-//   word 2..: literals, sorted ascending
-// The flags byte and the size bytes are written with separate byte/short
-// stores, so a flag update never rewrites the size and vice versa.
+//   word 2: LBD (learnt clauses)
+//   word 3..: literals, sorted ascending
+// A clause is named by its handle, the pool offset of its header (the
+// reference's S_REF, src/gpu/datatypes.hpp:35): occurrence lists hold
+// handles, so a visit costs no extra indirection.  Handles grow with the
+// clause order (clauses are laid out in order; resolvents are appended in
+// elected order; compaction is stable), so comparing handles compares
+// clause positions.  c_off lists the handles in clause order for the
+// clause-parallel kernels.  The flags byte and the size bytes are written
+// with separate byte/short stores, so a flag update never rewrites the size.
 struct DB {
   u32* lits;      // word pool: headers and literals
-  u32* c_off;     // per clause: pool offset of its header
-  u32* c_lbd;     // per clause: LBD of learnt clauses
+  u32* c_off;     // clause index -> handle
   u32* ot_off;
   u32* ot_size;
-  u32* ot_refs;
+  u32* ot_refs;   // clause handles
   u32* assign;    // per variable: 0 unassigned, else the true literal + 1
   u8* vstate;     // per variable
   u8* assumed;    // per variable
@@ -88,22 +94,24 @@ struct DB {
   u32 ncls;
 };
 
-constexpr u32 CL_HDR = 2;                 // header words per clause
+constexpr u32 CL_HDR = 3;                 // header words per clause
 constexpr u32 CL_MAX_SIZE = 0xFFFFFFu;     // largest clause the header holds
 PF_HD u32 make_hdr(u32 size, u8 flags) { return size | ((u32)flags << 24); }
 PF_HD u32 hdr_size(u32 h) { return h & CL_MAX_SIZE; }
 PF_HD u8 hdr_flags(u32 h) { return (u8)(h >> 24); }
-PF_HD u32* chead(const DB& db, u32 c) { return db.lits + db.c_off[c]; }
-PF_HD u32 csize(const DB& db, u32 c) { return hdr_size(chead(db, c)[0]); }
-PF_HD u8 cflags(const DB& db, u32 c) { return hdr_flags(chead(db, c)[0]); }
-PF_HD u32 csig(const DB& db, u32 c) { return chead(db, c)[1]; }
-PF_HD u32* clits(const DB& db, u32 c) { return chead(db, c) + CL_HDR; }
-PF_HD void set_cflags(const DB& db, u32 c, u8 f) { ((u8*)chead(db, c))[3] = f; }
+PF_HD u32* chead(const DB& db, u32 c) { return db.lits + c; }
+PF_HD u32 csize(const DB& db, u32 c) { return hdr_size(db.lits[c]); }
+PF_HD u8 cflags(const DB& db, u32 c) { return hdr_flags(db.lits[c]); }
+PF_HD u32 csig(const DB& db, u32 c) { return db.lits[c + 1]; }
+PF_HD u32 clbd(const DB& db, u32 c) { return db.lits[c + 2]; }
+PF_HD u32* clits(const DB& db, u32 c) { return db.lits + c + CL_HDR; }
+PF_HD void set_cflags(const DB& db, u32 c, u8 f) { ((u8*)(db.lits + c))[3] = f; }
 PF_HD void set_csize(const DB& db, u32 c, u32 n) {
-  u32* h = chead(db, c);
+  u32* h = db.lits + c;
   ((uint16_t*)h)[0] = (uint16_t)n;
   ((u8*)h)[2] = (u8)(n >> 16);
 }
-PF_HD void set_csig(const DB& db, u32 c, u32 sig) { chead(db, c)[1] = sig; }
+PF_HD void set_csig(const DB& db, u32 c, u32 sig) { db.lits[c + 1] = sig; }
+PF_HD void set_clbd(const DB& db, u32 c, u32 lbd) { db.lits[c + 2] = lbd; }
 
 }  // namespace pfs

@@ -56,13 +56,13 @@ Simplifier::Simplifier(int device, size_t budget, bool keep_input)
 }
 
 Simplifier::~Simplifier() {
-  Buf* all[] = {&c_off_[0], &c_lbd_[0], &lits_[0], &c_off_[1], &c_lbd_[1], &lits_[1],
+  Buf* all[] = {&c_off_[0], &lits_[0], &c_off_[1], &lits_[1],
                 &ot_off_, &ot_size_, &ot_refs_, &hist_, &assign_, &vstate_, &assumed_, &vorg_, &trail_,
                 &score_, &cflag_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_,
                 &def_, &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_ref_, &snap_flg_,
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
                 &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
-                &in_c_lbd_, &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
+                &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
                 &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
@@ -73,7 +73,6 @@ DB Simplifier::make_db(int w) {
   DB db;
   db.lits = P<u32>(lits_[w]);
   db.c_off = P<u32>(c_off_[w]);
-  db.c_lbd = P<u32>(c_lbd_[w]);
   db.ot_off = P<u32>(ot_off_);
   db.ot_size = P<u32>(ot_size_);
   db.ot_refs = P<u32>(ot_refs_);
@@ -112,7 +111,6 @@ void Simplifier::alloc_state() {
   const size_t V1 = (size_t)V_ + 1, L2 = 2 * (size_t)V_ + 3;
   for (int w = 0; w < 2; w++) {
     mem_.ensure(c_off_[w], ccap_ * 4, 0, st_);
-    mem_.ensure(c_lbd_[w], ccap_ * 4, 0, st_);
     mem_.ensure(lits_[w], lcap_ * 4, 0, st_);
   }
   mem_.ensure(ot_off_, (L2 + 1) * 4, 0, st_);
@@ -134,7 +132,7 @@ void Simplifier::alloc_state() {
   mem_.ensure(recs_, 3 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(subaff_, 3 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(ids_, (V1 + 1) * 4, 0, st_);
-  mem_.ensure(mark_, ccap_, 0, st_);
+  mem_.ensure(mark_, lcap_, 0, st_);  // per clause handle
   const size_t big = std::max<size_t>({V1 + 2, L2 + 1, ccap_});
   mem_.ensure(tmp_, std::max(scan_tmp_bytes(big), radix_tmp_bytes(V1 + 1)), 0, st_);
   mem_.ensure(scal_, 256, 0, st_);
@@ -157,7 +155,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   // pool words: a header per clause, then its literals
   const uint64_t L = offsets[ncls] - offsets[0] + CL_HDR * ncls;
   if (ncls >= INF / 2 || L >= INF / 4) throw AllocFail{"formula too large for 32-bit clause ids"};
-  std::vector<u32> h_off(ncls), h_lbd(ncls), h_lits(L);
+  std::vector<u32> h_off(ncls), h_lits(L);
   uint64_t w = 0;
   for (uint64_t k = 0; k < ncls; k++) {
     const uint64_t i = order[k];
@@ -168,7 +166,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     const bool learnt = f & 1;
     h_lits[w] = make_hdr((u32)n, learnt ? (u8)(ST_LEARNT | (f & USAGE_MASK)) : (u8)ST_ORIGINAL);
     h_lits[w + 1] = 0;  // signature: set by the literal sort on the device
-    h_lbd[k] = learnt && lbd ? (lbd[i] & 0x03FFFFFFu) : 0u;
+    h_lits[w + 2] = learnt && lbd ? (lbd[i] & 0x03FFFFFFu) : 0u;
     for (uint64_t j = 0; j < n; j++) {
       const u32 l = lits[offsets[i] + j];
       if (ABS(l) == 0 || ABS(l) > maxvar) throw std::invalid_argument("literal out of range");
@@ -183,7 +181,6 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   ncls_ = ncls;
   lits_end_ = L;
   h2d(c_off_[0].p, h_off.data(), ncls * 4, st_);
-  h2d(c_lbd_[0].p, h_lbd.data(), ncls * 4, st_);
   h2d(lits_[0].p, h_lits.data(), L * 4, st_);
   // per-variable state
   h_assign_.assign((size_t)V_ + 1, 0);
@@ -210,7 +207,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   units_out_.clear();
   refuted_ = false;
   resident_ = false;
-  fill_u8(P<u8>(mark_), 0, ccap_, st_);
+  fill_u8(P<u8>(mark_), 0, lcap_, st_);
   // extract: sort literals, signatures
   k_prep(make_db(0), 0, st_);
   if (keep_input_) {
@@ -219,13 +216,11 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     in_trail_ = ntrail;
     in_unassigned_ = unassigned_;
     mem_.ensure(in_c_off_, ncls * 4, 0, st_);
-    mem_.ensure(in_c_lbd_, ncls * 4, 0, st_);
     mem_.ensure(in_lits_, L * 4, 0, st_);
     mem_.ensure(in_assign_, ((size_t)V_ + 1) * 4, 0, st_);
     mem_.ensure(in_vstate_, (size_t)V_ + 1, 0, st_);
     mem_.ensure(in_trail_buf_, ((size_t)ntrail + 1) * 4, 0, st_);
     d2d(in_c_off_.p, c_off_[0].p, ncls * 4, st_);
-    d2d(in_c_lbd_.p, c_lbd_[0].p, ncls * 4, st_);
     d2d(in_lits_.p, lits_[0].p, L * 4, st_);
     d2d(in_assign_.p, assign_.p, ((size_t)V_ + 1) * 4, st_);
     d2d(in_vstate_.p, vstate_.p, (size_t)V_ + 1, st_);
@@ -241,7 +236,6 @@ void Simplifier::rewind() {
   lits_end_ = in_nlits_;
   ensure_clause_room(0, 0);
   d2d(c_off_[0].p, in_c_off_.p, in_ncls_ * 4, st_);
-  d2d(c_lbd_[0].p, in_c_lbd_.p, in_ncls_ * 4, st_);
   d2d(lits_[0].p, in_lits_.p, in_nlits_ * 4, st_);
   d2d(assign_.p, in_assign_.p, ((size_t)V_ + 1) * 4, st_);
   d2d(vstate_.p, in_vstate_.p, (size_t)V_ + 1, st_);
@@ -301,7 +295,8 @@ void Simplifier::append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     if (cl.size() > CL_MAX_SIZE) throw std::invalid_argument("clause longer than 2^24-1 literals");
     off.push_back((u32)w);
     words.push_back(make_hdr((u32)cl.size(), ST_ORIGINAL));
-    words.push_back(0);
+    words.push_back(0);  // signature: set by the literal sort
+    words.push_back(0);  // lbd
     words.insert(words.end(), cl.begin(), cl.end());
     w += CL_HDR + cl.size();
   }
@@ -310,7 +305,6 @@ void Simplifier::append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     ensure_clause_room(nadd, words.size());
     DB db0 = make_db(cur_);
     h2d(db0.c_off + ncls_, off.data(), nadd * 4, st_);
-    fill_u32(db0.c_lbd + ncls_, 0, nadd, st_);
     h2d(db0.lits + lits_end_, words.data(), words.size() * 4, st_);
     const u32 first = (u32)ncls_;
     ncls_ += nadd;
@@ -378,15 +372,14 @@ void Simplifier::ensure_clause_room(uint64_t add_cls, uint64_t add_lits) {
   for (int w = 0; w < 2; w++) {
     const bool keep = (w == cur_);
     mem_.ensure(c_off_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
-    mem_.ensure(c_lbd_[w], nc * 4, keep ? ncls_ * 4 : 0, st_);
     mem_.ensure(lits_[w], nl * 4, keep ? lits_end_ * 4 : 0, st_);
   }
   // the occurrence table is rebuilt every round; its refs array only needs
   // the new capacity from the next round on, but it is resized here too
   mem_.ensure(ot_refs_, nl * 4, lits_end_ * 4, st_);
   const size_t old_mark = mark_.cap;
-  mem_.ensure(mark_, nc, old_mark, st_);
-  if (nc > old_mark) fill_u8(P<u8>(mark_) + old_mark, 0, nc - old_mark, st_);
+  mem_.ensure(mark_, nl, old_mark, st_);
+  if (nl > old_mark) fill_u8(P<u8>(mark_) + old_mark, 0, nl - old_mark, st_);
   mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(nc)), 0, st_);
   ccap_ = nc;
   lcap_ = nl;
@@ -620,8 +613,8 @@ bool Simplifier::sub() {
   mem_.ensure(snap_sig_, ob * 4, 0, st_);
   mem_.ensure(snap_lbd_, ob * 4, 0, st_);
   mem_.ensure(subu_, ob * 8, 0, st_);
-  mem_.ensure(unit_by_, std::max<size_t>(ncls_, 2 * (size_t)V_ + 3) * 4, 0, st_);
-  fill_u32(P<u32>(unit_by_), INF, ncls_, st_);  // clause owners
+  mem_.ensure(unit_by_, std::max<size_t>(lits_end_, 2 * (size_t)V_ + 3) * 4, 0, st_);
+  fill_u32(P<u32>(unit_by_), INF, lits_end_, st_);  // clause owners, per handle
   SubSnap snap{P<u32>(vbase_),    P<u32>(snap_np_),  P<u32>(snap_nn_),  P<u32>(snap_ref_), P<u8>(snap_flg_),
                P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_), P<u32>(unit_by_)};
   k_sub_snapshot(db, P<u32>(pv_), E_, snap, st_);
@@ -999,7 +992,7 @@ void Simplifier::ere_per_var() {
   DB db = make_db(cur_);
   u32* sc = P<u32>(scal_);
   const u32 cap = V_ + 1;
-  mem_.ensure(unit_by_, std::max<size_t>(ncls_, 2 * (size_t)V_ + 3) * 4, 0, st_);
+  mem_.ensure(unit_by_, std::max<size_t>(lits_end_, 2 * (size_t)V_ + 3) * 4, 0, st_);
   u32 from = 0, passes = 0;
   std::vector<ProofEv> pev;
   while (true) {
@@ -1029,7 +1022,7 @@ void Simplifier::ere_per_var() {
       const u32 k = rec[3 * i], c = rec[3 * i + 2];
       if (first_by[c] < k) first_dirty = std::min(first_dirty, k);
     }
-    fill_u32(P<u32>(unit_by_), INF, ncls_, st_);
+    fill_u32(P<u32>(unit_by_), INF, lits_end_, st_);  // per handle
     k_del_by(P<u32>(recs_), nrec, P<u32>(unit_by_), st_);
     fill_u32(sc + 2, INF, 1, st_);
     k_ere_dirty(db, P<u32>(pv_), from, E_, P<u32>(unit_by_), sc + 2, st_);
@@ -1211,12 +1204,11 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
 
 // ---------------------------------------------------------------- results
 void Simplifier::download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd) {
-  // after the final compaction the pool holds every clause in id order:
+  // after the final compaction the pool holds every clause in order:
   // header words, then literals
   const size_t n = ncls_;
-  std::vector<u32> off(n), lb(n), pool(lits_end_);
+  std::vector<u32> off(n), pool(lits_end_);
   d2h(off.data(), c_off_[cur_].p, n * 4, st_);
-  d2h(lb.data(), c_lbd_[cur_].p, n * 4, st_);
   d2h(pool.data(), lits_[cur_].p, lits_end_ * 4, st_);
   stream_sync(st_);
   uint64_t a = 0;
@@ -1227,7 +1219,7 @@ void Simplifier::download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags,
     if (offsets) offsets[i] = a;
     if (lits) std::copy(h + CL_HDR, h + CL_HDR + sz, lits + a);
     if (flags) flags[i] = (u8)((st_learnt(fl) ? 1 : 0) | ((fl & F_ADDED) ? 2 : 0) | (fl & USAGE_MASK));
-    if (lbd) lbd[i] = st_learnt(fl) ? lb[i] : 0;
+    if (lbd) lbd[i] = st_learnt(fl) ? h[2] : 0;
     a += sz;
   }
   if (offsets) offsets[n] = a;

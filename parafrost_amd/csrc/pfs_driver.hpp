@@ -142,7 +142,7 @@ class Simplifier {
   uint64_t ncls_ = 0, lits_end_ = 0, ccap_ = 0, lcap_ = 0;  // lits_end_, lcap_: pool words
   int cur_ = 0;  // which clause block is live
   // clause blocks (dynamic region)
-  Buf c_off_[2], c_lbd_[2], lits_[2];  // lits_: word pool of headers + literals
+  Buf c_off_[2], lits_[2];  // lits_: word pool of headers + literals; c_off_: handles in clause order
   // occurrence table
   Buf ot_off_, ot_size_, ot_refs_, hist_;
   // per variable
@@ -167,7 +167,7 @@ class Simplifier {
   // temporaries
   Buf tmp_, tmp2_, tmp3_, scal_;
   // pristine input copy
-  Buf in_c_off_, in_c_lbd_, in_lits_, in_assign_, in_vstate_;
+  Buf in_c_off_, in_lits_, in_assign_, in_vstate_;
   uint64_t in_ncls_ = 0, in_nlits_ = 0, in_trail_ = 0;
   uint32_t in_unassigned_ = 0;
   Buf in_trail_buf_;

@@ -457,12 +457,12 @@ PF_HD void sub_local_oblivion(const DB& db, u32 p, u32 unit) {
   }
 }
 PF_HD void bump_shrunken(const DB& db, u32 c, int lbd_tier1) {  // elimination.cpp:28-39
-  const int old = (int)db.c_lbd[c];
+  const int old = (int)clbd(db, c);
   if (old <= lbd_tier1) return;
   int nl = (int)csize(db, c) - 1;
   if (nl > old) nl = old;
   if (nl >= old) return;
-  db.c_lbd[c] = (u32)nl;
+  set_clbd(db, c, (u32)nl);
   set_cflags(db, c, (u8)((cflags(db, c) & ~USAGE_MASK) | (1u << USAGE_SHIFT)));
 }
 // elimination.cpp:199-234
@@ -681,7 +681,7 @@ PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
       s.flg[e] = cflags(db, c);
       s.size[e] = csize(db, c);
       s.sig[e] = csig(db, c);
-      s.lbd[e] = db.c_lbd[c];
+      s.lbd[e] = clbd(db, c);
     }
   }
 }
@@ -696,7 +696,7 @@ PF_HD void sub_restore_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
       r[j] = c;
       set_cflags(db, c, s.flg[e]);
       set_csig(db, c, s.sig[e]);
-      db.c_lbd[c] = s.lbd[e];
+      set_clbd(db, c, s.lbd[e]);
       const u32 cur = csize(db, c);
       if (cur < s.size[e]) {
         // the only literal SUB removes from a clause of this list is 'lit'
@@ -1196,9 +1196,9 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
           merge_out(v, A, B, h + CL_HDR);
           h[0] = make_hdr(len, F_ADDED);
           h[1] = calc_sig(h + CL_HDR, len);
+          h[2] = 0;
           db.c_off[cid] = lpos;
-          db.c_lbd[cid] = 0;
-          push_proof(a.proof, k, seq++, PR_ADD, cid);
+          push_proof(a.proof, k, seq++, PR_ADD, lpos);
           cid++;
           lpos += CL_HDR + len;
         }

@@ -357,13 +357,15 @@ void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Strea
 }
 
 // ---------------------------------------------------------------- OT
+// clause-parallel kernels run over clause indices and work on the handles
+// c_off[i]
 __global__ void hist_k(DB db, u32* hist) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < db.ncls) hist_clause(db, c, hist);
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) hist_clause(db, db.c_off[i], hist);
 }
 __global__ void scatter_k(DB db, u32* cursor) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < db.ncls) scatter_clause(db, c, cursor);
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) scatter_clause(db, db.c_off[i], cursor);
 }
 void k_hist(const DB& db, u32* hist, Stream* st) {
   if (!db.ncls) return;
@@ -540,9 +542,12 @@ void k_bcp_mark(const DB& db, const u32* lits, u32 n, u8* mark, Stream* st) {
   if (!n) return;
   KLAUNCH(bcp_mark_k, n, 256, st, db, lits, mark);
 }
+// mark: per handle
 __global__ void bcp_clean_k(DB db, u8* mark) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= db.ncls || !mark[c]) return;
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= db.ncls) return;
+  const u32 c = db.c_off[i];
+  if (!mark[c]) return;
   mark[c] = 0;
   bcp_clean(db, c);
 }
@@ -843,8 +848,8 @@ void k_apply_deletions(const DB& db, const u32* ids, u32 n, Stream* st) {
   KLAUNCH(apply_del_k, grid_for(n, 256), 256, st, db, ids, n);
 }
 __global__ void ht_build_k(DB db, HashTable ht, int maxsize) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < db.ncls) ht_insert_clause(db, ht, c, maxsize);
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) ht_insert_clause(db, ht, db.c_off[i], maxsize);
 }
 void k_ht_build(const DB& db, const HashTable& ht, int maxsize, Stream* st) {
   fill_u32((u32*)ht.slot, 0xFFFFFFFFu, 2 * ((size_t)ht.mask + 1), st);
@@ -933,14 +938,14 @@ void k_reduce_lists(const DB& db, Stream* st) {
   KLAUNCH(reduce_lists_k, grid_for(2 * (size_t)db.nvars, 256), 256, st, db);
 }
 __global__ void prep_k(DB db, u32 first) {
-  const u32 c = first + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= db.ncls) return;
-  u32* h = chead(db, c);
+  const u32 i = first + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= db.ncls) return;
+  u32* h = chead(db, db.c_off[i]);
   u32* l = h + CL_HDR;
   const u32 n = hdr_size(h[0]);
-  for (u32 i = 1; i < n; i++) {
-    const u32 t = l[i];
-    u32 j = i;
+  for (u32 q = 1; q < n; q++) {
+    const u32 t = l[q];
+    u32 j = q;
     while (j > 0 && l[j - 1] > t) { l[j] = l[j - 1]; j--; }
     l[j] = t;
   }
@@ -951,8 +956,9 @@ void k_prep(const DB& db, u32 first, Stream* st) {
   KLAUNCH(prep_k, grid_for(db.ncls - first, 256), 256, st, db, first);
 }
 __global__ void clear_added_k(DB db) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= db.ncls) return;
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= db.ncls) return;
+  const u32 c = db.c_off[i];
   const u8 f = cflags(db, c);
   if (f & F_ADDED) set_cflags(db, c, (u8)(f & ~F_ADDED));
 }
@@ -999,8 +1005,8 @@ void k_set_stamps(u32* stamp, const u32* pairs, u32 n, bool clear, Stream* st) {
 // ---------------------------------------------------------------- count / compact
 __global__ void count_k(DB db, unsigned long long* out) {
   unsigned long long live = 0, lits = 0;
-  for (u32 c = blockIdx.x * blockDim.x + threadIdx.x; c < db.ncls; c += gridDim.x * blockDim.x) {
-    const u32 h = chead(db, c)[0];
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < db.ncls; i += gridDim.x * blockDim.x) {
+    const u32 h = chead(db, db.c_off[i])[0];
     if (!st_deleted(hdr_flags(h))) {
       live++;
       lits += hdr_size(h);
@@ -1047,22 +1053,21 @@ void k_count(const DB& db, u64* out2, Stream* st) {
   KLAUNCH(count_k, std::min(grid_for(db.ncls, ST), RED_BLOCKS), ST, st, db, (unsigned long long*)out2);
 }
 __global__ void live_k(DB db, u32* tc, u32* tl) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= db.ncls) return;
-  const u32 h = chead(db, c)[0];
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= db.ncls) return;
+  const u32 h = chead(db, db.c_off[i])[0];
   const bool live = !st_deleted(hdr_flags(h));
-  tc[c] = live ? 1u : 0u;
-  tl[c] = live ? CL_HDR + hdr_size(h) : 0u;  // pool words
+  tc[i] = live ? 1u : 0u;
+  tl[i] = live ? CL_HDR + hdr_size(h) : 0u;  // pool words
 }
 __global__ void compact_k(DB src, DB dst, const u32* tc, const u32* tl) {
-  const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= src.ncls) return;
-  const u32* a = chead(src, c);
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= src.ncls) return;
+  const u32* a = chead(src, src.c_off[i]);
   const u32 h = a[0];
   if (st_deleted(hdr_flags(h))) return;
-  const u32 nid = tc[c], noff = tl[c], n = hdr_size(h);
+  const u32 nid = tc[i], noff = tl[i], n = hdr_size(h);
   dst.c_off[nid] = noff;
-  dst.c_lbd[nid] = src.c_lbd[c];
   u32* b = dst.lits + noff;
   b[0] = make_hdr(n, (u8)(hdr_flags(h) & ~F_MOLTEN));
   for (u32 i = 1; i < CL_HDR + n; i++) b[i] = a[i];
