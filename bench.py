@@ -192,6 +192,47 @@ def bench_single(name, steps=3, warmup=1):
     return out
 
 
+def bench_handoff(s, cnf):
+    """The handoff after a simplify call on the same workload: the packed
+    download of the result (device packing + PCIe) and the device variable
+    map (pfs_map, forced).  The map kernel's roofline counts a read of the
+    clause handles and clauses (4C + 4(3C + L)) and a write of the
+    literals and signatures (4L + 4C), C and L of the result."""
+    import numpy as np
+    s.rewind()
+    status = s.simplify()
+    t0 = time.perf_counter()
+    res = s.result(status)
+    dl_ms = (time.perf_counter() - t0) * 1e3
+    C, L = int(res.sizes.shape[0]), int(res.lits.shape[0])
+    dl_bytes = 8 * (C + 1) + 4 * L + C + 4 * C + 4 * res.units.shape[0] + 4 * res.resolved.shape[0] + res.vstate.shape[0]
+    s.set_option("profilegpu", 1)
+    t0 = time.perf_counter()
+    info = s.map(force=True)
+    map_ms = (time.perf_counter() - t0) * 1e3
+    kt = s.kernel_times()
+    s.set_option("profilegpu", 0)
+    k_ms, k_n = kt.get("map_clauses_k", (0.0, 0))
+    alg = 4 * C + 4 * (3 * C + L) + 4 * L + 4 * C
+    out = {"download_ms": round(dl_ms, 3), "download_bytes": int(dl_bytes),
+           "download_gbs": round(dl_bytes / dl_ms / 1e6, 2) if dl_ms else None,
+           "map_ms": round(map_ms, 3), "old_vars": info["old_maxvar"], "new_vars": info["new_maxvar"],
+           "clauses": C, "literals": L,
+           "map_kernels_ms": {k: round(v[0], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])},
+           "map_clauses_k": {"bound": "hbm", "achieved": round(alg / (k_ms / k_n / 1e3) / 1e9, 1) if k_n else None,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(alg / (k_ms / k_n / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if k_n else None,
+                             "algorithmic_bytes_per_launch": int(alg),
+                             "avg_launch_us": round(k_ms / k_n * 1e3, 2) if k_n else None}}
+    # the map's own check: the mapped clauses are the map of the unmapped ones
+    m = s.download_map(info["old_maxvar"])
+    after = s.result(status)
+    expect = (2 * m[res.lits >> 1] + (res.lits & 1)).astype(np.uint32)
+    out["mapped_identical"] = bool(np.array_equal(after.lits, expect) and np.array_equal(after.sizes, res.sizes))
+    print("bench: handoff " + json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
 def bench_batched(n=512, workers=16, check=16):
     """Batched workload: n formulas (clause counts log-uniform in 1k-500k,
     Tseitin or heavy-tailed) resident in HBM, simplified once as a loop of
@@ -392,6 +433,7 @@ def main():
                     "traffic": int(traffic) if traffic else None, "traffic_scales": scales,
                     "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(b),
                     "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
+    handoff = bench_handoff(s, cnf)
     s.close()
 
     heavy = None if (args.no_heavytail or args.workload == "heavytail") else bench_single("heavytail")
@@ -423,6 +465,7 @@ def main():
                    "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": base,
+        "handoff": handoff,
         "heavytail": heavy,
         "batched": batched,
         "serving": serving,

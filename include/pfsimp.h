@@ -72,6 +72,8 @@ void pfs_destroy(pfs_ctx* ctx);
  * "eremaxoccurs", "electionsmin", "collectfreq", "velitsbound") are accepted.
  * "proof" (boolean, default off) records the binary DRAT lines of the run
  * (pfs_download_proof).
+ * "mapperc" (real in [0, 1], default 0.2) is the inactive share above which
+ * pfs_map renumbers.
  * "workorder" (boolean, default on; no reference counterpart) launches the
  * per-variable kernels heaviest lists first; results do not depend on it. */
 pfs_status pfs_set_option(pfs_ctx* ctx, const char* name, const char* value);
@@ -136,6 +138,36 @@ pfs_status pfs_download_stack(pfs_ctx* ctx, uint32_t* words);
  * redundancy.h:137, solver.h:251-254 and 475-479; the GPU build's cuPROOF,
  * src/gpu/proof.cu:161-254): 'a'/'d', literals 2*vorg(v)+sign as 7-bit
  * groups, 0.  Lines appended by the host solver's own proof go around it. */
+/* Compact variable renumbering after pfs_simplify, on the device: the
+ * reference's `if (canMap()) map(true)` (src/cpu/simplify.cpp:227;
+ * Solver::map, src/cpu/vmap.cpp:61-135; VMAP::initiate and mapLit,
+ * src/cpu/vmap.h:40-78).  Applied when force != 0 or when more than
+ * "mapperc" (option, default 0.2, src/cpu/options.cpp:146) of the variables
+ * are inactive (eliminated, root-assigned or substituted; canMap,
+ * src/cpu/solver.h:164-169); otherwise nothing changes and applied = 0.
+ * Variables in state 0 keep their order and become 1..new_maxvar; the first
+ * root-assigned variable is kept too and stands for every root-assigned
+ * literal (its new literal true at the root is trail_lit, the whole trail
+ * after the map; 0 when nothing is assigned).  Afterwards the resident
+ * clauses, vstate, vorg and the assumed set use the new numbering (so do
+ * pfs_download_cnf, pfs_download_vstate, pfs_download_units and later
+ * pfs_append/pfs_simplify calls); the stack and proof stay in original
+ * numbering through vorg.  Download units and vstate first when the old
+ * numbering is wanted.  Requires no pending root units (call it right after
+ * pfs_simplify); pfs_rewind is unavailable afterwards.  Variables added by
+ * a later pfs_append receive fresh original ids. */
+typedef struct pfs_map_info {
+  uint32_t applied, old_maxvar, new_maxvar, first_frozen, trail_lit, inactive;
+} pfs_map_info;
+pfs_status pfs_map(pfs_ctx* ctx, int32_t force, pfs_map_info* info);
+/* Old variable -> new variable of the last pfs_map (old_maxvar+1 words; 0:
+ * dropped; the identity when the map was not applied). */
+pfs_status pfs_download_map(pfs_ctx* ctx, uint32_t* mapped);
+/* VMAP::mapOrgs (src/cpu/vmap.h:112-131) on the device: literals of the old
+ * numbering (model.lits, assumptions) mapped in place; 0 stays 0, a
+ * root-assigned literal becomes trail_lit or its negation by value, a
+ * dropped unassigned literal becomes 0. */
+pfs_status pfs_map_lits(pfs_ctx* ctx, uint32_t* lits, uint64_t n);
 pfs_status pfs_proof_size(pfs_ctx* ctx, uint64_t* nbytes);
 pfs_status pfs_download_proof(pfs_ctx* ctx, uint8_t* bytes);
 pfs_status pfs_round_stats_get(pfs_ctx* ctx, pfs_round_stats* out, uint32_t max, uint32_t* n);

@@ -16,6 +16,8 @@
 // LCVE, sortOT, SUB, VE, BCE, ERE, countAll, filterPVs, shrinkSimp).
 //
 // Usage: pfref <in.cnf> <input.pfd> <output.pfd> [reference options...]
+//   PFREF_MAP_OUT=<path> also dumps the state after the host write-back
+//   (map or newBeginning, simplify.cpp:227-228)
 //   input.pfd : the simplifier input (clauses after parse + root BCP +
 //               rootify/shrinkTop, simplify.cpp:163-164)
 //   output.pfd: the simplifier output (see oracle/pfd_format.md)
@@ -85,6 +87,48 @@ void write_pfd(const std::string& path, uint32_t status, uint32_t maxvar,
   fwrite(resolved.data(), 4, resolved.size(), f);
   put_u64(f, vstate.size());
   fwrite(vstate.data(), 1, vstate.size(), f);
+  fclose(f);
+}
+
+// The host write-back that follows the rounds (simplify.cpp:217-228): the
+// reference's own Solver::map(true) when canMap() holds, else newBeginning().
+// Dump "PFM1": applied, maxVar, vorg[0..maxVar], trail, model.lits,
+// vstate[0..maxVar], then the clause arena (orgs, then learnts): sizes,
+// flags (bit0 learnt), lbd, literals.
+void write_map(const std::string& path, uint32_t applied, Solver* s) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) { perror("pfref: open map output"); exit(2); }
+  fwrite("PFM1", 1, 4, f);
+  put_u32(f, applied);
+  put_u32(f, inf.maxVar);
+  put_u64(f, (uint64_t)inf.maxVar + 1);
+  for (uint32_t v = 0; v <= inf.maxVar; v++) put_u32(f, s->vorg[v]);
+  put_u64(f, s->trail.size());
+  for (uint32_t i = 0; i < s->trail.size(); i++) put_u32(f, s->trail[i]);
+  put_u64(f, s->model.lits.size());
+  for (uint32_t i = 0; i < s->model.lits.size(); i++) put_u32(f, s->model.lits[i]);
+  put_u64(f, (uint64_t)inf.maxVar + 1);
+  for (uint32_t v = 0; v <= inf.maxVar; v++) {
+    const uint8_t st = v ? s->sp->vstate[v].state : 0;
+    fwrite(&st, 1, 1, f);
+  }
+  ClauseDump cd;
+  for (int pass = 0; pass < 2; pass++) {
+    const BCNF& list = pass ? s->learnts : s->orgs;
+    for (uint32_t i = 0; i < list.size(); i++) {
+      CLAUSE& c = s->cm[list[i]];
+      cd.sizes.push_back(c.size());
+      cd.flags.push_back(c.learnt() ? 1 : 0);
+      cd.lbds.push_back(c.learnt() ? c.lbd() : 0);
+      for (int k = 0; k < c.size(); k++) cd.lits.push_back(c[k]);
+    }
+  }
+  put_u64(f, cd.sizes.size());
+  fwrite(cd.sizes.data(), 4, cd.sizes.size(), f);
+  fwrite(cd.flags.data(), 1, cd.flags.size(), f);
+  fwrite(cd.lbds.data(), 4, cd.lbds.size(), f);
+  put_u64(f, cd.lits.size());
+  fwrite(cd.lits.data(), 4, cd.lits.size(), f);
   fclose(f);
 }
 
@@ -239,6 +283,18 @@ int main(int argc, char** argv) {
   }
   write_pfd(g_out_path, status, inf.maxVar, cd, units, resolved, vst);
   g_done = true;
+  // with PFREF_MAP_OUT set: the write-back after an unsolved run
+  const char* map_out = getenv("PFREF_MAP_OUT");
+  if (map_out && status == 2) {
+    uint32_t applied = 0;
+    if (s->canMap()) {
+      s->map(true);
+      applied = 1;
+    } else {
+      s->newBeginning();
+    }
+    write_map(map_out, applied, s);
+  }
 
   printf("{\"total_s\": %.6f, \"rounds\": %d, \"elected\": [", total, rounds);
   for (size_t i = 0; i < elected_per_round.size(); i++)

@@ -27,6 +27,7 @@ struct pfs_ctx {
           ere_max_resolvent = 1000, lcve_min = 1, shrink_rate = 2, lbd_tier1 = 2;
   bool ve = true, ve_plus = true, sub = true, bce = false, ere = true, all = false, velbound = false,
        profile = false, work_order = true, proof = false;
+  double map_perc = 0.2;  // mapperc, src/cpu/options.cpp:146
 };
 
 namespace {
@@ -192,6 +193,16 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       return PFS_OK;
     }
   }
+  if (n == "mapperc") {
+    char* end = nullptr;
+    const double d = value && *value ? strtod(value, &end) : -1.0;
+    if (!end || *end || !(d >= 0.0 && d <= 1.0)) {  // FP64R(0, 1)
+      c->err = "bad value for option " + n;
+      return PFS_BAD_INPUT;
+    }
+    c->map_perc = d;
+    return PFS_OK;
+  }
   c->err = "unknown option " + n;
   return PFS_UNKNOWN_OPTION;
 }
@@ -309,6 +320,46 @@ pfs_status pfs_download_vstate(pfs_ctx* c, uint8_t* vstate) {
   if (!c->simplified) return PFS_BAD_STATE;
   return guard(c, [&] {
     c->s->download_vstate(vstate);
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_map(pfs_ctx* c, int32_t force, pfs_map_info* info) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->simplified) {
+    c->err = "simplify has not run";
+    return PFS_BAD_STATE;
+  }
+  if (c->result == PFS_RESULT_UNSAT) {
+    c->err = "the formula is refuted";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    const Simplifier::MapInfo r = c->s->map(force != 0, c->map_perc);
+    if (info) {
+      info->applied = r.applied;
+      info->old_maxvar = r.old_nvars;
+      info->new_maxvar = r.new_nvars;
+      info->first_frozen = r.first;
+      info->trail_lit = r.trail_lit;
+      info->inactive = r.inactive;
+    }
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_download_map(pfs_ctx* c, uint32_t* mapped) {
+  if (!c || !mapped) return PFS_BAD_INPUT;
+  return guard(c, [&] {
+    c->s->download_map(mapped);
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_map_lits(pfs_ctx* c, uint32_t* lits, uint64_t n) {
+  if (!c || (!lits && n)) return PFS_BAD_INPUT;
+  return guard(c, [&] {
+    c->s->map_lits(lits, n);
     return PFS_OK;
   });
 }

@@ -63,7 +63,8 @@ Simplifier::~Simplifier() {
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
                 &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
                 &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
-                &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_};
+                &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_,
+                &map_, &map_assign_, &out_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
@@ -189,12 +190,15 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   std::vector<u8> vs((size_t)V_ + 1, 0), as((size_t)V_ + 1, 0);
   std::vector<u32> vo((size_t)V_ + 1);
   u32 melted = 0;
+  orgmax_ = V_;
   for (u32 v = 0; v <= V_; v++) {
     vs[v] = v && vstate ? vstate[v] : 0;
     as[v] = v && assumed ? assumed[v] : 0;
     vo[v] = vorg ? vorg[v] : v;
+    orgmax_ = std::max(orgmax_, vo[v]);
     melted += (vs[v] & (VS_MELTED | VS_SUBSTITUTED)) ? 1 : 0;
   }
+  map_called_ = map_applied_ = remapped_ = false;
   for (u32 i = 0; i < ntrail; i++) vs[ABS(trail[i])] = VS_FROZEN;
   h2d(assign_.p, h_assign_.data(), ((size_t)V_ + 1) * 4, st_);
   h2d(vstate_.p, vs.data(), (size_t)V_ + 1, st_);
@@ -231,6 +235,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
 
 void Simplifier::rewind() {
   if (!keep_input_) throw std::logic_error("context was created without keep_input");
+  if (remapped_) throw std::logic_error("the variables were renumbered since the upload");
   cur_ = 0;
   ncls_ = in_ncls_;
   lits_end_ = in_nlits_;
@@ -349,8 +354,13 @@ void Simplifier::grow_vars(uint32_t maxvar) {
   fill_u32(P<u32>(assign_) + o1, 0, n1 - o1, st_);
   fill_u8(P<u8>(vstate_) + o1, 0, n1 - o1, st_);
   fill_u8(P<u8>(assumed_) + o1, 0, n1 - o1, st_);
+  // new variables keep their number as original id (iadd, src/cpu/addinc.cpp:48-63);
+  // after a renumbering they get fresh original ids past every id handed out
   std::vector<u32> vo(n1 - o1);
-  for (size_t v = o1; v < n1; v++) vo[v - o1] = (u32)v;
+  for (size_t v = o1; v < n1; v++) {
+    vo[v - o1] = remapped_ ? ++orgmax_ : (u32)v;
+    if (!remapped_) orgmax_ = std::max(orgmax_, (u32)v);
+  }
   h2d(P<u32>(vorg_) + o1, vo.data(), vo.size() * 4, st_);
   if (pstamp_.p) {
     mem_.ensure(pstamp_, n1 * 4, o1 * 4, st_);
@@ -1204,26 +1214,130 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
 
 // ---------------------------------------------------------------- results
 void Simplifier::download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd) {
-  // after the final compaction the pool holds every clause in order:
-  // header words, then literals
-  const size_t n = ncls_;
-  std::vector<u32> off(n), pool(lits_end_);
-  d2h(off.data(), c_off_[cur_].p, n * 4, st_);
-  d2h(pool.data(), lits_[cur_].p, lits_end_ * 4, st_);
+  // after the final compaction c_off lists every clause in order; the
+  // clauses are packed on the device and only the requested arrays cross
+  // the bus
+  const size_t n = ncls_, nl = out_nlits();
+  const size_t o_off = 0, o_lits = o_off + 8 * (n + 1), o_lbd = o_lits + 4 * nl, o_flags = o_lbd + 4 * n;
+  mem_.ensure(out_, o_flags + n + 8, 0, st_);
+  mem_.ensure(tmp2_, std::max<size_t>(n, 1) * 4, 0, st_);
+  uint8_t* base = P<uint8_t>(out_);
+  k_pack_cnf(make_db(cur_), P<u32>(tmp2_), tmp_.p, P<u32>(scal_) + 56, (u64*)(base + o_off), (u32*)(base + o_lits),
+             base + o_flags, (u32*)(base + o_lbd), st_);
+  if (offsets) d2h(offsets, base + o_off, 8 * (n + 1), st_);
+  if (lits) d2h(lits, base + o_lits, 4 * nl, st_);
+  if (flags) d2h(flags, base + o_flags, n, st_);
+  if (lbd) d2h(lbd, base + o_lbd, 4 * n, st_);
   stream_sync(st_);
-  uint64_t a = 0;
-  for (size_t i = 0; i < n; i++) {
-    const u32* h = pool.data() + off[i];
-    const u32 sz = hdr_size(h[0]);
-    const u8 fl = hdr_flags(h[0]);
-    if (offsets) offsets[i] = a;
-    if (lits) std::copy(h + CL_HDR, h + CL_HDR + sz, lits + a);
-    if (flags) flags[i] = (u8)((st_learnt(fl) ? 1 : 0) | ((fl & F_ADDED) ? 2 : 0) | (fl & USAGE_MASK));
-    if (lbd) lbd[i] = st_learnt(fl) ? h[2] : 0;
-    a += sz;
-  }
-  if (offsets) offsets[n] = a;
 }
+
+// ---------------------------------------------------------------- variable map
+Simplifier::MapInfo Simplifier::map(bool force, double map_perc) {
+  if (refuted_) throw std::logic_error("the formula is refuted");
+  if (!pending_units_.empty() || trail_n_ != propagated_)
+    throw std::logic_error("root units are not propagated: simplify before mapping");
+  const u32 V = V_;
+  const size_t n1 = (size_t)V + 1;
+  // with profiling on, the map's kernels replace the per-kernel times
+  struct Prof {
+    Simplifier* s;
+    explicit Prof(Simplifier* x) : s(x) {
+      if (s->profile) s->kernel_times.clear(), prof_enable(s->st_, true);
+    }
+    ~Prof() {
+      if (!s->profile) return;
+      try {
+        prof_collect(s->st_, s->kernel_times);
+        prof_enable(s->st_, false);
+      } catch (...) {
+      }
+    }
+  } prof(this);
+  mem_.ensure(map_, n1 * 4, 0, st_);
+  mem_.ensure(tmp2_, n1 * 4, 0, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(n1)), 0, st_);
+  u32* info = P<u32>(scal_) + 56;
+  k_map_build(P<u8>(vstate_), P<u32>(assign_), V, P<u32>(tmp2_), P<u32>(map_), info, tmp_.p, st_);
+  u32 h[4];
+  d2h(h, info, sizeof(h), st_);
+  stream_sync(st_);
+  MapInfo r{0, V, V, h[0], 0, h[3]};
+  map_called_ = true;
+  map_applied_ = false;
+  map_old_nvars_ = V;
+  // canMap: inactive > map_perc * maxVar (DL() is 0 at the boundary)
+  if (!force && !((double)h[3] > map_perc * (double)V)) return r;
+  const u32 nV = h[1];
+  map_first_ = h[0];
+  map_first_true_ = h[2];
+  VarMap m{P<u32>(map_), P<u32>(assign_), map_first_, map_first_true_};
+  k_map_clauses(make_db(cur_), m, st_);
+  // per-variable state in the new numbering; the old assignments stay for map_lits
+  const size_t m1 = (size_t)nV + 1;
+  Buf vorg2, vstate2, assumed2, assign2;
+  mem_.ensure(vorg2, m1 * 4, 0, st_);
+  mem_.ensure(vstate2, m1, 0, st_);
+  mem_.ensure(assumed2, m1, 0, st_);
+  mem_.ensure(assign2, m1 * 4, 0, st_);
+  k_map_vars(m, V, P<u32>(vorg_), P<u8>(vstate_), P<u8>(assumed_), P<u32>(vorg2), P<u8>(vstate2), P<u8>(assumed2),
+             P<u32>(assign2), st_);
+  // trail: only 'first' survives (VMAP::mapShrinkLits, vmap.h:158-169)
+  trail_n_ = propagated_ = 0;
+  if (map_first_) {
+    h2d(P<u32>(trail_), &map_first_true_, 4, st_);
+    trail_n_ = propagated_ = 1;
+  }
+  stream_sync(st_);
+  mem_.release(map_assign_);
+  map_assign_ = assign_;
+  assign_ = assign2;
+  std::swap(vorg_, vorg2);
+  std::swap(vstate_, vstate2);
+  std::swap(assumed_, assumed2);
+  mem_.release(vorg2);
+  mem_.release(vstate2);
+  mem_.release(assumed2);
+  std::vector<u32> units;
+  for (u32 u : units_out_)
+    if (ABS(u) == map_first_) units.push_back(map_first_true_);
+  units_out_.swap(units);
+  h_assign_.assign(m1, 0);
+  if (map_first_) h_assign_[ABS(map_first_true_)] = map_first_true_ + 1;
+  V_ = nV;
+  remapped_ = true;
+  map_applied_ = true;
+  alloc_state();
+  stream_sync(st_);
+  r.applied = 1;
+  r.new_nvars = nV;
+  r.trail_lit = map_first_ ? map_first_true_ : 0;
+  return r;
+}
+
+void Simplifier::download_map(uint32_t* mapped) {
+  if (!map_called_) throw std::logic_error("no variable map was computed");
+  if (!map_applied_) {
+    for (u32 v = 0; v <= map_old_nvars_; v++) mapped[v] = v;
+    return;
+  }
+  d2h(mapped, map_.p, ((size_t)map_old_nvars_ + 1) * 4, st_);
+  stream_sync(st_);
+}
+
+void Simplifier::map_lits(uint32_t* lits, uint64_t n) {
+  if (!map_called_) throw std::logic_error("no variable map was computed");
+  for (uint64_t i = 0; i < n; i++)
+    if (lits[i] && (ABS(lits[i]) == 0 || ABS(lits[i]) > map_old_nvars_))
+      throw std::invalid_argument("literal out of range of the mapped numbering");
+  if (!map_applied_ || !n) return;
+  mem_.ensure(tmp3_, n * 4, 0, st_);
+  h2d(tmp3_.p, lits, n * 4, st_);
+  VarMap m{P<u32>(map_), P<u32>(map_assign_), map_first_, map_first_true_};
+  k_map_lits(m, P<u32>(tmp3_), n, st_);
+  d2h(lits, tmp3_.p, n * 4, st_);
+  stream_sync(st_);
+}
+
 void Simplifier::download_vstate(uint8_t* vs) {
   d2h(vs, vstate_.p, (size_t)V_ + 1, st_);
   stream_sync(st_);

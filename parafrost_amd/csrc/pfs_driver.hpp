@@ -90,6 +90,15 @@ class Simplifier {
   double stage_ms[S_NSTAGES] = {0};
   uint64_t stage_calls[S_NSTAGES] = {0};
   std::vector<KernelTime> kernel_times;  // per kernel, when profiling is enabled
+  // compact variable renumbering of the resident formula (Solver::map,
+  // src/cpu/vmap.cpp:61-135), applied when forced or when more than
+  // map_perc of the variables are inactive (canMap, src/cpu/solver.h:164-169)
+  struct MapInfo {
+    uint32_t applied, old_nvars, new_nvars, first, trail_lit, inactive;
+  };
+  MapInfo map(bool force, double map_perc);
+  void download_map(uint32_t* mapped);  // old variable count + 1 words
+  void map_lits(uint32_t* lits, uint64_t n);  // VMAP::mapOrgs on caller literals
   size_t mem_used() const { return mem_.used(); }
   size_t mem_peak() const { return mem_.peak(); }
   uint32_t nvars() const { return V_; }
@@ -171,6 +180,12 @@ class Simplifier {
   uint64_t in_ncls_ = 0, in_nlits_ = 0, in_trail_ = 0;
   uint32_t in_unassigned_ = 0;
   Buf in_trail_buf_;
+  // variable map of the last map() call: old -> new variables, the old
+  // root assignments (for map_lits) and the packed-output staging buffer
+  Buf map_, map_assign_, out_;
+  bool map_called_ = false, map_applied_ = false, remapped_ = false;
+  uint32_t map_old_nvars_ = 0, map_first_ = 0, map_first_true_ = 0;
+  uint32_t orgmax_ = 0;  // largest original variable id handed out
 
   // host mirrors
   std::vector<uint32_t> h_assign_;  // per variable

@@ -1482,4 +1482,58 @@ PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
   if (nm > (u32)ERE_K) at_add(a.overflow, 1u);
 }
 
+// ---------------------------------------------------------------- variable map
+// Compact renumbering after a simplify call (VMAP, src/cpu/vmap.h:26-78):
+// variables in state 0 keep their order and become 1..n; the first
+// root-assigned variable ('first') is kept as well and stands for every
+// root-assigned literal.
+struct VarMap {
+  const u32* mapped;  // old variable -> new variable, 0 when dropped
+  const u32* assign;  // root assignments in the old numbering (true literal + 1)
+  u32 first;          // first root-assigned old variable, 0 when there is none
+  u32 first_true;     // the new literal of 'first' that is true at the root
+};
+// variable v survives the map (VMAP::initiate, vmap.h:63-77)
+PF_HD u32 map_keep(const u8* vstate, u32 v, u32 first) {
+  return (v && (vstate[v] == 0 || v == first)) ? 1u : 0u;
+}
+// VMAP::mapLit (vmap.h:45-60): a surviving variable keeps its sign; a
+// root-assigned literal becomes the literal of 'first' with the same value;
+// a dropped unassigned literal becomes 0
+PF_HD u32 map_lit(const VarMap& m, u32 lit) {
+  const u32 v = ABS(lit), nv = m.mapped[v];
+  if (nv && v != m.first) return V2L(nv) | SIGN(lit);
+  const u32 a = m.assign[v];
+  if (!a || !m.first) return 0;
+  return (lit == a - 1) ? m.first_true : FLIP(m.first_true);
+}
+// one clause in place: literals and signature (VMAP::mapClause, vmap.h:183-193).
+// Clause literals are unassigned and the map keeps the variable order, so
+// sorted literals stay sorted.
+PF_HD void map_clause(const DB& db, u32 c, const VarMap& m) {
+  if (is_deleted(db, c)) return;
+  u32* l = clits(db, c);
+  const u32 n = csize(db, c);
+  u32 sig = 0;
+  for (u32 q = 0; q < n; q++) {
+    const u32 x = map_lit(m, l[q]);
+    l[q] = x;
+    sig |= MAPHASH(x);
+  }
+  set_csig(db, c, sig);
+}
+// output packing of clause index i (the host write-back, src/cpu/sclause.cpp:22-57):
+// literals at start[i], flags bit0 learnt, bit1 added, bits 4-5 usage
+PF_HD void pack_clause(const DB& db, u32 i, const u32* start, u64* offsets, u32* lits, u8* flags, u32* lbd) {
+  const u32 c = db.c_off[i];
+  const u32 n = csize(db, c), s = start[i];
+  const u8 f = cflags(db, c);
+  const u32* src = clits(db, c);
+  offsets[i] = s;
+  for (u32 q = 0; q < n; q++) lits[s + q] = src[q];
+  flags[i] = (u8)((st_learnt(f) ? 1 : 0) | ((f & F_ADDED) ? 2 : 0) | (f & USAGE_MASK));
+  lbd[i] = st_learnt(f) ? clbd(db, c) : 0u;
+  if (i + 1 == db.ncls) offsets[db.ncls] = (u64)s + n;
+}
+
 }  // namespace pfs

@@ -1083,4 +1083,108 @@ void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u3
   KLAUNCH(compact_k, grid_for(src.ncls, 256), 256, st, src, dst, tmp_cls, tmp_lit);
 }
 
+// ---------------------------------------------------------------- variable map
+// First root-assigned variable (VMAP::initiate, src/cpu/vmap.h:63-77): the
+// lowest set lane of a wave holds the wave's smallest candidate, so each
+// wave issues at most one atomic.
+__global__ void map_first_k(const u8* vstate, const u32* assign, u32 nvars, u32* first) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const bool cand = v <= nvars && vstate[v] && assign[v];
+  const unsigned long long b = __ballot(cand);
+  if (cand && (u32)(__ffsll(b) - 1) == (threadIdx.x & 63u)) atomicMin(first, v);
+}
+__global__ void map_keep_k(const u8* vstate, u32 nvars, const u32* first, u32* keep) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nvars) return;
+  const u32 f = *first == 0xFFFFFFFFu ? 0u : *first;
+  keep[v] = map_keep(vstate, v, f);
+}
+// pos: exclusive scan of the keep flags -> mapped
+__global__ void map_index_k(const u8* vstate, u32 nvars, const u32* first, const u32* pos, u32* mapped) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nvars) return;
+  const u32 f = *first == 0xFFFFFFFFu ? 0u : *first;
+  mapped[v] = map_keep(vstate, v, f) ? pos[v] + 1 : 0u;
+}
+// info = {first (0: none), new variable count, true literal of first, inactive variables}
+__global__ void map_info_k(const u32* assign, const u32* mapped, u32 nvars, const u32* total, u32* info) {
+  const u32 f = info[0] == 0xFFFFFFFFu ? 0u : info[0];
+  const u32 n = *total;
+  info[0] = f;
+  info[1] = n;
+  info[2] = f ? (V2L(mapped[f]) | SIGN(assign[f] - 1)) : 0u;
+  info[3] = nvars - (n - (f ? 1u : 0u));
+}
+void k_map_build(const u8* vstate, const u32* assign, u32 nvars, u32* pos_tmp, u32* mapped, u32* info,
+                 void* tmp, Stream* st) {
+  fill_u32(info, 0xFFFFFFFFu, 1, st);
+  if (nvars) KLAUNCH(map_first_k, grid_for(nvars, 256), 256, st, vstate, assign, nvars, info);
+  KLAUNCH(map_keep_k, grid_for((size_t)nvars + 1, 256), 256, st, vstate, nvars, info, pos_tmp);
+  scan_u32(pos_tmp, pos_tmp, (size_t)nvars + 1, info + 4, tmp, st);
+  KLAUNCH(map_index_k, grid_for((size_t)nvars + 1, 256), 256, st, vstate, nvars, info, pos_tmp, mapped);
+  KLAUNCH(map_info_k, 1, 1, st, assign, mapped, nvars, info + 4, info);
+}
+__global__ void map_clauses_k(DB db, VarMap m) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) map_clause(db, db.c_off[i], m);
+}
+void k_map_clauses(const DB& db, const VarMap& m, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(map_clauses_k, grid_for(db.ncls, 256), 256, st, db, m);
+}
+// per-variable arrays in the new numbering (VMAP::mapShrinkVars and mapSP,
+// vmap.h:80-100, 132-143): only 'first' stays assigned
+__global__ void map_vars_k(VarMap m, u32 nvars_old, const u32* vorg, const u8* vstate, const u8* assumed,
+                           u32* vorg2, u8* vstate2, u8* assumed2, u32* assign2) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nvars_old) return;
+  if (v == 0) {
+    vorg2[0] = vorg[0];
+    vstate2[0] = 0;
+    assumed2[0] = 0;
+    assign2[0] = 0;
+    return;
+  }
+  const u32 nv = m.mapped[v];
+  if (!nv) return;
+  vorg2[nv] = vorg[v];
+  vstate2[nv] = vstate[v];
+  assumed2[nv] = assumed[v];
+  assign2[nv] = v == m.first ? m.first_true + 1 : 0u;
+}
+void k_map_vars(const VarMap& m, u32 nvars_old, const u32* vorg, const u8* vstate, const u8* assumed, u32* vorg2,
+                u8* vstate2, u8* assumed2, u32* assign2, Stream* st) {
+  KLAUNCH(map_vars_k, grid_for((size_t)nvars_old + 1, 256), 256, st, m, nvars_old, vorg, vstate, assumed, vorg2,
+          vstate2, assumed2, assign2);
+}
+// VMAP::mapOrgs (vmap.h:112-131): 0 stays 0
+__global__ void map_lits_k(VarMap m, u32* lits, u64 n) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && lits[i]) lits[i] = map_lit(m, lits[i]);
+}
+void k_map_lits(const VarMap& m, u32* lits, u64 n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(map_lits_k, grid_for(n, 256), 256, st, m, lits, n);
+}
+
+// ---------------------------------------------------------------- output packing
+__global__ void pack_sizes_k(DB db, u32* sizes) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) sizes[i] = csize(db, db.c_off[i]);
+}
+__global__ void pack_k(DB db, const u32* start, u64* offsets, u32* lits, u8* flags, u32* lbd) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) pack_clause(db, i, start, offsets, lits, flags, lbd);
+}
+void k_pack_cnf(const DB& db, u32* start_tmp, void* tmp, u32* total, u64* offsets, u32* lits, u8* flags, u32* lbd,
+                Stream* st) {
+  if (!db.ncls) {
+    fill_u32((u32*)offsets, 0, 2, st);
+    return;
+  }
+  KLAUNCH(pack_sizes_k, grid_for(db.ncls, 256), 256, st, db, start_tmp);
+  scan_u32(start_tmp, start_tmp, db.ncls, total, tmp, st);
+  KLAUNCH(pack_k, grid_for(db.ncls, 256), 256, st, db, start_tmp, offsets, lits, flags, lbd);
+}
+
 }  // namespace pfs

@@ -150,4 +150,20 @@ void k_elected_work(const DB& db, const u32* pv, u32 n, u64* out2, Stream*);
 void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
                Stream*);
 
+// ---- variable map (Solver::map, src/cpu/vmap.cpp:61-135; VMAP, src/cpu/vmap.h)
+// mapped[0..nvars] (old variable -> new, 0 dropped) and
+// info[0..3] = {first root-assigned variable (0: none), new variable count,
+// true literal of first in the new numbering, inactive variables};
+// info[4] is scratch; pos_tmp holds nvars+1 words
+void k_map_build(const u8* vstate, const u32* assign, u32 nvars, u32* pos_tmp, u32* mapped, u32* info, void* tmp,
+                 Stream*);
+void k_map_clauses(const DB& db, const VarMap& m, Stream*);
+void k_map_vars(const VarMap& m, u32 nvars_old, const u32* vorg, const u8* vstate, const u8* assumed, u32* vorg2,
+                u8* vstate2, u8* assumed2, u32* assign2, Stream*);
+void k_map_lits(const VarMap& m, u32* lits, u64 n, Stream*);
+// ---- output packing: offsets[ncls+1], literals, flags, lbd of the live
+// clauses in clause order (start_tmp: ncls words)
+void k_pack_cnf(const DB& db, u32* start_tmp, void* tmp, u32* total, u64* offsets, u32* lits, u8* flags, u32* lbd,
+                Stream*);
+
 }  // namespace pfs

@@ -26,7 +26,8 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_download_stack", "pfs_proof_size", "pfs_download_proof", "pfs_round_stats_get", "pfs_stage_times", "pfs_stage_name", "pfs_stage_count",
            "pfs_kernel_count", "pfs_kernel_stats", "pfs_memory", "pfs_last_error", "pfs_version",
            "pfs_device_count", "pfs_batch_simplify", "pfs_append", "pfs_assume", "pfs_server_create",
-           "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy")
+           "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy", "pfs_map",
+           "pfs_download_map", "pfs_map_lits")
 
 
 class PfsError(RuntimeError):
@@ -44,6 +45,11 @@ class _RoundStats(ctypes.Structure):
                 ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("clauses_before", ctypes.c_uint64), ("literals_before", ctypes.c_uint64),
                 ("elected_occurs", ctypes.c_uint64), ("elected_bytes", ctypes.c_uint64)]
+
+
+class _MapInfo(ctypes.Structure):
+    _fields_ = [("applied", ctypes.c_uint32), ("old_maxvar", ctypes.c_uint32), ("new_maxvar", ctypes.c_uint32),
+                ("first_frozen", ctypes.c_uint32), ("trail_lit", ctypes.c_uint32), ("inactive", ctypes.c_uint32)]
 
 
 def lib_path() -> str:
@@ -96,6 +102,9 @@ def load_library():
     L.pfs_server_wait.argtypes = [vp, u32, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.pfs_server_destroy.argtypes = [vp]
+    L.pfs_map.argtypes = [vp, i32, ctypes.POINTER(_MapInfo)]
+    L.pfs_download_map.argtypes = [vp, vp]
+    L.pfs_map_lits.argtypes = [vp, vp, u64]
     L.pfs_device_count.restype = i32
     _LIB = L
     return L
@@ -229,6 +238,27 @@ class Simplifier:
             proof = pf.tobytes()
         return Result(status, self.maxvar, sizes, flags[: nc.value], lbd[: nc.value], lits[: nl.value],
                       units[: nu.value], stack[: ns.value], vst, rounds, proof)
+
+    def map(self, force: bool = False) -> dict:
+        """Compact variable renumbering of the resident formula (pfs_map)."""
+        info = _MapInfo()
+        self._check(self._L.pfs_map(self._h, 1 if force else 0, ctypes.byref(info)), "pfs_map")
+        out = {k: int(getattr(info, k)) for k, _ in _MapInfo._fields_}
+        if out["applied"]:
+            self.maxvar = out["new_maxvar"]
+        return out
+
+    def download_map(self, old_maxvar: int) -> np.ndarray:
+        """Old variable -> new variable of the last map (0: dropped)."""
+        m = np.zeros(old_maxvar + 1, np.uint32)
+        self._check(self._L.pfs_download_map(self._h, _ptr(m)), "pfs_download_map")
+        return m
+
+    def map_lits(self, lits) -> np.ndarray:
+        """VMAP::mapOrgs on a copy of 'lits' (pfs_map_lits)."""
+        a = np.array(lits, dtype=np.uint32, copy=True)
+        self._check(self._L.pfs_map_lits(self._h, _ptr(a), a.shape[0]), "pfs_map_lits")
+        return a
 
     def stage_times(self) -> dict:
         L = self._L

@@ -9,6 +9,9 @@ Each fixture is a pair of PFD1 dumps (parafrost_amd/pfd.py) and a proof:
                     new root units, eliminated-variable stack, variable states
   <case>.<opts>.proof.gz  -- the binary DRAT lines the reference simplifier
                     wrote (run with -proof; the part after rootify/shrinkTop)
+  <case>.<opts>.map.gz    -- the state after the reference's host write-back
+                    (Solver::map(true) when canMap(), else newBeginning();
+                    PFM1, oracle/vmap.py read_pfm), for unsolved runs
 manifest.json lists every fixture with the reference options it was run
 with.  Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -76,8 +79,10 @@ def main():
             tag = f"{name}.{oname}"
             out = os.path.join(HERE, tag + ".out.pfd")
             drat = os.path.join(HERE, tag + ".drat")
+            mp = os.path.join(HERE, tag + ".map")
+            env = dict(os.environ, PFREF_MAP_OUT=mp)
             r = subprocess.run([REF, cnf_path, inp, out] + ropts + ["-proof", f"--proofout={drat}"],
-                               capture_output=True, text=True, timeout=600)
+                               capture_output=True, text=True, timeout=600, env=env)
             if not os.path.exists(out):
                 print("skip (refuted at parse time)", tag)
                 continue
@@ -91,8 +96,12 @@ def main():
             with open(os.path.join(HERE, tag + ".proof"), "wb") as f:
                 f.write(proof)
             gz(os.path.join(HERE, tag + ".proof"))
-            manifest.append({"name": tag, "input": name + ".in.pfd.gz", "output": tag + ".out.pfd.gz",
-                             "proof": tag + ".proof.gz", "ref_options": ropts, "options": kw})
+            entry = {"name": tag, "input": name + ".in.pfd.gz", "output": tag + ".out.pfd.gz",
+                     "proof": tag + ".proof.gz", "ref_options": ropts, "options": kw}
+            if os.path.exists(mp):
+                gz(mp)
+                entry["map"] = tag + ".map.gz"
+            manifest.append(entry)
             print(tag, r.stdout.strip().splitlines()[-1][:100])
         os.remove(cnf_path)
         if os.path.exists(inp):
