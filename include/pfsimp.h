@@ -93,6 +93,40 @@ pfs_status pfs_upload(pfs_ctx* ctx, uint32_t maxvar, uint64_t ncls, const uint64
                       const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
                       const uint32_t* trail, uint32_t ntrail);
 
+/* DIMACS ingestion on the device: the text (nbytes, need not be
+ * NUL-terminated) becomes the resident formula, replacing pfs_upload.
+ * Replaces the reference's parser (Solver::parse/toClause,
+ * src/cpu/dimacs.cpp:41-170) together with the root propagation and the
+ * top-level shrink that precede the first simplify (Solver(), solver.cpp:49;
+ * rootify/shrinkTop, simplify.cpp:163-164, shrink.cpp:152-216).  The
+ * header ("p cnf V C", after blanks and 'c' comment lines) is read on the
+ * host; the clause part is lexed, split, deduplicated and propagated on the
+ * device.  Duplicate literals are dropped, tautologies removed, unit
+ * clauses assigned in text order, root units propagated, satisfied clauses
+ * removed and false literals dropped; the remaining clauses keep their text
+ * order.  Comment lines may appear anywhere; the clause part ends at the
+ * first NUL byte or '%' token (SATLIB files).  Errors (PFS_BAD_INPUT, the
+ * reason in pfs_last_error): no or malformed header, a malformed token
+ * (numbers are separated by blanks), a literal above V, an unterminated last
+ * clause, more remaining clauses of two or more literals than C.  An empty
+ * clause or opposite units make info.status 0 and the next pfs_simplify
+ * report UNSAT.  The root trail is pfs_download_trail; the clauses,
+ * vstate and sizes are readable before pfs_simplify through
+ * pfs_result_sizes, pfs_download_cnf and pfs_download_vstate. */
+typedef struct pfs_parse_info {
+  uint32_t maxvar;          /* V of the header */
+  uint32_t status;          /* 0: refuted while parsing/propagating, 2: otherwise */
+  uint64_t header_clauses;  /* C of the header */
+  uint64_t read_clauses;    /* clauses in the text */
+  uint64_t clauses, literals;  /* resident formula after propagation */
+  uint64_t trail;           /* root units */
+} pfs_parse_info;
+pfs_status pfs_parse_dimacs(pfs_ctx* ctx, const char* text, uint64_t nbytes, pfs_parse_info* info);
+/* The root trail of the resident formula (*n literals; lits nullable to
+ * query the size): the input trail, the units of a parse, or after pfs_map
+ * its one stand-in literal. */
+pfs_status pfs_download_trail(pfs_ctx* ctx, uint32_t* lits, uint64_t* n);
+
 /* Incremental use (the reference's incremental front end iadd/itoClause,
  * src/cpu/addinc.cpp:44-160, and assumptions, src/cpu/assume.cpp): the
  * formula stays resident in device memory between calls.
@@ -114,7 +148,8 @@ pfs_status pfs_rewind(pfs_ctx* ctx);
  * simplifying(), src/gpu/simplify.cu:136-241).  *result receives a pfs_result. */
 pfs_status pfs_simplify(pfs_ctx* ctx, int32_t* result);
 
-/* Result sizes: clauses, literals, new root units, stack words, rounds. */
+/* Result sizes: clauses, literals, new root units, stack words, rounds.
+ * Before the first pfs_simplify they describe the resident input. */
 pfs_status pfs_result_sizes(pfs_ctx* ctx, uint64_t* ncls, uint64_t* nlits, uint64_t* nunits,
                             uint64_t* nstack, uint32_t* nrounds);
 /* Simplified clauses in simplifier order (the reference's scnf order):

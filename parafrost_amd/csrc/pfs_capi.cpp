@@ -217,9 +217,45 @@ pfs_status pfs_upload(pfs_ctx* c, uint32_t maxvar, uint64_t ncls, const uint64_t
       if (offsets[i + 1] < offsets[i] + 1) throw std::invalid_argument("empty clause or bad offsets");
     for (uint32_t i = 0; i < ntrail; i++)
       if ((trail[i] >> 1) == 0 || (trail[i] >> 1) > maxvar) throw std::invalid_argument("trail literal out of range");
+    c->uploaded = false;
     c->s->upload(maxvar, ncls, offsets, lits, flags, lbd, vstate, assumed, vorg, trail, ntrail);
     c->uploaded = true;
     c->simplified = false;
+    c->result = PFS_RESULT_UNSOLVED;
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_parse_dimacs(pfs_ctx* c, const char* text, uint64_t nbytes, pfs_parse_info* info) {
+  if (!c || (!text && nbytes)) return PFS_BAD_INPUT;
+  return guard(c, [&] {
+    c->uploaded = false;
+    c->simplified = false;
+    const Simplifier::ParseInfo r = c->s->parse_dimacs(text, nbytes);
+    c->uploaded = true;
+    c->result = r.status == 0 ? PFS_RESULT_UNSAT : PFS_RESULT_UNSOLVED;
+    if (info) {
+      info->maxvar = r.maxvar;
+      info->status = r.status;
+      info->header_clauses = r.header_clauses;
+      info->read_clauses = r.read_clauses;
+      info->clauses = r.clauses;
+      info->literals = r.literals;
+      info->trail = r.trail;
+    }
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_download_trail(pfs_ctx* c, uint32_t* lits, uint64_t* n) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    if (n) *n = c->s->trail_size();
+    if (lits) c->s->download_trail(lits);
     return PFS_OK;
   });
 }
@@ -281,8 +317,8 @@ pfs_status pfs_simplify(pfs_ctx* c, int32_t* result) {
 pfs_status pfs_result_sizes(pfs_ctx* c, uint64_t* ncls, uint64_t* nlits, uint64_t* nunits, uint64_t* nstack,
                             uint32_t* nrounds) {
   if (!c) return PFS_BAD_INPUT;
-  if (!c->simplified) {
-    c->err = "simplify has not run";
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
     return PFS_BAD_STATE;
   }
   const bool unsat = c->result == PFS_RESULT_UNSAT;
@@ -296,7 +332,7 @@ pfs_status pfs_result_sizes(pfs_ctx* c, uint64_t* ncls, uint64_t* nlits, uint64_
 
 pfs_status pfs_download_cnf(pfs_ctx* c, uint64_t* offsets, uint32_t* lits, uint8_t* flags, uint32_t* lbd) {
   if (!c) return PFS_BAD_INPUT;
-  if (!c->simplified) return PFS_BAD_STATE;
+  if (!c->uploaded) return PFS_BAD_STATE;
   if (c->result == PFS_RESULT_UNSAT) {
     if (offsets) offsets[0] = 0;
     return PFS_OK;
@@ -309,7 +345,7 @@ pfs_status pfs_download_cnf(pfs_ctx* c, uint64_t* offsets, uint32_t* lits, uint8
 
 pfs_status pfs_download_units(pfs_ctx* c, uint32_t* lits) {
   if (!c || !lits) return PFS_BAD_INPUT;
-  if (!c->simplified) return PFS_BAD_STATE;
+  if (!c->uploaded) return PFS_BAD_STATE;
   const auto& u = c->s->new_units();
   std::memcpy(lits, u.data(), u.size() * 4);
   return PFS_OK;
@@ -317,7 +353,7 @@ pfs_status pfs_download_units(pfs_ctx* c, uint32_t* lits) {
 
 pfs_status pfs_download_vstate(pfs_ctx* c, uint8_t* vstate) {
   if (!c || !vstate) return PFS_BAD_INPUT;
-  if (!c->simplified) return PFS_BAD_STATE;
+  if (!c->uploaded) return PFS_BAD_STATE;
   return guard(c, [&] {
     c->s->download_vstate(vstate);
     return PFS_OK;
@@ -366,7 +402,7 @@ pfs_status pfs_map_lits(pfs_ctx* c, uint32_t* lits, uint64_t n) {
 
 pfs_status pfs_download_stack(pfs_ctx* c, uint32_t* words) {
   if (!c || !words) return PFS_BAD_INPUT;
-  if (!c->simplified) return PFS_BAD_STATE;
+  if (!c->uploaded) return PFS_BAD_STATE;
   return guard(c, [&] {
     c->s->download_stack(words);
     return PFS_OK;
@@ -375,14 +411,14 @@ pfs_status pfs_download_stack(pfs_ctx* c, uint32_t* words) {
 
 pfs_status pfs_proof_size(pfs_ctx* c, uint64_t* nbytes) {
   if (!c || !nbytes) return PFS_BAD_INPUT;
-  if (!c->simplified) return PFS_BAD_STATE;
+  if (!c->uploaded) return PFS_BAD_STATE;
   *nbytes = c->s->proof_bytes();
   return PFS_OK;
 }
 
 pfs_status pfs_download_proof(pfs_ctx* c, uint8_t* bytes) {
   if (!c || (!bytes && c->simplified && c->s->proof_bytes())) return PFS_BAD_INPUT;
-  if (!c->simplified) return PFS_BAD_STATE;
+  if (!c->uploaded) return PFS_BAD_STATE;
   return guard(c, [&] {
     c->s->download_proof(bytes);
     return PFS_OK;

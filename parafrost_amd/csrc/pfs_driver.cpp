@@ -209,27 +209,252 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   unassigned_ = V_ - ntrail - melted;
   stack_n_ = 0;
   units_out_.clear();
+  rounds_.clear();
+  proof_n_ = 0;
   refuted_ = false;
   resident_ = false;
   fill_u8(P<u8>(mark_), 0, lcap_, st_);
   // extract: sort literals, signatures
   k_prep(make_db(0), 0, st_);
-  if (keep_input_) {
-    in_ncls_ = ncls;
-    in_nlits_ = L;
-    in_trail_ = ntrail;
-    in_unassigned_ = unassigned_;
-    mem_.ensure(in_c_off_, ncls * 4, 0, st_);
-    mem_.ensure(in_lits_, L * 4, 0, st_);
-    mem_.ensure(in_assign_, ((size_t)V_ + 1) * 4, 0, st_);
-    mem_.ensure(in_vstate_, (size_t)V_ + 1, 0, st_);
-    mem_.ensure(in_trail_buf_, ((size_t)ntrail + 1) * 4, 0, st_);
-    d2d(in_c_off_.p, c_off_[0].p, ncls * 4, st_);
-    d2d(in_lits_.p, lits_[0].p, L * 4, st_);
-    d2d(in_assign_.p, assign_.p, ((size_t)V_ + 1) * 4, st_);
-    d2d(in_vstate_.p, vstate_.p, (size_t)V_ + 1, st_);
-    d2d(in_trail_buf_.p, trail_.p, (size_t)ntrail * 4, st_);
+  save_input();
+  stream_sync(st_);
+}
+
+void Simplifier::save_input() {
+  if (!keep_input_) return;
+  const uint64_t ncls = ncls_, L = lits_end_, ntrail = trail_n_;
+  in_ncls_ = ncls;
+  in_nlits_ = L;
+  in_trail_ = ntrail;
+  in_unassigned_ = unassigned_;
+  mem_.ensure(in_c_off_, std::max<uint64_t>(ncls, 1) * 4, 0, st_);
+  mem_.ensure(in_lits_, std::max<uint64_t>(L, 1) * 4, 0, st_);
+  mem_.ensure(in_assign_, ((size_t)V_ + 1) * 4, 0, st_);
+  mem_.ensure(in_vstate_, (size_t)V_ + 1, 0, st_);
+  mem_.ensure(in_trail_buf_, ((size_t)ntrail + 1) * 4, 0, st_);
+  d2d(in_c_off_.p, c_off_[cur_].p, ncls * 4, st_);
+  d2d(in_lits_.p, lits_[cur_].p, L * 4, st_);
+  d2d(in_assign_.p, assign_.p, ((size_t)V_ + 1) * 4, st_);
+  d2d(in_vstate_.p, vstate_.p, (size_t)V_ + 1, st_);
+  d2d(in_trail_buf_.p, trail_.p, (size_t)ntrail * 4, st_);
+}
+
+// ---------------------------------------------------------------- DIMACS ingestion
+namespace {
+bool dimacs_blank(unsigned char c) { return (c >= 9 && c <= 13) || c == 32; }  // dimacs.h:33
+bool dimacs_digit(unsigned char c) { return c >= '0' && c <= '9'; }
+// releases temporaries when the scope ends, also on an exception
+struct BufScope {
+  DeviceMemory& m;
+  std::vector<DeviceMemory::Buf*> bufs;
+  ~BufScope() {
+    for (DeviceMemory::Buf* b : bufs) m.release(*b);
   }
+};
+}  // namespace
+
+// The header on the host (dimacs.cpp:81-108): blanks, comment lines, then
+// "p cnf V C"; the clause part after it on the device.  Parsing drops
+// duplicate literals, tautologies and unit clauses (the units are assigned
+// in text order) and reports empty or conflicting clauses (toClause,
+// dimacs.cpp:113-170); then the root units are propagated and the clauses
+// shrunk at the top level like Solver() + rootify/shrinkTop before the
+// first simplify (solver.cpp:49, simplify.cpp:163-164,
+// shrink.cpp:152-216).  The result equals the reference's simplifier input
+// up to the order of literals in a clause (the simplifier sorts them) and
+// the order of the trail.
+Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
+  ParseInfo info;
+  const unsigned char* t = (const unsigned char*)text;
+  uint64_t i = 0, V = 0, C = 0;
+  bool header = false;
+  while (i < n) {
+    while (i < n && dimacs_blank(t[i])) i++;
+    if (i >= n || t[i] == 0 || t[i] == '%') break;
+    if (t[i] == 'c') {  // eatLine
+      while (i < n && t[i] != 0 && t[i++] != '\n') {
+      }
+      continue;
+    }
+    if (t[i] != 'p') throw std::invalid_argument("clauses before the 'p cnf' header");
+    if (n - i < 5 || memcmp(t + i, "p cnf", 5)) throw std::invalid_argument("header has wrong format");
+    i += 5;
+    auto number = [&](const char* what) -> uint64_t {  // toInteger, dimacs.h:101-111
+      while (i < n && dimacs_blank(t[i])) i++;
+      bool neg = false;
+      if (i < n && (t[i] == '-' || t[i] == '+')) neg = t[i++] == '-';
+      if (i >= n || !dimacs_digit(t[i])) throw std::invalid_argument("expected a digit in the header");
+      uint64_t v = 0;
+      while (i < n && dimacs_digit(t[i])) v = std::min<uint64_t>(v * 10 + (t[i++] - '0'), 1ull << 40);
+      if (neg) throw std::invalid_argument(std::string("number of ") + what + " in header is negative");
+      return v;
+    };
+    V = number("variables");
+    if (V == 0) throw std::invalid_argument("zero number of variables in header");
+    if (V >= 0x7FFFFFFEull) throw std::invalid_argument("number of variables not supported");
+    C = number("clauses");
+    if (C == 0) throw std::invalid_argument("zero number of clauses in header");
+    header = true;
+    break;
+  }
+  if (!header) throw std::invalid_argument("no 'p cnf' header");
+  if (n >= (1ull << 40)) throw std::invalid_argument("text too large");
+  info.maxvar = (u32)V;
+  info.header_clauses = C;
+
+  // ---- lexing on the device
+  const uint64_t begin = i;
+  const uint64_t ntiles = lex_tiles(n);
+  const size_t padded = (size_t)ntiles * 4096 + 16;
+  Buf text_d, cnt_d, pmark_d, pos_d, mask_d, lits_d, ends_d, zpos_d, ulit_d, first_d, win_d, upos_d;
+  BufScope scope{mem_, {&text_d, &cnt_d, &pmark_d, &pos_d, &mask_d, &lits_d, &ends_d, &zpos_d, &ulit_d, &first_d,
+                        &win_d, &upos_d}};
+  mem_.ensure(text_d, padded, 0, st_);
+  mem_.ensure(cnt_d, ((size_t)ntiles + 1) * 4, 0, st_);
+  mem_.ensure(pmark_d, 64, 0, st_);
+  fill_u8(P<u8>(text_d) + n, ' ', padded - n, st_);
+  h2d(text_d.p, text, n, st_);
+  unsigned long long* mark = (unsigned long long*)pmark_d.p;  // 3 words
+  u32* err = (u32*)(mark + 3);
+  u32* word = err + 1;  // scratch words: totals
+  fill_u32((u32*)mark, 0xFFFFFFFFu, 6, st_);
+  fill_u32(err, 0, 5, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes((size_t)ntiles + 1)), 0, st_);
+  LexArgs a{P<u8>(text_d), begin, n, nullptr, P<u32>(cnt_d), nullptr, nullptr, mark, err, (u32)V};
+  k_lex(LEX_SPECIAL, a, ntiles, st_);
+  scan_u32(P<u32>(cnt_d), P<u32>(cnt_d), ntiles, word, tmp_.p, st_);
+  struct {
+    unsigned long long mark[3];
+    u32 err, word[4];
+  } h{};
+  d2h(&h, pmark_d.p, sizeof(h), st_);
+  stream_sync(st_);
+  uint64_t end = std::min<uint64_t>({n, h.mark[0], h.mark[1]});
+  const u32 ncom = h.word[0];
+  if (ncom) {
+    // comment lines: their bytes masked, then the first '%' token outside them
+    mem_.ensure(pos_d, (size_t)ncom * 8, 0, st_);
+    mem_.ensure(mask_d, padded, 0, st_);
+    fill_u8(P<u8>(mask_d), 0, padded, st_);
+    a.pos = P<u64>(pos_d);
+    k_lex(LEX_COMMENTS, a, ntiles, st_);
+    k_lex_mask(P<u8>(text_d), n, P<u64>(pos_d), ncom, P<u8>(mask_d), st_);
+    a.mask = P<u8>(mask_d);
+    fill_u32((u32*)(mark + 1), 0xFFFFFFFFu, 2, st_);
+    k_lex(LEX_PERCENT, a, ntiles, st_);
+    d2h(&h, pmark_d.p, sizeof(h), st_);
+    stream_sync(st_);
+    end = std::min<uint64_t>({n, h.mark[0], h.mark[1]});
+  }
+  a.end = end;
+  k_lex(LEX_COUNT, a, ntiles, st_);
+  scan_u32(P<u32>(cnt_d), P<u32>(cnt_d), ntiles, word, tmp_.p, st_);
+  u32 T = 0;
+  d2h(&T, word, 4, st_);
+  stream_sync(st_);
+  mem_.ensure(lits_d, ((size_t)T + 1) * 4, 0, st_);
+  a.lits = P<u32>(lits_d);
+  k_lex(LEX_EMIT, a, ntiles, st_);
+  u32 last = 0;
+  if (T) d2h(&last, P<u32>(lits_d) + T - 1, 4, st_);
+  d2h(&h, pmark_d.p, sizeof(h), st_);
+  stream_sync(st_);
+  if (h.err & LEXE_TOKEN)
+    throw std::invalid_argument("malformed token at byte " + std::to_string(h.mark[2]));
+  if (h.err & LEXE_VAR) throw std::invalid_argument("too many variables (a literal above the header's count)");
+  if (T && last != 0) throw std::invalid_argument("the last clause is not terminated by 0");
+
+  // ---- clauses into the pool
+  mem_.ensure(ends_d, ((size_t)T + 1) * 4, 0, st_);
+  mem_.ensure(zpos_d, ((size_t)T + 1) * 4, 0, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes((size_t)T + 1)), 0, st_);
+  k_clause_ends(P<u32>(lits_d), T, P<u32>(zpos_d), tmp_.p, word, P<u32>(ends_d), st_);
+  u32 NC = 0;
+  d2h(&NC, word, 4, st_);
+  stream_sync(st_);
+  mem_.release(zpos_d);
+  const uint64_t L = (uint64_t)T - NC + (uint64_t)CL_HDR * NC;
+  if (NC >= INF / 2 || L >= INF / 4) throw AllocFail{"formula too large for 32-bit clause ids"};
+  info.read_clauses = NC;
+  V_ = (u32)V;
+  ccap_ = std::max<uint64_t>(2 * (uint64_t)NC + 1024, 4096);
+  lcap_ = std::max<uint64_t>(2 * L + 4096, 16384);
+  alloc_state();
+  cur_ = 0;
+  ncls_ = NC;
+  lits_end_ = L;
+  DB db = make_db(0);
+  k_parse_build(db, P<u32>(lits_d), P<u32>(ends_d), err, st_);
+  const size_t V1 = (size_t)V_ + 1;
+  fill_u32(P<u32>(assign_), 0, V1, st_);
+  fill_u8(P<u8>(vstate_), 0, V1, st_);
+  fill_u8(P<u8>(assumed_), 0, V1, st_);
+  iota_u32(P<u32>(vorg_), V1, st_);
+  fill_u8(P<u8>(mark_), 0, lcap_, st_);
+  k_prep(db, 0, st_);
+  mem_.release(lits_d);
+  mem_.release(ends_d);
+  mem_.ensure(ulit_d, ((size_t)NC + 1) * 4, 0, st_);
+  mem_.ensure(first_d, V1 * 4, 0, st_);
+  mem_.ensure(win_d, ((size_t)NC + 1) * 4, 0, st_);
+  mem_.ensure(upos_d, ((size_t)NC + 1) * 4, 0, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes((size_t)NC + 1)), 0, st_);
+  fill_u32(word, 0, 2, st_);
+  k_parse_dedup(db, P<u32>(ulit_d), word + 1, err, st_);
+  k_parse_units(db, P<u32>(ulit_d), NC, P<u32>(first_d), P<u32>(win_d), P<u32>(upos_d), tmp_.p, word,
+                P<u32>(trail_), err, st_);
+  d2h(&h, pmark_d.p, sizeof(h), st_);
+  stream_sync(st_);
+  if (h.err & LEXE_SIZE) throw std::invalid_argument("clause longer than 2^24-1 literals");
+  if (h.word[1] > C) throw std::invalid_argument("too many clauses");
+  const u32 ntrail = h.word[0];
+
+  // ---- the state of a fresh upload, then root propagation and the shrink
+  h_assign_.assign(V1, 0);
+  std::vector<u32> tr(ntrail);
+  if (ntrail) d2h(tr.data(), trail_.p, (size_t)ntrail * 4, st_);
+  stream_sync(st_);
+  for (u32 u : tr) h_assign_[ABS(u)] = u + 1;
+  trail_n_ = ntrail;
+  propagated_ = 0;
+  unassigned_ = (u32)V - ntrail;
+  stack_n_ = 0;
+  units_out_.clear();
+  rounds_.clear();
+  pending_units_.clear();
+  refuted_ = (h.err & LEXE_UNSAT) != 0;
+  resident_ = false;
+  map_called_ = map_applied_ = remapped_ = false;
+  orgmax_ = (u32)V;
+  proof_n_ = 0;
+  pl_.clear();
+  pst_.clear();
+  if (!refuted_) {
+    const int32_t proof = opt.proof_en;
+    opt.proof_en = 0;  // the parse's own proof lines are not part of the simplifier's proof
+    create_ot();
+    const bool ok = prop();
+    opt.proof_en = proof;
+    if (!ok) refuted_ = true;
+  }
+  if (!refuted_) gc();
+  units_out_.clear();
+  rounds_.clear();
+  in_units_.resize(trail_n_);
+  if (trail_n_) d2h(in_units_.data(), trail_.p, trail_n_ * 4, st_);
+  stream_sync(st_);
+  if (refuted_) ncls_ = 0, lits_end_ = 0;
+  save_input();
+  stream_sync(st_);
+  info.status = refuted_ ? 0 : 2;
+  info.clauses = ncls_;
+  info.literals = out_nlits();
+  info.trail = trail_n_;
+  return info;
+}
+
+void Simplifier::download_trail(uint32_t* lits) {
+  if (trail_n_) d2h(lits, trail_.p, trail_n_ * 4, st_);
   stream_sync(st_);
 }
 

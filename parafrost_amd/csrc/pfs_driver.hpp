@@ -66,6 +66,15 @@ class Simplifier {
   void upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits,
               const uint8_t* flags, const uint32_t* lbd, const uint8_t* vstate, const uint8_t* assumed,
               const uint32_t* vorg, const uint32_t* trail, uint32_t ntrail);
+  // DIMACS text -> resident formula on the device (parse, src/cpu/dimacs.cpp:41-170,
+  // then root propagation and the top-level shrink, src/cpu/shrink.cpp:152-216)
+  struct ParseInfo {
+    uint32_t maxvar = 0, status = 2;
+    uint64_t header_clauses = 0, read_clauses = 0, clauses = 0, literals = 0, trail = 0;
+  };
+  ParseInfo parse_dimacs(const char* text, uint64_t n);
+  uint64_t trail_size() const { return trail_n_; }
+  void download_trail(uint32_t* lits);  // the root trail of the resident formula
   void rewind();                    // restore the last uploaded input (device copy)
   void bind() { stream_bind(st_); }  // select this context's device on the calling thread
   // incremental use: add clauses / replace the assumed variables of the
@@ -183,6 +192,7 @@ class Simplifier {
   // variable map of the last map() call: old -> new variables, the old
   // root assignments (for map_lits) and the packed-output staging buffer
   Buf map_, map_assign_, out_;
+  void save_input();  // device copy of the resident input for rewind()
   bool map_called_ = false, map_applied_ = false, remapped_ = false;
   uint32_t map_old_nvars_ = 0, map_first_ = 0, map_first_true_ = 0;
   uint32_t orgmax_ = 0;  // largest original variable id handed out

@@ -1536,4 +1536,68 @@ PF_HD void pack_clause(const DB& db, u32 i, const u32* start, u64* offsets, u32*
   if (i + 1 == db.ncls) offsets[db.ncls] = (u64)s + n;
 }
 
+// ---------------------------------------------------------------- DIMACS ingestion
+PF_HD bool lex_blank(u32 c) { return (c >= 9 && c <= 13) || c == 32; }  // isSpace, dimacs.h:33
+PF_HD bool lex_digit(u32 c) { return c >= '0' && c <= '9'; }
+// byte i (value c, byte before it 'prev') starts a token of the clause part [begin, end)
+PF_HD bool lex_start(u64 i, u32 c, u32 prev, u64 begin, u64 end) {
+  return i >= begin && i < end && !lex_blank(c) && (i == begin || lex_blank(prev));
+}
+// toInteger (dimacs.h:101-111) for the token at byte i: the literal
+// 2 v + sign (0 ends a clause); bad: not a signed number followed by a
+// blank; big: v above maxvar
+PF_HD u32 lex_literal(const u8* text, u64 i, u64 end, u32 maxvar, bool& bad, bool& big) {
+  u64 p = i;
+  u32 sign = 0, c = text[p];
+  if (c == '-' || c == '+') sign = c == '-', p++;
+  u64 v = 0;
+  u32 nd = 0;
+  while (p < end && lex_digit(c = text[p])) {
+    v = v * 10 + (c - '0');
+    if (v > 0x7FFFFFFFull) v = 0x80000000ull;  // saturates: reported as above maxvar
+    p++, nd++;
+  }
+  bad = nd == 0 || (p < end && !lex_blank(text[p]));
+  big = !bad && v > maxvar;
+  return bad || !v ? 0u : (u32)(2 * v + sign);
+}
+// clause i of a parse from the literals between the (i-1)-th and the i-th 0:
+// in the pool it starts at its literal offset plus two words per earlier
+// clause; returns false when it is too long for the header
+PF_HD bool parse_build_clause(const DB& db, const u32* lits, const u32* ends, u32 i) {
+  const u32 s = i ? ends[i - 1] + 1 : 0u, n = ends[i] - s, off = s + 2 * i;
+  u32* h = db.lits + off;
+  const u32 m = n < CL_MAX_SIZE ? n : CL_MAX_SIZE;
+  h[0] = make_hdr(m, ST_ORIGINAL);
+  h[1] = 0;
+  h[2] = 0;
+  for (u32 q = 0; q < m; q++) h[CL_HDR + q] = lits[s + q];
+  db.c_off[i] = off;
+  return n <= CL_MAX_SIZE;
+}
+// a parsed clause with sorted literals (toClause, dimacs.cpp:113-170):
+// duplicates dropped; a tautology is removed, a unit clause becomes the
+// unit (*unit) and is removed too.  Returns 0 kept, 1 tautology, 2 empty, 3 unit.
+PF_HD u32 parse_dedup_clause(const DB& db, u32 c, u32* unit) {
+  u32* l = clits(db, c);
+  const u32 n = csize(db, c);
+  u32 m = 0;
+  bool taut = false;
+  for (u32 q = 0; q < n; q++) {
+    const u32 x = l[q];
+    if (m && l[m - 1] == x) continue;
+    if (m && (l[m - 1] ^ 1u) == x) taut = true;
+    l[m++] = x;
+  }
+  *unit = 0;
+  if (taut || m < 2) {
+    if (m == 1 && !taut) *unit = l[0];
+    set_cflags(db, c, ST_DELETED);
+    return taut ? 1u : (m == 0 ? 2u : 3u);
+  }
+  set_csize(db, c, m);
+  set_csig(db, c, calc_sig(l, m));
+  return 0;
+}
+
 }  // namespace pfs

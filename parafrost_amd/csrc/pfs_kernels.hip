@@ -1187,4 +1187,178 @@ void k_pack_cnf(const DB& db, u32* start_tmp, void* tmp, u32* total, u64* offset
   KLAUNCH(pack_k, grid_for(db.ncls, 256), 256, st, db, start_tmp, offsets, lits, flags, lbd);
 }
 
+// ---------------------------------------------------------------- DIMACS ingestion
+// The clause part of a DIMACS text (src/cpu/dimacs.cpp:41-110, toClause
+// :113-170) is lexed on the device in 4096-byte tiles: workgroup b owns
+// bytes [4096 b, 4096 b + 4096) of the text, each of its 256 threads 16
+// consecutive bytes read with one 16-byte load (the text buffer is padded
+// with blanks to whole tiles).  A token starts at a byte that is not blank
+// (bytes 9-13 and 32, dimacs.h:33) and follows a blank or the start of the
+// clause part.  Bytes of comment lines ('c' tokens, to the end of the line)
+// are masked; the clause part ends at the first NUL byte or '%' token.
+constexpr u32 LEX_T = 256, LEX_B = 16, LEX_TILE = LEX_T * LEX_B;
+template <u32 MODE>
+__global__ void __launch_bounds__(LEX_T) lex_k(LexArgs a) {
+  __shared__ u32 lw[NW];
+  const u64 base = (u64)blockIdx.x * LEX_TILE + (u64)threadIdx.x * LEX_B;
+  const uint4 w = *(const uint4*)(a.text + base);
+  u8 b[LEX_B];
+  *(uint4*)b = w;
+  u32 prev = base ? a.text[base - 1] : 32u;
+  u32 mine = 0;  // starts of the kind this mode counts or emits
+  u32 kinds = 0;  // bit j: byte j starts such a token
+#pragma unroll
+  for (u32 j = 0; j < LEX_B; j++) {
+    const u64 i = base + j;
+    const u32 c = b[j];
+    const bool start = lex_start(i, c, prev, a.begin, a.end);
+    prev = c;
+    if (MODE == LEX_SPECIAL) {
+      if (i >= a.begin && i < a.end && c == 0) atomicMin(&a.mark[0], (unsigned long long)i);
+      if (start && c == '%') atomicMin(&a.mark[1], (unsigned long long)i);
+      if (start && c == 'c') kinds |= 1u << j;
+    } else if (MODE == LEX_COMMENTS) {
+      if (start && c == 'c') kinds |= 1u << j;
+    } else if (MODE == LEX_PERCENT) {
+      if (start && c == '%' && !a.mask[i]) atomicMin(&a.mark[1], (unsigned long long)i);
+    } else {
+      if (start && c != 0 && !(a.mask && a.mask[i])) kinds |= 1u << j;
+    }
+  }
+  mine = __popc(kinds);
+  if (MODE == LEX_PERCENT) return;
+  u32 tot;
+  const u32 pre = block_excl_scan(mine, lw, &tot);
+  if (MODE == LEX_SPECIAL || MODE == LEX_COUNT) {
+    if (threadIdx.x == 0) a.cnt[blockIdx.x] = tot;
+    return;
+  }
+  u32 k = a.cnt[blockIdx.x] + pre;
+  while (kinds) {
+    const u32 j = __ffs(kinds) - 1;
+    kinds &= kinds - 1;
+    const u64 i = base + j;
+    if (MODE == LEX_COMMENTS) {
+      a.pos[k++] = i;
+      continue;
+    }
+    bool bad, big;
+    a.lits[k++] = lex_literal(a.text, i, a.end, a.maxvar, bad, big);
+    if (bad) {
+      atomicOr(a.err, (u32)LEXE_TOKEN);
+      atomicMin(&a.mark[2], (unsigned long long)i);
+    }
+    if (big) atomicOr(a.err, (u32)LEXE_VAR);  // "too many variables"
+  }
+}
+// one thread per comment start: its bytes up to and including the newline
+__global__ void lex_mask_k(const u8* text, u64 end, const u64* pos, u32 n, u8* mask) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (u64 i = pos[k]; i < end; i++) {
+    mask[i] = 1;
+    if (text[i] == '\n') break;
+  }
+}
+u64 lex_tiles(u64 n) { return (n + LEX_TILE - 1) / LEX_TILE; }
+void k_lex(u32 mode, const LexArgs& a, u64 ntiles, Stream* st) {
+  if (!ntiles) return;
+  switch (mode) {
+    case LEX_SPECIAL: KLAUNCH(lex_k<LEX_SPECIAL>, (unsigned)ntiles, LEX_T, st, a); break;
+    case LEX_COMMENTS: KLAUNCH(lex_k<LEX_COMMENTS>, (unsigned)ntiles, LEX_T, st, a); break;
+    case LEX_PERCENT: KLAUNCH(lex_k<LEX_PERCENT>, (unsigned)ntiles, LEX_T, st, a); break;
+    case LEX_COUNT: KLAUNCH(lex_k<LEX_COUNT>, (unsigned)ntiles, LEX_T, st, a); break;
+    default: KLAUNCH(lex_k<LEX_EMIT>, (unsigned)ntiles, LEX_T, st, a); break;
+  }
+}
+void k_lex_mask(const u8* text, u64 end, const u64* pos, u32 n, u8* mask, Stream* st) {
+  if (!n) return;
+  KLAUNCH(lex_mask_k, grid_for(n, 256), 256, st, text, end, pos, n, mask);
+}
+// clause ends: ends[c] = index of the c-th 0 among the literals
+__global__ void zero_flags_k(const u32* lits, u64 n, u32* z) {
+  const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) z[k] = lits[k] == 0;
+}
+__global__ void zero_emit_k(const u32* lits, u64 n, const u32* pos, u32* ends) {
+  const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && lits[k] == 0) ends[pos[k]] = (u32)k;
+}
+void k_clause_ends(const u32* lits, u64 n, u32* tmp_pos, void* tmp, u32* total, u32* ends, Stream* st) {
+  if (!n) {
+    fill_u32(total, 0, 1, st);
+    return;
+  }
+  KLAUNCH(zero_flags_k, grid_for(n, 256), 256, st, lits, n, tmp_pos);
+  scan_u32(tmp_pos, tmp_pos, n, total, tmp, st);
+  KLAUNCH(zero_emit_k, grid_for(n, 256), 256, st, lits, n, tmp_pos, ends);
+}
+// clause i holds the literals between the (i-1)-th and the i-th 0; in the
+// pool it starts at that literal offset plus two words per earlier clause
+__global__ void parse_build_k(DB db, const u32* lits, const u32* ends, u32* err) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls && !parse_build_clause(db, lits, ends, i)) atomicOr(err, (u32)LEXE_SIZE);
+}
+void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(parse_build_k, grid_for(db.ncls, 256), 256, st, db, lits, ends, err);
+}
+// after the literal sort: duplicates dropped, tautologies removed, empty
+// clauses reported, unit clauses turned into unit records (toClause,
+// dimacs.cpp:120-166); ulit[i] = the unit literal of clause i or 0;
+// kept[0] counts the clauses that stay
+__global__ void parse_dedup_k(DB db, u32* ulit, u32* kept, u32* err) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= db.ncls) return;
+  const u32 kind = parse_dedup_clause(db, db.c_off[i], &ulit[i]);
+  if (kind == 0) atomicAdd(kept, 1u);
+  if (kind == 2) atomicOr(err, (u32)LEXE_UNSAT);
+}
+void k_parse_dedup(const DB& db, u32* ulit, u32* kept, u32* err, Stream* st) {
+  if (!db.ncls) return;
+  KLAUNCH(parse_dedup_k, grid_for(db.ncls, 256), 256, st, db, ulit, kept, err);
+}
+// unit clauses in text order: the first one of each variable is assigned
+// (enqueueUnit); a later one of the other sign is a conflict
+__global__ void parse_claim_k(const u32* ulit, u32 n, u32* first) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && ulit[i]) atomicMin(&first[ABS(ulit[i])], i);
+}
+__global__ void parse_settle_k(const u32* ulit, u32 n, const u32* first, u32* win, u32* err) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 u = ulit[i];
+  win[i] = 0;
+  if (!u) return;
+  const u32 f = first[ABS(u)];
+  if (ulit[f] != u) atomicOr(err, (u32)LEXE_UNSAT);
+  win[i] = f == i;
+}
+__global__ void parse_trail_k(DB db, const u32* ulit, u32 n, const u32* win, const u32* pos, u32* trail) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !win[i]) return;
+  const u32 u = ulit[i];
+  trail[pos[i]] = u;
+  db.assign[ABS(u)] = u + 1;
+  db.vstate[ABS(u)] = VS_FROZEN;
+}
+void k_parse_units(const DB& db, const u32* ulit, u32 n, u32* first, u32* win, u32* pos, void* tmp, u32* total,
+                   u32* trail, u32* err, Stream* st) {
+  fill_u32(total, 0, 1, st);
+  if (!n) return;
+  fill_u32(first, 0xFFFFFFFFu, (size_t)db.nvars + 1, st);
+  KLAUNCH(parse_claim_k, grid_for(n, 256), 256, st, ulit, n, first);
+  KLAUNCH(parse_settle_k, grid_for(n, 256), 256, st, ulit, n, first, win, err);
+  scan_u32(win, pos, n, total, tmp, st);
+  KLAUNCH(parse_trail_k, grid_for(n, 256), 256, st, db, ulit, n, win, pos, trail);
+}
+__global__ void iota_u32_k(u32* a, u64 n) {
+  const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = (u32)k;
+}
+void iota_u32(u32* a, size_t n, Stream* st) {
+  if (!n) return;
+  KLAUNCH(iota_u32_k, grid_for(n, 256), 256, st, a, (u64)n);
+}
+
 }  // namespace pfs

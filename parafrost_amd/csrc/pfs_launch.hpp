@@ -166,4 +166,42 @@ void k_map_lits(const VarMap& m, u32* lits, u64 n, Stream*);
 void k_pack_cnf(const DB& db, u32* start_tmp, void* tmp, u32* total, u64* offsets, u32* lits, u8* flags, u32* lbd,
                 Stream*);
 
+// ---- DIMACS ingestion (src/cpu/dimacs.cpp:41-170)
+// lexer over the clause part of a text in 4096-byte tiles (text padded with
+// blanks to whole tiles); modes: 0 special bytes (comment starts counted per
+// tile, first NUL and '%' token in mark[0], mark[1]), 1 comment start
+// positions, 2 first '%' token outside comments, 3 literal tokens counted
+// per tile, 4 literals written (cnt: exclusive scan of mode 3's counts);
+// malformed tokens set err bit 1 (first at mark[2]), variables above maxvar
+// bit 2
+enum LexMode : u32 { LEX_SPECIAL = 0, LEX_COMMENTS = 1, LEX_PERCENT = 2, LEX_COUNT = 3, LEX_EMIT = 4 };
+enum LexErr : u32 { LEXE_TOKEN = 1, LEXE_VAR = 2, LEXE_SIZE = 4, LEXE_UNSAT = 8 };
+struct LexArgs {
+  const u8* text;
+  u64 begin, end;                // clause part [begin, end)
+  const u8* mask;                // comment bytes (null: none)
+  u32* cnt;                      // per tile: counts, or their exclusive scan (emit modes)
+  u64* pos;                      // comment starts (mode 1)
+  u32* lits;                     // literals, 0 ends a clause (mode 4)
+  unsigned long long* mark;      // first NUL, first '%' token, first malformed token
+  u32* err;
+  u32 maxvar;
+};
+u64 lex_tiles(u64 n);
+void k_lex(u32 mode, const LexArgs& a, u64 ntiles, Stream*);
+void k_lex_mask(const u8* text, u64 end, const u64* pos, u32 n, u8* mask, Stream*);
+// ends[c] = index of the c-th 0 in lits[0..n); *total = number of 0s
+void k_clause_ends(const u32* lits, u64 n, u32* tmp_pos, void* tmp, u32* total, u32* ends, Stream*);
+// clause i of the pool from the literals between 0s (err bit 4: too long)
+void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Stream*);
+// sorted clauses: duplicates dropped, tautologies and units removed
+// (ulit[i]: the unit), *kept = clauses that stay, err bit 8: empty clause
+void k_parse_dedup(const DB& db, u32* ulit, u32* kept, u32* err, Stream*);
+// unit clauses in order: first of each variable assigned and written to
+// the trail (*total of them), opposite units set err bit 8; first needs
+// nvars+1 words, win and pos n words
+void k_parse_units(const DB& db, const u32* ulit, u32 n, u32* first, u32* win, u32* pos, void* tmp, u32* total,
+                   u32* trail, u32* err, Stream*);
+void iota_u32(u32* a, size_t n, Stream*);
+
 }  // namespace pfs

@@ -27,7 +27,7 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_kernel_count", "pfs_kernel_stats", "pfs_memory", "pfs_last_error", "pfs_version",
            "pfs_device_count", "pfs_batch_simplify", "pfs_append", "pfs_assume", "pfs_server_create",
            "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy", "pfs_map",
-           "pfs_download_map", "pfs_map_lits")
+           "pfs_download_map", "pfs_map_lits", "pfs_parse_dimacs", "pfs_download_trail")
 
 
 class PfsError(RuntimeError):
@@ -45,6 +45,12 @@ class _RoundStats(ctypes.Structure):
                 ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("clauses_before", ctypes.c_uint64), ("literals_before", ctypes.c_uint64),
                 ("elected_occurs", ctypes.c_uint64), ("elected_bytes", ctypes.c_uint64)]
+
+
+class _ParseInfo(ctypes.Structure):
+    _fields_ = [("maxvar", ctypes.c_uint32), ("status", ctypes.c_uint32), ("header_clauses", ctypes.c_uint64),
+                ("read_clauses", ctypes.c_uint64), ("clauses", ctypes.c_uint64), ("literals", ctypes.c_uint64),
+                ("trail", ctypes.c_uint64)]
 
 
 class _MapInfo(ctypes.Structure):
@@ -105,6 +111,8 @@ def load_library():
     L.pfs_map.argtypes = [vp, i32, ctypes.POINTER(_MapInfo)]
     L.pfs_download_map.argtypes = [vp, vp]
     L.pfs_map_lits.argtypes = [vp, vp, u64]
+    L.pfs_parse_dimacs.argtypes = [vp, ctypes.c_char_p, u64, ctypes.POINTER(_ParseInfo)]
+    L.pfs_download_trail.argtypes = [vp, vp, ctypes.POINTER(u64)]
     L.pfs_device_count.restype = i32
     _LIB = L
     return L
@@ -198,6 +206,24 @@ class Simplifier:
         """Replaces the assumed (never eliminated) variables (pfs_assume)."""
         v = np.ascontiguousarray(variables, np.uint32)
         self._check(self._L.pfs_assume(self._h, _ptr(v), v.shape[0]), "pfs_assume")
+
+    def parse_dimacs(self, text: bytes) -> dict:
+        """DIMACS text -> resident formula on the device (pfs_parse_dimacs)."""
+        if isinstance(text, str):
+            text = text.encode()
+        info = _ParseInfo()
+        self._check(self._L.pfs_parse_dimacs(self._h, text, len(text), ctypes.byref(info)), "pfs_parse_dimacs")
+        out = {k: int(getattr(info, k)) for k, _ in _ParseInfo._fields_}
+        self.maxvar = out["maxvar"]
+        return out
+
+    def trail(self) -> np.ndarray:
+        """The root trail of the resident formula (pfs_download_trail)."""
+        n = ctypes.c_uint64()
+        self._check(self._L.pfs_download_trail(self._h, None, ctypes.byref(n)), "pfs_download_trail")
+        t = np.zeros(max(n.value, 1), np.uint32)
+        self._check(self._L.pfs_download_trail(self._h, _ptr(t), ctypes.byref(n)), "pfs_download_trail")
+        return t[: n.value]
 
     def rewind(self) -> None:
         self._check(self._L.pfs_rewind(self._h), "pfs_rewind")

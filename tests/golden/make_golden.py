@@ -13,7 +13,7 @@ Each fixture is a pair of PFD1 dumps (parafrost_amd/pfd.py) and a proof:
                     (Solver::map(true) when canMap(), else newBeginning();
                     PFM1, oracle/vmap.py read_pfm), for unsolved runs
 manifest.json lists every fixture with the reference options it was run
-with.  Run from the repository root:  python tests/golden/make_golden.py
+with; dimacs_manifest.json the parser fixtures (main_dimacs).  Run from the repository root:  python tests/golden/make_golden.py
 """
 import gzip
 import json
@@ -50,6 +50,129 @@ def cases():
     yield "ht_v300", gen.heavytail(300, 1200, seed=31)
     yield "ht_v2000", gen.heavytail(2000, 8000, seed=32)
     yield "ht_v3000", gen.heavytail(3000, 12000, seed=33)
+
+
+def messy_text(seed, nvars, nclauses, k=3, units=4, unsat=None, percent=False, crlf=False):
+    """A DIMACS text that exercises the parser: comment lines before the
+    header and between clauses, comments after a clause's 0, blank lines,
+    tabs and several blanks, CRLF line ends, '+' signs, leading zeros,
+    clauses spread over lines, duplicate literals, tautologies, unit clauses
+    (some repeated, some satisfying or shrinking later clauses) and an
+    optional SATLIB '%' ending.  unsat: None, "empty" (an empty clause),
+    "units" (opposite units) or "bcp" (units that propagate to a conflict)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cls = []
+    for _ in range(nclauses):
+        vs = rng.choice(np.arange(1, nvars + 1), size=k, replace=False)
+        cl = [int(v) if rng.random() < 0.5 else -int(v) for v in vs]
+        r = rng.random()
+        if r < 0.08:
+            cl.insert(int(rng.integers(0, len(cl) + 1)), cl[int(rng.integers(0, len(cl)))])  # duplicate
+        elif r < 0.12:
+            cl.insert(int(rng.integers(0, len(cl) + 1)), -cl[int(rng.integers(0, len(cl)))])  # tautology
+        cls.append(cl)
+    for _ in range(units):
+        u = int(rng.integers(1, nvars + 1)) * (1 if rng.random() < 0.5 else -1)
+        at = int(rng.integers(0, len(cls) + 1))
+        cls.insert(at, [u])
+        if rng.random() < 0.5:
+            cls.insert(int(rng.integers(at + 1, len(cls) + 1)), [u, u])  # the same unit again
+    if unsat == "empty":
+        cls.insert(int(rng.integers(0, len(cls) + 1)), [])
+    elif unsat == "units":
+        u = int(rng.integers(1, nvars + 1))
+        cls.insert(int(rng.integers(0, len(cls) // 2)), [u])
+        cls.insert(int(rng.integers(len(cls) // 2, len(cls) + 1)), [-u])
+    elif unsat == "bcp":
+        a, b = (int(x) for x in rng.choice(np.arange(1, nvars + 1), 2, replace=False))
+        cls += [[-a, b], [-a, -b], [a]]
+    nl = "\r\n" if crlf else "\n"
+    out = ["c messy text, seed %d%s" % (seed, nl), "c p cnf 1 1 is only a comment%s" % nl, nl,
+           "p cnf  %d\t%d %s" % (nvars, len(cls), nl)]
+
+    def num(x):
+        t = str(abs(x))
+        if rng.random() < 0.05:
+            t = "0" + t
+        if x < 0:
+            return "-" + t
+        return ("+" + t) if rng.random() < 0.1 else t
+
+    for cl in cls:
+        if rng.random() < 0.04:
+            out.append("c a comment line between clauses 12 -3 0 %% x%s" % nl)
+        if rng.random() < 0.03:
+            out.append(nl)
+        parts = [num(x) for x in cl] + ["0"]
+        line = ""
+        for j, t in enumerate(parts):
+            line += t
+            if j + 1 < len(parts):
+                line += nl if rng.random() < 0.05 else (" \t"[int(rng.integers(0, 2))] * int(rng.integers(1, 3)))
+        if rng.random() < 0.03:
+            line += " c trailing comment"
+        out.append(line + nl)
+    if percent:
+        out.append("%" + nl + "0" + nl)
+    return "".join(out).encode()
+
+
+def dimacs_cases():
+    """(name, text) of the parser fixtures; the last ones are rejected."""
+    yield "mx_u3_a", messy_text(101, 60, 200)
+    yield "mx_u3_b", messy_text(102, 300, 1200, units=12, crlf=True)
+    yield "mx_u3_c", messy_text(103, 2000, 8000, units=40, percent=True)
+    yield "mx_u4_d", messy_text(104, 500, 1500, k=4, units=25)
+    yield "mx_u2_e", messy_text(105, 400, 450, k=2, units=4)
+    yield "mx_unsat_empty", messy_text(106, 100, 300, unsat="empty")
+    yield "mx_unsat_units", messy_text(107, 100, 300, unsat="units")
+    yield "mx_unsat_bcp", messy_text(108, 100, 300, unsat="bcp")
+    yield "err_var", b"p cnf 3 1\n1 4 0\n"
+    yield "err_token", b"p cnf 3 1\n1 x 0\n"
+    yield "err_noheader", b"1 2 0\n"
+    yield "err_unterminated", b"p cnf 3 1\n1 2"
+    yield "err_clauses", b"p cnf 3 1\n1 2 0\n2 3 0\n"
+    yield "err_zerovars", b"p cnf 0 1\n"
+
+
+def main_dimacs():
+    """Parser fixtures: <name>.dimacs (the text), <name>.in.pfd.gz (the
+    reference's simplifier input after its parse, BCP and shrinkTop),
+    <name>.p1bce.out.pfd.gz (its simplifier output with --phases=1 -bce),
+    listed in dimacs_manifest.json with the outcome: "ok", "unsat" (the text
+    refutes itself) or "error" (the reference rejects it)."""
+    manifest = []
+    for name, text in dimacs_cases():
+        path = os.path.join(HERE, name + ".dimacs")
+        with open(path, "wb") as f:
+            f.write(text)
+        inp = os.path.join(HERE, name + ".in.pfd")
+        out = os.path.join(HERE, name + ".p1bce.out.pfd")
+        for x in (inp, out):
+            if os.path.exists(x):
+                os.remove(x)
+        r = subprocess.run([REF, path, inp, out, "--phases=1", "-bce"], capture_output=True, text=True, timeout=600)
+        entry = {"name": name, "dimacs": name + ".dimacs", "options": {"phases": 1, "bce": 1}}
+        if r.returncode != 0:
+            entry["outcome"] = "error"
+            msg = [ln for ln in (r.stdout + r.stderr).splitlines() if "ERROR" in ln]
+            entry["reference_error"] = msg[0].split("ERROR - ", 1)[-1].strip() if msg else ""
+        elif not os.path.exists(inp):
+            entry["outcome"] = "unsat"
+        else:
+            entry["outcome"] = "ok"
+            gz(inp)
+            entry["input"] = name + ".in.pfd.gz"
+        if os.path.exists(out) and entry["outcome"] == "ok":
+            gz(out)
+            entry["output"] = name + ".p1bce.out.pfd.gz"
+        elif os.path.exists(out):
+            os.remove(out)
+        manifest.append(entry)
+        print(name, entry["outcome"], entry.get("reference_error", ""))
+    with open(os.path.join(HERE, "dimacs_manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
 
 
 def gz(path):
@@ -108,6 +231,7 @@ def main():
             gz(inp)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1)
+    main_dimacs()
 
 
 if __name__ == "__main__":
