@@ -122,6 +122,69 @@ def calibrate_copy(nbytes=256 << 20, reps=3):
     print(f"bench: calibration copy {nbytes} bytes x {reps}", file=sys.stderr, flush=True)
 
 
+def with_units(cnf):
+    """The prepared formula with its root units as unit clauses: the text
+    a host solver would read (its parser and root propagation give back the
+    prepared formula)."""
+    import numpy as np
+    from parafrost_amd import gen
+    units = np.asarray(cnf.trail, np.uint32)
+    if not units.shape[0]:
+        return cnf
+    lits = np.concatenate([cnf.lits, units])
+    off = np.concatenate([cnf.offsets, cnf.offsets[-1] + 1 + np.arange(units.shape[0], dtype=np.uint64)])
+    return gen.Cnf(cnf.maxvar, off, lits)
+
+
+def bench_ingest(cnf, steps=3):
+    """DIMACS ingestion (pfs_parse_dimacs) of the workload's text, the text
+    the CPU baseline's reference parses.  Wall time includes the host-to-
+    device copy of the text; the emit lexer's roofline counts the text read
+    once and the literal stream written (N + 4 T bytes, T tokens)."""
+    import numpy as np
+    from parafrost_amd import gen
+    from parafrost_amd.simplifier import Simplifier
+    full = with_units(cnf)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "w.cnf")
+        gen.write_dimacs(full, p)
+        with open(p, "rb") as f:
+            text = f.read()
+    s = Simplifier(0)
+    try:
+        s.parse_dimacs(text)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            info = s.parse_dimacs(text)
+        dt = (time.perf_counter() - t0) / steps
+        s.set_option("profilegpu", 1)
+        s.parse_dimacs(text)
+        kt = s.kernel_times()
+        s.set_option("profilegpu", 0)
+        r = s.result(2)
+        trail = s.trail()
+    finally:
+        s.close()
+    sizes = np.diff(cnf.offsets).astype(np.int64)
+    order = np.lexsort((cnf.lits, np.repeat(np.arange(cnf.nclauses), sizes)))
+    same = (np.array_equal(r.sizes, sizes) and np.array_equal(r.lits, cnf.lits[order])
+            and np.array_equal(np.sort(trail), np.sort(np.asarray(cnf.trail, np.uint32))))
+    T = full.nlits + full.nclauses
+    emit = kt.get("lex_k<LEX_EMIT>")
+    out = {"text_bytes": len(text), "ms": round(dt * 1e3, 3), "text_gbs_pcie_inclusive": round(len(text) / dt / 1e9, 2),
+           "clauses_per_s": round(cnf.nclauses / dt, 1), "device_kernel_ms": round(sum(v[0] for v in kt.values()), 3),
+           "kernels_ms": {k: round(v[0], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:8]},
+           "identical_to_prepared": bool(same), "read_clauses": info["read_clauses"]}
+    if emit and emit[1]:
+        us = emit[0] / emit[1] * 1e3
+        ach = (len(text) + 4 * T) / (us * 1e-6) / 1e9
+        out["lex_emit"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(us, 2),
+                           "algorithmic_bytes_per_launch": len(text) + 4 * T}
+    print("bench: ingest " + json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(cnf, name, budget_clauses=7_500_000):
     """The reference CPU simplifier (compiled from its sources into
     oracle/_ref/pfref) on the same workload, or a bounded sample of it, on one
@@ -140,21 +203,15 @@ def cpu_baseline(cnf, name, budget_clauses=7_500_000):
             p = os.path.join(d, "w.cnf")
             # the reference runs its own parser and root propagation on the
             # raw formula; the prepared formula has the same clauses and units
-            raw = sample
-            off_units = list(raw.trail)
-            if off_units:
-                import numpy as np
-                lits = np.concatenate([raw.lits, np.array(off_units, np.uint32)])
-                off = np.concatenate([raw.offsets, raw.offsets[-1] + 1 + np.arange(len(off_units), dtype=np.uint64)])
-                raw = gen.Cnf(raw.maxvar, off, lits)
-            gen.write_dimacs(raw, p)
+            gen.write_dimacs(with_units(sample), p)
             r = subprocess.run(["taskset", "-c", "0", ref, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd")]
                                + REF_OPTIONS, capture_output=True, text=True, timeout=600)
             line = [x for x in r.stdout.splitlines() if x.startswith("{")][-1]
             t = json.loads(line)["total_s"]
+            tp = json.loads(line).get("parse_s")
         return {"value": sample.nclauses / t, "unit": "clauses/s", "cores": 1, "kind": "reference",
                 "sample": note + f"; reference CPU simplifier (ParaFROST src/cpu) round time {t:.3f} s, taskset 1 core",
-                "seconds": t}
+                "seconds": t, "parse_seconds": tp, "parse_sample": note}
     from oracle import oracle
     r = oracle.simplify(sample.maxvar, sample.offsets, sample.lits, nassigned=int(sample.trail.shape[0]), det=0,
                         phases=1, bce_en=1)
@@ -355,6 +412,7 @@ def main():
     ap.add_argument("--no-heavytail", action="store_true", help="skip the heavy-tailed single-formula leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the DIMACS ingestion leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 256 MiB device copies first (reference kernel for counter runs)")
     args = ap.parse_args()
@@ -435,6 +493,7 @@ def main():
                     "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
     handoff = bench_handoff(s, cnf)
     s.close()
+    ingest = None if args.no_ingest else bench_ingest(cnf)
 
     heavy = None if (args.no_heavytail or args.workload == "heavytail") else bench_single("heavytail")
     batched = None if args.no_batched else bench_batched()
@@ -446,6 +505,9 @@ def main():
             base = cpu_baseline(cnf, args.workload)
         except Exception as e:  # the baseline is reported, never required
             print(f"bench: cpu baseline failed: {e}", file=sys.stderr)
+    if ingest is not None and base and base.get("parse_seconds"):
+        ingest["cpu_reference_parse_s"] = base["parse_seconds"]
+        ingest["cpu_reference_parse_sample"] = base.get("parse_sample")
     value = cnf.nclauses / dt
     out = {
         "metric": METRIC,
@@ -466,6 +528,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": base,
         "handoff": handoff,
+        "ingest": ingest,
         "heavytail": heavy,
         "batched": batched,
         "serving": serving,

@@ -174,6 +174,7 @@ int main(int argc, char** argv) {
   verbose = 0;
   atexit(on_exit_unsat);
 
+  const double tp0 = now_s();
   Solver* s = new Solver(cnf);  // parse + root BCP (src/cpu/solver.cpp:28-48)
   solver = s;
   g_maxvar = inf.maxVar;
@@ -181,6 +182,7 @@ int main(int argc, char** argv) {
   // --- simplify.cpp:163-164: root-level cleanup before extraction
   s->rootify();
   s->shrinkTop(false);
+  const double parse_s = now_s() - tp0;
 
   // --- dump the simplifier input (the clauses extract() will read)
   {
@@ -296,7 +298,7 @@ int main(int argc, char** argv) {
     write_map(map_out, applied, s);
   }
 
-  printf("{\"total_s\": %.6f, \"rounds\": %d, \"elected\": [", total, rounds);
+  printf("{\"total_s\": %.6f, \"parse_s\": %.6f, \"rounds\": %d, \"elected\": [", total, parse_s, rounds);
   for (size_t i = 0; i < elected_per_round.size(); i++)
     printf("%s%u", i ? ", " : "", elected_per_round[i]);
   printf("]");

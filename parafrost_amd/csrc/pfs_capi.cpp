@@ -231,6 +231,7 @@ pfs_status pfs_parse_dimacs(pfs_ctx* c, const char* text, uint64_t nbytes, pfs_p
   return guard(c, [&] {
     c->uploaded = false;
     c->simplified = false;
+    c->s->profile = c->profile;
     const Simplifier::ParseInfo r = c->s->parse_dimacs(text, nbytes);
     c->uploaded = true;
     c->result = r.status == 0 ? PFS_RESULT_UNSAT : PFS_RESULT_UNSOLVED;
@@ -371,6 +372,7 @@ pfs_status pfs_map(pfs_ctx* c, int32_t force, pfs_map_info* info) {
     return PFS_BAD_STATE;
   }
   return guard(c, [&] {
+    c->s->profile = c->profile;
     const Simplifier::MapInfo r = c->s->map(force != 0, c->map_perc);
     if (info) {
       info->applied = r.applied;

@@ -239,6 +239,18 @@ void Simplifier::save_input() {
   d2d(in_trail_buf_.p, trail_.p, (size_t)ntrail * 4, st_);
 }
 
+Simplifier::ProfScope::ProfScope(Simplifier* x) : s(x) {
+  if (s->profile) s->kernel_times.clear(), prof_enable(s->st_, true);
+}
+Simplifier::ProfScope::~ProfScope() {
+  if (!s->profile) return;
+  try {
+    prof_collect(s->st_, s->kernel_times);
+    prof_enable(s->st_, false);
+  } catch (...) {
+  }
+}
+
 // ---------------------------------------------------------------- DIMACS ingestion
 namespace {
 bool dimacs_blank(unsigned char c) { return (c >= 9 && c <= 13) || c == 32; }  // dimacs.h:33
@@ -301,6 +313,7 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
   if (n >= (1ull << 40)) throw std::invalid_argument("text too large");
   info.maxvar = (u32)V;
   info.header_clauses = C;
+  ProfScope prof(this);
 
   // ---- lexing on the device
   const uint64_t begin = i;
@@ -1463,21 +1476,7 @@ Simplifier::MapInfo Simplifier::map(bool force, double map_perc) {
     throw std::logic_error("root units are not propagated: simplify before mapping");
   const u32 V = V_;
   const size_t n1 = (size_t)V + 1;
-  // with profiling on, the map's kernels replace the per-kernel times
-  struct Prof {
-    Simplifier* s;
-    explicit Prof(Simplifier* x) : s(x) {
-      if (s->profile) s->kernel_times.clear(), prof_enable(s->st_, true);
-    }
-    ~Prof() {
-      if (!s->profile) return;
-      try {
-        prof_collect(s->st_, s->kernel_times);
-        prof_enable(s->st_, false);
-      } catch (...) {
-      }
-    }
-  } prof(this);
+  ProfScope prof(this);
   mem_.ensure(map_, n1 * 4, 0, st_);
   mem_.ensure(tmp2_, n1 * 4, 0, st_);
   mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(n1)), 0, st_);

@@ -193,6 +193,12 @@ class Simplifier {
   // root assignments (for map_lits) and the packed-output staging buffer
   Buf map_, map_assign_, out_;
   void save_input();  // device copy of the resident input for rewind()
+  // with profiling on, per-kernel event times of one call replace kernel_times
+  struct ProfScope {
+    Simplifier* s;
+    explicit ProfScope(Simplifier* x);
+    ~ProfScope();
+  };
   bool map_called_ = false, map_applied_ = false, remapped_ = false;
   uint32_t map_old_nvars_ = 0, map_first_ = 0, map_first_true_ = 0;
   uint32_t orgmax_ = 0;  // largest original variable id handed out
