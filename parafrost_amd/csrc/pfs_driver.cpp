@@ -442,7 +442,7 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
   proof_n_ = 0;
   pl_.clear();
   pst_.clear();
-  if (!refuted_) {
+  if (!refuted_ && ntrail) {
     const int32_t proof = opt.proof_en;
     opt.proof_en = 0;  // the parse's own proof lines are not part of the simplifier's proof
     create_ot();

@@ -1307,11 +1307,12 @@ void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Str
 // clauses reported, unit clauses turned into unit records (toClause,
 // dimacs.cpp:120-166); ulit[i] = the unit literal of clause i or 0;
 // kept[0] counts the clauses that stay
+// the kept count takes one atomic per wavefront
 __global__ void parse_dedup_k(DB db, u32* ulit, u32* kept, u32* err) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= db.ncls) return;
-  const u32 kind = parse_dedup_clause(db, db.c_off[i], &ulit[i]);
-  if (kind == 0) atomicAdd(kept, 1u);
+  const u32 kind = i < db.ncls ? parse_dedup_clause(db, db.c_off[i], &ulit[i]) : 4u;
+  const unsigned long long kb = __ballot(kind == 0);
+  if ((threadIdx.x & 63u) == 0 && kb) atomicAdd(kept, (u32)__popcll(kb));
   if (kind == 2) atomicOr(err, (u32)LEXE_UNSAT);
 }
 void k_parse_dedup(const DB& db, u32* ulit, u32* kept, u32* err, Stream* st) {
