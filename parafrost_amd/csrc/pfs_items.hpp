@@ -1581,22 +1581,26 @@ PF_HD bool parse_build_clause(const DB& db, const u32* lits, const u32* ends, u3
 PF_HD u32 parse_dedup_clause(const DB& db, u32 c, u32* unit) {
   u32* l = clits(db, c);
   const u32 n = csize(db, c);
-  u32 m = 0;
+  u32 m = 0, prev = 0;
   bool taut = false;
   for (u32 q = 0; q < n; q++) {
     const u32 x = l[q];
-    if (m && l[m - 1] == x) continue;
-    if (m && (l[m - 1] ^ 1u) == x) taut = true;
-    l[m++] = x;
+    if (m) {
+      if (x == prev) continue;
+      if ((prev ^ 1u) == x) taut = true;
+    }
+    if (m != q) l[m] = x;  // a literal moves only after a dropped duplicate
+    m++;
+    prev = x;
   }
   *unit = 0;
   if (taut || m < 2) {
-    if (m == 1 && !taut) *unit = l[0];
+    if (m == 1 && !taut) *unit = prev;
     set_cflags(db, c, ST_DELETED);
     return taut ? 1u : (m == 0 ? 2u : 3u);
   }
-  set_csize(db, c, m);
-  set_csig(db, c, calc_sig(l, m));
+  // dropping duplicates keeps the signature (the same set of literals)
+  if (m != n) set_csize(db, c, m);
   return 0;
 }
 

@@ -1091,7 +1091,9 @@ __global__ void map_first_k(const u8* vstate, const u32* assign, u32 nvars, u32*
   const u32 v = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const bool cand = v <= nvars && vstate[v] && assign[v];
   const unsigned long long b = __ballot(cand);
-  if (cand && (u32)(__ffsll(b) - 1) == (threadIdx.x & 63u)) atomicMin(first, v);
+  // a wave whose smallest candidate cannot win skips the atomic
+  if (cand && (u32)(__ffsll(b) - 1) == (threadIdx.x & 63u) && v < __atomic_load_n(first, __ATOMIC_RELAXED))
+    atomicMin(first, v);
 }
 __global__ void map_keep_k(const u8* vstate, u32 nvars, const u32* first, u32* keep) {
   const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
