@@ -72,6 +72,7 @@ void pfs_destroy(pfs_ctx* ctx);
  * "eremaxoccurs", "electionsmin", "collectfreq", "velitsbound") are accepted.
  * "proof" (boolean, default off) records the binary DRAT lines of the run
  * (pfs_download_proof).
+ * "decomposemin" (1-10, default 2) is pfs_decompose's default round count.
  * "mapperc" (real in [0, 1], default 0.2) is the inactive share above which
  * pfs_map renumbers.
  * "workorder" (boolean, default on; no reference counterpart) launches the
@@ -92,6 +93,30 @@ pfs_status pfs_upload(pfs_ctx* ctx, uint32_t maxvar, uint64_t ncls, const uint64
                       const uint32_t* lits, const uint8_t* flags, const uint32_t* lbd,
                       const uint8_t* vstate, const uint8_t* assumed, const uint32_t* vorg,
                       const uint32_t* trail, uint32_t ntrail);
+
+/* Equivalent-literal substitution on the resident formula, on the device:
+ * the CDCL side's ELS (Solver::ELS/decompose/substitute,
+ * src/cpu/equivalence.cpp:41-323).  Up to max_rounds (0: option
+ * "decomposemin", default 2, src/cpu/options.cpp:87) decompositions while
+ * one succeeds (new units or binary clauses): the SCCs of the binary
+ * implication graph over active variables, each represented by its literal
+ * of smallest variable (assumed variables keep themselves; a literal and
+ * its negation in one SCC refute), clauses rewritten with the
+ * representatives (learnts first; duplicates dropped, tautologies and
+ * root-satisfied clauses removed, units assigned), the units propagated,
+ * substituted variables marked (state 4) with binary records on the stack
+ * (MODEL::saveBinary).  Afterwards the clauses are shrunk at the top level
+ * (root-satisfied clauses removed, false literals dropped).  Requires a
+ * propagated formula (after pfs_upload, pfs_parse_dimacs or pfs_simplify).
+ * New units accumulate in pfs_download_units, stack records in
+ * pfs_download_stack; info.status 0 means refuted (the next pfs_simplify
+ * reports UNSAT).  No proof lines are recorded for it.  Learnt clauses keep
+ * their LBD and usage when rewritten (the reference may re-create them). */
+typedef struct pfs_els_info {
+  uint32_t rounds, status;  /* status 0: refuted, 2: otherwise */
+  uint64_t substituted, units, stack_words;
+} pfs_els_info;
+pfs_status pfs_decompose(pfs_ctx* ctx, uint32_t max_rounds, pfs_els_info* info);
 
 /* DIMACS ingestion on the device: the text (nbytes, need not be
  * NUL-terminated) becomes the resident formula, replacing pfs_upload.

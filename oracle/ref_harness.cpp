@@ -203,6 +203,52 @@ int main(int argc, char** argv) {
     std::vector<uint32_t> none;
     write_pfd(in_path, 2u, inf.maxVar, cd, units, none, vst);
   }
+  // --- PFREF_ELS_OUT: the CDCL side's equivalent-literal substitution on this
+  // root-clean formula instead of the simplifier: fresh watches (rebuildWT,
+  // watch.cpp:132-151), Solver::decompose (equivalence.cpp:41-199) repeated
+  // while it succeeds, at most PFREF_ELS_ROUNDS times (default: decomposemin,
+  // as ELS, equivalence.cpp:316-323), then rootify + shrinkTop.  Dumped as a
+  // PFD: status, clauses (orgs then learnts), the whole trail, the stack
+  // (model.resolved), vstate.
+  if (const char* els_out = getenv("PFREF_ELS_OUT")) {
+    g_out_path = els_out;
+    s->wt.clear(true);
+    s->rebuildWT(s->opts.sigma_priorbins);
+    const char* rr = getenv("PFREF_ELS_ROUNDS");
+    const int rounds = rr ? atoi(rr) : s->opts.decompose_min;
+    int done = 0;
+    for (bool ok = true; ok && done < rounds && s->cnfstate; done++) ok = s->decompose();
+    if (s->cnfstate) {
+      s->rootify();
+      if (s->cnfstate) s->shrinkTop(false);
+    }
+    ClauseDump ed;
+    std::vector<uint32_t> eu, er;
+    std::vector<uint8_t> ev;
+    uint32_t st = s->cnfstate ? 2u : 0u;
+    if (st) {
+      for (int pass = 0; pass < 2; pass++) {
+        const BCNF& list = pass ? s->learnts : s->orgs;
+        for (uint32_t i = 0; i < list.size(); i++) {
+          if (s->cm.deleted(list[i])) continue;
+          CLAUSE& c = s->cm[list[i]];
+          ed.sizes.push_back(c.size());
+          ed.flags.push_back(c.learnt() ? 1 : 0);
+          ed.lbds.push_back(c.learnt() ? c.lbd() : 0);
+          for (int k = 0; k < c.size(); k++) ed.lits.push_back(c[k]);
+        }
+      }
+      for (uint32_t i = 0; i < s->trail.size(); i++) eu.push_back(s->trail[i]);
+      for (uint32_t i = 0; i < s->model.resolved.size(); i++) er.push_back(s->model.resolved[i]);
+      ev.assign(inf.maxVar + 1, 0);
+      for (uint32_t v = 1; v <= inf.maxVar; v++) ev[v] = s->sp->vstate[v].state;
+    }
+    write_pfd(els_out, st, inf.maxVar, ed, eu, er, ev);
+    g_done = true;
+    printf("{\"els_rounds\": %d, \"status\": %u}\n", done, st);
+    fflush(stdout);
+    _exit(0);
+  }
   const uint32_t trail_start = s->trail.size();
   // with -proof: the simplifier's proof lines start here; the position is
   // written next to the output dump as <output.pfd>.proofpos

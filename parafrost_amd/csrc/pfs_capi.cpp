@@ -28,6 +28,7 @@ struct pfs_ctx {
   bool ve = true, ve_plus = true, sub = true, bce = false, ere = true, all = false, velbound = false,
        profile = false, work_order = true, proof = false;
   double map_perc = 0.2;  // mapperc, src/cpu/options.cpp:146
+  int32_t decompose_min = 2;  // decomposemin, src/cpu/options.cpp:87
 };
 
 namespace {
@@ -160,6 +161,7 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"lcvemin", "electionsmin", &pfs_ctx::lcve_min, 1, INT32_MAX},
       {"gcfreq", "collectfreq", &pfs_ctx::shrink_rate, 0, 5},
       {"lbdtier1", nullptr, &pfs_ctx::lbd_tier1, 1, INT32_MAX},
+      {"decomposemin", nullptr, &pfs_ctx::decompose_min, 1, 10},
   };
   for (const IntOpt& o : ints) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -381,6 +383,32 @@ pfs_status pfs_map(pfs_ctx* c, int32_t force, pfs_map_info* info) {
       info->first_frozen = r.first;
       info->trail_lit = r.trail_lit;
       info->inactive = r.inactive;
+    }
+    return PFS_OK;
+  });
+}
+
+pfs_status pfs_decompose(pfs_ctx* c, uint32_t max_rounds, pfs_els_info* info) {
+  if (!c) return PFS_BAD_INPUT;
+  if (!c->uploaded) {
+    c->err = "nothing uploaded";
+    return PFS_BAD_STATE;
+  }
+  if (c->result == PFS_RESULT_UNSAT) {
+    c->err = "the formula is refuted";
+    return PFS_BAD_STATE;
+  }
+  return guard(c, [&] {
+    c->s->opt = normalise(c);
+    c->s->profile = c->profile;
+    const Simplifier::ElsInfo r = c->s->decompose(max_rounds ? max_rounds : (uint32_t)c->decompose_min);
+    if (r.status == 0) c->result = PFS_RESULT_UNSAT;
+    if (info) {
+      info->rounds = r.rounds;
+      info->status = r.status;
+      info->substituted = r.substituted;
+      info->units = r.units;
+      info->stack_words = r.stack_words;
     }
     return PFS_OK;
   });

@@ -700,11 +700,22 @@ bool Simplifier::prop() {
   if (trail_n_ == propagated_) return true;
   stage_begin(S_BCP);
   prop_changed_ = true;
+  if (!prop_visit(propagated_)) {
+    stage_end();
+    return false;
+  }
+  prop_clean(propagated_);
+  propagated_ = trail_n_;
+  stage_end();
+  return true;
+}
+
+bool Simplifier::prop_visit(uint64_t front) {
   DB db = make_db(cur_);
   u32* trail = P<u32>(trail_);
   u32* sc = P<u32>(scal_);
   const uint64_t start_new = trail_n_;
-  uint64_t front = propagated_, nfront = trail_n_ - propagated_;
+  uint64_t nfront = trail_n_ - front;
   while (nfront) {
     fill_u32(sc, 0, 2, st_);
     k_bcp_visit(db, trail + front, (u32)nfront, trail + trail_n_, sc, sc + 1, st_);
@@ -714,7 +725,6 @@ bool Simplifier::prop() {
     if (h[1]) {
       proof_line(PR_EMPTY, 0);  // learnEmpty (elimination.cpp:80-86)
       proof_flush();
-      stage_end();
       return false;
     }
     front = trail_n_;
@@ -736,15 +746,18 @@ bool Simplifier::prop() {
     unassigned_ -= (u32)nnew;
     if (!rounds_.empty()) rounds_.back().units += (u32)nnew;
   }
-  const u32 nprop = (u32)(trail_n_ - propagated_);
-  k_bcp_mark(db, trail + propagated_, nprop, P<u8>(mark_), st_);
-  k_clear_lists(db, trail + propagated_, nprop, st_);
+  return true;
+}
+
+void Simplifier::prop_clean(uint64_t from) {
+  DB db = make_db(cur_);
+  u32* trail = P<u32>(trail_);
+  const u32 nprop = (u32)(trail_n_ - from);
+  k_bcp_mark(db, trail + from, nprop, P<u8>(mark_), st_);
+  k_clear_lists(db, trail + from, nprop, st_);
   k_bcp_clean(db, P<u8>(mark_), st_);
-  propagated_ = trail_n_;
   if (!opt.sub_en) k_reduce_lists(db, st_);
   stream_sync(st_);
-  stage_end();
-  return true;
 }
 
 // ---------------------------------------------------------------- LCVE (lcve.cpp)
@@ -1448,6 +1461,190 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
   gc();
   if (!unassigned_ || !nC) return 1;
   return 2;
+}
+
+// ---------------------------------------------------------------- ELS
+// Solver::ELS (src/cpu/equivalence.cpp:316-323) on the resident formula:
+// decompose (:41-199) repeated while it succeeds.  One round: the binary
+// implication graph over active literals, its SCCs (trimming, then
+// coloring), representatives (smallest variable of the SCC; assumed
+// variables keep themselves; a literal with its negation refutes), the
+// substitution pass (substitute, :201-306, learnts before originals, with
+// the units it finds visible to the clauses after them: the times of
+// those units are the fixpoint of re-running the pass's unit detection),
+// propagation of the new units (assignments only, as the reference's BCP),
+// then SUBSTITUTED marks and binary stack records (:173-190).  After the
+// last round the clauses are shrunk at the top level (rootify/shrinkTop)
+// and compacted.
+Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
+  if (refuted_) throw std::logic_error("the formula is refuted");
+  if (!pending_units_.empty() || trail_n_ != propagated_)
+    throw std::logic_error("root units are not propagated: simplify or parse first");
+  ElsInfo info;
+  ProfScope prof(this);
+  // the substitution's own proof lines are not recorded
+  struct NoProof {
+    Options& o;
+    int32_t saved;
+    explicit NoProof(Options& x) : o(x), saved(x.proof_en) { o.proof_en = 0; }
+    ~NoProof() { o.proof_en = saved; }
+  } noproof(opt);
+  const uint64_t start = trail_n_, stack0 = stack_n_;
+  const size_t n2 = 2 * (size_t)V_ + 2, n1 = (size_t)V_ + 1;
+  Buf deg, off, cur, adj, alive, indeg, outdeg, root, color, mark, fa, fb, list, rep, utime, ptime, lrank, tl, tpos,
+      ukey, uval, sflag, spos, words;
+  BufScope scope{mem_, {&deg, &off, &cur, &adj, &alive, &indeg, &outdeg, &root, &color, &mark, &fa, &fb, &list, &rep,
+                        &utime, &ptime, &lrank, &tl, &tpos, &ukey, &uval, &sflag, &spos, &words}};
+  for (Buf* b : {&deg, &cur, &alive, &indeg, &outdeg, &root, &color, &mark, &fa, &fb, &list, &rep})
+    mem_.ensure(*b, n2 * 4, 0, st_);
+  mem_.ensure(off, (n2 + 1) * 4, 0, st_);
+  for (Buf* b : {&utime, &ptime, &sflag, &spos}) mem_.ensure(*b, n1 * 4, 0, st_);
+  mem_.ensure(words, 64, 0, st_);
+  u32* w = P<u32>(words);  // [0] count A, [1] count B, [2] changed, [3] alive, [4..5] rep info, [6] learnts,
+                           // [7] touched, [8] units, [9..10] apply info, [11] substituted vars, [12] marked
+  auto rd = [&](u32 k) {
+    u32 x;
+    d2h(&x, w + k, 4, st_);
+    stream_sync(st_);
+    return x;
+  };
+  auto trim = [&](u32 nf) {  // frontier in fa, count in w[0]
+    for (u32 it = 0; nf && it <= n2; it++) {
+      fill_u32(w + 1, 0, 1, st_);
+      k_els_trim(P<u32>(fa), nf, P<u32>(off), P<u32>(adj), P<u32>(alive), P<u32>(indeg), P<u32>(outdeg),
+                 P<u32>(root), P<u32>(fb), w + 1, st_);
+      nf = rd(1);
+      std::swap(fa, fb);
+    }
+  };
+  bool unsat = false;
+  for (uint32_t r = 0; r < max_rounds && !unsat; r++) {
+    info.rounds++;
+    DB db = make_db(cur_);
+    mem_.ensure(tmp_, std::max({tmp_.cap, scan_tmp_bytes(n2 + 1), scan_tmp_bytes((size_t)ncls_ + 1),
+                                radix_tmp_bytes((size_t)ncls_ + 1)}), 0, st_);
+    // ---- graph and SCCs
+    k_els_graph(db, P<u32>(deg), P<u32>(off), tmp_.p, st_);
+    u32 E = 0;
+    d2h(&E, P<u32>(off) + n2, 4, st_);
+    stream_sync(st_);
+    mem_.ensure(adj, ((size_t)E + 1) * 4, 0, st_);
+    k_els_fill(db, P<u32>(off), P<u32>(cur), P<u32>(adj), st_);
+    k_els_trim_init(db, P<u32>(off), P<u32>(alive), P<u32>(indeg), P<u32>(outdeg), P<u32>(root), P<u32>(fa), w, st_);
+    trim(rd(0));
+    for (u32 it = 0; it <= n2; it++) {  // each pass removes at least one SCC
+      k_els_alive((u32)n2, P<u32>(alive), P<u32>(list), w + 3, st_);
+      const u32 na = rd(3);
+      if (!na) break;
+      k_els_color_init(P<u32>(list), na, P<u32>(color), P<u32>(mark), st_);
+      for (u32 k = 0; k <= na; k++) {
+        k_els_fw(P<u32>(list), na, P<u32>(off), P<u32>(adj), P<u32>(alive), P<u32>(color), w + 2, st_);
+        if (!rd(2)) break;
+      }
+      k_els_roots(P<u32>(list), na, P<u32>(color), P<u32>(mark), st_);
+      for (u32 k = 0; k <= na; k++) {
+        k_els_bw(P<u32>(list), na, P<u32>(off), P<u32>(adj), P<u32>(alive), P<u32>(color), P<u32>(mark), w + 2, st_);
+        if (!rd(2)) break;
+      }
+      k_els_take(P<u32>(list), na, P<u32>(mark), P<u32>(color), P<u32>(alive), P<u32>(root), st_);
+      k_els_redeg(P<u32>(list), na, P<u32>(off), P<u32>(adj), P<u32>(alive), P<u32>(indeg), P<u32>(outdeg),
+                  P<u32>(root), P<u32>(fa), w, st_);
+      trim(rd(0));
+    }
+    fill_u32(w + 4, 0, 2, st_);
+    k_els_rep(db, P<u32>(root), P<u32>(rep), w + 4, st_);
+    u32 ri[2];
+    d2h(ri, w + 4, 8, st_);
+    stream_sync(st_);
+    if (ri[0]) {
+      unsat = true;
+      break;
+    }
+    const u32 nsub = ri[1];
+    u32 nunits = 0, nbin = 0;
+    const uint64_t visited = trail_n_;
+    if (nsub) {
+      // ---- substitution pass
+      mem_.ensure(lrank, ((size_t)ncls_ + 1) * 4, 0, st_);
+      mem_.ensure(tl, ((size_t)ncls_ + 1) * 4, 0, st_);
+      mem_.ensure(tpos, ((size_t)ncls_ + 1) * 4, 0, st_);
+      k_els_touch(db, P<u32>(rep), P<u32>(lrank), tmp_.p, w + 6, P<u32>(tl), P<u32>(tpos), w + 7, st_);
+      const u32 nlearnt = rd(6);
+      k_els_touch_list(db, P<u32>(rep), P<u32>(lrank), nlearnt, P<u32>(tl), P<u32>(tpos), w + 7, st_);
+      const u32 nt = rd(7);
+      fill_u32(P<u32>(utime), NOPOS, n1, st_);
+      const ElsView ev{P<u32>(rep), P<u32>(utime)};
+      for (u32 it = 0; it <= nt + 1; it++) {
+        k_els_check(db, P<u32>(tl), P<u32>(tpos), nt, ev, P<u32>(ptime), st_);
+        k_els_commit((u32)n1, P<u32>(ptime), P<u32>(utime), w + 2, st_);
+        if (!rd(2)) break;
+      }
+      mem_.ensure(ukey, ((size_t)nt + 1) * 4, 0, st_);
+      mem_.ensure(uval, ((size_t)nt + 1) * 4, 0, st_);
+      fill_u32(w + 9, 0, 2, st_);
+      k_els_apply(db, P<u32>(tl), P<u32>(tpos), nt, ev, P<u32>(ukey), P<u32>(uval), w + 8, w + 9, st_);
+      u32 ai[3];
+      d2h(ai, w + 8, 12, st_);
+      stream_sync(st_);
+      nunits = ai[0];
+      nbin = ai[2];
+      if (ai[1]) {
+        unsat = true;
+        break;
+      }
+      if (nunits) {
+        radix_sort_pairs(P<u32>(ukey), P<u32>(uval), nunits, 32, tmp_.p, st_);
+        k_els_enqueue(db, P<u32>(uval), nunits, P<u32>(trail_) + trail_n_, st_);
+        std::vector<u32> us(nunits);
+        d2h(us.data(), P<u32>(uval), (size_t)nunits * 4, st_);
+        stream_sync(st_);
+        for (u32 u : us) {
+          h_assign_[ABS(u)] = u + 1;
+          units_out_.push_back(u);
+        }
+        trail_n_ += nunits;
+        unassigned_ -= nunits;
+      }
+      // ---- propagation of the new units (assignments only)
+      if (trail_n_ > visited) {
+        create_ot();
+        if (!prop_visit(visited)) {
+          unsat = true;
+          break;
+        }
+      }
+    }
+    // ---- substituted variables and their stack records
+    DB db2 = make_db(cur_);
+    k_els_subst_flags(db2, P<u32>(rep), P<u32>(sflag), P<u32>(spos), tmp_.p, w + 11, st_);
+    const u32 ns = rd(11);
+    if (ns) {
+      ensure_stack(stack_n_ + 5 * (uint64_t)ns);
+      k_els_subst(db2, P<u32>(rep), P<u32>(sflag), P<u32>(spos), P<u32>(stack_) + stack_n_, w + 12, st_);
+      const u32 marked = rd(12);
+      stack_n_ += 5 * (uint64_t)ns;
+      unassigned_ -= marked;
+      info.substituted += marked;
+    }
+    const bool success = nsub && (nunits || nbin);
+    if (!success) break;
+  }
+  info.units = trail_n_ - start;
+  info.stack_words = stack_n_ - stack0;
+  if (unsat) {
+    refuted_ = true;
+    info.status = 0;
+    return info;
+  }
+  // rootify + shrinkTop over the units of the rounds, then compaction
+  if (trail_n_ > start) {
+    create_ot();
+    prop_clean(start);
+  }
+  propagated_ = trail_n_;
+  prop_changed_ = true;
+  gc();
+  return info;
 }
 
 // ---------------------------------------------------------------- results

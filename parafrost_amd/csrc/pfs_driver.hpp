@@ -74,6 +74,14 @@ class Simplifier {
   };
   ParseInfo parse_dimacs(const char* text, uint64_t n);
   uint64_t trail_size() const { return trail_n_; }
+  // equivalent-literal substitution on the resident formula (Solver::ELS /
+  // decompose / substitute, src/cpu/equivalence.cpp:41-323): up to
+  // max_rounds decompositions while one succeeds, then the top-level shrink
+  struct ElsInfo {
+    uint32_t rounds = 0, status = 2;
+    uint64_t substituted = 0, units = 0, stack_words = 0;
+  };
+  ElsInfo decompose(uint32_t max_rounds);
   void download_trail(uint32_t* lits);  // the root trail of the resident formula
   void rewind();                    // restore the last uploaded input (device copy)
   void bind() { stream_bind(st_); }  // select this context's device on the calling thread
@@ -124,6 +132,8 @@ class Simplifier {
   void count(uint64_t& cls, uint64_t& lits);
   void create_ot();
   bool prop();
+  bool prop_visit(uint64_t front);  // propagation from trail[front..): assignments only
+  void prop_clean(uint64_t from);   // satisfied clauses deleted, false literals dropped
   bool lcve();
   void sort_ot();
   bool sub();

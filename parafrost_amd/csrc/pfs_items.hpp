@@ -1604,4 +1604,93 @@ PF_HD u32 parse_dedup_clause(const DB& db, u32 c, u32* unit) {
   return 0;
 }
 
+// ---------------------------------------------------------------- ELS
+// Equivalent-literal substitution (Solver::decompose/substitute,
+// src/cpu/equivalence.cpp:41-306).  rep[l]: the representative of literal
+// l (its SCC's literal of smallest variable; l itself for assumed
+// variables; 0 for inactive literals, as the reference's fresh array).
+// The substitution pass runs over the clauses in the order learnts, then
+// originals; a unit it finds at position p is visible to the clauses after
+// p: utime[x] = 2 p + sign of that unit, NOPOS when there is none.
+struct ElsView {
+  const u32* rep;
+  const u32* utime;
+};
+// 1 true, 0 false, -1 unassigned at position p
+PF_HD int els_value(const DB& db, const ElsView& e, u32 lit, u32 p) {
+  const u32 a = db.assign[ABS(lit)];
+  if (a) return (a - 1) == lit ? 1 : 0;
+  const u32 t = e.utime[ABS(lit)];
+  if (t != NOPOS && (t >> 1) < p) return (t & 1u) == SIGN(lit) ? 1 : 0;
+  return -1;
+}
+// one clause at position p (equivalence.cpp:214-256).  Returns 0 untouched
+// (every literal is its own representative), 1 satisfied or tautological,
+// 2 empty, 3 unit (*unit), 4 rewritten with *m >= 2 literals.  With write,
+// the kept literals (mapped, duplicates dropped, first occurrence order)
+// are compacted in place, sorted, and size and signature updated; satisfied
+// and unit clauses are marked deleted.  Without write the clause is only
+// classified (an earlier kept literal is found again from the original
+// literals).
+PF_HD u32 els_clause(const DB& db, u32 c, const ElsView& e, u32 p, bool write, u32* unit, u32* m_out) {
+  u32* l = clits(db, c);
+  const u32 n = csize(db, c);
+  bool touched = false;
+  for (u32 q = 0; q < n && !touched; q++) touched = e.rep[l[q]] != l[q];
+  if (!touched) return 0;
+  u32 m = 0, first = 0;
+  bool sat = false;
+  for (u32 q = 0; q < n && !sat; q++) {
+    const u32 lit = l[q];
+    const int x = els_value(db, e, lit, p);
+    if (x > 0) sat = true;
+    if (x != -1) continue;
+    const u32 other = e.rep[lit];
+    const int y = els_value(db, e, other, p);
+    if (y > 0) sat = true;
+    if (y != -1) continue;
+    // the marks of equivalence.cpp:231-236: same variable kept before?
+    int seen = -1;
+    if (write) {
+      for (u32 r = 0; r < m && seen < 0; r++)
+        if (ABS(l[r]) == ABS(other)) seen = l[r] == other ? 0 : 1;
+    } else {
+      for (u32 r = 0; r < q && seen < 0; r++) {
+        const u32 lr = l[r];
+        if (els_value(db, e, lr, p) != -1) continue;
+        const u32 orr = e.rep[lr];
+        if (els_value(db, e, orr, p) != -1) continue;
+        if (ABS(orr) == ABS(other)) seen = orr == other ? 0 : 1;
+      }
+    }
+    if (seen == 1) sat = true;
+    if (seen != -1) continue;
+    if (write) l[m] = other;
+    if (!m) first = other;
+    m++;
+  }
+  *m_out = m;
+  if (sat) {
+    if (write) set_deleted(db, c);
+    return 1;
+  }
+  if (!m) return 2;
+  if (m == 1) {
+    *unit = first;
+    if (write) set_deleted(db, c);
+    return 3;
+  }
+  if (write) {
+    for (u32 q = 1; q < m; q++) {
+      const u32 t = l[q];
+      u32 j = q;
+      while (j > 0 && l[j - 1] > t) l[j] = l[j - 1], j--;
+      l[j] = t;
+    }
+    set_csize(db, c, m);
+    set_csig(db, c, calc_sig(l, m));
+  }
+  return 4;
+}
+
 }  // namespace pfs

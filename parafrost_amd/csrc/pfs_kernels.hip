@@ -1364,4 +1364,342 @@ void iota_u32(u32* a, size_t n, Stream* st) {
   KLAUNCH(iota_u32_k, grid_for(n, 256), 256, st, a, (u64)n);
 }
 
+// ---------------------------------------------------------------- ELS
+// Binary implication graph over literals (equivalence.cpp:26-40, 92-139):
+// a live clause (a, b) of two active variables gives a -> ... edges
+// -a -> b and -b -> a.  Out-edges are a CSR (off, adj); the in-edges of l
+// are the negations of the out-edges of -l (the graph is skew-symmetric).
+PF_DEV bool els_binary(const DB& db, u32 c, u32& a, u32& b) {
+  if (is_deleted(db, c) || csize(db, c) != 2) return false;
+  const u32* l = clits(db, c);
+  a = l[0], b = l[1];
+  return !db.vstate[ABS(a)] && !db.vstate[ABS(b)];
+}
+__global__ void els_deg_k(DB db, u32* deg) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 a, b;
+  if (i < db.ncls && els_binary(db, db.c_off[i], a, b)) {
+    atomicAdd(&deg[FLIP(a)], 1u);
+    atomicAdd(&deg[FLIP(b)], 1u);
+  }
+}
+__global__ void els_fill_k(DB db, const u32* off, u32* cur, u32* adj) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 a, b;
+  if (i < db.ncls && els_binary(db, db.c_off[i], a, b)) {
+    adj[off[FLIP(a)] + atomicAdd(&cur[FLIP(a)], 1u)] = b;
+    adj[off[FLIP(b)] + atomicAdd(&cur[FLIP(b)], 1u)] = a;
+  }
+}
+void k_els_graph(const DB& db, u32* deg, u32* off, void* tmp, Stream* st) {
+  const size_t n2 = 2 * (size_t)db.nvars + 2;
+  fill_u32(deg, 0, n2, st);
+  if (db.ncls) KLAUNCH(els_deg_k, grid_for(db.ncls, 256), 256, st, db, deg);
+  scan_u32(deg, off, n2, off + n2, tmp, st);
+}
+void k_els_fill(const DB& db, const u32* off, u32* cur, u32* adj, Stream* st) {
+  fill_u32(cur, 0, 2 * (size_t)db.nvars + 2, st);
+  if (db.ncls) KLAUNCH(els_fill_k, grid_for(db.ncls, 256), 256, st, db, off, cur, adj);
+}
+// Trimming: a literal without remaining in-edges or out-edges is an SCC of
+// its own.  alive[l] = 1 for active literals still in the graph; a removed
+// literal is its own root.  The first frontier holds the literals that
+// start with no in- or out-edge.
+__global__ void els_trim_init_k(DB db, const u32* off, u32* alive, u32* indeg, u32* outdeg, u32* root, u32* front,
+                                u32* nfront) {
+  const u32 l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= 2 * db.nvars + 2) return;
+  const bool act = l >= 2 && !db.vstate[ABS(l)];
+  const u32 out = off[l + 1] - off[l], in = off[FLIP(l) + 1] - off[FLIP(l)];
+  indeg[l] = in;
+  outdeg[l] = out;
+  root[l] = l;
+  alive[l] = act && in && out ? 1u : 0u;
+  if (act && !(in && out)) front[atomicAdd(nfront, 1u)] = l;
+}
+PF_DEV void els_drop(u32 w, u32* alive, u32* root, u32* next, u32* nnext) {
+  if (atomicCAS(&alive[w], 1u, 0u) == 1u) {
+    root[w] = w;
+    next[atomicAdd(nnext, 1u)] = w;
+  }
+}
+__global__ void els_trim_k(const u32* front, u32 n, const u32* off, const u32* adj, u32* alive, u32* indeg,
+                           u32* outdeg, u32* root, u32* next, u32* nnext) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 v = front[k];
+  for (u32 j = off[v]; j < off[v + 1]; j++) {  // out-neighbours lose an in-edge
+    const u32 w = adj[j];
+    if (atomicSub(&indeg[w], 1u) == 1u) els_drop(w, alive, root, next, nnext);
+  }
+  const u32 nv = FLIP(v);
+  for (u32 j = off[nv]; j < off[nv + 1]; j++) {  // in-neighbours lose an out-edge
+    const u32 u = FLIP(adj[j]);
+    if (atomicSub(&outdeg[u], 1u) == 1u) els_drop(u, alive, root, next, nnext);
+  }
+}
+void k_els_trim_init(const DB& db, const u32* off, u32* alive, u32* indeg, u32* outdeg, u32* root, u32* front,
+                     u32* nfront, Stream* st) {
+  fill_u32(nfront, 0, 1, st);
+  KLAUNCH(els_trim_init_k, grid_for(2 * (size_t)db.nvars + 2, 256), 256, st, db, off, alive, indeg, outdeg, root,
+          front, nfront);
+}
+void k_els_trim(const u32* front, u32 n, const u32* off, const u32* adj, u32* alive, u32* indeg, u32* outdeg, u32* root,
+                u32* next, u32* nnext, Stream* st) {
+  if (!n) return;
+  KLAUNCH(els_trim_k, grid_for(n, 256), 256, st, front, n, off, adj, alive, indeg, outdeg, root, next, nnext);
+}
+// Coloring over the literals that survive trimming (list[0..n)): forward
+// minimum labels, then from each root (color == itself) backwards inside
+// its color; the marked literals form the root's SCC, whose root is its
+// smallest literal.
+__global__ void els_alive_k(u32 n2, const u32* alive, u32* list, u32* n) {
+  const u32 l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < n2 && alive[l]) list[atomicAdd(n, 1u)] = l;
+}
+__global__ void els_color_init_k(const u32* list, u32 n, u32* color, u32* mark) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  color[list[k]] = list[k];
+  mark[list[k]] = 0;
+}
+__global__ void els_fw_k(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, u32* color,
+                         u32* changed) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 v = list[k], c = color[v];
+  for (u32 j = off[v]; j < off[v + 1]; j++) {
+    const u32 w = adj[j];
+    if (alive[w] && c < color[w] && atomicMin(&color[w], c) > c) *changed = 1;
+  }
+}
+__global__ void els_roots_k(const u32* list, u32 n, const u32* color, u32* mark) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && color[list[k]] == list[k]) mark[list[k]] = 1;
+}
+__global__ void els_bw_k(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, const u32* color,
+                         u32* mark, u32* changed) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 v = list[k];
+  if (mark[v]) return;
+  const u32 c = color[v];
+  for (u32 j = off[v]; j < off[v + 1]; j++) {
+    const u32 w = adj[j];
+    if (alive[w] && mark[w] && color[w] == c) {
+      mark[v] = 1;
+      *changed = 1;
+      return;
+    }
+  }
+}
+__global__ void els_take_k(const u32* list, u32 n, const u32* mark, const u32* color, u32* alive, u32* root) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 v = list[k];
+  if (mark[v]) {
+    root[v] = color[v];
+    alive[v] = 0;
+  }
+}
+// degrees inside the literals still alive (all counted before any is
+// dropped), then the next trimming frontier
+__global__ void els_redeg_k(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, u32* indeg,
+                            u32* outdeg) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 v = list[k];
+  if (!alive[v]) return;
+  u32 out = 0, in = 0;
+  for (u32 j = off[v]; j < off[v + 1]; j++) out += alive[adj[j]];
+  const u32 nv = FLIP(v);
+  for (u32 j = off[nv]; j < off[nv + 1]; j++) in += alive[FLIP(adj[j])];
+  indeg[v] = in;
+  outdeg[v] = out;
+}
+__global__ void els_redrop_k(const u32* list, u32 n, const u32* indeg, const u32* outdeg, u32* alive, u32* root,
+                             u32* front, u32* nfront) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 v = list[k];
+  if (alive[v] && !(indeg[v] && outdeg[v])) els_drop(v, alive, root, front, nfront);
+}
+void k_els_alive(u32 n2, const u32* alive, u32* list, u32* n, Stream* st) {
+  fill_u32(n, 0, 1, st);
+  KLAUNCH(els_alive_k, grid_for(n2, 256), 256, st, n2, alive, list, n);
+}
+void k_els_color_init(const u32* list, u32 n, u32* color, u32* mark, Stream* st) {
+  if (n) KLAUNCH(els_color_init_k, grid_for(n, 256), 256, st, list, n, color, mark);
+}
+void k_els_fw(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, u32* color, u32* changed,
+              Stream* st) {
+  fill_u32(changed, 0, 1, st);
+  if (n) KLAUNCH(els_fw_k, grid_for(n, 256), 256, st, list, n, off, adj, alive, color, changed);
+}
+void k_els_roots(const u32* list, u32 n, const u32* color, u32* mark, Stream* st) {
+  if (n) KLAUNCH(els_roots_k, grid_for(n, 256), 256, st, list, n, color, mark);
+}
+void k_els_bw(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, const u32* color, u32* mark,
+              u32* changed, Stream* st) {
+  fill_u32(changed, 0, 1, st);
+  if (n) KLAUNCH(els_bw_k, grid_for(n, 256), 256, st, list, n, off, adj, alive, color, mark, changed);
+}
+void k_els_take(const u32* list, u32 n, const u32* mark, const u32* color, u32* alive, u32* root, Stream* st) {
+  if (n) KLAUNCH(els_take_k, grid_for(n, 256), 256, st, list, n, mark, color, alive, root);
+}
+void k_els_redeg(const u32* list, u32 n, const u32* off, const u32* adj, u32* alive, u32* indeg, u32* outdeg,
+                 u32* root, u32* front, u32* nfront, Stream* st) {
+  fill_u32(nfront, 0, 1, st);
+  if (!n) return;
+  KLAUNCH(els_redeg_k, grid_for(n, 256), 256, st, list, n, off, adj, alive, indeg, outdeg);
+  KLAUNCH(els_redrop_k, grid_for(n, 256), 256, st, list, n, indeg, outdeg, alive, root, front, nfront);
+}
+// representatives (equivalence.cpp:66-118): rep[l] = root of l's SCC, l
+// for assumed variables, 0 for inactive literals; info[0] |= 1 when a
+// literal and its negation share an SCC, info[1] counts substituted literals
+__global__ void els_rep_k(DB db, const u32* root, u32* rep, u32* info) {
+  const u32 l = blockIdx.x * blockDim.x + threadIdx.x;
+  bool sub = false;
+  if (l < 2 * db.nvars + 2) {
+    const u32 v = ABS(l);
+    if (l < 2 || db.vstate[v]) {
+      rep[l] = 0;
+    } else {
+      const u32 r = root[l];
+      if (root[FLIP(l)] == r) atomicOr(&info[0], 1u);
+      rep[l] = db.assumed[v] ? l : r;
+      sub = rep[l] != l;
+    }
+  }
+  const unsigned long long b = __ballot(sub);
+  if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&info[1], (u32)__popcll(b));
+}
+void k_els_rep(const DB& db, const u32* root, u32* rep, u32* info, Stream* st) {
+  KLAUNCH(els_rep_k, grid_for(2 * (size_t)db.nvars + 2, 256), 256, st, db, root, rep, info);
+}
+// the clauses a substitution touches, with their position in the order
+// learnts, then originals (lrank: exclusive scan of the learnt flags)
+__global__ void els_learnt_k(DB db, u32* flag) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < db.ncls) flag[i] = st_learnt(cflags(db, db.c_off[i])) && !is_deleted(db, db.c_off[i]) ? 1u : 0u;
+}
+__global__ void els_touch_k(DB db, const u32* rep, const u32* lrank, u32 nlearnt, u32* list, u32* pos, u32* n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= db.ncls) return;
+  const u32 c = db.c_off[i];
+  if (is_deleted(db, c)) return;
+  const u32* l = clits(db, c);
+  const u32 sz = csize(db, c);
+  bool t = false;
+  for (u32 q = 0; q < sz && !t; q++) t = rep[l[q]] != l[q];
+  if (!t) return;
+  const u32 k = atomicAdd(n, 1u);
+  list[k] = c;
+  pos[k] = st_learnt(cflags(db, c)) ? lrank[i] : nlearnt + (i - lrank[i]);
+}
+void k_els_touch(const DB& db, const u32* rep, u32* lrank, void* tmp, u32* nlearnt, u32* list, u32* pos, u32* n,
+                 Stream* st) {
+  fill_u32(n, 0, 1, st);
+  fill_u32(nlearnt, 0, 1, st);
+  if (!db.ncls) return;
+  KLAUNCH(els_learnt_k, grid_for(db.ncls, 256), 256, st, db, lrank);
+  scan_u32(lrank, lrank, db.ncls, nlearnt, tmp, st);
+}
+void k_els_touch_list(const DB& db, const u32* rep, const u32* lrank, u32 nlearnt, u32* list, u32* pos, u32* n,
+                      Stream* st) {
+  if (db.ncls) KLAUNCH(els_touch_k, grid_for(db.ncls, 256), 256, st, db, rep, lrank, nlearnt, list, pos, n);
+}
+// one pass of the unit-time fixpoint: the unit each touched clause yields
+// under the current times (ptime: 2 p + sign, the smallest position wins)
+__global__ void els_check_k(DB db, const u32* list, const u32* pos, u32 n, ElsView e, u32* ptime) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  u32 u = 0, m = 0;
+  if (els_clause(db, list[k], e, pos[k], false, &u, &m) == 3) atomicMin(&ptime[ABS(u)], 2 * pos[k] + SIGN(u));
+}
+__global__ void els_commit_k(u32 n1, const u32* ptime, u32* utime, u32* changed) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n1 && ptime[v] != utime[v]) {
+    utime[v] = ptime[v];
+    *changed = 1;
+  }
+}
+void k_els_check(const DB& db, const u32* list, const u32* pos, u32 n, const ElsView& e, u32* ptime, Stream* st) {
+  fill_u32(ptime, NOPOS, (size_t)db.nvars + 1, st);
+  if (n) KLAUNCH(els_check_k, grid_for(n, 256), 256, st, db, list, pos, n, e, ptime);
+}
+void k_els_commit(u32 n1, const u32* ptime, u32* utime, u32* changed, Stream* st) {
+  fill_u32(changed, 0, 1, st);
+  KLAUNCH(els_commit_k, grid_for(n1, 256), 256, st, n1, ptime, utime, changed);
+}
+// the substitution itself; units as (position, literal) records; info[0]
+// |= 1 on an empty clause, info[1] counts clauses rewritten to two literals
+__global__ void els_apply_k(DB db, const u32* list, const u32* pos, u32 n, ElsView e, u32* ukey, u32* uval, u32* nu,
+                            u32* info) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  u32 u = 0, m = 0;
+  const u32 kind = els_clause(db, list[k], e, pos[k], true, &u, &m);
+  if (kind == 2) atomicOr(&info[0], 1u);
+  if (kind == 3) {
+    const u32 r = atomicAdd(nu, 1u);
+    ukey[r] = pos[k];
+    uval[r] = u;
+  }
+  if (kind == 4 && m == 2) atomicAdd(&info[1], 1u);
+}
+void k_els_apply(const DB& db, const u32* list, const u32* pos, u32 n, const ElsView& e, u32* ukey, u32* uval, u32* nu,
+                 u32* info, Stream* st) {
+  fill_u32(nu, 0, 1, st);
+  if (n) KLAUNCH(els_apply_k, grid_for(n, 256), 256, st, db, list, pos, n, e, ukey, uval, nu, info);
+}
+// units in position order onto the trail (enqueueUnit)
+__global__ void els_enqueue_k(DB db, const u32* uval, u32 n, u32* trail) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 u = uval[k];
+  trail[k] = u;
+  db.assign[ABS(u)] = u + 1;
+  db.vstate[ABS(u)] = VS_FROZEN;
+}
+void k_els_enqueue(const DB& db, const u32* uval, u32 n, u32* trail, Stream* st) {
+  if (n) KLAUNCH(els_enqueue_k, grid_for(n, 256), 256, st, db, uval, n, trail);
+}
+// substituted variables (equivalence.cpp:173-190): an active variable whose
+// positive literal has another representative gets a binary record on the
+// stack (MODEL::saveBinary, model.cpp:280-288) and, when the
+// representative's variable is active, the state SUBSTITUTED
+__global__ void els_subst_flags_k(DB db, const u32* rep, u32* flag) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > db.nvars) return;
+  flag[v] = v && !db.vstate[v] && rep[V2L(v)] != V2L(v) ? 1u : 0u;
+}
+__global__ void els_subst_k(DB db, const u32* rep, const u32* flag, const u32* pos, u32* stack, u32* nmarked) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  bool marked = false;
+  if (v <= db.nvars && flag[v]) {
+    const u32 p = V2L(v), other = rep[p];
+    u32* w = stack + 5 * (size_t)pos[v];
+    w[0] = V2L(db.vorg[v]);
+    w[1] = V2L(db.vorg[ABS(other)]) | SIGN(FLIP(other));
+    w[2] = 2;
+    w[3] = V2L(db.vorg[v]) | 1u;
+    w[4] = 1;
+    // the representative's variable is the smallest of its SCC, never marked here
+    marked = !db.vstate[ABS(other)];
+    if (marked) db.vstate[v] = VS_SUBSTITUTED;
+  }
+  const unsigned long long b = __ballot(marked);
+  if ((threadIdx.x & 63u) == 0 && b) atomicAdd(nmarked, (u32)__popcll(b));
+}
+void k_els_subst_flags(const DB& db, const u32* rep, u32* flag, u32* pos, void* tmp, u32* total, Stream* st) {
+  KLAUNCH(els_subst_flags_k, grid_for((size_t)db.nvars + 1, 256), 256, st, db, rep, flag);
+  scan_u32(flag, pos, (size_t)db.nvars + 1, total, tmp, st);
+}
+void k_els_subst(const DB& db, const u32* rep, const u32* flag, const u32* pos, u32* stack, u32* nmarked,
+                 Stream* st) {
+  fill_u32(nmarked, 0, 1, st);
+  KLAUNCH(els_subst_k, grid_for((size_t)db.nvars + 1, 256), 256, st, db, rep, flag, pos, stack, nmarked);
+}
+
 }  // namespace pfs

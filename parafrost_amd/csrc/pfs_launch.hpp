@@ -204,4 +204,41 @@ void k_parse_units(const DB& db, const u32* ulit, u32 n, u32* first, u32* win, u
                    u32* trail, u32* err, Stream*);
 void iota_u32(u32* a, size_t n, Stream*);
 
+// ---- equivalent-literal substitution (src/cpu/equivalence.cpp:41-306)
+// implication graph CSR over literals: deg and off (2V+3 words, off[2V+2] =
+// edges), then adj (k_els_fill)
+void k_els_graph(const DB& db, u32* deg, u32* off, void* tmp, Stream*);
+void k_els_fill(const DB& db, const u32* off, u32* cur, u32* adj, Stream*);
+// trimming: alive, degrees and roots of every literal, first frontier
+void k_els_trim_init(const DB& db, const u32* off, u32* alive, u32* indeg, u32* outdeg, u32* root, u32* front,
+                     u32* nfront, Stream*);
+void k_els_trim(const u32* front, u32 n, const u32* off, const u32* adj, u32* alive, u32* indeg, u32* outdeg,
+                u32* root, u32* next, u32* nnext, Stream*);
+// coloring over the alive literals
+void k_els_alive(u32 n2, const u32* alive, u32* list, u32* n, Stream*);
+void k_els_color_init(const u32* list, u32 n, u32* color, u32* mark, Stream*);
+void k_els_fw(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, u32* color, u32* changed,
+              Stream*);
+void k_els_roots(const u32* list, u32 n, const u32* color, u32* mark, Stream*);
+void k_els_bw(const u32* list, u32 n, const u32* off, const u32* adj, const u32* alive, const u32* color, u32* mark,
+              u32* changed, Stream*);
+void k_els_take(const u32* list, u32 n, const u32* mark, const u32* color, u32* alive, u32* root, Stream*);
+void k_els_redeg(const u32* list, u32 n, const u32* off, const u32* adj, u32* alive, u32* indeg, u32* outdeg,
+                 u32* root, u32* front, u32* nfront, Stream*);
+// representatives; info[0]: conflict, info[1]: substituted literals
+void k_els_rep(const DB& db, const u32* root, u32* rep, u32* info, Stream*);
+// touched clauses and their positions (learnts first): lrank = exclusive
+// scan of the learnt flags (k_els_touch), then the list (k_els_touch_list)
+void k_els_touch(const DB& db, const u32* rep, u32* lrank, void* tmp, u32* nlearnt, u32* list, u32* pos, u32* n,
+                 Stream*);
+void k_els_touch_list(const DB& db, const u32* rep, const u32* lrank, u32 nlearnt, u32* list, u32* pos, u32* n,
+                      Stream*);
+void k_els_check(const DB& db, const u32* list, const u32* pos, u32 n, const ElsView& e, u32* ptime, Stream*);
+void k_els_commit(u32 n1, const u32* ptime, u32* utime, u32* changed, Stream*);
+void k_els_apply(const DB& db, const u32* list, const u32* pos, u32 n, const ElsView& e, u32* ukey, u32* uval,
+                 u32* nu, u32* info, Stream*);
+void k_els_enqueue(const DB& db, const u32* uval, u32 n, u32* trail, Stream*);
+void k_els_subst_flags(const DB& db, const u32* rep, u32* flag, u32* pos, void* tmp, u32* total, Stream*);
+void k_els_subst(const DB& db, const u32* rep, const u32* flag, const u32* pos, u32* stack, u32* nmarked, Stream*);
+
 }  // namespace pfs

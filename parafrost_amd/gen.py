@@ -143,6 +143,63 @@ def from_clauses(maxvar: int, clauses) -> Cnf:
     return Cnf(maxvar, np.array(off, np.uint64), np.array(lits, np.uint32))
 
 
+def equivalences(nvars: int, nclauses: int, nclasses: int, size: int, k: int = 3, seed: int = BASE_SEED + 5,
+                 chords: float = 0.5, units: int = 0, conflict: bool = False, anti: int = 0) -> Cnf:
+    """Random k-SAT with planted equivalent literals: nclasses classes of
+    'size' literals (distinct variables, random signs), each closed into a
+    cycle of implications by binary clauses (-a | b), plus chords; units
+    unit clauses of random literals; conflict puts a literal and its
+    negation into one class; anti clauses (a | b) or (-a | -b) over two
+    variables of one class, each in its own class (they become units or
+    tautologies once the class is substituted).  Clause order is shuffled.
+    Vectorised: millions of clauses take seconds."""
+    rng = np.random.default_rng(seed)
+    if nclasses * size > nvars or size < 2:
+        raise ValueError("classes need more variables than there are")
+    perm = rng.permutation(np.arange(1, nvars + 1, dtype=np.int64))
+    members = perm[: nclasses * size].reshape(nclasses, size)
+    lits = 2 * members + rng.integers(0, 2, (nclasses, size))
+    order = np.argsort(rng.random((nclasses, size)), axis=1)
+    cyc = np.take_along_axis(lits, order, axis=1)
+    parts = [np.stack([cyc ^ 1, np.roll(cyc, -1, axis=1)], axis=2).reshape(-1, 2)]  # (-a | next a)
+    nch = int(chords * size)
+    if nch:
+        ci = rng.integers(0, size, (nclasses, nch))
+        cj = (ci + rng.integers(1, size, (nclasses, nch))) % size
+        parts.append(np.stack([np.take_along_axis(lits, ci, axis=1) ^ 1, np.take_along_axis(lits, cj, axis=1)],
+                              axis=2).reshape(-1, 2))
+    if conflict:
+        parts.append(np.array([[lits[0, 0] ^ 1, lits[0, 1] ^ 1], [lits[0, 1], lits[0, 0]]], np.int64))
+    na = min(anti, nclasses)
+    if na:
+        cl = rng.choice(nclasses, na, replace=False)
+        ai = rng.integers(0, size, na)
+        aj = (ai + rng.integers(1, size, na)) % size
+        neg = rng.integers(0, 2, na)
+        parts.append(np.stack([2 * members[cl, ai] ^ neg, 2 * members[cl, aj] ^ neg], axis=1))
+    bins = np.concatenate(parts)
+    # k distinct variables per clause (rows with a repeat are drawn again)
+    vs = rng.integers(1, nvars + 1, (nclauses, k))
+    while True:
+        srt = np.sort(vs, axis=1)
+        bad = np.nonzero((srt[:, 1:] == srt[:, :-1]).any(axis=1))[0] if k > 1 else np.zeros(0, np.int64)
+        if not bad.shape[0]:
+            break
+        vs[bad] = rng.integers(1, nvars + 1, (bad.shape[0], k))
+    kcl = 2 * vs + rng.integers(0, 2, (nclauses, k))
+    ucl = (2 * rng.integers(1, nvars + 1, units) + rng.integers(0, 2, units)).reshape(-1, 1)
+    flat = np.concatenate([bins.ravel(), kcl.ravel(), ucl.ravel()]).astype(np.uint32)
+    sizes = np.concatenate([np.full(bins.shape[0], 2), np.full(nclauses, k), np.ones(units, np.int64)])
+    starts = np.zeros(sizes.shape[0], np.int64)
+    np.cumsum(sizes[:-1], out=starts[1:])
+    p = rng.permutation(sizes.shape[0])
+    ns = sizes[p]
+    off = np.zeros(ns.shape[0] + 1, np.uint64)
+    np.cumsum(ns, out=off[1:])
+    idx = np.repeat(starts[p], ns) + (np.arange(int(off[-1])) - np.repeat(off[:-1].astype(np.int64), ns))
+    return Cnf(nvars, off, flat[idx])
+
+
 def batch(n: int = 512, seed: int = BASE_SEED + 4, cmin: float = 1e3, cmax: float = 5e5) -> list[Cnf]:
     """The batched workload: n formulas with clause counts log-uniform in
     [cmin, cmax]; each is, with probability 1/2, a Tseitin circuit scaled to

@@ -27,7 +27,8 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_kernel_count", "pfs_kernel_stats", "pfs_memory", "pfs_last_error", "pfs_version",
            "pfs_device_count", "pfs_batch_simplify", "pfs_append", "pfs_assume", "pfs_server_create",
            "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy", "pfs_map",
-           "pfs_download_map", "pfs_map_lits", "pfs_parse_dimacs", "pfs_download_trail")
+           "pfs_download_map", "pfs_map_lits", "pfs_parse_dimacs", "pfs_download_trail",
+           "pfs_decompose")
 
 
 class PfsError(RuntimeError):
@@ -51,6 +52,11 @@ class _ParseInfo(ctypes.Structure):
     _fields_ = [("maxvar", ctypes.c_uint32), ("status", ctypes.c_uint32), ("header_clauses", ctypes.c_uint64),
                 ("read_clauses", ctypes.c_uint64), ("clauses", ctypes.c_uint64), ("literals", ctypes.c_uint64),
                 ("trail", ctypes.c_uint64)]
+
+
+class _ElsInfo(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_uint32), ("status", ctypes.c_uint32), ("substituted", ctypes.c_uint64),
+                ("units", ctypes.c_uint64), ("stack_words", ctypes.c_uint64)]
 
 
 class _MapInfo(ctypes.Structure):
@@ -113,6 +119,7 @@ def load_library():
     L.pfs_map_lits.argtypes = [vp, vp, u64]
     L.pfs_parse_dimacs.argtypes = [vp, ctypes.c_char_p, u64, ctypes.POINTER(_ParseInfo)]
     L.pfs_download_trail.argtypes = [vp, vp, ctypes.POINTER(u64)]
+    L.pfs_decompose.argtypes = [vp, u32, ctypes.POINTER(_ElsInfo)]
     L.pfs_device_count.restype = i32
     _LIB = L
     return L
@@ -216,6 +223,12 @@ class Simplifier:
         out = {k: int(getattr(info, k)) for k, _ in _ParseInfo._fields_}
         self.maxvar = out["maxvar"]
         return out
+
+    def decompose(self, rounds: int = 0) -> dict:
+        """Equivalent-literal substitution on the resident formula (pfs_decompose)."""
+        info = _ElsInfo()
+        self._check(self._L.pfs_decompose(self._h, rounds, ctypes.byref(info)), "pfs_decompose")
+        return {k: int(getattr(info, k)) for k, _ in _ElsInfo._fields_}
 
     def trail(self) -> np.ndarray:
         """The root trail of the resident formula (pfs_download_trail)."""

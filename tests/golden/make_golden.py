@@ -175,6 +175,58 @@ def main_dimacs():
         json.dump(manifest, f, indent=1)
 
 
+def els_cases():
+    """(name, formula, rounds) of the ELS fixtures; rounds None: the
+    reference's default (decomposemin)."""
+    yield "els_eq_a", gen.equivalences(200, 500, 10, 4, seed=1), None
+    yield "els_eq_b", gen.equivalences(2000, 6000, 80, 6, seed=2, units=5), None
+    yield "els_eq_anti", gen.equivalences(1000, 3000, 60, 5, seed=16, anti=6), None
+    yield "els_eq_anti_r1", gen.equivalences(1000, 3000, 60, 5, seed=16, anti=6), 1
+    yield "els_eq_anti_r5", gen.equivalences(1000, 3000, 60, 5, seed=16, anti=6), 5
+    yield "els_eq_anti_unsat", gen.equivalences(1000, 3000, 60, 5, seed=6, anti=40), None
+    yield "els_eq_units", gen.equivalences(1500, 4000, 100, 4, seed=7, anti=10, units=3), None
+    yield "els_eq_units_b", gen.equivalences(1500, 4000, 100, 4, seed=17, anti=10, units=2), None
+    yield "els_eq_conflict", gen.equivalences(500, 1500, 30, 5, seed=3, conflict=True), None
+    yield "els_eq_2sat", gen.equivalences(800, 300, 40, 5, k=2, seed=8, anti=10), None
+    yield "els_eq_big", gen.equivalences(20000, 60000, 1500, 5, seed=9, anti=200, units=20), None
+    yield "els_u3_n2000", gen.uniform_ksat(2000, 8520, 3, seed=13), None
+    yield "els_ts_g2000", gen.tseitin(2000, 60, 10, seed=22), None
+    yield "els_u2_n400", gen.uniform_ksat(400, 500, 2, seed=15), None
+    yield "els_ht_v2000", gen.heavytail(2000, 8000, seed=32), None
+
+
+def main_els():
+    """ELS fixtures: <name>.in.pfd.gz (the reference's root-clean input),
+    <name>.els.pfd.gz (its state after decompose rounds + rootify +
+    shrinkTop, PFREF_ELS_OUT of oracle/ref_harness.cpp), listed in
+    els_manifest.json."""
+    manifest = []
+    for name, cnf, rounds in els_cases():
+        cnf_path = os.path.join(HERE, name + ".cnf")
+        gen.write_dimacs(cnf, cnf_path)
+        inp = os.path.join(HERE, name + ".in.pfd")
+        out = os.path.join(HERE, name + ".els.pfd")
+        env = dict(os.environ, PFREF_ELS_OUT=out)
+        if rounds:
+            env["PFREF_ELS_ROUNDS"] = str(rounds)
+        r = subprocess.run([REF, cnf_path, inp, os.path.join(HERE, name + ".unused.pfd")], capture_output=True,
+                           text=True, timeout=600, env=env)
+        os.remove(cnf_path)
+        entry = {"name": name, "rounds": rounds}
+        if not os.path.exists(inp):
+            print("skip (refuted at parse time)", name)
+            continue
+        gz(inp)
+        gz(out)
+        entry.update({"input": name + ".in.pfd.gz", "output": name + ".els.pfd.gz"})
+        line = [x for x in r.stdout.splitlines() if x.startswith("{")]
+        entry["reference"] = json.loads(line[-1]) if line else None
+        manifest.append(entry)
+        print(name, entry["reference"])
+    with open(os.path.join(HERE, "els_manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
+
+
 def gz(path):
     """Compresses path to path.gz (no timestamp in the header) unless an
     existing path.gz already holds the same bytes."""
@@ -232,6 +284,7 @@ def main():
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1)
     main_dimacs()
+    main_els()
 
 
 if __name__ == "__main__":
