@@ -185,6 +185,53 @@ def bench_ingest(cnf, steps=3):
     return out
 
 
+def bench_els(steps=3, rounds=2):
+    """Equivalent-literal substitution (pfs_decompose) on a planted-
+    equivalence formula: 2,000,000 variables, 6,000,000 ternary clauses,
+    200,000 classes of 5 literals closed by binary clauses (+ chords).
+    Inputs resident in HBM (pfs_rewind between calls).  The reference's own
+    decompose rounds on the same text (oracle/_ref/pfref, PFREF_ELS_OUT, one
+    core) are timed beside it."""
+    from parafrost_amd import gen
+    from parafrost_amd.simplifier import Simplifier
+    cnf = gen.equivalences(2_000_000, 6_000_000, 200_000, 5, seed=0x5EED0007, chords=0.4)
+    s = Simplifier(0, keep_input=True, decomposemin=rounds)
+    try:
+        s.upload(cnf.maxvar, cnf.offsets, cnf.lits)
+        s.decompose()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.rewind()
+            info = s.decompose()
+        dt = (time.perf_counter() - t0) / steps
+        s.set_option("profilegpu", 1)
+        s.rewind()
+        s.decompose()
+        kt = s.kernel_times()
+        s.set_option("profilegpu", 0)
+    finally:
+        s.close()
+    out = {"variables": cnf.maxvar, "clauses": cnf.nclauses, "rounds": info["rounds"],
+           "substituted": info["substituted"], "ms": round(dt * 1e3, 3), "clauses_per_s": round(cnf.nclauses / dt, 1),
+           "device_kernel_ms": round(sum(v[0] for v in kt.values()), 3),
+           "kernels_ms": {k: round(v[0], 4) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:8]}}
+    ref = os.path.join(ROOT, "oracle", "_ref", "pfref")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "e.cnf")
+            gen.write_dimacs(cnf, p)
+            env = dict(os.environ, PFREF_ELS_OUT=os.path.join(d, "e.els"), PFREF_ELS_ROUNDS=str(rounds))
+            r = subprocess.run(["taskset", "-c", "0", ref, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd")],
+                               capture_output=True, text=True, timeout=600, env=env)
+            line = [x for x in r.stdout.splitlines() if x.startswith("{")]
+            if line:
+                j = json.loads(line[-1])
+                out["cpu_reference_s"] = j.get("els_s")
+                out["cpu_reference_rounds"] = j.get("els_rounds")
+    print("bench: els " + json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(cnf, name, budget_clauses=7_500_000):
     """The reference CPU simplifier (compiled from its sources into
     oracle/_ref/pfref) on the same workload, or a bounded sample of it, on one
@@ -413,6 +460,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-ingest", action="store_true", help="skip the DIMACS ingestion leg")
+    ap.add_argument("--no-els", action="store_true", help="skip the equivalent-literal substitution leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 256 MiB device copies first (reference kernel for counter runs)")
     args = ap.parse_args()
@@ -494,6 +542,7 @@ def main():
     handoff = bench_handoff(s, cnf)
     s.close()
     ingest = None if args.no_ingest else bench_ingest(cnf)
+    els_leg = None if args.no_els else bench_els()
 
     heavy = None if (args.no_heavytail or args.workload == "heavytail") else bench_single("heavytail")
     batched = None if args.no_batched else bench_batched()
@@ -529,6 +578,7 @@ def main():
         "cpu_baseline": base,
         "handoff": handoff,
         "ingest": ingest,
+        "els": els_leg,
         "heavytail": heavy,
         "batched": batched,
         "serving": serving,

@@ -217,7 +217,9 @@ int main(int argc, char** argv) {
     const char* rr = getenv("PFREF_ELS_ROUNDS");
     const int rounds = rr ? atoi(rr) : s->opts.decompose_min;
     int done = 0;
+    const double te0 = now_s();
     for (bool ok = true; ok && done < rounds && s->cnfstate; done++) ok = s->decompose();
+    const double els_s = now_s() - te0;
     if (s->cnfstate) {
       s->rootify();
       if (s->cnfstate) s->shrinkTop(false);
@@ -245,7 +247,7 @@ int main(int argc, char** argv) {
     }
     write_pfd(els_out, st, inf.maxVar, ed, eu, er, ev);
     g_done = true;
-    printf("{\"els_rounds\": %d, \"status\": %u}\n", done, st);
+    printf("{\"els_rounds\": %d, \"status\": %u, \"els_s\": %.6f}\n", done, st, els_s);
     fflush(stdout);
     _exit(0);
   }

@@ -413,13 +413,16 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
   mem_.ensure(upos_d, ((size_t)NC + 1) * 4, 0, st_);
   mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes((size_t)NC + 1)), 0, st_);
   fill_u32(word, 0, 2, st_);
-  k_parse_dedup(db, P<u32>(ulit_d), word + 1, err, st_);
+  k_parse_dedup(db, P<u32>(ulit_d), err, st_);
   k_parse_units(db, P<u32>(ulit_d), NC, P<u32>(first_d), P<u32>(win_d), P<u32>(upos_d), tmp_.p, word,
                 P<u32>(trail_), err, st_);
+  k_count(db, (u64*)(P<u32>(scal_) + 40), st_);  // clauses that stay (duplicates, tautologies, units gone)
+  uint64_t kept[2];
+  d2h(kept, P<u32>(scal_) + 40, sizeof(kept), st_);
   d2h(&h, pmark_d.p, sizeof(h), st_);
   stream_sync(st_);
   if (h.err & LEXE_SIZE) throw std::invalid_argument("clause longer than 2^24-1 literals");
-  if (h.word[1] > C) throw std::invalid_argument("too many clauses");
+  if (kept[0] > C) throw std::invalid_argument("too many clauses");
   const u32 ntrail = h.word[0];
 
   // ---- the state of a fresh upload, then root propagation and the shrink

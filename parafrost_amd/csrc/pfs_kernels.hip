@@ -1309,17 +1309,13 @@ void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Str
 // clauses reported, unit clauses turned into unit records (toClause,
 // dimacs.cpp:120-166); ulit[i] = the unit literal of clause i or 0;
 // kept[0] counts the clauses that stay
-// the kept count takes one atomic per wavefront
-__global__ void parse_dedup_k(DB db, u32* ulit, u32* kept, u32* err) {
+__global__ void parse_dedup_k(DB db, u32* ulit, u32* err) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  const u32 kind = i < db.ncls ? parse_dedup_clause(db, db.c_off[i], &ulit[i]) : 4u;
-  const unsigned long long kb = __ballot(kind == 0);
-  if ((threadIdx.x & 63u) == 0 && kb) atomicAdd(kept, (u32)__popcll(kb));
-  if (kind == 2) atomicOr(err, (u32)LEXE_UNSAT);
+  if (i < db.ncls && parse_dedup_clause(db, db.c_off[i], &ulit[i]) == 2) atomicOr(err, (u32)LEXE_UNSAT);
 }
-void k_parse_dedup(const DB& db, u32* ulit, u32* kept, u32* err, Stream* st) {
+void k_parse_dedup(const DB& db, u32* ulit, u32* err, Stream* st) {
   if (!db.ncls) return;
-  KLAUNCH(parse_dedup_k, grid_for(db.ncls, 256), 256, st, db, ulit, kept, err);
+  KLAUNCH(parse_dedup_k, grid_for(db.ncls, 256), 256, st, db, ulit, err);
 }
 // unit clauses in text order: the first one of each variable is assigned
 // (enqueueUnit); a later one of the other sign is a conflict
@@ -1557,25 +1553,32 @@ void k_els_redeg(const u32* list, u32 n, const u32* off, const u32* adj, u32* al
 // representatives (equivalence.cpp:66-118): rep[l] = root of l's SCC, l
 // for assumed variables, 0 for inactive literals; info[0] |= 1 when a
 // literal and its negation share an SCC, info[1] counts substituted literals
+// grid-stride over a bounded number of workgroups: the count takes one
+// atomic per workgroup (a single counter serialises its atomics)
+__device__ inline void block_count_add(u32 mine, u32* lw, u32* out) {
+  u32 tot;
+  block_excl_scan(mine, lw, &tot);
+  if (threadIdx.x == 0 && tot) atomicAdd(out, tot);
+}
 __global__ void els_rep_k(DB db, const u32* root, u32* rep, u32* info) {
-  const u32 l = blockIdx.x * blockDim.x + threadIdx.x;
-  bool sub = false;
-  if (l < 2 * db.nvars + 2) {
+  __shared__ u32 lw[NW];
+  const u32 n2 = 2 * db.nvars + 2;
+  u32 subs = 0;
+  for (u32 l = blockIdx.x * blockDim.x + threadIdx.x; l < n2; l += gridDim.x * blockDim.x) {
     const u32 v = ABS(l);
     if (l < 2 || db.vstate[v]) {
       rep[l] = 0;
-    } else {
-      const u32 r = root[l];
-      if (root[FLIP(l)] == r) atomicOr(&info[0], 1u);
-      rep[l] = db.assumed[v] ? l : r;
-      sub = rep[l] != l;
+      continue;
     }
+    const u32 r = root[l];
+    if (root[FLIP(l)] == r) atomicOr(&info[0], 1u);
+    rep[l] = db.assumed[v] ? l : r;
+    subs += rep[l] != l;
   }
-  const unsigned long long b = __ballot(sub);
-  if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&info[1], (u32)__popcll(b));
+  block_count_add(subs, lw, &info[1]);
 }
 void k_els_rep(const DB& db, const u32* root, u32* rep, u32* info, Stream* st) {
-  KLAUNCH(els_rep_k, grid_for(2 * (size_t)db.nvars + 2, 256), 256, st, db, root, rep, info);
+  KLAUNCH(els_rep_k, std::min(grid_for(2 * (size_t)db.nvars + 2, ST), RED_BLOCKS), ST, st, db, root, rep, info);
 }
 // the clauses a substitution touches, with their position in the order
 // learnts, then originals (lrank: exclusive scan of the learnt flags)
@@ -1675,9 +1678,10 @@ __global__ void els_subst_flags_k(DB db, const u32* rep, u32* flag) {
   flag[v] = v && !db.vstate[v] && rep[V2L(v)] != V2L(v) ? 1u : 0u;
 }
 __global__ void els_subst_k(DB db, const u32* rep, const u32* flag, const u32* pos, u32* stack, u32* nmarked) {
-  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
-  bool marked = false;
-  if (v <= db.nvars && flag[v]) {
+  __shared__ u32 lw[NW];
+  u32 marked = 0;
+  for (u32 v = blockIdx.x * blockDim.x + threadIdx.x; v <= db.nvars; v += gridDim.x * blockDim.x) {
+    if (!flag[v]) continue;
     const u32 p = V2L(v), other = rep[p];
     u32* w = stack + 5 * (size_t)pos[v];
     w[0] = V2L(db.vorg[v]);
@@ -1686,11 +1690,12 @@ __global__ void els_subst_k(DB db, const u32* rep, const u32* flag, const u32* p
     w[3] = V2L(db.vorg[v]) | 1u;
     w[4] = 1;
     // the representative's variable is the smallest of its SCC, never marked here
-    marked = !db.vstate[ABS(other)];
-    if (marked) db.vstate[v] = VS_SUBSTITUTED;
+    if (!db.vstate[ABS(other)]) {
+      db.vstate[v] = VS_SUBSTITUTED;
+      marked++;
+    }
   }
-  const unsigned long long b = __ballot(marked);
-  if ((threadIdx.x & 63u) == 0 && b) atomicAdd(nmarked, (u32)__popcll(b));
+  block_count_add(marked, lw, nmarked);
 }
 void k_els_subst_flags(const DB& db, const u32* rep, u32* flag, u32* pos, void* tmp, u32* total, Stream* st) {
   KLAUNCH(els_subst_flags_k, grid_for((size_t)db.nvars + 1, 256), 256, st, db, rep, flag);
@@ -1699,7 +1704,8 @@ void k_els_subst_flags(const DB& db, const u32* rep, u32* flag, u32* pos, void* 
 void k_els_subst(const DB& db, const u32* rep, const u32* flag, const u32* pos, u32* stack, u32* nmarked,
                  Stream* st) {
   fill_u32(nmarked, 0, 1, st);
-  KLAUNCH(els_subst_k, grid_for((size_t)db.nvars + 1, 256), 256, st, db, rep, flag, pos, stack, nmarked);
+  KLAUNCH(els_subst_k, std::min(grid_for((size_t)db.nvars + 1, ST), RED_BLOCKS), ST, st, db, rep, flag, pos, stack,
+          nmarked);
 }
 
 }  // namespace pfs

@@ -195,8 +195,8 @@ void k_clause_ends(const u32* lits, u64 n, u32* tmp_pos, void* tmp, u32* total, 
 // clause i of the pool from the literals between 0s (err bit 4: too long)
 void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Stream*);
 // sorted clauses: duplicates dropped, tautologies and units removed
-// (ulit[i]: the unit), *kept = clauses that stay, err bit 8: empty clause
-void k_parse_dedup(const DB& db, u32* ulit, u32* kept, u32* err, Stream*);
+// (ulit[i]: the unit), err bit 8: empty clause
+void k_parse_dedup(const DB& db, u32* ulit, u32* err, Stream*);
 // unit clauses in order: first of each variable assigned and written to
 // the trail (*total of them), opposite units set err bit 8; first needs
 // nvars+1 words, win and pos n words
