@@ -202,6 +202,17 @@ __device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) {
   return pre + x - v;
 }
 
+// order-free compaction: each thread holds the keep bits of SIPT
+// consecutive items; the workgroup reserves its slots with one atomic and
+// returns each thread's first slot
+__device__ inline u32 block_reserve(u32 bits, u32* lw, u32* counter) {
+  __shared__ u32 base;
+  u32 tot;
+  const u32 pre = block_excl_scan(__popc(bits), lw, &tot);
+  if (threadIdx.x == 0) base = tot ? atomicAdd(counter, tot) : 0u;
+  __syncthreads();
+  return base + pre;
+}
 // grid-stride reductions: a bounded number of workgroups, one atomic per
 // wavefront, so a single result word never sees more than a few thousand
 // atomics
@@ -1403,15 +1414,27 @@ void k_els_fill(const DB& db, const u32* off, u32* cur, u32* adj, Stream* st) {
 // start with no in- or out-edge.
 __global__ void els_trim_init_k(DB db, const u32* off, u32* alive, u32* indeg, u32* outdeg, u32* root, u32* front,
                                 u32* nfront) {
-  const u32 l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= 2 * db.nvars + 2) return;
-  const bool act = l >= 2 && !db.vstate[ABS(l)];
-  const u32 out = off[l + 1] - off[l], in = off[FLIP(l) + 1] - off[FLIP(l)];
-  indeg[l] = in;
-  outdeg[l] = out;
-  root[l] = l;
-  alive[l] = act && in && out ? 1u : 0u;
-  if (act && !(in && out)) front[atomicAdd(nfront, 1u)] = l;
+  __shared__ u32 lw[NW];
+  const u32 n2 = 2 * db.nvars + 2, first = (blockIdx.x * blockDim.x + threadIdx.x) * SIPT;
+  u32 bits = 0;
+#pragma unroll
+  for (u32 t = 0; t < (u32)SIPT; t++) {
+    const u32 l = first + t;
+    if (l >= n2) break;
+    const bool act = l >= 2 && !db.vstate[ABS(l)];
+    const u32 out = off[l + 1] - off[l], in = off[FLIP(l) + 1] - off[FLIP(l)];
+    indeg[l] = in;
+    outdeg[l] = out;
+    root[l] = l;
+    alive[l] = act && in && out ? 1u : 0u;
+    if (act && !(in && out)) bits |= 1u << t;
+  }
+  u32 k = block_reserve(bits, lw, nfront);
+  while (bits) {
+    const u32 t = __ffs(bits) - 1;
+    bits &= bits - 1;
+    front[k++] = first + t;
+  }
 }
 PF_DEV void els_drop(u32 w, u32* alive, u32* root, u32* next, u32* nnext) {
   if (atomicCAS(&alive[w], 1u, 0u) == 1u) {
@@ -1437,7 +1460,7 @@ __global__ void els_trim_k(const u32* front, u32 n, const u32* off, const u32* a
 void k_els_trim_init(const DB& db, const u32* off, u32* alive, u32* indeg, u32* outdeg, u32* root, u32* front,
                      u32* nfront, Stream* st) {
   fill_u32(nfront, 0, 1, st);
-  KLAUNCH(els_trim_init_k, grid_for(2 * (size_t)db.nvars + 2, 256), 256, st, db, off, alive, indeg, outdeg, root,
+  KLAUNCH(els_trim_init_k, grid_for(2 * (size_t)db.nvars + 2, STILE), ST, st, db, off, alive, indeg, outdeg, root,
           front, nfront);
 }
 void k_els_trim(const u32* front, u32 n, const u32* off, const u32* adj, u32* alive, u32* indeg, u32* outdeg, u32* root,
@@ -1450,8 +1473,18 @@ void k_els_trim(const u32* front, u32 n, const u32* off, const u32* adj, u32* al
 // its color; the marked literals form the root's SCC, whose root is its
 // smallest literal.
 __global__ void els_alive_k(u32 n2, const u32* alive, u32* list, u32* n) {
-  const u32 l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < n2 && alive[l]) list[atomicAdd(n, 1u)] = l;
+  __shared__ u32 lw[NW];
+  const u32 first = (blockIdx.x * blockDim.x + threadIdx.x) * SIPT;
+  u32 bits = 0;
+#pragma unroll
+  for (u32 t = 0; t < (u32)SIPT; t++)
+    if (first + t < n2 && alive[first + t]) bits |= 1u << t;
+  u32 k = block_reserve(bits, lw, n);
+  while (bits) {
+    const u32 t = __ffs(bits) - 1;
+    bits &= bits - 1;
+    list[k++] = first + t;
+  }
 }
 __global__ void els_color_init_k(const u32* list, u32 n, u32* color, u32* mark) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1522,7 +1555,7 @@ __global__ void els_redrop_k(const u32* list, u32 n, const u32* indeg, const u32
 }
 void k_els_alive(u32 n2, const u32* alive, u32* list, u32* n, Stream* st) {
   fill_u32(n, 0, 1, st);
-  KLAUNCH(els_alive_k, grid_for(n2, 256), 256, st, n2, alive, list, n);
+  KLAUNCH(els_alive_k, grid_for(n2, STILE), ST, st, n2, alive, list, n);
 }
 void k_els_color_init(const u32* list, u32 n, u32* color, u32* mark, Stream* st) {
   if (n) KLAUNCH(els_color_init_k, grid_for(n, 256), 256, st, list, n, color, mark);
@@ -1587,18 +1620,29 @@ __global__ void els_learnt_k(DB db, u32* flag) {
   if (i < db.ncls) flag[i] = st_learnt(cflags(db, db.c_off[i])) && !is_deleted(db, db.c_off[i]) ? 1u : 0u;
 }
 __global__ void els_touch_k(DB db, const u32* rep, const u32* lrank, u32 nlearnt, u32* list, u32* pos, u32* n) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= db.ncls) return;
-  const u32 c = db.c_off[i];
-  if (is_deleted(db, c)) return;
-  const u32* l = clits(db, c);
-  const u32 sz = csize(db, c);
-  bool t = false;
-  for (u32 q = 0; q < sz && !t; q++) t = rep[l[q]] != l[q];
-  if (!t) return;
-  const u32 k = atomicAdd(n, 1u);
-  list[k] = c;
-  pos[k] = st_learnt(cflags(db, c)) ? lrank[i] : nlearnt + (i - lrank[i]);
+  __shared__ u32 lw[NW];
+  const u32 first = (blockIdx.x * blockDim.x + threadIdx.x) * SIPT;
+  u32 bits = 0;
+#pragma unroll
+  for (u32 t = 0; t < (u32)SIPT; t++) {
+    const u32 i = first + t;
+    if (i >= db.ncls) break;
+    const u32 c = db.c_off[i];
+    if (is_deleted(db, c)) continue;
+    const u32* l = clits(db, c);
+    const u32 sz = csize(db, c);
+    bool hit = false;
+    for (u32 q = 0; q < sz && !hit; q++) hit = rep[l[q]] != l[q];
+    if (hit) bits |= 1u << t;
+  }
+  u32 k = block_reserve(bits, lw, n);
+  while (bits) {
+    const u32 t = __ffs(bits) - 1;
+    bits &= bits - 1;
+    const u32 i = first + t, c = db.c_off[i];
+    list[k] = c;
+    pos[k++] = st_learnt(cflags(db, c)) ? lrank[i] : nlearnt + (i - lrank[i]);
+  }
 }
 void k_els_touch(const DB& db, const u32* rep, u32* lrank, void* tmp, u32* nlearnt, u32* list, u32* pos, u32* n,
                  Stream* st) {
@@ -1610,7 +1654,7 @@ void k_els_touch(const DB& db, const u32* rep, u32* lrank, void* tmp, u32* nlear
 }
 void k_els_touch_list(const DB& db, const u32* rep, const u32* lrank, u32 nlearnt, u32* list, u32* pos, u32* n,
                       Stream* st) {
-  if (db.ncls) KLAUNCH(els_touch_k, grid_for(db.ncls, 256), 256, st, db, rep, lrank, nlearnt, list, pos, n);
+  if (db.ncls) KLAUNCH(els_touch_k, grid_for(db.ncls, STILE), ST, st, db, rep, lrank, nlearnt, list, pos, n);
 }
 // one pass of the unit-time fixpoint: the unit each touched clause yields
 // under the current times (ptime: 2 p + sign, the smallest position wins)
