@@ -50,14 +50,16 @@ static inline unsigned grid_for(size_t n, unsigned block) {
 }
 #define LAUNCH_CHECK() PFS_HIP(hipGetLastError())
 // launch on the context stream; with profiling on, bracket it with events
-#define KLAUNCH(kern, grid, block, st, ...)                   \
+#define KLAUNCH(kern, grid, block, st, ...) KLAUNCH_LDS(kern, grid, block, 0, st, __VA_ARGS__)
+// the same with 'lds' bytes of dynamic LDS reserved per workgroup
+#define KLAUNCH_LDS(kern, grid, block, lds, st, ...)          \
   do {                                                         \
     hipEvent_t e0_ = nullptr;                                  \
     if ((st)->prof) {                                          \
       PFS_HIP(hipEventCreate(&e0_));                           \
       PFS_HIP(hipEventRecord(e0_, (st)->s));                   \
     }                                                          \
-    kern<<<(grid), (block), 0, (st)->s>>>(__VA_ARGS__);        \
+    kern<<<(grid), (block), (lds), (st)->s>>>(__VA_ARGS__);    \
     LAUNCH_CHECK();                                            \
     if ((st)->prof) {                                          \
       hipEvent_t e1_;                                          \
@@ -679,6 +681,15 @@ void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk_unused, u32
 
 // ---------------------------------------------------------------- per elected variable
 constexpr int VB = 64;  // one wavefront per workgroup for divergent per-variable work
+// Occupancy of the list-walking per-variable kernels (SUB, VE decisions):
+// each of their one-wavefront workgroups reserves PV_LDS bytes of LDS, so at
+// most 160 KiB / PV_LDS of them share a CU: 20 KiB gives 8 wavefronts per
+// CU, 2 per SIMD.  Fewer waves keep more of each wave's clauses in L1/L2
+// while it re-reads them (Tseitin round on MI355X: 0 / 12 / 20 / 40 KiB ->
+// 18.1 / 17.4 / 17.25 / 17.35 ms).
+#ifndef PV_LDS
+#define PV_LDS 20480
+#endif
 
 __global__ void list_sizes_k(DB db, const u32* pv, u32 n, u32 extra, u32* out) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -698,7 +709,7 @@ __global__ void sub_k(DB db, Options o, const u32* pv, const u32* idx, u32 n, Su
 }
 void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, Stream* st) {
   if (!n) return;
-  KLAUNCH(sub_k, grid_for(n, VB), VB, st, db, o, pv, idx, n, out);
+  KLAUNCH_LDS(sub_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, idx, n, out);
 }
 __global__ void sub_snapshot_k(DB db, const u32* pv, u32 n, SubSnap s) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -781,7 +792,7 @@ __global__ void ve_decide_k(DB db, Options o, const u32* pv, u32 n, VeDecision d
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
                  Stream* st) {
   if (!n) return;
-  KLAUNCH(ve_decide_k, grid_for(n, VB), VB, st, db, o, pv, n, d, order);
+  KLAUNCH_LDS(ve_decide_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, n, d, order);
 }
 __global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a, const u32* order) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
