@@ -130,21 +130,31 @@ void d2h(void* dst, const void* src, size_t bytes, Stream* st) {
 void d2d(void* dst, const void* src, size_t bytes, Stream* st) {
   if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st->s));
 }
+// 16 bytes per thread: one 16-byte store where the address allows it
 __global__ void fill_u32_k(u32* d, u32 v, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = v;
+  const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 4 <= n && ((uintptr_t)(d + i) & 15u) == 0) {
+    *(uint4*)(d + i) = make_uint4(v, v, v, v);
+    return;
+  }
+  for (size_t k = i; k < n && k < i + 4; k++) d[k] = v;
 }
 __global__ void fill_u8_k(u8* d, u8 v, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = v;
+  const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  if (i + 16 <= n && ((uintptr_t)(d + i) & 15u) == 0) {
+    const u32 w = v * 0x01010101u;
+    *(uint4*)(d + i) = make_uint4(w, w, w, w);
+    return;
+  }
+  for (size_t k = i; k < n && k < i + 16; k++) d[k] = v;
 }
 void fill_u32(u32* dst, u32 value, size_t n, Stream* st) {
   if (!n) return;
-  KLAUNCH(fill_u32_k, grid_for(n, 256), 256, st, dst, value, n);
+  KLAUNCH(fill_u32_k, grid_for((n + 3) / 4, 256), 256, st, dst, value, n);
 }
 void fill_u8(u8* dst, u8 value, size_t n, Stream* st) {
   if (!n) return;
-  KLAUNCH(fill_u8_k, grid_for(n, 256), 256, st, dst, value, n);
+  KLAUNCH(fill_u8_k, grid_for((n + 15) / 16, 256), 256, st, dst, value, n);
 }
 Event* event_record(Stream* st) {
   Event* e = new Event();
@@ -636,6 +646,12 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
 }
 // one election step over the undecided candidates idx[0..n) (all when idx
 // is null); the ones still undecided are appended to 'next'
+// LDS reserved per one-wavefront workgroup of the MIS step: 2 wavefronts per
+// SIMD, as PV_LDS (Tseitin round: 0 / 20 / 40 KiB -> mis_step_k 3.34 / 2.76
+// / 3.18 ms)
+#ifndef MIS_LDS
+#define MIS_LDS 20480
+#endif
 __global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
                            u32* next, u32* nnext) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -647,7 +663,7 @@ __global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, const 
 void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
                 u32* next, u32* nnext, Stream* st) {
   if (!n) return;
-  KLAUNCH(mis_step_k, grid_for(n, 64), 64, st, db, cand, idx, n, pos, state, blk, next, nnext);
+  KLAUNCH_LDS(mis_step_k, grid_for(n, 64), 64, MIS_LDS, st, db, cand, idx, n, pos, state, blk, next, nnext);
 }
 __global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
