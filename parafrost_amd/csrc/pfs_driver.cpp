@@ -805,19 +805,29 @@ bool Simplifier::lcve() {
   k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
   fill_u32(P<u32>(state_), 0, ncand, st_);
   fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
-  u32 iters = 0;
-  // worklists of undecided candidates (the selection buffers are free now)
+  // worklists of undecided candidates (the selection buffers are free now);
+  // after the first pass over all candidates, passes are queued four at a
+  // time with the list lengths kept on the device (sc[4], sc[5]; sc[6]
+  // counts the passes that had work)
   u32* wl[2] = {P<u32>(flag_), P<u32>(score_)};
-  const u32* idx = nullptr;
-  u32 nundec = ncand;
-  while (nundec) {
-    u32* next = wl[iters & 1];
-    fill_u32(sc + 4, 0, 1, st_);
-    k_mis_step(db, P<u32>(cand_), idx, nundec, P<u32>(pos_), P<u32>(state_), P<u32>(key_), next, sc + 4, st_);
-    iters++;
-    if (iters > ncand + 2) throw DeviceFail{"LCVE election did not converge"};
-    nundec = read_u32(sc + 4);
-    idx = next;
+  u32* cnt[2] = {sc + 4, sc + 5};
+  u32 iters = 0;
+  if (ncand) {
+    fill_u32(sc + 4, 0, 3, st_);
+    k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), wl[0], cnt[0], st_);
+    u32 cur = 0, bound = read_u32(cnt[0]), launched = 0;
+    while (bound) {
+      for (int b = 0; b < 4; b++, launched++) {
+        const u32 nx = cur ^ 1;
+        fill_u32(cnt[nx], 0, 1, st_);
+        k_mis_step_dev(db, P<u32>(cand_), wl[cur], cnt[cur], bound, P<u32>(pos_), P<u32>(state_), P<u32>(key_),
+                       wl[nx], cnt[nx], sc + 6, st_);
+        cur = nx;
+      }
+      if (launched > ncand + 8) throw DeviceFail{"LCVE election did not converge"};
+      bound = read_u32(cnt[cur]);
+    }
+    iters = 1 + read_u32(sc + 6);
   }
   fill_u32(sc, 0, 1, st_);
   fill_u32(sc + 1, ncand, 1, st_);

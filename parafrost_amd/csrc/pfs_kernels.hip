@@ -665,6 +665,26 @@ void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32*
   if (!n) return;
   KLAUNCH_LDS(mis_step_k, grid_for(n, 64), 64, MIS_LDS, st, db, cand, idx, n, pos, state, blk, next, nnext);
 }
+// A worklist pass whose length is read on the device (n_in): a bounded grid
+// strides over the list, so several passes are queued without a host read;
+// passes over an empty list return at once.  iters counts the non-empty passes.
+__global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32* n_in, const u32* pos, u32* state,
+                               u32* blk, u32* next, u32* nnext, u32* iters) {
+  const u32 n = *n_in;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(iters, 1u);
+  for (u32 base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {  // wave-uniform
+    const u32 t = base + threadIdx.x;
+    const u32 i = t < n ? idx[t] : 0u;
+    const bool undecided = t < n && mis_step(db, i, cand, pos, state, blk) != 0;
+    wave_append(undecided, i, next, nnext);
+  }
+}
+void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, const u32* pos,
+                    u32* state, u32* blk, u32* next, u32* nnext, u32* iters, Stream* st) {
+  if (!n_bound) return;
+  const unsigned grid = std::min(grid_for(n_bound, 64), 2048u);
+  KLAUNCH_LDS(mis_step_dev_k, grid, 64, MIS_LDS, st, db, cand, idx, n_in, pos, state, blk, next, nnext, iters);
+}
 __global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && state[i] == 1 && cflag[cand[i]] == 3) atomicMin(brk, i);

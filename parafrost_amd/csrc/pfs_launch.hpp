@@ -82,6 +82,10 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream*);
 // one election step over the candidates idx[0..n) (all when null); blk: per
 // candidate, an undecided earlier neighbour (NOPOS before the first step);
 // the still undecided candidates are appended to next (count in *nnext)
+// the same over the list idx[0..*n_in) (n_in on the device, at most n_bound);
+// *iters counts the passes over a non-empty list
+void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, const u32* pos,
+                    u32* state, u32* blk, u32* next, u32* nnext, u32* iters, Stream*);
 void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
                 u32* next, u32* nnext, Stream*);
 void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
