@@ -64,7 +64,7 @@ Simplifier::~Simplifier() {
                 &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
                 &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
                 &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_,
-                &map_, &map_assign_, &out_};
+                &map_, &map_assign_, &out_, &vrest_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
@@ -133,6 +133,7 @@ void Simplifier::alloc_state() {
   mem_.ensure(recs_, 3 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(subaff_, 3 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(ids_, (V1 + 1) * 4, 0, st_);
+  mem_.ensure(vrest_, (2 * (V1 + 1) + 32) * 4, 0, st_);
   mem_.ensure(mark_, lcap_, 0, st_);  // per clause handle
   const size_t big = std::max<size_t>({V1 + 2, L2 + 1, ccap_});
   mem_.ensure(tmp_, std::max(scan_tmp_bytes(big), radix_tmp_bytes(V1 + 1)), 0, st_);
@@ -1067,7 +1068,7 @@ bool Simplifier::ve() {
   mem_.ensure(scratch_, std::max<size_t>(S, 1) * 4, 0, st_);
   VeDecision d{P<u32>(type_), P<u32>(def_), P<u32>(vncls_), P<u32>(vnlits_), P<u32>(vnstack_),
                P<u32>(scratch_), P<u32>(vbase_)};
-  k_ve_decide(db, opt, P<u32>(pv_), E_, d, work_order(), st_);
+  k_ve_decide(db, opt, P<u32>(pv_), E_, d, work_order(), P<u32>(ids_), P<u32>(vrest_), st_);
   scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, sc, tmp_.p, st_);
   scan_u32(P<u32>(vnlits_), P<u32>(litb_), E_, sc + 1, tmp_.p, st_);
   scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc + 2, tmp_.p, st_);

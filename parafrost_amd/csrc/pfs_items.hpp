@@ -95,6 +95,73 @@ struct OLV {
   u32 n;
 };
 PF_HD OLV olist(const DB& db, u32 lit) { return OLV{db.ot_refs + db.ot_off[lit], db.ot_size[lit]}; }
+PF_HD u32 cid(const DB&, u32 c) { return c; }  // clause identity (the pool handle)
+
+// A variable's two occurrence lists copied to a private buffer (LDS on the
+// device) for the searches that read them many times.  Layout: the sizes of
+// the positive and the negative list, then each list's record offsets, then
+// the records: pool handle, header word (size and flags), literals.  A
+// clause is named by its record offset; cid() gives its pool handle, which
+// orders clauses as in the pool.  Only the variable's own literals have
+// lists here, and only the molten mark changes (lview_commit returns it).
+struct LView {
+  u32* w;
+  u32 lit;  // the variable's positive literal
+};
+PF_HD OLV olist(const LView& v, u32 lit) {
+  return lit == v.lit ? OLV{v.w + 2, v.w[0]} : OLV{v.w + 2 + v.w[0], v.w[1]};
+}
+PF_HD u32 cid(const LView& v, u32 c) { return v.w[c]; }
+PF_HD u32 csize(const LView& v, u32 c) { return hdr_size(v.w[c + 1]); }
+PF_HD u8 cflags(const LView& v, u32 c) { return hdr_flags(v.w[c + 1]); }
+PF_HD void set_cflags(const LView& v, u32 c, u8 f) { v.w[c + 1] = (v.w[c + 1] & CL_MAX_SIZE) | ((u32)f << 24); }
+PF_HD u32* clits(const LView& v, u32 c) { return v.w + c + 2; }
+PF_HD CL clause(const LView& v, u32 c) { return CL{clits(v, c), csize(v, c)}; }
+PF_HD bool is_original(const LView& v, u32 c) { return st_original(cflags(v, c)); }
+PF_HD bool is_molten(const LView& v, u32 c) { return (cflags(v, c) & F_MOLTEN) != 0; }
+PF_HD void melt(const LView& v, u32 c) { set_cflags(v, c, (u8)(cflags(v, c) | F_MOLTEN)); }
+PF_HD void freeze(const LView& v, u32 c) { set_cflags(v, c, (u8)(cflags(v, c) & ~F_MOLTEN)); }
+// words the view of variable v takes
+PF_HD u32 lview_words(const DB& db, u32 v) {
+  const OLV a = olist(db, V2L(v)), b = olist(db, NEGL(V2L(v)));
+  u32 n = 2 + a.n + b.n;
+  for (u32 i = 0; i < a.n; i++) n += 2 + csize(db, a.r[i]);
+  for (u32 i = 0; i < b.n; i++) n += 2 + csize(db, b.r[i]);
+  return n;
+}
+PF_HD LView lview_stage(const DB& db, u32 v, u32* w) {
+  const u32 p = V2L(v);
+  const OLV a = olist(db, p), b = olist(db, NEGL(p));
+  w[0] = a.n;
+  w[1] = b.n;
+  u32 at = 2 + a.n + b.n;
+  for (u32 s = 0; s < 2; s++) {
+    const OLV ol = s ? b : a;
+    u32* offs = w + 2 + (s ? a.n : 0);
+    for (u32 i = 0; i < ol.n; i++) {
+      const u32 c = ol.r[i];
+      const u32 h = db.lits[c];
+      const u32 n = hdr_size(h);
+      const u32* l = clits(db, c);
+      offs[i] = at;
+      w[at] = c;
+      w[at + 1] = h;
+      for (u32 q = 0; q < n; q++) w[at + 2 + q] = l[q];
+      at += 2 + n;
+    }
+  }
+  return LView{w, p};
+}
+// writes the molten marks the searches changed back to the pool
+PF_HD void lview_commit(const DB& db, const LView& v) {
+  const u32 n = v.w[0] + v.w[1];
+  for (u32 i = 0; i < n; i++) {
+    const u32 r = v.w[2 + i];
+    const u32 c = v.w[r];
+    const u8 lf = hdr_flags(v.w[r + 1]), gf = cflags(db, c);
+    if ((lf ^ gf) & F_MOLTEN) set_cflags(db, c, (u8)((gf & ~F_MOLTEN) | (lf & F_MOLTEN)));
+  }
+}
 
 // ---------------------------------------------------------------- OT construction
 // histogram of live literals (simplify.cpp:45-54 counts, histogram.cpp:82-95)
@@ -735,19 +802,22 @@ struct VeDecision {
   u32* scr_base;  // per variable start in scratch
 };
 
-PF_HD u32 count_orgs(const DB& db, OLV ol) {
+template <class D>
+PF_HD u32 count_orgs(const D& db, OLV ol) {
   u32 n = 0;
   for (u32 i = 0; i < ol.n; i++) n += is_original(db, ol.r[i]);
   return n;
 }
-PF_HD u32 count_lits_before(const DB& db, OLV ol) {
+template <class D>
+PF_HD u32 count_lits_before(const D& db, OLV ol) {
   u32 n = 0;
   for (u32 i = 0; i < ol.n; i++)
     if (is_original(db, ol.r[i])) n += csize(db, ol.r[i]);
   return n;
 }
 // simplify.h:237-295: true when the bound is exceeded; nAddedCls / lits
-PF_HD bool count_res(const DB& db, const Options& o, u32 x, int clsbefore, OLV me, OLV other,
+template <class D>
+PF_HD bool count_res(const D& db, const Options& o, u32 x, int clsbefore, OLV me, OLV other,
                      int& nAddedCls, u32& nAddedLits, bool substitute) {
   nAddedLits = 0;
   const int rlimit = o.ve_clause_limit;
@@ -863,27 +933,32 @@ PF_HD bool find_ao_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, u32
   return false;
 }
 // ifthenelse.h:25-45: first (lowest id) live non-molten original ternary
-// equal to {x, y, z}, searched in the shortest of the three lists; NOPOS
-// if none
-PF_HD u32 fast_equality_check(const DB& db, u32 x, u32 y, u32 z) {
-  if (db.ot_size[y] > db.ot_size[z]) { const u32 t = y; y = z; z = t; }
-  if (db.ot_size[x] > db.ot_size[y]) { const u32 t = x; x = y; y = t; }
+// equal to {x, y, z}; NOPOS if none.  The reference scans the shortest of
+// the three lists; every such clause is in all three and the lowest id is
+// taken, so scanning the list of x (the eliminated variable's literal,
+// whose clauses the gate search reads anyway) gives the same clause.
+template <class D>
+PF_HD u32 fast_equality_check(const D& db, u32 x, u32 y, u32 z) {
   const OLV ol = olist(db, x);
   u32 a = x, b = y, c = z, t;
   if (b > c) { t = b; b = c; c = t; }
   if (a > c) { t = a; a = c; c = t; }
   if (a > b) { t = a; a = b; b = t; }
-  u32 best = NOPOS;
+  u32 best = NOPOS, best_id = NOPOS;
   for (u32 i = 0; i < ol.n; i++) {
     const u32 cl = ol.r[i];
     if (is_molten(db, cl)) continue;
     if (!is_original(db, cl) || csize(db, cl) != 3) continue;
     const u32* l = clits(db, cl);
-    if (l[0] == a && l[1] == b && l[2] == c && cl < best) best = cl;
+    if (l[0] == a && l[1] == b && l[2] == c && cid(db, cl) < best_id) {
+      best = cl;
+      best_id = cid(db, cl);
+    }
   }
   return best;
 }
-PF_HD bool find_ite_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
+template <class D>
+PF_HD bool find_ite_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
                          u32& nAddedLits) {
   const u32 fx = FLIP(dx);
   const OLV it = olist(db, dx);
@@ -920,7 +995,8 @@ PF_HD bool find_ite_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
   return false;
 }
 // xor.h:31-43
-PF_HD void freeze_arities(const DB& db, OLV me, OLV other) {
+template <class D>
+PF_HD void freeze_arities(const D& db, OLV me, OLV other) {
   for (u32 i = 0; i < me.n; i++) {
     const u32 c = me.r[i];
     if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
@@ -931,38 +1007,39 @@ PF_HD void freeze_arities(const DB& db, OLV me, OLV other) {
   }
 }
 // xor.h:70-102: flip the literals for the next parity pattern and look for
-// the matching clause (lowest id) in the shortest list.  The pattern is the
-// base clause with literal k flipped where bit k of 'parity' is set; clauses
-// hold distinct variables in ascending order, so a clause of the same size
-// holds exactly the pattern's literals iff it equals the pattern position by
-// position.
+// the matching clause (lowest id).  The pattern is the base clause with
+// literal k flipped where bit k of 'parity' is set; clauses hold distinct
+// variables in ascending order, so a clause of the same size holds exactly
+// the pattern's literals iff it equals the pattern position by position.
+// The reference scans the shortest of the pattern's lists; the matches are
+// in every one of them and the lowest id is taken, so the list of the
+// pattern's literal of the eliminated variable (at position kx of the base)
+// gives the same clause.
 PF_HD u32 xor_lit(const u32* base, u32 parity, u32 k) { return base[k] ^ ((parity >> k) & 1u); }
-PF_HD bool make_arity(const DB& db, u32& parity, const u32* base, u32 size) {
+template <class D>
+PF_HD bool make_arity(const D& db, u32& parity, const u32* base, u32 size, u32 kx) {
   do { parity++; } while (__builtin_popcount(parity) & 1);
-  u32 best = xor_lit(base, parity, 0);
-  u32 minsize = db.ot_size[best];
-  for (u32 k = 1; k < size; k++) {
-    const u32 lk = xor_lit(base, parity, k);
-    const u32 s = db.ot_size[lk];
-    if (s < minsize) { minsize = s; best = lk; }
-  }
-  const OLV ol = olist(db, best);
-  u32 found = NOPOS;
+  const OLV ol = olist(db, xor_lit(base, parity, kx));
+  u32 found = NOPOS, found_id = NOPOS;
   for (u32 i = 0; i < ol.n; i++) {
     const u32 c = ol.r[i];
-    if (c >= found) continue;
-    const u32* h = chead(db, c);
-    if (!st_original(hdr_flags(h[0])) || hdr_size(h[0]) != size) continue;
-    const u32* l = h + CL_HDR;
+    const u32 id = cid(db, c);
+    if (id >= found_id) continue;
+    if (!is_original(db, c) || csize(db, c) != size) continue;
+    const u32* l = clits(db, c);
     bool all = true;
     for (u32 q = 0; q < size && all; q++) all = (l[q] == xor_lit(base, parity, q));
-    if (all) found = c;
+    if (all) {
+      found = c;
+      found_id = id;
+    }
   }
   if (found == NOPOS) return false;
   melt(db, found);
   return true;
 }
-PF_HD bool find_xor_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
+template <class D>
+PF_HD bool find_xor_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
                          u32& nAddedLits) {
   const u32 fx = FLIP(dx), v = ABS(dx);
   const OLV it = olist(db, dx), ot2 = olist(db, fx);
@@ -973,9 +1050,12 @@ PF_HD bool find_xor_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
     const int arity = (int)size - 1;
     if (size < 3 || arity > o.xor_max_arity || size > 32) continue;
     const u32* base = clits(db, ci);
+    u32 kx = 0;
+    while (kx < size && base[kx] != dx) kx++;
+    if (kx == size) continue;  // not reached: the clause is in the list of dx
     u32 parity = 0;
     int itargets = 1 << arity;
-    while (--itargets && make_arity(db, parity, base, size)) {
+    while (--itargets && make_arity(db, parity, base, size, kx)) {
     }
     if (itargets) freeze_arities(db, it, ot2);
     else {
@@ -992,7 +1072,8 @@ PF_HD bool find_xor_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, in
 }
 // stack words written by toblivion / save_BN_gate (simplify.h:305-332,
 // equivalence.h:133-154): the originals of the side with fewer originals
-PF_HD u32 stack_words(const DB& db, OLV side) {
+template <class D>
+PF_HD u32 stack_words(const D& db, OLV side) {
   u32 w = 2;
   for (u32 i = 0; i < side.n; i++) {
     const u32 c = side.r[i];
@@ -1000,8 +1081,24 @@ PF_HD u32 stack_words(const DB& db, OLV side) {
   }
   return w;
 }
-// bve.cpp:138-241, decision part, for elected variable v at index k
-PF_HD void ve_decide(const DB& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
+// bve.cpp:138-241, decision part, for elected variable v at index k, in
+// two parts: the pure, equivalence and AND/OR checks (ve_decide_gates) for
+// every elected variable, then the ITE and XOR checks and the resolvent
+// count (ve_decide_rest) for the variables the first part leaves open.
+// The first part of a failed substitution count (nAddedCls > 0 disables
+// the negative-side searches and the plain count) is carried in d.ncls[k].
+template <class D>
+PF_HD void ve_write(const D& db, u32 k, const VeDecision& d, OLV side, u32 type, u32 def, int nAddedCls,
+                    u32 nAddedLits) {
+  d.type[k] = type;
+  d.def[k] = def;
+  const bool adds = (type == VT_RES || type == VT_SUBST) && nAddedCls;
+  d.ncls[k] = adds ? (u32)nAddedCls : 0;
+  d.nlits[k] = adds ? nAddedLits + CL_HDR * (u32)nAddedCls : 0;  // pool words
+  d.nstack[k] = type ? stack_words(db, side) : 0;
+}
+// returns false when the variable is left to ve_decide_rest
+PF_HD bool ve_decide_gates(const DB& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
   const u32 p = V2L(v), n = NEGL(p);
   const OLV poss = olist(db, p), negs = olist(db, n);
   const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);
@@ -1021,20 +1118,36 @@ PF_HD void ve_decide(const DB& db, const Options& o, u32 v, u32 k, const VeDecis
       else if (!nAddedCls && find_ao_gate(db, o, p, nOrgCls, out, nAddedCls, nAddedLits)) type = VT_SUBST;
     }
     if (!type && nOrgCls > 3) {
-      if (find_ite_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-      else if (!nAddedCls && find_ite_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-      else if (find_xor_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-      else if (!nAddedCls && find_xor_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+      d.ncls[k] = (u32)nAddedCls;
+      return false;
     }
     if (!type && !nAddedCls && !count_res(db, o, v, nOrgCls, poss, negs, nAddedCls, nAddedLits, false))
       type = VT_RES;
   }
-  d.type[k] = type;
-  d.def[k] = def;
-  const bool adds = (type == VT_RES || type == VT_SUBST) && nAddedCls;
-  d.ncls[k] = adds ? (u32)nAddedCls : 0;
-  d.nlits[k] = adds ? nAddedLits + CL_HDR * (u32)nAddedCls : 0;  // pool words
-  d.nstack[k] = type ? stack_words(db, side) : 0;
+  ve_write(db, k, d, side, type, def, nAddedCls, nAddedLits);
+  return true;
+}
+// D: the pool (DB) or the variable's lists staged in an LView
+template <class D>
+PF_HD void ve_decide_rest(const D& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
+  const u32 p = V2L(v), n = NEGL(p);
+  const OLV poss = olist(db, p), negs = olist(db, n);
+  const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);
+  const OLV side = pOrgs > nOrgs ? negs : poss;
+  const int nOrgCls = (int)(pOrgs + nOrgs);
+  u32 type = VT_NONE;
+  int nAddedCls = (int)d.ncls[k];
+  u32 nAddedLits = 0;
+  if (find_ite_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+  else if (!nAddedCls && find_ite_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+  else if (find_xor_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+  else if (!nAddedCls && find_xor_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
+  if (!type && !nAddedCls && !count_res(db, o, v, nOrgCls, poss, negs, nAddedCls, nAddedLits, false))
+    type = VT_RES;
+  ve_write(db, k, d, side, type, 0, nAddedCls, nAddedLits);
+}
+PF_HD void ve_decide(const DB& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
+  if (!ve_decide_gates(db, o, v, k, d)) ve_decide_rest(db, o, v, k, d);
 }
 
 // unit records appended by VE / BCP: (elected index, sequence, literal)

@@ -108,8 +108,10 @@ void k_set_vis(u32* ust, u32* ulit, const u32* entries, u32 n, Stream*);  // tri
 // indices of the nonzero flags, in order; pos: n words of scratch
 void k_pack_flags(const u32* flags, u32 n, u32* pos, u32* out, u32* count, void* tmp, Stream*);
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
+// open: n words of scratch; work: 2 n + 32 words of scratch (the classes
+// and placement of the variables the gate checks leave open, and counters)
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
-                 Stream*);
+                 u32* open, u32* work, Stream*);
 void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, const u32* order,
                 Stream*);
 void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
