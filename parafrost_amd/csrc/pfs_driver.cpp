@@ -905,7 +905,7 @@ bool Simplifier::sub() {
     fill_u32(sc, 0, 1, st_);
     SubOut out{P<u32>(recs_), sc, cap, P<u32>(subu_), P<u32>(vbase_), proof_out(2 * O + E_ + 64),
                SubVis{vis ? P<u32>(ust_) : nullptr, vis ? P<u32>(ulit_) : nullptr}};
-    k_sub(db, opt, P<u32>(pv_), idx, n, out, st_);
+    k_sub(db, opt, P<u32>(pv_), idx, n, out, P<u32>(vrest_), st_);
     const u32 nrec = read_u32(sc);
     if (nrec > cap) throw DeviceFail{"SUB unit record buffer overflow"};
     std::vector<u32> rec(3 * (size_t)nrec);

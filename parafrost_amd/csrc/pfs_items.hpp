@@ -152,6 +152,104 @@ PF_HD LView lview_stage(const DB& db, u32 v, u32* w) {
   }
   return LView{w, p};
 }
+// The same for SUB, which also changes clauses and compacts the lists:
+// records hold the pool handle, the three header words and the literals;
+// w[0] and w[1] are the current sizes of the positive and the negative
+// list, w[2] and w[3] their first words (offset arrays), w[4] the first
+// record.  sview_commit writes back what changed.
+struct SView {
+  u32* w;
+  u32 lit;
+  const u32* assign;
+};
+PF_HD OLV olist(const SView& v, u32 lit) {
+  return lit == v.lit ? OLV{v.w + v.w[2], v.w[0]} : OLV{v.w + v.w[3], v.w[1]};
+}
+PF_HD u32 cid(const SView& v, u32 c) { return v.w[c]; }
+PF_HD u32 csize(const SView& v, u32 c) { return hdr_size(v.w[c + 1]); }
+PF_HD u8 cflags(const SView& v, u32 c) { return hdr_flags(v.w[c + 1]); }
+PF_HD u32 csig(const SView& v, u32 c) { return v.w[c + 2]; }
+PF_HD u32 clbd(const SView& v, u32 c) { return v.w[c + 3]; }
+PF_HD u32* clits(const SView& v, u32 c) { return v.w + c + 4; }
+PF_HD void set_cflags(const SView& v, u32 c, u8 f) { v.w[c + 1] = (v.w[c + 1] & CL_MAX_SIZE) | ((u32)f << 24); }
+PF_HD void set_csize(const SView& v, u32 c, u32 n) { v.w[c + 1] = (v.w[c + 1] & ~CL_MAX_SIZE) | n; }
+PF_HD void set_csig(const SView& v, u32 c, u32 sig) { v.w[c + 2] = sig; }
+PF_HD void set_clbd(const SView& v, u32 c, u32 lbd) { v.w[c + 3] = lbd; }
+PF_HD CL clause(const SView& v, u32 c) { return CL{clits(v, c), csize(v, c)}; }
+PF_HD bool is_original(const SView& v, u32 c) { return st_original(cflags(v, c)); }
+PF_HD bool is_learnt(const SView& v, u32 c) { return st_learnt(cflags(v, c)); }
+PF_HD bool is_deleted(const SView& v, u32 c) { return st_deleted(cflags(v, c)); }
+PF_HD bool is_molten(const SView& v, u32 c) { return (cflags(v, c) & F_MOLTEN) != 0; }
+PF_HD void set_deleted(const SView& v, u32 c) {
+  const u8 f = cflags(v, c);
+  if (!st_deleted(f)) set_cflags(v, c, (u8)((f & ~ST_MASK) | ST_DELETED));
+}
+PF_HD void set_status(const SView& v, u32 c, u8 st) { set_cflags(v, c, (u8)((cflags(v, c) & ~ST_MASK) | st)); }
+PF_HD void melt(const SView& v, u32 c) { set_cflags(v, c, (u8)(cflags(v, c) | F_MOLTEN)); }
+PF_HD void freeze(const SView& v, u32 c) { set_cflags(v, c, (u8)(cflags(v, c) & ~F_MOLTEN)); }
+PF_HD int lit_value(const SView& v, u32 l) {
+  const u32 a = v.assign[ABS(l)];
+  if (!a) return -1;
+  return (a - 1) == l ? 1 : 0;
+}
+PF_HD u32 sview_words(const DB& db, u32 v) {
+  const OLV a = olist(db, V2L(v)), b = olist(db, NEGL(V2L(v)));
+  u32 n = 4 + a.n + b.n;
+  for (u32 i = 0; i < a.n; i++) n += CL_HDR + 1 + csize(db, a.r[i]);
+  for (u32 i = 0; i < b.n; i++) n += CL_HDR + 1 + csize(db, b.r[i]);
+  return n;
+}
+PF_HD SView sview_stage(const DB& db, u32 v, u32* w) {
+  const u32 p = V2L(v);
+  const OLV a = olist(db, p), b = olist(db, NEGL(p));
+  w[0] = a.n;
+  w[1] = b.n;
+  w[2] = 4;
+  w[3] = 4 + a.n;
+  u32 at = 4 + a.n + b.n;
+  for (u32 s = 0; s < 2; s++) {
+    const OLV ol = s ? b : a;
+    u32* offs = w + w[2 + s];
+    for (u32 i = 0; i < ol.n; i++) {
+      const u32 c = ol.r[i];
+      const u32* h = chead(db, c);
+      const u32 n = hdr_size(h[0]);
+      offs[i] = at;
+      w[at] = c;
+      for (u32 q = 0; q < CL_HDR + n; q++) w[at + 1 + q] = h[q];
+      at += CL_HDR + 1 + n;
+    }
+  }
+  return SView{w, p, db.assign};
+}
+// writes back the changed clauses (walked in staging order: a record spans
+// the clause's size in the pool, which changes only here) and the lists
+// that lost entries
+PF_HD void sview_commit(const DB& db, const SView& v, u32 np0, u32 nn0) {
+  u32 at = 4 + np0 + nn0;
+  for (u32 i = 0; i < np0 + nn0; i++) {
+    const u32 c = v.w[at];
+    u32* h = chead(db, c);
+    const u32 n0 = hdr_size(h[0]);
+    const u32 hl = v.w[at + 1];
+    if (hdr_size(hl) != n0) {
+      const u32 n = hdr_size(hl);
+      for (u32 q = 0; q < n; q++) h[CL_HDR + q] = v.w[at + 4 + q];
+    }
+    if (hl != h[0]) h[0] = hl;
+    if (v.w[at + 2] != h[1]) h[1] = v.w[at + 2];
+    if (v.w[at + 3] != h[2]) h[2] = v.w[at + 3];
+    at += CL_HDR + 1 + n0;
+  }
+  for (u32 s = 0; s < 2; s++) {
+    const u32 lit = v.lit | s, n = v.w[s];
+    if (n == (s ? nn0 : np0)) continue;
+    u32* r = db.ot_refs + db.ot_off[lit];
+    const u32* offs = v.w + v.w[2 + s];
+    for (u32 j = 0; j < n; j++) r[j] = v.w[offs[j]];
+    db.ot_size[lit] = n;
+  }
+}
 // writes the molten marks the searches changed back to the pool
 PF_HD void lview_commit(const DB& db, const LView& v) {
   const u32 n = v.w[0] + v.w[1];
@@ -497,7 +595,8 @@ struct SubLocal {
 };
 // value of a literal seen by one variable: committed root assignment plus
 // the units this variable derived so far
-PF_HD int sub_value(const DB& db, const SubLocal& sl, u32 l) {
+template <class D>
+PF_HD int sub_value(const D& db, const SubLocal& sl, u32 l) {
   const int v = lit_value(db, l);
   if (v >= 0) return v;
   if (sl.vis.ust && sl.vis.ust[ABS(l)] < sl.k) return sl.vis.ulit[ABS(l)] == l ? 1 : 0;
@@ -514,7 +613,8 @@ PF_HD bool has_lit(CL c, u32 l) {
 }
 // deletes the clauses of this variable's lists that contain 'unit'
 // (the part of toblivion(ot[unit]) this variable observes)
-PF_HD void sub_local_oblivion(const DB& db, u32 p, u32 unit) {
+template <class D>
+PF_HD void sub_local_oblivion(const D& db, u32 p, u32 unit) {
   for (u32 s = 0; s < 2; s++) {
     const OLV ol = olist(db, p | s);
     for (u32 k = 0; k < ol.n; k++) {
@@ -523,7 +623,8 @@ PF_HD void sub_local_oblivion(const DB& db, u32 p, u32 unit) {
     }
   }
 }
-PF_HD void bump_shrunken(const DB& db, u32 c, int lbd_tier1) {  // elimination.cpp:28-39
+template <class D>
+PF_HD void bump_shrunken(const D& db, u32 c, int lbd_tier1) {  // elimination.cpp:28-39
   const int old = (int)clbd(db, c);
   if (old <= lbd_tier1) return;
   int nl = (int)csize(db, c) - 1;
@@ -533,7 +634,8 @@ PF_HD void bump_shrunken(const DB& db, u32 c, int lbd_tier1) {  // elimination.c
   set_cflags(db, c, (u8)((cflags(db, c) & ~USAGE_MASK) | (1u << USAGE_SHIFT)));
 }
 // elimination.cpp:199-234
-PF_HD void strengthen(const DB& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_tier1) {
+template <class D>
+PF_HD void strengthen(const D& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_tier1) {
   u32* l = clits(db, c);
   const u32 n0 = csize(db, c);
   u32 n = 0, sig = 0;
@@ -559,7 +661,7 @@ PF_HD void strengthen(const DB& db, u32 c, u32 me, u32 p, SubLocal& sl, int lbd_
       sl.unsat = true;
     }
   } else {
-    push_proof(sl.proof, sl.k, sl.ev++, PR_ADD, c);
+    push_proof(sl.proof, sl.k, sl.ev++, PR_ADD, cid(db, c));
     if (is_learnt(db, c)) bump_shrunken(db, c, lbd_tier1);
   }
 }
@@ -596,7 +698,8 @@ PF_HD bool selfsub_lits(u32 x, u32 fx, CL sm, CL lr) {
   return false;
 }
 // subsume.h:105-121: is 'cand' subsumed by an earlier entry of the list
-PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand, SubLocal& sl) {
+template <class D>
+PF_HD bool subsume_cand(const D& db, OLV ol, u32 end, u32 cand, SubLocal& sl) {
   const u32 csz = csize(db, cand);
   const bool cm = is_molten(db, cand);
   for (u32 j = 0; j < end; j++) {
@@ -606,7 +709,7 @@ PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand, SubLocal& sl) {
     if (cm && ssz > csz) continue;
     if (ssz > 1 && sub_sig(csig(db, s), csig(db, cand)) && sub_lits(clause(db, s), clause(db, cand))) {
       if (is_learnt(db, s) && is_original(db, cand)) set_status(db, s, ST_ORIGINAL);
-      push_proof(sl.proof, sl.k, sl.ev++, PR_DEL, cand);
+      push_proof(sl.proof, sl.k, sl.ev++, PR_DEL, cid(db, cand));
       set_deleted(db, cand);
       return true;
     }
@@ -615,7 +718,8 @@ PF_HD bool subsume_cand(const DB& db, OLV ol, u32 end, u32 cand, SubLocal& sl) {
 }
 // subsume.h:123-142: strengthen 'cand' by removing x if a clause of 'ol'
 // self-subsumes it
-PF_HD bool selfsubsume_cand(const DB& db, u32 x, u32 fx, OLV ol, u32 cand, u32 p, SubLocal& sl,
+template <class D>
+PF_HD bool selfsubsume_cand(const D& db, u32 x, u32 fx, OLV ol, u32 cand, u32 p, SubLocal& sl,
                             int lbd_tier1) {
   const u32 csz = csize(db, cand);
   const u32 cand_sig = csig(db, cand);
@@ -644,6 +748,18 @@ PF_HD void update_ol(const DB& db, u32 lit) {
   }
   db.ot_size[lit] = j;
 }
+PF_HD void update_ol(const SView& v, u32 lit) {
+  const u32 s = lit == v.lit ? 0u : 1u;
+  u32* r = v.w + v.w[2 + s];
+  const u32 n = v.w[s];
+  u32 j = 0;
+  for (u32 i = 0; i < n; i++) {
+    const u32 c = r[i];
+    if (is_molten(v, c)) freeze(v, c);
+    else if (!is_deleted(v, c)) r[j++] = c;
+  }
+  v.w[s] = j;
+}
 // subsume.h:144-179 + the list-size gate of elimination.cpp:121-124, for
 // elected variable v (index k in elected order)
 PF_HD bool sub_visible_unit(const SubVis& vis, u32 k, CL c) {
@@ -653,9 +769,11 @@ PF_HD bool sub_visible_unit(const SubVis& vis, u32 k, CL c) {
   }
   return false;
 }
-PF_HD void sub_var(const DB& db, const Options& o, u32 v, u32 k, const SubOut& out) {
+// D: the pool (DB) or the variable's lists staged in an SView
+template <class D>
+PF_HD void sub_var(const D& db, const Options& o, u32 v, u32 k, const SubOut& out) {
   const u32 p = V2L(v), n = NEGL(p);
-  u32 np = db.ot_size[p], nn = db.ot_size[n];
+  u32 np = olist(db, p).n, nn = olist(db, n).n;
   if (out.vis.ust) {
     // toblivion(ot[unit]) of the units earlier variables derived: their
     // clauses are deleted and the unit's own list emptied before this

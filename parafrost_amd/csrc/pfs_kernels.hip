@@ -737,16 +737,7 @@ void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Strea
 }
 // 'order' (nullable) permutes the elected indices so that the heaviest
 // variables start first and share wavefronts with variables of similar work
-__global__ void sub_k(DB db, Options o, const u32* pv, const u32* idx, u32 n, SubOut out) {
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const u32 k = idx ? idx[t] : t;
-  sub_var(db, o, pv[k], k, out);
-}
-void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, Stream* st) {
-  if (!n) return;
-  KLAUNCH_LDS(sub_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, idx, n, out);
-}
+
 __global__ void sub_snapshot_k(DB db, const u32* pv, u32 n, SubSnap s) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) sub_snapshot_var(db, pv[k], k, s);
@@ -819,23 +810,28 @@ void k_oblivion(const DB& db, const u32* lits, u32 n, Stream* st) {
   if (!n) return;
   KLAUNCH(oblivion_k, n, 256, st, db, lits);
 }
-// VE decisions in two launches: the gate checks most variables end with,
-// then the variables left open (no AND/OR gate, more than three original
-// clauses).  The open ones run the ITE and XOR searches and the full
-// resolvent count, several times the work of the others and re-reading
-// their two lists all along: each copies its lists (lview_words words) into
-// LDS and searches there.  A wavefront's LDS (cap words) is shared by the
-// variables of one size class: class q < VE_CLASSES - 1 gives 64 >> q lanes
-// cap / (64 >> q) words each; the last class (lists larger than cap) reads
-// the pool, 64 lanes to a wavefront.  A wavefront then runs variables of
-// like size, never waiting on one lane that reads the pool.
-constexpr u32 VE_CLASSES = 8;
-__device__ inline u32 ve_class(u32 need, u32 cap) {
+// Per-variable searches over a variable's own two occurrence lists (SUB,
+// the VE decisions that are not settled by the gate checks) re-read those
+// lists all along.  Each variable copies them (lview_words / sview_words
+// words) into LDS and works there.  A wavefront's LDS (cap words) is
+// shared by the variables of one size class: class q < LV_CLASSES - 1
+// gives 64 >> q lanes cap / (64 >> q) words each; the last class (lists
+// larger than cap) works on the pool, 64 lanes to a wavefront.  A
+// wavefront thus runs variables of like size, the largest first, and no
+// lane that reads the pool holds up lanes that work in LDS.
+//
+// work: LV_WORK_HDR counter words (0: variables, 1 + c: variables of class
+// c, 1 + LV_CLASSES + c: placement cursor of class c), then the class of
+// each variable slot (n words), then the variables placed by class (n).
+constexpr u32 LV_CLASSES = 8;
+constexpr u32 LV_WORK_HDR = 32;
+static_assert(1 + 2 * LV_CLASSES <= LV_WORK_HDR, "class counters overlap the class array");
+__device__ inline u32 lv_class(u32 need, u32 cap) {
   u32 q = 0;
-  while (q < VE_CLASSES - 1 && need * (64u >> q) > cap) q++;
+  while (q < LV_CLASSES - 1 && need * (64u >> q) > cap) q++;
   return q;
 }
-__device__ inline u32 ve_class_lanes(u32 q) { return q < VE_CLASSES - 1 ? 64u >> q : 64u; }
+__device__ inline u32 lv_class_lanes(u32 q) { return q < LV_CLASSES - 1 ? 64u >> q : 64u; }
 // wave-aggregated append that returns the slot of each appending lane
 __device__ inline u32 wave_slot(bool pred, u32* cnt) {
   const u64 mask = __ballot(pred);
@@ -847,11 +843,61 @@ __device__ inline u32 wave_slot(bool pred, u32* cnt) {
   base = __shfl(base, leader, 64);
   return base + (u32)__popcll(mask & ((1ull << lane) - 1));
 }
-// work: VE_WORK_HDR counter words (0: open variables, 1 + c: variables of
-// class c, 1 + VE_CLASSES + c: placement cursor of class c), then the class
-// of each open slot (n words), then the placed variables (n words)
-constexpr u32 VE_WORK_HDR = 32;
-static_assert(1 + 2 * VE_CLASSES <= VE_WORK_HDR, "VE counters overlap the class array");
+// records class q of slot 'at' (every lane of the wavefront calls this)
+__device__ inline void lv_count(bool ok, u32 at, u32 q, u32* work) {
+  if (ok) work[LV_WORK_HDR + at] = q;
+  for (u32 c = 0; c < LV_CLASSES; c++) wave_slot(ok && q == c, work + 1 + c);
+}
+// places the variables of the slots by class, the largest lists first;
+// items[i] is the elected index of slot i (null: i itself)
+__global__ void lv_place_k(const u32* items, u32 nmax, u32* work) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 n = work[0];
+  const bool ok = i < n && i < nmax;
+  const u32 q = ok ? work[LV_WORK_HDR + i] : 0u;
+  u32 off = 0;
+  for (u32 c = LV_CLASSES - 1; c > q; c--) off += work[1 + c];
+  u32 pos = 0;
+  for (u32 c = 0; c < LV_CLASSES; c++) {
+    const u32 at = wave_slot(ok && q == c, work + 1 + LV_CLASSES + c);
+    if (ok && q == c) pos = at;
+  }
+  if (ok) work[LV_WORK_HDR + nmax + off + pos] = items ? items[i] : i;
+}
+// one wavefront per workgroup, grid-stride over the chunks of all classes;
+// f(k, w) runs elected index k with LDS slice w (null: on the pool)
+template <class F>
+__device__ inline void lv_chunks(const u32* work, u32 nmax, u32 cap, u32* lds, F f) {
+  const u32 lane = threadIdx.x;
+  const u32* placed = work + LV_WORK_HDR + nmax;
+  u32 chunks = 0;
+  for (u32 c = 0; c < LV_CLASSES; c++) {
+    const u32 L = lv_class_lanes(c);
+    chunks += (work[1 + c] + L - 1) / L;
+  }
+  for (u32 J = blockIdx.x; J < chunks; J += gridDim.x) {
+    u32 c = LV_CLASSES - 1, j = J, off = 0;
+    for (;; c--) {
+      const u32 L = lv_class_lanes(c), m = (work[1 + c] + L - 1) / L;
+      if (j < m || c == 0) break;
+      j -= m;
+      off += work[1 + c];
+    }
+    const u32 L = lv_class_lanes(c), r = j * L + lane;
+    if (lane < L && r < work[1 + c]) f(placed[off + r], c < LV_CLASSES - 1 ? lds + lane * (cap / L) : nullptr);
+    __syncthreads();  // the next chunk reuses the LDS
+  }
+}
+// LDS of each one-wavefront workgroup of the staged launches, and their
+// workgroups: as many as fit the CUs
+#ifndef VR_LDS
+#define VR_LDS PV_LDS
+#endif
+constexpr u32 LV_BLOCKS = 256 * (160 * 1024 / VR_LDS);
+
+// VE decisions in two launches: the gate checks (pure, equivalence,
+// AND/OR) for every elected variable, then, staged, the variables they
+// leave open (ITE and XOR searches, resolvent count)
 __global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order, u32* open,
                            u32* work, u32 cap) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -860,77 +906,66 @@ __global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d,
   if (t < n) {
     k = order ? order[t] : t;
     left = !ve_decide_gates(db, o, pv[k], k, d);
-    if (left) q = ve_class(min(lview_words(db, pv[k]), cap + 1), cap);
+    if (left) q = lv_class(min(lview_words(db, pv[k]), cap + 1), cap);
   }
   const u32 at = wave_slot(left, work);
-  if (left) {
-    open[at] = k;
-    work[VE_WORK_HDR + at] = q;
-  }
-  for (u32 c = 0; c < VE_CLASSES; c++) wave_slot(left && q == c, work + 1 + c);
+  if (left) open[at] = k;
+  lv_count(left, at, q, work);
 }
-// places the open variables by class, the largest lists first
-__global__ void ve_place_k(const u32* open, u32 nmax, u32* work) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  const u32 n = work[0];
-  const bool ok = i < n && i < nmax;
-  const u32 q = ok ? work[VE_WORK_HDR + i] : 0u;
-  u32 off = 0;
-  for (u32 c = VE_CLASSES - 1; c > q; c--) off += work[1 + c];
-  u32 pos = 0;
-  for (u32 c = 0; c < VE_CLASSES; c++) {
-    const u32 p = wave_slot(ok && q == c, work + 1 + VE_CLASSES + c);
-    if (ok && q == c) pos = p;
-  }
-  if (ok) work[VE_WORK_HDR + nmax + off + pos] = open[i];
-}
-// one wavefront per workgroup, grid-stride over the chunks of all classes
 __global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 nmax, u32 cap) {
   extern __shared__ u32 lds[];
-  const u32 lane = threadIdx.x;
-  const u32* placed = work + VE_WORK_HDR + nmax;
-  u32 chunks = 0;
-  for (u32 c = 0; c < VE_CLASSES; c++) {
-    const u32 L = ve_class_lanes(c);
-    chunks += (work[1 + c] + L - 1) / L;
-  }
-  for (u32 J = blockIdx.x; J < chunks; J += gridDim.x) {
-    u32 c = VE_CLASSES - 1, j = J, off = 0;
-    for (;; c--) {
-      const u32 L = ve_class_lanes(c), m = (work[1 + c] + L - 1) / L;
-      if (j < m || c == 0) break;
-      j -= m;
-      off += work[1 + c];
+  lv_chunks(work, nmax, cap, lds, [&](u32 k, u32* w) {
+    const u32 v = pv[k];
+    if (w) {
+      const LView lv = lview_stage(db, v, w);
+      ve_decide_rest(lv, o, v, k, d);
+      lview_commit(db, lv);
+    } else {
+      ve_decide_rest(db, o, v, k, d);
     }
-    const u32 L = ve_class_lanes(c), r = j * L + lane;
-    if (lane < L && r < work[1 + c]) {
-      const u32 k = placed[off + r], v = pv[k];
-      if (c < VE_CLASSES - 1) {
-        const LView lv = lview_stage(db, v, lds + lane * (cap / L));
-        ve_decide_rest(lv, o, v, k, d);
-        lview_commit(db, lv);
-      } else {
-        ve_decide_rest(db, o, v, k, d);
-      }
-    }
-    __syncthreads();  // the next chunk reuses the LDS
-  }
+  });
 }
-// LDS of each one-wavefront workgroup of the second launch, and its
-// workgroups: as many as fit the CUs
-#ifndef VR_LDS
-#define VR_LDS PV_LDS
-#endif
-constexpr u32 VE_REST_BLOCKS = 256 * (160 * 1024 / VR_LDS);
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
                  u32* open, u32* work, Stream* st) {
   if (!n) return;
   const u32 cap = VR_LDS / 4;
-  fill_u32(work, 0, VE_WORK_HDR, st);
+  fill_u32(work, 0, LV_WORK_HDR, st);
   KLAUNCH_LDS(ve_gates_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, n, d, order, open, work, cap);
-  KLAUNCH(ve_place_k, grid_for(n, 256), 256, st, open, n, work);
-  KLAUNCH_LDS(ve_rest_k, std::min<u32>(grid_for(n, VB), VE_REST_BLOCKS), VB, VR_LDS, st, db, o, pv, d, work, n,
-              cap);
+  KLAUNCH(lv_place_k, grid_for(n, 256), 256, st, open, n, work);
+  KLAUNCH_LDS(ve_rest_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, d, work, n, cap);
+}
+// SUB, staged as the VE searches (the class of a variable from the words of
+// its SView)
+__global__ void sub_class_k(DB db, const u32* pv, const u32* idx, u32 n, u32* work, u32 cap) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = t < n;
+  u32 q = 0;
+  if (ok) q = lv_class(min(sview_words(db, pv[idx ? idx[t] : t]), cap + 1), cap);
+  wave_slot(ok, work);
+  lv_count(ok, t, q, work);
+}
+__global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 nmax, u32 cap, SubOut out) {
+  extern __shared__ u32 lds[];
+  lv_chunks(work, nmax, cap, lds, [&](u32 k, u32* w) {
+    const u32 v = pv[k];
+    if (w) {
+      const SView sv = sview_stage(db, v, w);
+      const u32 np0 = sv.w[0], nn0 = sv.w[1];
+      sub_var(sv, o, v, k, out);
+      sview_commit(db, sv, np0, nn0);
+    } else {
+      sub_var(db, o, v, k, out);
+    }
+  });
+}
+void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, u32* work,
+           Stream* st) {
+  if (!n) return;
+  const u32 cap = VR_LDS / 4;
+  fill_u32(work, 0, LV_WORK_HDR, st);
+  KLAUNCH(sub_class_k, grid_for(n, VB), VB, st, db, pv, idx, n, work, cap);
+  KLAUNCH(lv_place_k, grid_for(n, 256), 256, st, idx, n, work);
+  KLAUNCH_LDS(sub_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, work, n, cap, out);
 }
 __global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a, const u32* order) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;

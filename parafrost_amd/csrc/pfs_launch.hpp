@@ -95,7 +95,9 @@ void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_
 // ---- per elected variable stages
 void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Stream*);  // |poss|+|negs|+extra
 // SUB over the elected indices idx[0..n) (all of 0..n when idx is null)
-void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, Stream*);
+// work: 2 n + 32 words of scratch (size classes and placement)
+void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, u32* work,
+           Stream*);
 void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
 void k_sub_restore(const DB& db, const u32* pv, const u32* idx, u32 n, const SubSnap& s, Stream*);
 // aff[o] = 1 for every owner o of a clause holding a changed unit (changes:
