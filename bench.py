@@ -305,6 +305,8 @@ def bench_handoff(s, cnf):
     import numpy as np
     s.rewind()
     status = s.simplify()
+    if status == 0:  # refuted: nothing to hand off (pfs_map refuses a refuted formula)
+        return {"skipped": "the formula is refuted"}
     t0 = time.perf_counter()
     res = s.result(status)
     dl_ms = (time.perf_counter() - t0) * 1e3

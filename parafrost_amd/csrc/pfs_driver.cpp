@@ -64,7 +64,7 @@ Simplifier::~Simplifier() {
                 &recs_, &ids_, &mark_, &ht_, &erecs_, &order_, &stack_, &tmp_, &tmp2_, &tmp3_, &scal_, &in_c_off_,
                 &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
                 &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_,
-                &map_, &map_assign_, &out_, &vrest_};
+                &map_, &map_assign_, &out_, &vrest_, &lsum_};
   if (st_) stream_sync(st_);
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
@@ -134,6 +134,7 @@ void Simplifier::alloc_state() {
   mem_.ensure(subaff_, 3 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(ids_, (V1 + 1) * 4, 0, st_);
   mem_.ensure(vrest_, (2 * (V1 + 1) + 32) * 4, 0, st_);
+  mem_.ensure(lsum_, 2 * (V1 + 1) * 4, 0, st_);
   mem_.ensure(mark_, lcap_, 0, st_);  // per clause handle
   const size_t big = std::max<size_t>({V1 + 2, L2 + 1, ccap_});
   mem_.ensure(tmp_, std::max(scan_tmp_bytes(big), radix_tmp_bytes(V1 + 1)), 0, st_);
@@ -849,7 +850,7 @@ void Simplifier::sort_ot() {
   k_pv_lits(P<u32>(pv_), E_, P<u32>(elits_), st_);
   mem_.ensure(scratch_, std::max<size_t>(lits_end_, 6 * (size_t)E_ + 3) * 4, 0, st_);
   mem_.ensure(tmp2_, (6 * (size_t)E_ + 3) * 4, 0, st_);
-  k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), st_);
+  k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), P<u32>(lsum_), st_);
   if (opt.work_order) k_work_order(db, P<u32>(pv_), E_, P<u32>(tmp2_), P<u32>(order_), tmp_.p, st_);
   if (profile) {
     uint64_t w[2];
@@ -905,7 +906,7 @@ bool Simplifier::sub() {
     fill_u32(sc, 0, 1, st_);
     SubOut out{P<u32>(recs_), sc, cap, P<u32>(subu_), P<u32>(vbase_), proof_out(2 * O + E_ + 64),
                SubVis{vis ? P<u32>(ust_) : nullptr, vis ? P<u32>(ulit_) : nullptr}};
-    k_sub(db, opt, P<u32>(pv_), idx, n, out, P<u32>(vrest_), st_);
+    k_sub(db, opt, P<u32>(pv_), idx, n, out, P<u32>(vrest_), P<u32>(lsum_), tmp_.p, st_);
     const u32 nrec = read_u32(sc);
     if (nrec > cap) throw DeviceFail{"SUB unit record buffer overflow"};
     std::vector<u32> rec(3 * (size_t)nrec);
@@ -1068,7 +1069,7 @@ bool Simplifier::ve() {
   mem_.ensure(scratch_, std::max<size_t>(S, 1) * 4, 0, st_);
   VeDecision d{P<u32>(type_), P<u32>(def_), P<u32>(vncls_), P<u32>(vnlits_), P<u32>(vnstack_),
                P<u32>(scratch_), P<u32>(vbase_)};
-  k_ve_decide(db, opt, P<u32>(pv_), E_, d, work_order(), P<u32>(ids_), P<u32>(vrest_), st_);
+  k_ve_decide(db, opt, P<u32>(pv_), E_, d, work_order(), P<u32>(vrest_), P<u32>(lsum_), tmp_.p, st_);
   scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, sc, tmp_.p, st_);
   scan_u32(P<u32>(vnlits_), P<u32>(litb_), E_, sc + 1, tmp_.p, st_);
   scan_u32(P<u32>(vnstack_), P<u32>(stkb_), E_, sc + 2, tmp_.p, st_);

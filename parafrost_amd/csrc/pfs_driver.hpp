@@ -183,7 +183,8 @@ class Simplifier {
   Buf vsz_, vbase_, type_, def_, vncls_, vnlits_, vnstack_, clsb_, litb_, stkb_;
   Buf snap_ref_, snap_flg_, snap_size_, snap_sig_, snap_lbd_, snap_np_, snap_nn_, scratch_;
   Buf elits_, unit_by_, recs_, ids_, mark_, ht_, erecs_, order_, subu_;
-  Buf vrest_;  // VE: classes and placement of the variables the gate checks leave open
+  Buf vrest_;  // SUB, VE: size classes and placement of the variables
+  Buf lsum_;   // per elected literal (2k, 2k + 1): clause size sum of its list at the list sort
   Buf subaff_, sublast_, ust_, ulit_;  // SUB re-runs: affected flags, last pass, visible stage units
   // proof
   Buf proof_, precs_, pstamp_, plines_, pbytes_;

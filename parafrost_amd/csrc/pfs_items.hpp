@@ -121,7 +121,9 @@ PF_HD bool is_original(const LView& v, u32 c) { return st_original(cflags(v, c))
 PF_HD bool is_molten(const LView& v, u32 c) { return (cflags(v, c) & F_MOLTEN) != 0; }
 PF_HD void melt(const LView& v, u32 c) { set_cflags(v, c, (u8)(cflags(v, c) | F_MOLTEN)); }
 PF_HD void freeze(const LView& v, u32 c) { set_cflags(v, c, (u8)(cflags(v, c) & ~F_MOLTEN)); }
-// words the view of variable v takes
+// words the view of variable v takes; lview_bound / sview_bound bound them
+// from the list lengths and a bound on the sum of the clause sizes of the
+// two lists
 PF_HD u32 lview_words(const DB& db, u32 v) {
   const OLV a = olist(db, V2L(v)), b = olist(db, NEGL(V2L(v)));
   u32 n = 2 + a.n + b.n;
@@ -129,12 +131,16 @@ PF_HD u32 lview_words(const DB& db, u32 v) {
   for (u32 i = 0; i < b.n; i++) n += 2 + csize(db, b.r[i]);
   return n;
 }
-PF_HD LView lview_stage(const DB& db, u32 v, u32* w) {
+PF_HD u64 lview_bound(u32 nlists, u64 sizes) { return 2 + 3 * (u64)nlists + sizes; }
+// copies the lists of v into w[0, cap); false (w partly written) when they
+// do not fit
+PF_HD bool lview_stage(const DB& db, u32 v, u32* w, u32 cap, LView& lv) {
   const u32 p = V2L(v);
   const OLV a = olist(db, p), b = olist(db, NEGL(p));
+  u32 at = 2 + a.n + b.n;
+  if ((u64)at > cap) return false;
   w[0] = a.n;
   w[1] = b.n;
-  u32 at = 2 + a.n + b.n;
   for (u32 s = 0; s < 2; s++) {
     const OLV ol = s ? b : a;
     u32* offs = w + 2 + (s ? a.n : 0);
@@ -142,6 +148,7 @@ PF_HD LView lview_stage(const DB& db, u32 v, u32* w) {
       const u32 c = ol.r[i];
       const u32 h = db.lits[c];
       const u32 n = hdr_size(h);
+      if ((u64)at + 2 + n > cap) return false;
       const u32* l = clits(db, c);
       offs[i] = at;
       w[at] = c;
@@ -150,7 +157,8 @@ PF_HD LView lview_stage(const DB& db, u32 v, u32* w) {
       at += 2 + n;
     }
   }
-  return LView{w, p};
+  lv = LView{w, p};
+  return true;
 }
 // The same for SUB, which also changes clauses and compacts the lists:
 // records hold the pool handle, the three header words and the literals;
@@ -199,14 +207,16 @@ PF_HD u32 sview_words(const DB& db, u32 v) {
   for (u32 i = 0; i < b.n; i++) n += CL_HDR + 1 + csize(db, b.r[i]);
   return n;
 }
-PF_HD SView sview_stage(const DB& db, u32 v, u32* w) {
+PF_HD u64 sview_bound(u32 nlists, u64 sizes) { return 4 + (CL_HDR + 2) * (u64)nlists + sizes; }
+PF_HD bool sview_stage(const DB& db, u32 v, u32* w, u32 cap, SView& sv) {
   const u32 p = V2L(v);
   const OLV a = olist(db, p), b = olist(db, NEGL(p));
+  u32 at = 4 + a.n + b.n;
+  if ((u64)at > cap) return false;
   w[0] = a.n;
   w[1] = b.n;
   w[2] = 4;
   w[3] = 4 + a.n;
-  u32 at = 4 + a.n + b.n;
   for (u32 s = 0; s < 2; s++) {
     const OLV ol = s ? b : a;
     u32* offs = w + w[2 + s];
@@ -214,13 +224,15 @@ PF_HD SView sview_stage(const DB& db, u32 v, u32* w) {
       const u32 c = ol.r[i];
       const u32* h = chead(db, c);
       const u32 n = hdr_size(h[0]);
+      if ((u64)at + CL_HDR + 1 + n > cap) return false;
       offs[i] = at;
       w[at] = c;
       for (u32 q = 0; q < CL_HDR + n; q++) w[at + 1 + q] = h[q];
       at += CL_HDR + 1 + n;
     }
   }
-  return SView{w, p, db.assign};
+  sv = SView{w, p, db.assign};
+  return true;
 }
 // writes back the changed clauses (walked in staging order: a record spans
 // the clause's size in the pool, which changes only here) and the lists

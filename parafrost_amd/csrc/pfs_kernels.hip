@@ -412,31 +412,40 @@ __device__ inline void wave_append(bool pred, u32 val, u32* arr, u32* cnt) {
   base = __shfl(base, leader, 64);
   if (pred) arr[base + __popcll(mask & ((1ull << lane) - 1))] = val;
 }
-// collects the lists longer than SORT_SMALL (few: elected variables have
-// at least one short list) for the workgroup and long-list sorts
+// collects the positions of the lists longer than SORT_SMALL (few: elected
+// variables have at least one short list) for the workgroup and long-list
+// sorts
 __global__ void sort_classify_k(DB db, const u32* lits, u32 n, u32* big, u32* cnt) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 lit = i < n ? lits[i] : 0;
   const u32 len = i < n ? db.ot_size[lit] : 0;
-  wave_append(len > SORT_SMALL, lit, big, cnt);
+  wave_append(len > SORT_SMALL, i, big, cnt);
 }
-// 16 lanes per list: bitonic network over wave64 shuffles, keys in registers
-__global__ void sort_small_k(DB db, const u32* lits, u32 n) {
+// 16 lanes per list: bitonic network over wave64 shuffles, keys in registers.
+// The sorts also leave in lsum[g] the sum of the clause sizes of list g (a
+// bound for later stages: the lists of elected variables only shrink).
+__global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum) {
   const u32 g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const u32 j = threadIdx.x & 15;
   const u32 lit = g < n ? lits[g] : 0;
   const u32 len0 = g < n ? db.ot_size[lit] : 0;
-  const bool valid = len0 >= 2 && len0 <= SORT_SMALL;  // longer lists go to sort_block_k
-  const u32 len = valid ? len0 : 0;
-  u32* r = db.ot_refs + (valid ? db.ot_off[lit] : 0);
+  const u32 small = len0 <= SORT_SMALL ? len0 : 0;  // longer lists go to sort_block_k
+  const u32 len = small >= 2 ? small : 0;
+  u32* r = db.ot_refs + (small ? db.ot_off[lit] : 0);
   u64 a = ~0ull, b = ~0ull;
-  u32 id = 0xFFFFFFFFu;
-  if (j < len) {
+  u32 id = 0xFFFFFFFFu, sz = 0;
+  if (j < small) {
     const SortKey k = make_key(db, r[j]);
-    a = ((u64)k.size << 32) | k.lit0;
-    b = ((u64)k.back << 32) | k.sig;
-    id = k.id;
+    sz = k.size;
+    if (len) {
+      a = ((u64)k.size << 32) | k.lit0;
+      b = ((u64)k.back << 32) | k.sig;
+      id = k.id;
+    }
   }
+#pragma unroll
+  for (u32 d = 8; d > 0; d >>= 1) sz += __shfl_xor(sz, d, 64);
+  if (j == 0 && g < n && len0 <= SORT_SMALL) lsum[g] = sz;
 #pragma unroll
   for (u32 k = 2; k <= 16; k <<= 1) {
 #pragma unroll
@@ -455,30 +464,36 @@ __global__ void sort_small_k(DB db, const u32* lits, u32 n) {
   }
   if (j < len) r[j] = id;
 }
-// bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK)
-__global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, u32 n) {
+// bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK);
+// big: positions in lits
+__global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, const u32* big, u32* lsum) {
   __shared__ u64 ka[SORT_BLOCK];
   __shared__ u64 kb[SORT_BLOCK];
   __shared__ u32 kid[SORT_BLOCK];
-  const u32 lit = lits[blockIdx.x];
+  __shared__ u32 lw[NW];
+  const u32 g = big[blockIdx.x], lit = lits[g];
   const u32 len = db.ot_size[lit];
   if (len <= SORT_SMALL || len > SORT_BLOCK) return;
   u32* r = db.ot_refs + db.ot_off[lit];
   u32 N = 1;
   while (N < len) N <<= 1;
+  u32 sz = 0;
   for (u32 i = threadIdx.x; i < N; i += blockDim.x) {
     if (i < len) {
       const SortKey k = make_key(db, r[i]);
       ka[i] = ((u64)k.size << 32) | k.lit0;
       kb[i] = ((u64)k.back << 32) | k.sig;
       kid[i] = k.id;
+      sz += k.size;
     } else {
       ka[i] = ~0ull;
       kb[i] = ~0ull;
       kid[i] = 0xFFFFFFFFu;
     }
   }
-  __syncthreads();
+  u32 tot;
+  block_excl_scan(sz, lw, &tot);
+  if (threadIdx.x == 0) lsum[g] = tot;
   for (u32 k = 2; k <= N; k <<= 1) {
     for (u32 j = k >> 1; j > 0; j >>= 1) {
       for (u32 i = threadIdx.x; i < N; i += blockDim.x) {
@@ -502,13 +517,16 @@ __global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, u32 
 }
 // bottom-up merge sort of one long list by one thread (rare lists above
 // SORT_BLOCK entries); 'scratch' has one slot per occurrence entry
-__global__ void sort_large_k(DB db, const u32* lits, u32 n, u32* scratch) {
+__global__ void sort_large_k(DB db, const u32* lits, const u32* big, u32 n, u32* scratch, u32* lsum) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const u32 lit = lits[i];
+  const u32 g = big[i], lit = lits[g];
   const u32 len = db.ot_size[lit];
   if (len <= SORT_BLOCK) return;
   u32* a = db.ot_refs + db.ot_off[lit];
+  u64 sz = 0;
+  for (u32 k = 0; k < len; k++) sz += csize(db, a[k]);
+  lsum[g] = sz > 0xFFFFFFFFull ? 0xFFFFFFFFu : (u32)sz;
   u32* b = scratch + db.ot_off[lit];
   u32 *src = a, *dst = b;
   for (u32 w = 1; w < len; w <<= 1) {
@@ -528,19 +546,20 @@ __global__ void sort_large_k(DB db, const u32* lits, u32 n, u32* scratch) {
   if (src != a)
     for (u32 k = 0; k < len; k++) a[k] = src[k];
 }
-void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, Stream* st) {
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum,
+                  Stream* st) {
   if (!n) return;
   u32* big = classes;
   fill_u32(cnt, 0, 1, st);
   // short lists (the bulk) are sorted in place by 16-lane groups, longer
   // ones are collected and sorted per workgroup (or by the long-list merge)
-  KLAUNCH(sort_small_k, grid_for((size_t)n * 16, 256), 256, st, db, lits, n);
+  KLAUNCH(sort_small_k, grid_for((size_t)n * 16, 256), 256, st, db, lits, n, lsum);
   KLAUNCH(sort_classify_k, grid_for(n, 256), 256, st, db, lits, n, big, cnt);
   u32 nb = 0;
   PFS_HIP(hipMemcpyAsync(&nb, cnt, 4, hipMemcpyDeviceToHost, st->s));
   PFS_HIP(hipStreamSynchronize(st->s));
-  if (nb) KLAUNCH(sort_block_k, nb, 256, st, db, big, nb);
-  if (nb) KLAUNCH(sort_large_k, grid_for(nb, 64), 64, st, db, big, nb, scratch);
+  if (nb) KLAUNCH(sort_block_k, nb, 256, st, db, lits, big, lsum);
+  if (nb) KLAUNCH(sort_large_k, grid_for(nb, 64), 64, st, db, lits, big, nb, scratch, lsum);
 }
 
 // ---------------------------------------------------------------- BCP
@@ -820,71 +839,64 @@ void k_oblivion(const DB& db, const u32* lits, u32 n, Stream* st) {
 // wavefront thus runs variables of like size, the largest first, and no
 // lane that reads the pool holds up lanes that work in LDS.
 //
-// work: LV_WORK_HDR counter words (0: variables, 1 + c: variables of class
-// c, 1 + LV_CLASSES + c: placement cursor of class c), then the class of
-// each variable slot (n words), then the variables placed by class (n).
+// work: LV_WORK_HDR words (the first positions of the sort keys 0..8 in
+// the sorted order), then n sort keys, then n elected indices.  The key
+// of a variable of class c is LV_CLASSES - 1 - c (the largest lists
+// first); VE gives the variables its gate checks settle key LV_CLASSES.
+// A stable radix sort orders the pairs, lv_bounds_k finds where each key
+// starts.  No counter is shared by the variables.
 constexpr u32 LV_CLASSES = 8;
 constexpr u32 LV_WORK_HDR = 32;
-static_assert(1 + 2 * LV_CLASSES <= LV_WORK_HDR, "class counters overlap the class array");
+constexpr int LV_KEY_BITS = 4;
+static_assert(LV_CLASSES + 2 <= LV_WORK_HDR && LV_CLASSES < (1u << LV_KEY_BITS), "class keys");
 __device__ inline u32 lv_class(u32 need, u32 cap) {
   u32 q = 0;
   while (q < LV_CLASSES - 1 && need * (64u >> q) > cap) q++;
   return q;
 }
 __device__ inline u32 lv_class_lanes(u32 q) { return q < LV_CLASSES - 1 ? 64u >> q : 64u; }
-// wave-aggregated append that returns the slot of each appending lane
-__device__ inline u32 wave_slot(bool pred, u32* cnt) {
-  const u64 mask = __ballot(pred);
-  if (!mask) return 0;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  u32 base = 0;
-  if (lane == leader) base = atomicAdd(cnt, (u32)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  return base + (u32)__popcll(mask & ((1ull << lane) - 1));
-}
-// records class q of slot 'at' (every lane of the wavefront calls this)
-__device__ inline void lv_count(bool ok, u32 at, u32 q, u32* work) {
-  if (ok) work[LV_WORK_HDR + at] = q;
-  for (u32 c = 0; c < LV_CLASSES; c++) wave_slot(ok && q == c, work + 1 + c);
-}
-// places the variables of the slots by class, the largest lists first;
-// items[i] is the elected index of slot i (null: i itself)
-__global__ void lv_place_k(const u32* items, u32 nmax, u32* work) {
+__global__ void lv_bounds_k(const u32* keys, u32 n, u32* starts) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  const u32 n = work[0];
-  const bool ok = i < n && i < nmax;
-  const u32 q = ok ? work[LV_WORK_HDR + i] : 0u;
-  u32 off = 0;
-  for (u32 c = LV_CLASSES - 1; c > q; c--) off += work[1 + c];
-  u32 pos = 0;
-  for (u32 c = 0; c < LV_CLASSES; c++) {
-    const u32 at = wave_slot(ok && q == c, work + 1 + LV_CLASSES + c);
-    if (ok && q == c) pos = at;
-  }
-  if (ok) work[LV_WORK_HDR + nmax + off + pos] = items ? items[i] : i;
+  if (i < n && (i == 0 || keys[i] != keys[i - 1])) starts[keys[i]] = i;
+}
+// sorts the (key, elected index) pairs and finds the key starts
+void lv_sort(u32* work, u32 n, void* tmp, Stream* st) {
+  radix_sort_pairs(work + LV_WORK_HDR, work + LV_WORK_HDR + n, n, LV_KEY_BITS, tmp, st);
+  KLAUNCH(lv_bounds_k, grid_for(n, 256), 256, st, work + LV_WORK_HDR, n, work);
 }
 // one wavefront per workgroup, grid-stride over the chunks of all classes;
-// f(k, w) runs elected index k with LDS slice w (null: on the pool)
+// f(k, w, slice) runs elected index k with the LDS words [w, w + slice)
+// (w null: on the pool).  A class is chosen from a bound on the words the
+// variable needs; the staging checks the slice and falls back to the pool.
 template <class F>
-__device__ inline void lv_chunks(const u32* work, u32 nmax, u32 cap, u32* lds, F f) {
+__device__ inline void lv_chunks(const u32* work, u32 n, u32 cap, u32* lds, F f) {
   const u32 lane = threadIdx.x;
-  const u32* placed = work + LV_WORK_HDR + nmax;
+  const u32* placed = work + LV_WORK_HDR + n;
+  // key starts (absent keys start where the next key starts)
+  u32 start[LV_CLASSES + 1];
+  u32 next = n;
+#pragma unroll
+  for (int key = LV_CLASSES; key >= 0; key--) {
+    const u32 s0 = work[key];
+    start[key] = s0 < n ? s0 : next;
+    next = start[key];
+  }
   u32 chunks = 0;
-  for (u32 c = 0; c < LV_CLASSES; c++) {
-    const u32 L = lv_class_lanes(c);
-    chunks += (work[1 + c] + L - 1) / L;
+#pragma unroll
+  for (u32 key = 0; key < LV_CLASSES; key++) {
+    const u32 L = lv_class_lanes(LV_CLASSES - 1 - key);
+    chunks += (start[key + 1] - start[key] + L - 1) / L;
   }
   for (u32 J = blockIdx.x; J < chunks; J += gridDim.x) {
-    u32 c = LV_CLASSES - 1, j = J, off = 0;
-    for (;; c--) {
-      const u32 L = lv_class_lanes(c), m = (work[1 + c] + L - 1) / L;
-      if (j < m || c == 0) break;
+    u32 key = 0, j = J;
+    for (; key < LV_CLASSES - 1; key++) {
+      const u32 L = lv_class_lanes(LV_CLASSES - 1 - key), m = (start[key + 1] - start[key] + L - 1) / L;
+      if (j < m) break;
       j -= m;
-      off += work[1 + c];
     }
-    const u32 L = lv_class_lanes(c), r = j * L + lane;
-    if (lane < L && r < work[1 + c]) f(placed[off + r], c < LV_CLASSES - 1 ? lds + lane * (cap / L) : nullptr);
+    const u32 c = LV_CLASSES - 1 - key, L = lv_class_lanes(c), r = start[key] + j * L + lane;
+    if (lane < L && r < start[key + 1])
+      f(placed[r], c < LV_CLASSES - 1 ? lds + lane * (cap / L) : nullptr, cap / L);
     __syncthreads();  // the next chunk reuses the LDS
   }
 }
@@ -898,26 +910,28 @@ constexpr u32 LV_BLOCKS = 256 * (160 * 1024 / VR_LDS);
 // VE decisions in two launches: the gate checks (pure, equivalence,
 // AND/OR) for every elected variable, then, staged, the variables they
 // leave open (ITE and XOR searches, resolvent count)
-__global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order, u32* open,
-                           u32* work, u32 cap) {
+// lsum[2k], lsum[2k + 1]: bounds on the clause size sums of the lists of
+// elected variable k (from the list sort)
+__global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order, u32* work,
+                           const u32* lsum, u32 cap) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  bool left = false;
-  u32 k = 0, q = 0;
-  if (t < n) {
-    k = order ? order[t] : t;
-    left = !ve_decide_gates(db, o, pv[k], k, d);
-    if (left) q = lv_class(min(lview_words(db, pv[k]), cap + 1), cap);
+  if (t >= n) return;
+  const u32 k = order ? order[t] : t;
+  u32 key = LV_CLASSES;  // settled by the gate checks
+  if (!ve_decide_gates(db, o, pv[k], k, d)) {
+    const u32 p = V2L(pv[k]);
+    const u64 b = lview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
+    key = LV_CLASSES - 1 - lv_class((u32)min(b, (u64)cap + 1), cap);
   }
-  const u32 at = wave_slot(left, work);
-  if (left) open[at] = k;
-  lv_count(left, at, q, work);
+  work[LV_WORK_HDR + t] = key;
+  work[LV_WORK_HDR + n + t] = k;
 }
-__global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 nmax, u32 cap) {
+__global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 n, u32 cap) {
   extern __shared__ u32 lds[];
-  lv_chunks(work, nmax, cap, lds, [&](u32 k, u32* w) {
+  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
     const u32 v = pv[k];
-    if (w) {
-      const LView lv = lview_stage(db, v, w);
+    LView lv;
+    if (w && lview_stage(db, v, w, slice, lv)) {
       ve_decide_rest(lv, o, v, k, d);
       lview_commit(db, lv);
     } else {
@@ -926,30 +940,30 @@ __global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u
   });
 }
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
-                 u32* open, u32* work, Stream* st) {
+                 u32* work, const u32* lsum, void* tmp, Stream* st) {
   if (!n) return;
   const u32 cap = VR_LDS / 4;
-  fill_u32(work, 0, LV_WORK_HDR, st);
-  KLAUNCH_LDS(ve_gates_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, n, d, order, open, work, cap);
-  KLAUNCH(lv_place_k, grid_for(n, 256), 256, st, open, n, work);
+  fill_u32(work, n, LV_WORK_HDR, st);
+  KLAUNCH_LDS(ve_gates_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, n, d, order, work, lsum, cap);
+  lv_sort(work, n, tmp, st);
   KLAUNCH_LDS(ve_rest_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, d, work, n, cap);
 }
 // SUB, staged as the VE searches (the class of a variable from the words of
 // its SView)
-__global__ void sub_class_k(DB db, const u32* pv, const u32* idx, u32 n, u32* work, u32 cap) {
+__global__ void sub_class_k(DB db, const u32* pv, const u32* idx, u32 n, u32* work, const u32* lsum, u32 cap) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = t < n;
-  u32 q = 0;
-  if (ok) q = lv_class(min(sview_words(db, pv[idx ? idx[t] : t]), cap + 1), cap);
-  wave_slot(ok, work);
-  lv_count(ok, t, q, work);
+  if (t >= n) return;
+  const u32 k = idx ? idx[t] : t, p = V2L(pv[k]);
+  const u64 b = sview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
+  work[LV_WORK_HDR + t] = LV_CLASSES - 1 - lv_class((u32)min(b, (u64)cap + 1), cap);
+  work[LV_WORK_HDR + n + t] = k;
 }
-__global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 nmax, u32 cap, SubOut out) {
+__global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 n, u32 cap, SubOut out) {
   extern __shared__ u32 lds[];
-  lv_chunks(work, nmax, cap, lds, [&](u32 k, u32* w) {
+  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
     const u32 v = pv[k];
-    if (w) {
-      const SView sv = sview_stage(db, v, w);
+    SView sv;
+    if (w && sview_stage(db, v, w, slice, sv)) {
       const u32 np0 = sv.w[0], nn0 = sv.w[1];
       sub_var(sv, o, v, k, out);
       sview_commit(db, sv, np0, nn0);
@@ -959,12 +973,12 @@ __global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 nmax
   });
 }
 void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, u32* work,
-           Stream* st) {
+           const u32* lsum, void* tmp, Stream* st) {
   if (!n) return;
   const u32 cap = VR_LDS / 4;
-  fill_u32(work, 0, LV_WORK_HDR, st);
-  KLAUNCH(sub_class_k, grid_for(n, VB), VB, st, db, pv, idx, n, work, cap);
-  KLAUNCH(lv_place_k, grid_for(n, 256), 256, st, idx, n, work);
+  fill_u32(work, n, LV_WORK_HDR, st);
+  KLAUNCH(sub_class_k, grid_for(n, 256), 256, st, db, pv, idx, n, work, lsum, cap);
+  lv_sort(work, n, tmp, st);
   KLAUNCH_LDS(sub_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, work, n, cap, out);
 }
 __global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a, const u32* order) {

@@ -56,7 +56,8 @@ void k_hist(const DB& db, u32* hist, Stream*);
 void k_scatter(const DB& db, u32* cursor, Stream*);
 // sort the lists of the given literals (lits[0..n)) by the total order
 // classes: 3n words of scratch for the size classes; cnt: 3 device words
-void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, Stream*);
+// lsum[i]: sum of the clause sizes of list lits[i] (n words)
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum, Stream*);
 
 // ---- BCP
 void k_bcp_visit(const DB& db, const u32* front, u32 nfront, u32* next, u32* nnext, u32* conflict,
@@ -95,9 +96,11 @@ void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_
 // ---- per elected variable stages
 void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Stream*);  // |poss|+|negs|+extra
 // SUB over the elected indices idx[0..n) (all of 0..n when idx is null)
-// work: 2 n + 32 words of scratch (size classes and placement)
+// work: 2 n + 32 words of scratch (size classes and placement); lsum: the
+// list size sums of k_sort_lists over the elected literals (2k, 2k + 1);
+// tmp: radix_tmp_bytes(n)
 void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, u32* work,
-           Stream*);
+           const u32* lsum, void* tmp, Stream*);
 void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
 void k_sub_restore(const DB& db, const u32* pv, const u32* idx, u32 n, const SubSnap& s, Stream*);
 // aff[o] = 1 for every owner o of a clause holding a changed unit (changes:
@@ -110,10 +113,10 @@ void k_set_vis(u32* ust, u32* ulit, const u32* entries, u32 n, Stream*);  // tri
 // indices of the nonzero flags, in order; pos: n words of scratch
 void k_pack_flags(const u32* flags, u32 n, u32* pos, u32* out, u32* count, void* tmp, Stream*);
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
-// open: n words of scratch; work: 2 n + 32 words of scratch (the classes
-// and placement of the variables the gate checks leave open, and counters)
+// work, lsum, tmp: as for k_sub (the variables the gate checks leave open
+// are staged by size class)
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
-                 u32* open, u32* work, Stream*);
+                 u32* work, const u32* lsum, void* tmp, Stream*);
 void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, const u32* order,
                 Stream*);
 void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
