@@ -803,6 +803,8 @@ bool Simplifier::lcve() {
     while (bits < 32 && (hs >> bits)) bits++;
   }
   radix_sort_pairs(P<u32>(key_), P<u32>(cand_), ncand, bits, tmp_.p, st_);
+  // positions share a word with the candidate's state (position << 2)
+  if (ncand >= (1u << 30)) throw DeviceFail{"LCVE: too many candidates for the packed positions"};
   fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
   k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
   fill_u32(P<u32>(state_), 0, ncand, st_);

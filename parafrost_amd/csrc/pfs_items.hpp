@@ -390,18 +390,25 @@ PF_HD void bcp_clean(const DB& db, u32 c) {
 // while it stays undecided the candidate stays blocked without a rescan.
 // The elected set does not depend on the schedule (the lexicographically
 // first independent set is unique), only the number of iterations does.
-PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* state, u32* blk) {
+// pw[u] packs the candidate position of variable u and its state
+// (position << 2 | state; NOPOS for variables that are not candidates), so
+// a neighbour costs one load; state[] holds the same states by position.
+PF_HD void mis_decide(u32* state, u32* pw, u32 i, u32 v, u32 s) {
+  state[i] = s;
+  pw[v] = (i << 2) | s;
+}
+PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, u32* pw, u32* state, u32* blk) {
   if (at_load(&state[i])) return 0;
   const u32 b = blk[i];
+  const u32 v = cand[i];
   if (b != NOPOS) {
     const u32 sb = at_load(&state[b]);
     if (sb == 0) return 1;
     if (sb == 1) {
-      state[i] = 2;
+      mis_decide(state, pw, i, v, 2);
       return 0;
     }
   }
-  const u32 v = cand[i];
   u32 blocker = NOPOS;
   for (u32 s = 0; s < 2; s++) {
     const OLV ol = olist(db, V2L(v) | s);
@@ -412,11 +419,12 @@ PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* st
       for (u32 q = 0; q < cl.n; q++) {
         const u32 u = ABS(cl.l[q]);
         if (u == v) continue;
-        const u32 pu = pos[u];
+        const u32 w = at_load(&pw[u]);
+        const u32 pu = w >> 2;
         if (pu >= i) continue;
-        const u32 su = at_load(&state[pu]);
+        const u32 su = w & 3u;
         if (su == 1) {
-          state[i] = 2;
+          mis_decide(state, pw, i, v, 2);
           return 0;
         }
         if (su == 0 && blocker == NOPOS) blocker = pu;
@@ -427,7 +435,7 @@ PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, const u32* pos, u32* st
     blk[i] = blocker;
     return 1;
   }
-  state[i] = 1;
+  mis_decide(state, pw, i, v, 1);
   return 0;
 }
 

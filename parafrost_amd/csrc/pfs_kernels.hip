@@ -655,9 +655,10 @@ void k_lcve_select(const DB& db, const LcveArgs& a, u32* flag_tmp, u32* cand, u3
   scan_u32(flag_tmp, flag_tmp, n, ncand, tmp, st);
   KLAUNCH(lcve_pack_k, grid_for(n, 256), 256, st, db, a, flag_tmp, flag_tmp, cand, key);
 }
+// packed position and state (undecided) of each candidate, for mis_step
 __global__ void lcve_pos_k(const u32* cand, u32 n, u32* pos) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) pos[cand[i]] = i;
+  if (i < n) pos[cand[i]] = i << 2;
 }
 void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
   if (!n) return;
@@ -671,7 +672,7 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
 #ifndef MIS_LDS
 #define MIS_LDS 20480
 #endif
-__global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
+__global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
                            u32* next, u32* nnext) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -679,7 +680,7 @@ __global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, const 
   const bool undecided = mis_step(db, i, cand, pos, state, blk) != 0;
   wave_append(undecided, i, next, nnext);
 }
-void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
+void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
                 u32* next, u32* nnext, Stream* st) {
   if (!n) return;
   KLAUNCH_LDS(mis_step_k, grid_for(n, 64), 64, MIS_LDS, st, db, cand, idx, n, pos, state, blk, next, nnext);
@@ -687,7 +688,7 @@ void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32*
 // A worklist pass whose length is read on the device (n_in): a bounded grid
 // strides over the list, so several passes are queued without a host read;
 // passes over an empty list return at once.  iters counts the non-empty passes.
-__global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32* n_in, const u32* pos, u32* state,
+__global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32* n_in, u32* pos, u32* state,
                                u32* blk, u32* next, u32* nnext, u32* iters) {
   const u32 n = *n_in;
   if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(iters, 1u);
@@ -698,7 +699,7 @@ __global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32
     wave_append(undecided, i, next, nnext);
   }
 }
-void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, const u32* pos,
+void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
                     u32* state, u32* blk, u32* next, u32* nnext, u32* iters, Stream* st) {
   if (!n_bound) return;
   const unsigned grid = std::min(grid_for(n_bound, 64), 2048u);

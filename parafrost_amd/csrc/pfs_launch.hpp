@@ -85,9 +85,9 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream*);
 // the still undecided candidates are appended to next (count in *nnext)
 // the same over the list idx[0..*n_in) (n_in on the device, at most n_bound);
 // *iters counts the passes over a non-empty list
-void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, const u32* pos,
+void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
                     u32* state, u32* blk, u32* next, u32* nnext, u32* iters, Stream*);
-void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, const u32* pos, u32* state, u32* blk,
+void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
                 u32* next, u32* nnext, Stream*);
 void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
 void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_tmp, u32* out,
