@@ -737,12 +737,14 @@ void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk_unused, u32
 
 // ---------------------------------------------------------------- per elected variable
 constexpr int VB = 64;  // one wavefront per workgroup for divergent per-variable work
-// Occupancy of the list-walking per-variable kernels (SUB, VE decisions):
-// each of their one-wavefront workgroups reserves PV_LDS bytes of LDS, so at
-// most 160 KiB / PV_LDS of them share a CU: 20 KiB gives 8 wavefronts per
-// CU, 2 per SIMD.  Fewer waves keep more of each wave's clauses in L1/L2
-// while it re-reads them (Tseitin round on MI355X: 0 / 12 / 20 / 40 KiB ->
-// 18.1 / 17.4 / 17.25 / 17.35 ms).
+// Occupancy of the per-variable kernels that walk the pool (the VE gate
+// checks): each of their one-wavefront workgroups reserves PV_LDS bytes of
+// LDS, so at most 160 KiB / PV_LDS of them share a CU: 20 KiB gives 8
+// wavefronts per CU, 2 per SIMD.  Fewer waves keep more of each wave's
+// clauses in L1/L2 while it re-reads them (Tseitin round on MI355X, when
+// SUB and all VE decisions walked the pool: 0 / 12 / 20 / 40 KiB -> 18.1 /
+// 17.4 / 17.25 / 17.35 ms).  The staged kernels use the same size as their
+// working LDS (VR_LDS).
 #ifndef PV_LDS
 #define PV_LDS 20480
 #endif
