@@ -895,7 +895,8 @@ bool Simplifier::sub() {
   fill_u32(P<u32>(unit_by_), INF, lits_end_, st_);  // clause owners, per handle
   SubSnap snap{P<u32>(vbase_),    P<u32>(snap_np_),  P<u32>(snap_nn_),  P<u32>(snap_ref_), P<u8>(snap_flg_),
                P<u32>(snap_size_), P<u32>(snap_sig_), P<u32>(snap_lbd_), P<u32>(unit_by_)};
-  k_sub_snapshot(db, P<u32>(pv_), E_, snap, st_);
+  // the snapshot for the re-runs is taken by the first pass (every
+  // variable, just before its run)
 
   // per variable (elected index): its latest units (sequence, literal;
   // literal 0 = refutation) and proof events
@@ -908,7 +909,7 @@ bool Simplifier::sub() {
     fill_u32(sc, 0, 1, st_);
     SubOut out{P<u32>(recs_), sc, cap, P<u32>(subu_), P<u32>(vbase_), proof_out(2 * O + E_ + 64),
                SubVis{vis ? P<u32>(ust_) : nullptr, vis ? P<u32>(ulit_) : nullptr}};
-    k_sub(db, opt, P<u32>(pv_), idx, n, out, P<u32>(vrest_), P<u32>(lsum_), tmp_.p, st_);
+    k_sub(db, opt, P<u32>(pv_), idx, n, out, P<u32>(vrest_), P<u32>(lsum_), tmp_.p, vis ? nullptr : &snap, st_);
     const u32 nrec = read_u32(sc);
     if (nrec > cap) throw DeviceFail{"SUB unit record buffer overflow"};
     std::vector<u32> rec(3 * (size_t)nrec);

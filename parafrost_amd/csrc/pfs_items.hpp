@@ -873,7 +873,9 @@ struct SubSnap {
   u32* lbd;
   u32* owner;  // per clause: elected index of the variable whose lists hold it live
 };
-PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
+// D: the pool, or the variable's staged SView before its SUB run changes it
+template <class D>
+PF_HD void sub_snapshot_var(const D& db, u32 v, u32 k, const SubSnap& s) {
   u32 e = s.base[k];
   for (u32 q = 0; q < 2; q++) {
     const OLV ol = olist(db, V2L(v) | q);
@@ -881,8 +883,8 @@ PF_HD void sub_snapshot_var(const DB& db, u32 v, u32 k, const SubSnap& s) {
     else s.nn[k] = ol.n;
     for (u32 j = 0; j < ol.n; j++, e++) {
       const u32 c = ol.r[j];
-      if (!is_deleted(db, c)) s.owner[c] = k;  // live clauses have one elected owner
-      s.ref[e] = c;
+      if (!is_deleted(db, c)) s.owner[cid(db, c)] = k;  // live clauses have one elected owner
+      s.ref[e] = cid(db, c);
       s.flg[e] = cflags(db, c);
       s.size[e] = csize(db, c);
       s.sig[e] = csig(db, c);

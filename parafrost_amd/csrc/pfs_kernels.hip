@@ -961,28 +961,35 @@ __global__ void sub_class_k(DB db, const u32* pv, const u32* idx, u32 n, u32* wo
   work[LV_WORK_HDR + t] = LV_CLASSES - 1 - lv_class((u32)min(b, (u64)cap + 1), cap);
   work[LV_WORK_HDR + n + t] = k;
 }
-__global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 n, u32 cap, SubOut out) {
+// take_snap: record each variable's state for the SUB re-runs just before
+// its run (a run changes only the variable's own lists and clauses, so this
+// equals a snapshot before the launch); the staged copy provides it
+__global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 n, u32 cap, SubOut out, SubSnap snap,
+                      u32 take_snap) {
   extern __shared__ u32 lds[];
   lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
     const u32 v = pv[k];
     SView sv;
     if (w && sview_stage(db, v, w, slice, sv)) {
       const u32 np0 = sv.w[0], nn0 = sv.w[1];
+      if (take_snap) sub_snapshot_var(sv, v, k, snap);
       sub_var(sv, o, v, k, out);
       sview_commit(db, sv, np0, nn0);
     } else {
+      if (take_snap) sub_snapshot_var(db, v, k, snap);
       sub_var(db, o, v, k, out);
     }
   });
 }
 void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, u32* work,
-           const u32* lsum, void* tmp, Stream* st) {
+           const u32* lsum, void* tmp, const SubSnap* snap, Stream* st) {
   if (!n) return;
   const u32 cap = VR_LDS / 4;
   fill_u32(work, n, LV_WORK_HDR, st);
   KLAUNCH(sub_class_k, grid_for(n, 256), 256, st, db, pv, idx, n, work, lsum, cap);
   lv_sort(work, n, tmp, st);
-  KLAUNCH_LDS(sub_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, work, n, cap, out);
+  KLAUNCH_LDS(sub_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, work, n, cap, out,
+              snap ? *snap : SubSnap{}, snap ? 1u : 0u);
 }
 __global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a, const u32* order) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -98,9 +98,10 @@ void k_list_sizes(const DB& db, const u32* pv, u32 n, u32 extra, u32* out, Strea
 // SUB over the elected indices idx[0..n) (all of 0..n when idx is null)
 // work: 2 n + 32 words of scratch (size classes and placement); lsum: the
 // list size sums of k_sort_lists over the elected literals (2k, 2k + 1);
-// tmp: radix_tmp_bytes(n)
+// tmp: radix_tmp_bytes(n); snap (nullable): take the re-run snapshot of
+// every variable run (as k_sub_snapshot would before the launch)
 void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n, const SubOut& out, u32* work,
-           const u32* lsum, void* tmp, Stream*);
+           const u32* lsum, void* tmp, const SubSnap* snap, Stream*);
 void k_sub_snapshot(const DB& db, const u32* pv, u32 n, const SubSnap& s, Stream*);
 void k_sub_restore(const DB& db, const u32* pv, const u32* idx, u32 n, const SubSnap& s, Stream*);
 // aff[o] = 1 for every owner o of a clause holding a changed unit (changes:
