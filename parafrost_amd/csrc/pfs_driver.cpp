@@ -1098,7 +1098,7 @@ bool Simplifier::ve() {
   ap.error = sc + 6;
   ap.proof = proof_out(tot[0] + S + V_ + 64);
   fill_u32(sc + 6, 0, 1, st_);
-  k_ve_apply(db, P<u32>(pv_), E_, d, ap, work_order(), st_);
+  k_ve_apply(db, P<u32>(pv_), E_, d, ap, P<u32>(vrest_), P<u32>(lsum_), tmp_.p, st_);
   if (read_u32(sc + 6)) throw DeviceFail{"BVE resolvent writes disagree with the counted space"};
   ncls_ += tot[0];
   lits_end_ += tot[1];

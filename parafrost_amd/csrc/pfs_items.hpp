@@ -1307,8 +1307,11 @@ PF_HD void push_unit(const UnitOut& u, u32 k, u32 seq, u32 lit) {
 
 // model.cpp:261-305
 PF_HD u32 org_lit(const DB& db, u32 l) { return V2L(db.vorg[ABS(l)]) | SIGN(l); }
-PF_HD u32 save_clause(const DB& db, u32 c, u32 witlit, u32* st) {
-  const CL cl = clause(db, c);
+// D: where the clause is read (the pool or a staged LView); the original
+// variable numbers come from the pool's vorg
+template <class D>
+PF_HD u32 save_clause(const D& view, const DB& db, u32 c, u32 witlit, u32* st) {
+  const CL cl = clause(view, c);
   u32 pos = 0;
   for (u32 i = 0; i < cl.n; i++) {
     st[i] = org_lit(db, cl.l[i]);
@@ -1324,15 +1327,16 @@ PF_HD u32 save_witness(const DB& db, u32 w, u32* st) {
   return 2;
 }
 // writes the eliminated-stack records of toblivion / save_BN_gate
-PF_HD void save_side(const DB& db, u32 p, u32 pOrgs, u32 nOrgs, u32* st) {
+template <class D>
+PF_HD void save_side(const D& view, const DB& db, u32 p, u32 pOrgs, u32 nOrgs, u32* st) {
   const u32 n = NEGL(p);
   const bool negside = pOrgs > nOrgs;
-  const OLV side = olist(db, negside ? n : p);
+  const OLV side = olist(view, negside ? n : p);
   const u32 wit = negside ? n : p;
   u32 w = 0;
   for (u32 i = 0; i < side.n; i++) {
     const u32 c = side.r[i];
-    if (is_original(db, c)) w += save_clause(db, c, wit, st + w);
+    if (is_original(view, c)) w += save_clause(view, db, c, wit, st + w);
   }
   save_witness(db, negside ? p : n, st + w);
 }
@@ -1377,12 +1381,16 @@ struct VeApply {
   ProofOut proof;
 };
 // bve.cpp:138-241, execution part
-PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApply& a) {
+// L: where the variable's lists and clauses are read: the pool (DB), or a
+// staged LView for the resolvent path (which changes none of them until it
+// deletes the lists; the EQU path rewrites its clauses and reads the pool)
+template <class L>
+PF_HD void ve_apply(const L& lv, const DB& db, u32 v, u32 k, const VeDecision& d, const VeApply& a) {
   const u32 type = d.type[k];
   if (!type) return;
   const u32 p = V2L(v), n = NEGL(p);
-  const OLV poss = olist(db, p), negs = olist(db, n);
-  const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);
+  const OLV poss = olist(lv, p), negs = olist(lv, n);
+  const u32 pOrgs = count_orgs(lv, poss), nOrgs = count_orgs(lv, negs);
   const u32 sbase = a.stack0 + a.stack_base[k];
   if (sbase + d.nstack[k] > a.stack_cap) {
     at_add(a.error, 1u);
@@ -1391,7 +1399,7 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
   u32* st = a.stack + sbase;
   u32 seq = 0;
   if (type == VT_EQU) {
-    save_side(db, p, pOrgs, nOrgs, st);
+    save_side(db, db, p, pOrgs, nOrgs, st);
     // equivalence.h:45-87
     const u32 def = d.def[k], def_f = FLIP(def);
     for (u32 q = 0; q < 2; q++) {
@@ -1416,19 +1424,19 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
     if ((type == VT_RES || type == VT_SUBST) && d.ncls[k]) {
       const bool substitute = type == VT_SUBST;
       u32 dx = p, fx = n;
-      if (db.ot_size[dx] > db.ot_size[fx]) { dx = n; fx = p; }
-      const OLV me = olist(db, dx), other = olist(db, fx);
+      if (olist(lv, dx).n > olist(lv, fx).n) { dx = n; fx = p; }
+      const OLV me = olist(lv, dx), other = olist(lv, fx);
       u32 cid = a.cls0 + a.cls_base[k], lpos = a.lit0 + a.lit_base[k];
       for (u32 i = 0; i < me.n; i++) {
         const u32 ci = me.r[i];
-        if (!is_original(db, ci)) continue;
-        const bool ma = is_molten(db, ci);
-        const CL A = clause(db, ci);
+        if (!is_original(lv, ci)) continue;
+        const bool ma = is_molten(lv, ci);
+        const CL A = clause(lv, ci);
         for (u32 j = 0; j < other.n; j++) {
           const u32 cj = other.r[j];
-          if (!is_original(db, cj)) continue;
-          if (substitute && ma == is_molten(db, cj)) continue;
-          const CL B = clause(db, cj);
+          if (!is_original(lv, cj)) continue;
+          if (substitute && ma == is_molten(lv, cj)) continue;
+          const CL B = clause(lv, cj);
           const u32 len = merge_len(v, A, B);
           if (!len) {
             // tautology, or a unit resolvent of two literals (len counts
@@ -1457,7 +1465,7 @@ PF_HD void ve_apply(const DB& db, u32 v, u32 k, const VeDecision& d, const VeApp
         }
       }
     }
-    save_side(db, p, pOrgs, nOrgs, st);
+    save_side(lv, db, p, pOrgs, nOrgs, st);
     delete_lists(db, p);
   }
   db.vstate[v] = VS_MELTED;
@@ -1482,7 +1490,7 @@ PF_HD u32 bce_var(const DB& db, const Options& o, u32 v, bool count, u32* st) {
     if (all) {
       if (count) w += A.n + 1;
       else {
-        w += save_clause(db, ng, n, st + w);
+        w += save_clause(db, db, ng, n, st + w);
         set_deleted(db, ng);
       }
     }

@@ -991,16 +991,42 @@ void k_sub(const DB& db, const Options& o, const u32* pv, const u32* idx, u32 n,
   KLAUNCH_LDS(sub_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, work, n, cap, out,
               snap ? *snap : SubSnap{}, snap ? 1u : 0u);
 }
-__global__ void ve_apply_k(DB db, const u32* pv, u32 n, VeDecision d, VeApply a, const u32* order) {
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const u32 k = order ? order[t] : t;
-  ve_apply(db, pv[k], k, d, a);
+// VE execution, staged as the searches: the resolvent path reads the
+// variable's lists from LDS; EQU substitutions (which rewrite their clauses
+// in place) take the last class and work on the pool; variables VE keeps
+// are not placed
+__global__ void ve_apply_class_k(DB db, const u32* pv, u32 n, const u32* type, u32* work, const u32* lsum,
+                                 u32 cap) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  u32 key = LV_CLASSES;
+  if (type[k] == VT_EQU) {
+    key = 0;
+  } else if (type[k]) {
+    const u32 p = V2L(pv[k]);
+    const u64 b = lview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
+    key = LV_CLASSES - 1 - lv_class((u32)min(b, (u64)cap + 1), cap);
+  }
+  work[LV_WORK_HDR + k] = key;
+  work[LV_WORK_HDR + n + k] = k;
 }
-void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, const u32* order,
-                Stream* st) {
+__global__ void ve_apply_k(DB db, const u32* pv, VeDecision d, VeApply a, const u32* work, u32 n, u32 cap) {
+  extern __shared__ u32 lds[];
+  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
+    const u32 v = pv[k];
+    LView lv;
+    if (w && d.type[k] != VT_EQU && lview_stage(db, v, w, slice, lv)) ve_apply(lv, db, v, k, d, a);
+    else ve_apply(db, db, v, k, d, a);
+  });
+}
+void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, u32* work,
+                const u32* lsum, void* tmp, Stream* st) {
   if (!n) return;
-  KLAUNCH(ve_apply_k, grid_for(n, VB), VB, st, db, pv, n, d, a, order);
+  const u32 cap = VR_LDS / 4;
+  fill_u32(work, n, LV_WORK_HDR, st);
+  KLAUNCH(ve_apply_class_k, grid_for(n, 256), 256, st, db, pv, n, d.type, work, lsum, cap);
+  lv_sort(work, n, tmp, st);
+  KLAUNCH_LDS(ve_apply_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, pv, d, a, work, n, cap);
 }
 __global__ void bce_k(DB db, Options o, const u32* pv, const u32* type, u32 n, u32* words,
                       const u32* base, u32* stack, bool count, const u32* order) {

@@ -118,7 +118,9 @@ void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all c
 // are staged by size class)
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
                  u32* work, const u32* lsum, void* tmp, Stream*);
-void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, const u32* order,
+// work, lsum, tmp: as for k_sub
+void k_ve_apply(const DB& db, const u32* pv, u32 n, const VeDecision& d, const VeApply& a, u32* work,
+                const u32* lsum, void* tmp,
                 Stream*);
 void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n, u32* words,
            const u32* base, u32* stack, bool count, const u32* order, Stream*);
