@@ -810,7 +810,7 @@ bool Simplifier::lcve() {
   fill_u32(P<u32>(state_), 0, ncand, st_);
   fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
   // worklists of undecided candidates (the selection buffers are free now);
-  // after the first pass over all candidates, passes are queued four at a
+  // after the first pass over all candidates, passes are queued eight at a
   // time with the list lengths kept on the device (sc[4], sc[5]; sc[6]
   // counts the passes that had work)
   u32* wl[2] = {P<u32>(flag_), P<u32>(score_)};
@@ -821,7 +821,7 @@ bool Simplifier::lcve() {
     k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), wl[0], cnt[0], st_);
     u32 cur = 0, bound = read_u32(cnt[0]), launched = 0;
     while (bound) {
-      for (int b = 0; b < 4; b++, launched++) {
+      for (int b = 0; b < 8; b++, launched++) {
         const u32 nx = cur ^ 1;
         fill_u32(cnt[nx], 0, 1, st_);
         k_mis_step_dev(db, P<u32>(cand_), wl[cur], cnt[cur], bound, P<u32>(pos_), P<u32>(state_), P<u32>(key_),
@@ -1142,11 +1142,8 @@ bool Simplifier::ve() {
   proof_events(pev);
   proof_flush();
   // eliminated variables (markEliminated, solver.h:535-541)
-  std::vector<u32> types(E_);
-  d2h(types.data(), type_.p, (size_t)E_ * 4, st_);
-  stream_sync(st_);
-  u32 elim = 0;
-  for (u32 t : types) elim += t ? 1 : 0;
+  k_count_nonzero(P<u32>(type_), E_, sc + 7, st_);
+  const u32 elim = read_u32(sc + 7);
   unassigned_ -= elim;
   rounds_.back().eliminated = elim;
   rounds_.back().resolvents = tot[0];

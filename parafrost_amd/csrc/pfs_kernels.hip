@@ -466,12 +466,8 @@ __global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum) {
 }
 // bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK);
 // big: positions in lits
-__global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, const u32* big, u32* lsum) {
-  __shared__ u64 ka[SORT_BLOCK];
-  __shared__ u64 kb[SORT_BLOCK];
-  __shared__ u32 kid[SORT_BLOCK];
-  __shared__ u32 lw[NW];
-  const u32 g = big[blockIdx.x], lit = lits[g];
+__device__ void sort_block_list(DB db, const u32* lits, u32 g, u32* lsum, u64* ka, u64* kb, u32* kid, u32* lw) {
+  const u32 lit = lits[g];
   const u32 len = db.ot_size[lit];
   if (len <= SORT_SMALL || len > SORT_BLOCK) return;
   u32* r = db.ot_refs + db.ot_off[lit];
@@ -515,12 +511,24 @@ __global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, cons
   }
   for (u32 i = threadIdx.x; i < len; i += blockDim.x) r[i] = kid[i];
 }
+// big: positions in lits, *nbig of them (read on the device: a bounded grid
+// strides over them, so the launch needs no host read of the count)
+__global__ void __launch_bounds__(256) sort_block_k(DB db, const u32* lits, const u32* big, const u32* nbig,
+                                                    u32* lsum) {
+  __shared__ u64 ka[SORT_BLOCK];
+  __shared__ u64 kb[SORT_BLOCK];
+  __shared__ u32 kid[SORT_BLOCK];
+  __shared__ u32 lw[NW];
+  const u32 nb = *nbig;
+  for (u32 b = blockIdx.x; b < nb; b += gridDim.x) {
+    sort_block_list(db, lits, big[b], lsum, ka, kb, kid, lw);
+    __syncthreads();
+  }
+}
 // bottom-up merge sort of one long list by one thread (rare lists above
 // SORT_BLOCK entries); 'scratch' has one slot per occurrence entry
-__global__ void sort_large_k(DB db, const u32* lits, const u32* big, u32 n, u32* scratch, u32* lsum) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u32 g = big[i], lit = lits[g];
+__device__ void sort_large_list(DB db, const u32* lits, u32 g, u32* scratch, u32* lsum) {
+  const u32 lit = lits[g];
   const u32 len = db.ot_size[lit];
   if (len <= SORT_BLOCK) return;
   u32* a = db.ot_refs + db.ot_off[lit];
@@ -546,6 +554,12 @@ __global__ void sort_large_k(DB db, const u32* lits, const u32* big, u32 n, u32*
   if (src != a)
     for (u32 k = 0; k < len; k++) a[k] = src[k];
 }
+// grid-stride over the *nbig collected lists (count read on the device)
+__global__ void sort_large_k(DB db, const u32* lits, const u32* big, const u32* nbig, u32* scratch, u32* lsum) {
+  const u32 nb = *nbig;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x)
+    sort_large_list(db, lits, big[i], scratch, lsum);
+}
 void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum,
                   Stream* st) {
   if (!n) return;
@@ -555,11 +569,9 @@ void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, 
   // ones are collected and sorted per workgroup (or by the long-list merge)
   KLAUNCH(sort_small_k, grid_for((size_t)n * 16, 256), 256, st, db, lits, n, lsum);
   KLAUNCH(sort_classify_k, grid_for(n, 256), 256, st, db, lits, n, big, cnt);
-  u32 nb = 0;
-  PFS_HIP(hipMemcpyAsync(&nb, cnt, 4, hipMemcpyDeviceToHost, st->s));
-  PFS_HIP(hipStreamSynchronize(st->s));
-  if (nb) KLAUNCH(sort_block_k, nb, 256, st, db, lits, big, lsum);
-  if (nb) KLAUNCH(sort_large_k, grid_for(nb, 64), 64, st, db, lits, big, nb, scratch, lsum);
+  // the long lists are few; both launches read their count on the device
+  KLAUNCH(sort_block_k, std::min(grid_for(n, 1), 1024u), 256, st, db, lits, big, cnt, lsum);
+  KLAUNCH(sort_large_k, std::min(grid_for(n, 64), 64u), 64, st, db, lits, big, cnt, scratch, lsum);
 }
 
 // ---------------------------------------------------------------- BCP
@@ -1843,6 +1855,16 @@ __global__ void els_rep_k(DB db, const u32* root, u32* rep, u32* info) {
     subs += rep[l] != l;
   }
   block_count_add(subs, lw, &info[1]);
+}
+__global__ void count_nonzero_k(const u32* a, u32 n, u32* out) {
+  __shared__ u32 lw[NW];
+  u32 mine = 0;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) mine += a[i] != 0;
+  block_count_add(mine, lw, out);
+}
+void k_count_nonzero(const u32* a, u32 n, u32* out, Stream* st) {
+  fill_u32(out, 0, 1, st);
+  if (n) KLAUNCH(count_nonzero_k, std::min(grid_for(n, ST), RED_BLOCKS), ST, st, a, n, out);
 }
 void k_els_rep(const DB& db, const u32* root, u32* rep, u32* info, Stream* st) {
   KLAUNCH(els_rep_k, std::min(grid_for(2 * (size_t)db.nvars + 2, ST), RED_BLOCKS), ST, st, db, root, rep, info);

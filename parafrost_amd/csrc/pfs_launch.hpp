@@ -114,6 +114,7 @@ void k_set_vis(u32* ust, u32* ulit, const u32* entries, u32 n, Stream*);  // tri
 // indices of the nonzero flags, in order; pos: n words of scratch
 void k_pack_flags(const u32* flags, u32 n, u32* pos, u32* out, u32* count, void* tmp, Stream*);
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream*);  // delete all clauses of the lists
+void k_count_nonzero(const u32* a, u32 n, u32* out, Stream*);    // *out = number of nonzero a[i]
 // work, lsum, tmp: as for k_sub (the variables the gate checks leave open
 // are staged by size class)
 void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeDecision& d, const u32* order,
