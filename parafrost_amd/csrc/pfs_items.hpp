@@ -718,43 +718,84 @@ PF_HD bool selfsub_lits(u32 x, u32 fx, CL sm, CL lr) {
   return false;
 }
 // subsume.h:105-121: is 'cand' subsumed by an earlier entry of the list
-template <class D>
-PF_HD bool subsume_cand(const D& db, OLV ol, u32 end, u32 cand, SubLocal& sl) {
-  const u32 csz = csize(db, cand);
-  const bool cm = is_molten(db, cand);
-  for (u32 j = 0; j < end; j++) {
-    const u32 s = ol.r[j];
-    if (is_deleted(db, s)) continue;
-    const u32 ssz = csize(db, s);
-    if (cm && ssz > csz) continue;
-    if (ssz > 1 && sub_sig(csig(db, s), csig(db, cand)) && sub_lits(clause(db, s), clause(db, cand))) {
-      if (is_learnt(db, s) && is_original(db, cand)) set_status(db, s, ST_ORIGINAL);
-      push_proof(sl.proof, sl.k, sl.ev++, PR_DEL, cid(db, cand));
-      set_deleted(db, cand);
-      return true;
+// Groups of G consecutive lanes of a wavefront may share one variable
+// (gl = 0..G-1 is the lane's place in its group; G = 1 is one lane per
+// variable).  group_first returns the lowest j in [0, n) with pred(j), n if
+// none: the group tests j = gl, gl + G, ... and takes the lowest hit of the
+// first stride that has one, which is the sequential scan's answer; pred
+// must not change memory.  The group's first lane does every write (the
+// SUB actions), group_sync orders those writes before the group's next
+// reads of the variable's LDS copy.
+template <class P>
+PF_HD u32 group_first(u32 n, u32 gl, u32 G, P pred) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if (G > 1) {
+    const u32 g0 = (threadIdx.x & 63) - gl;
+    const u64 gmask = (G >= 64 ? ~0ull : ((1ull << G) - 1)) << g0;
+    for (u32 j0 = 0; j0 < n; j0 += G) {
+      const u32 j = j0 + gl;
+      const bool hit = j < n && pred(j);
+      const u64 m = __ballot(hit) & gmask;
+      if (m) return j0 + (u32)(__ffsll((long long)m) - 1) - g0;
     }
+    return n;
   }
-  return false;
+#endif
+  for (u32 j = 0; j < n; j++)
+    if (pred(j)) return j;
+  return n;
+}
+PF_HD void group_sync() {
+#ifdef __HIP_DEVICE_COMPILE__
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#endif
+}
+// subsume.h:105-121: is 'cand' subsumed by an earlier entry of the list
+template <class D>
+PF_HD bool subsume_cand(const D& db, OLV ol, u32 end, u32 cand, SubLocal& sl, u32 gl = 0, u32 G = 1) {
+  const u32 csz = csize(db, cand), csg = csig(db, cand);
+  const bool cm = is_molten(db, cand);
+  const CL cc = clause(db, cand);
+  const u32 j = group_first(end, gl, G, [&](u32 j) {
+    const u32 s = ol.r[j];
+    if (is_deleted(db, s)) return false;
+    const u32 ssz = csize(db, s);
+    if (cm && ssz > csz) return false;
+    return ssz > 1 && sub_sig(csig(db, s), csg) && sub_lits(clause(db, s), cc);
+  });
+  if (j == end) return false;
+  if (gl == 0) {
+    const u32 s = ol.r[j];
+    if (is_learnt(db, s) && is_original(db, cand)) set_status(db, s, ST_ORIGINAL);
+    push_proof(sl.proof, sl.k, sl.ev++, PR_DEL, cid(db, cand));
+    set_deleted(db, cand);
+  }
+  group_sync();
+  return true;
 }
 // subsume.h:123-142: strengthen 'cand' by removing x if a clause of 'ol'
-// self-subsumes it
+// self-subsumes it (the scan stops at the first clause longer than cand)
 template <class D>
 PF_HD bool selfsubsume_cand(const D& db, u32 x, u32 fx, OLV ol, u32 cand, u32 p, SubLocal& sl,
-                            int lbd_tier1) {
+                            int lbd_tier1, u32 gl = 0, u32 G = 1) {
   const u32 csz = csize(db, cand);
   const u32 cand_sig = csig(db, cand);
-  for (u32 j = 0; j < ol.n; j++) {
+  const CL cc = clause(db, cand);
+  const u32 j = group_first(ol.n, gl, G, [&](u32 j) {
     const u32 s = ol.r[j];
     const u32 ssz = csize(db, s);
-    if (ssz > csz) break;
-    if (is_deleted(db, s)) continue;
-    if (ssz > 1 && selfsub_sig(csig(db, s), cand_sig) && selfsub_lits(x, fx, clause(db, s), clause(db, cand))) {
-      strengthen(db, cand, x, p, sl, lbd_tier1);
-      melt(db, cand);
-      return true;
-    }
+    if (ssz > csz) return true;  // the stop
+    if (is_deleted(db, s)) return false;
+    return ssz > 1 && selfsub_sig(csig(db, s), cand_sig) && selfsub_lits(x, fx, clause(db, s), cc);
+  });
+  if (j == ol.n || csize(db, ol.r[j]) > csz) return false;
+  if (gl == 0) {
+    strengthen(db, cand, x, p, sl, lbd_tier1);
+    melt(db, cand);
   }
-  return false;
+  group_sync();
+  return true;
 }
 // subsume.h:27-37
 PF_HD void update_ol(const DB& db, u32 lit) {
@@ -790,8 +831,9 @@ PF_HD bool sub_visible_unit(const SubVis& vis, u32 k, CL c) {
   return false;
 }
 // D: the pool (DB) or the variable's lists staged in an SView
+// gl, G: the lane's place in the group of lanes running this variable
 template <class D>
-PF_HD void sub_var(const D& db, const Options& o, u32 v, u32 k, const SubOut& out) {
+PF_HD void sub_var(const D& db, const Options& o, u32 v, u32 k, const SubOut& out, u32 gl = 0, u32 G = 1) {
   const u32 p = V2L(v), n = NEGL(p);
   u32 np = olist(db, p).n, nn = olist(db, n).n;
   if (out.vis.ust) {
@@ -802,13 +844,15 @@ PF_HD void sub_var(const D& db, const Options& o, u32 v, u32 k, const SubOut& ou
       if (out.vis.ulit[v] == p) np = 0;
       else nn = 0;
     }
-    for (u32 q = 0; q < 2; q++) {
-      const OLV ol = olist(db, p | q);
-      for (u32 i = 0; i < ol.n; i++) {
-        const u32 c = ol.r[i];
-        if (!is_deleted(db, c) && sub_visible_unit(out.vis, k, clause(db, c))) set_deleted(db, c);
+    if (gl == 0)
+      for (u32 q = 0; q < 2; q++) {
+        const OLV ol = olist(db, p | q);
+        for (u32 i = 0; i < ol.n; i++) {
+          const u32 c = ol.r[i];
+          if (!is_deleted(db, c) && sub_visible_unit(out.vis, k, clause(db, c))) set_deleted(db, c);
+        }
       }
-    }
+    group_sync();
   }
   if ((int)np > o.sub_limit || (int)nn > o.sub_limit) return;
   SubLocal sl;
@@ -826,10 +870,11 @@ PF_HD void sub_var(const D& db, const Options& o, u32 v, u32 k, const SubOut& ou
       const u32 c = poss.r[i];
       if ((int)csize(db, c) > HSE_MAX_CL_SIZE) break;
       if (is_deleted(db, c)) continue;
-      selfsubsume_cand(db, p, n, olist(db, n), c, p, sl, o.lbd_tier1);
-      subsume_cand(db, poss, i, c, sl);
+      selfsubsume_cand(db, p, n, olist(db, n), c, p, sl, o.lbd_tier1, gl, G);
+      subsume_cand(db, poss, i, c, sl, gl, G);
     }
-    update_ol(db, p);
+    if (gl == 0) update_ol(db, p);
+    group_sync();
   }
   {
     const OLV negs = olist(db, n);
@@ -837,11 +882,13 @@ PF_HD void sub_var(const D& db, const Options& o, u32 v, u32 k, const SubOut& ou
       const u32 c = negs.r[i];
       if ((int)csize(db, c) > HSE_MAX_CL_SIZE) break;
       if (is_deleted(db, c)) continue;
-      selfsubsume_cand(db, n, p, olist(db, p), c, p, sl, o.lbd_tier1);
-      subsume_cand(db, negs, i, c, sl);
+      selfsubsume_cand(db, n, p, olist(db, p), c, p, sl, o.lbd_tier1, gl, G);
+      subsume_cand(db, negs, i, c, sl, gl, G);
     }
-    update_ol(db, n);
+    if (gl == 0) update_ol(db, n);
+    group_sync();
   }
+  if (gl) return;  // the group's first lane holds the units
   for (u32 i = 0; i < sl.n; i++) {
     const u32 r = at_add(out.n, 1u);
     if (r < out.cap) {

@@ -880,9 +880,12 @@ void lv_sort(u32* work, u32 n, void* tmp, Stream* st) {
   KLAUNCH(lv_bounds_k, grid_for(n, 256), 256, st, work + LV_WORK_HDR, n, work);
 }
 // one wavefront per workgroup, grid-stride over the chunks of all classes;
-// f(k, w, slice) runs elected index k with the LDS words [w, w + slice)
-// (w null: on the pool).  A class is chosen from a bound on the words the
-// variable needs; the staging checks the slice and falls back to the pool.
+// f(k, w, slice, gl, G) runs elected index k with the LDS words
+// [w, w + slice) (w null: on the pool) on a group of G lanes (gl = the
+// lane's place in it): a chunk of class c < LV_CLASSES - 1 holds 64 >> c
+// variables, each on 1 << c lanes.  A class is chosen from a bound on the
+// words the variable needs; the staging checks the slice and falls back to
+// the pool (one lane per variable).
 template <class F>
 __device__ inline void lv_chunks(const u32* work, u32 n, u32 cap, u32* lds, F f) {
   const u32 lane = threadIdx.x;
@@ -909,9 +912,10 @@ __device__ inline void lv_chunks(const u32* work, u32 n, u32 cap, u32* lds, F f)
       if (j < m) break;
       j -= m;
     }
-    const u32 c = LV_CLASSES - 1 - key, L = lv_class_lanes(c), r = start[key] + j * L + lane;
-    if (lane < L && r < start[key + 1])
-      f(placed[r], c < LV_CLASSES - 1 ? lds + lane * (cap / L) : nullptr, cap / L);
+    const u32 c = LV_CLASSES - 1 - key, L = lv_class_lanes(c), G = 64 / L;
+    const u32 slot = lane / G, r = start[key] + j * L + slot;
+    if (r < start[key + 1])
+      f(placed[r], c < LV_CLASSES - 1 ? lds + slot * (cap / L) : nullptr, cap / L, lane % G, G);
     __syncthreads();  // the next chunk reuses the LDS
   }
 }
@@ -943,7 +947,8 @@ __global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d,
 }
 __global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 n, u32 cap) {
   extern __shared__ u32 lds[];
-  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
+  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice, u32 gl, u32 G) {
+    if (gl) return;  // one lane per variable
     const u32 v = pv[k];
     LView lv;
     if (w && lview_stage(db, v, w, slice, lv)) {
@@ -979,15 +984,18 @@ __global__ void sub_class_k(DB db, const u32* pv, const u32* idx, u32 n, u32* wo
 __global__ void sub_k(DB db, Options o, const u32* pv, const u32* work, u32 n, u32 cap, SubOut out, SubSnap snap,
                       u32 take_snap) {
   extern __shared__ u32 lds[];
-  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
+  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice, u32 gl, u32 G) {
     const u32 v = pv[k];
     SView sv;
+    // every lane of the group stages the same words, so the outcome is the
+    // group's; the scans are shared, the writes are the first lane's
     if (w && sview_stage(db, v, w, slice, sv)) {
       const u32 np0 = sv.w[0], nn0 = sv.w[1];
-      if (take_snap) sub_snapshot_var(sv, v, k, snap);
-      sub_var(sv, o, v, k, out);
-      sview_commit(db, sv, np0, nn0);
-    } else {
+      if (take_snap && gl == 0) sub_snapshot_var(sv, v, k, snap);
+      group_sync();
+      sub_var(sv, o, v, k, out, gl, G);
+      if (gl == 0) sview_commit(db, sv, np0, nn0);
+    } else if (gl == 0) {
       if (take_snap) sub_snapshot_var(db, v, k, snap);
       sub_var(db, o, v, k, out);
     }
@@ -1024,7 +1032,8 @@ __global__ void ve_apply_class_k(DB db, const u32* pv, u32 n, const u32* type, u
 }
 __global__ void ve_apply_k(DB db, const u32* pv, VeDecision d, VeApply a, const u32* work, u32 n, u32 cap) {
   extern __shared__ u32 lds[];
-  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice) {
+  lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice, u32 gl, u32 G) {
+    if (gl) return;  // one lane per variable
     const u32 v = pv[k];
     LView lv;
     if (w && d.type[k] != VT_EQU && lview_stage(db, v, w, slice, lv)) ve_apply(lv, db, v, k, d, a);
