@@ -751,6 +751,25 @@ PF_HD void group_sync() {
   __builtin_amdgcn_wave_barrier();
 #endif
 }
+// the lowest j in [from, n) with pred(j), n if none (as group_first)
+template <class P>
+PF_HD u32 group_first_from(u32 from, u32 n, u32 gl, u32 G, P pred) {
+  if (from >= n) return n;
+  return from + group_first(n - from, gl, G, [&](u32 j) { return pred(from + j); });
+}
+// reductions over the G lanes of a group (G a power of two, groups aligned)
+PF_HD u32 group_min(u32 x, u32 G) {
+#ifdef __HIP_DEVICE_COMPILE__
+  for (u32 d = 1; d < G; d <<= 1) x = min(x, (u32)__shfl_xor(x, d, 64));
+#endif
+  return x;
+}
+PF_HD u32 group_sum(u32 x, u32 G) {
+#ifdef __HIP_DEVICE_COMPILE__
+  for (u32 d = 1; d < G; d <<= 1) x += (u32)__shfl_xor(x, d, 64);
+#endif
+  return x;
+}
 // subsume.h:105-121: is 'cand' subsumed by an earlier entry of the list
 template <class D>
 PF_HD bool subsume_cand(const D& db, OLV ol, u32 end, u32 cand, SubLocal& sl, u32 gl = 0, u32 G = 1) {
@@ -1003,11 +1022,33 @@ PF_HD u32 count_lits_before(const D& db, OLV ol) {
   return n;
 }
 // simplify.h:237-295: true when the bound is exceeded; nAddedCls / lits
+// On a group (G > 1) the pairs are shared out and the counts summed: the
+// bound is exceeded at some pair iff the total exceeds it, and a caller
+// reads nAddedCls after a true return only as zero / not zero (every call
+// starts from zero; a true return has counted at least one resolvent).
 template <class D>
 PF_HD bool count_res(const D& db, const Options& o, u32 x, int clsbefore, OLV me, OLV other,
-                     int& nAddedCls, u32& nAddedLits, bool substitute) {
+                     int& nAddedCls, u32& nAddedLits, bool substitute, u32 gl = 0, u32 G = 1) {
   nAddedLits = 0;
   const int rlimit = o.ve_clause_limit;
+  if (G > 1) {
+    u32 cls = 0, lits = 0, over = 0;
+    const u32 np = me.n * other.n;
+    for (u32 q = gl; q < np; q += G) {
+      const u32 ci = me.r[q / other.n], cj = other.r[q % other.n];
+      if (!is_original(db, ci) || !is_original(db, cj)) continue;
+      if (substitute && is_molten(db, ci) == is_molten(db, cj)) continue;
+      const u32 rsize = merge_len(x, clause(db, ci), clause(db, cj));
+      if (rsize > 1) {
+        cls++;
+        lits += rsize;
+        if (rlimit && (int)rsize > rlimit) over = 1;
+      }
+    }
+    nAddedCls += (int)group_sum(cls, G);
+    nAddedLits = group_sum(lits, G);
+    if (nAddedCls > clsbefore || group_sum(over, G)) return true;
+  } else {
   for (u32 i = 0; i < me.n; i++) {
     const u32 ci = me.r[i];
     if (!is_original(db, ci)) continue;
@@ -1023,6 +1064,7 @@ PF_HD bool count_res(const D& db, const Options& o, u32 x, int clsbefore, OLV me
         nAddedLits += rsize;
       }
     }
+  }
   }
   if (o.ve_lbound_en) {
     const u32 before = count_lits_before(db, me) + count_lits_before(db, other);
@@ -1125,14 +1167,14 @@ PF_HD bool find_ao_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, u32
 // taken, so scanning the list of x (the eliminated variable's literal,
 // whose clauses the gate search reads anyway) gives the same clause.
 template <class D>
-PF_HD u32 fast_equality_check(const D& db, u32 x, u32 y, u32 z) {
+PF_HD u32 fast_equality_check(const D& db, u32 x, u32 y, u32 z, u32 gl = 0, u32 G = 1) {
   const OLV ol = olist(db, x);
   u32 a = x, b = y, c = z, t;
   if (b > c) { t = b; b = c; c = t; }
   if (a > c) { t = a; a = c; c = t; }
   if (a > b) { t = a; a = b; b = t; }
   u32 best = NOPOS, best_id = NOPOS;
-  for (u32 i = 0; i < ol.n; i++) {
+  for (u32 i = gl; i < ol.n; i += G) {
     const u32 cl = ol.r[i];
     if (is_molten(db, cl)) continue;
     if (!is_original(db, cl) || csize(db, cl) != 3) continue;
@@ -1142,11 +1184,35 @@ PF_HD u32 fast_equality_check(const D& db, u32 x, u32 y, u32 z) {
       best_id = cid(db, cl);
     }
   }
+  if (G > 1) {  // the lowest id over the group, then its clause
+    const u32 m = group_min(best_id, G);
+    best = group_min(best_id == m ? best : NOPOS, G);
+  }
   return best;
 }
+// the second clause of an ITE candidate pair: cj ternary original, and after
+// ordering (x first, then y matching yi's variable) yi = -yj, zi and zj of
+// different variables
+template <class D>
+PF_HD bool ite_pair(const D& db, u32 dx, u32 yi, u32 zi, u32 cj, u32& yj_out, u32& zj_out) {
+  if (!(is_original(db, cj) && csize(db, cj) == 3)) return false;
+  const u32* lj = clits(db, cj);
+  u32 xj = lj[0], yj = lj[1], zj = lj[2], t;
+  if (yj == dx) { t = xj; xj = yj; yj = t; }
+  if (zj == dx) { t = xj; xj = zj; zj = t; }
+  if (ABS(yi) == ABS(zj)) { t = yj; yj = zj; zj = t; }
+  if (ABS(zi) == ABS(zj)) return false;
+  if (yi != FLIP(yj)) return false;
+  yj_out = yj;
+  zj_out = zj;
+  return true;
+}
+// pairs (i, j) in the reference's order; on a group, the candidates j of
+// each i are found with group_first_from and the two equality checks are
+// shared
 template <class D>
 PF_HD bool find_ite_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
-                         u32& nAddedLits) {
+                         u32& nAddedLits, u32 gl = 0, u32 G = 1) {
   const u32 fx = FLIP(dx);
   const OLV it = olist(db, dx);
   for (u32 i = 0; i < it.n; i++) {
@@ -1157,23 +1223,21 @@ PF_HD bool find_ite_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int
     if (yi == dx) { t = xi; xi = yi; yi = t; }
     if (zi == dx) { t = xi; xi = zi; zi = t; }
     for (u32 j = i + 1; j < it.n; j++) {
+      u32 yj = 0, zj = 0;
+      j = group_first_from(j, it.n, gl, G, [&](u32 q) { u32 a, b; return ite_pair(db, dx, yi, zi, it.r[q], a, b); });
+      if (j == it.n) break;
       const u32 cj = it.r[j];
-      if (!(is_original(db, cj) && csize(db, cj) == 3)) continue;
-      const u32* lj = clits(db, cj);
-      u32 xj = lj[0], yj = lj[1], zj = lj[2];
-      if (yj == dx) { t = xj; xj = yj; yj = t; }
-      if (zj == dx) { t = xj; xj = zj; zj = t; }
-      if (ABS(yi) == ABS(zj)) { t = yj; yj = zj; zj = t; }
-      if (ABS(zi) == ABS(zj)) continue;
-      if (yi != FLIP(yj)) continue;
-      const u32 d1 = fast_equality_check(db, fx, yi, FLIP(zi));
+      ite_pair(db, dx, yi, zi, cj, yj, zj);
+      const u32 d1 = fast_equality_check(db, fx, yi, FLIP(zi), gl, G);
       if (d1 == NOPOS) continue;
-      const u32 d2 = fast_equality_check(db, fx, yj, FLIP(zj));
+      const u32 d2 = fast_equality_check(db, fx, yj, FLIP(zj), gl, G);
       if (d2 == NOPOS) continue;
-      melt(db, ci); melt(db, cj); melt(db, d1); melt(db, d2);
+      if (gl == 0) { melt(db, ci); melt(db, cj); melt(db, d1); melt(db, d2); }
+      group_sync();
       nAddedCls = 0;
-      if (count_res(db, o, ABS(dx), nOrgCls, it, olist(db, fx), nAddedCls, nAddedLits, true)) {
-        freeze(db, ci); freeze(db, cj); freeze(db, d1); freeze(db, d2);
+      if (count_res(db, o, ABS(dx), nOrgCls, it, olist(db, fx), nAddedCls, nAddedLits, true, gl, G)) {
+        if (gl == 0) { freeze(db, ci); freeze(db, cj); freeze(db, d1); freeze(db, d2); }
+        group_sync();
         return false;
       }
       return true;
@@ -1183,15 +1247,18 @@ PF_HD bool find_ite_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int
 }
 // xor.h:31-43
 template <class D>
-PF_HD void freeze_arities(const D& db, OLV me, OLV other) {
-  for (u32 i = 0; i < me.n; i++) {
-    const u32 c = me.r[i];
-    if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
+PF_HD void freeze_arities(const D& db, OLV me, OLV other, u32 gl = 0, u32 G = 1) {
+  if (gl == 0) {
+    for (u32 i = 0; i < me.n; i++) {
+      const u32 c = me.r[i];
+      if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
+    }
+    for (u32 i = 0; i < other.n; i++) {
+      const u32 c = other.r[i];
+      if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
+    }
   }
-  for (u32 i = 0; i < other.n; i++) {
-    const u32 c = other.r[i];
-    if (csize(db, c) > 2 && is_molten(db, c)) freeze(db, c);
-  }
+  group_sync();
 }
 // xor.h:70-102: flip the literals for the next parity pattern and look for
 // the matching clause (lowest id).  The pattern is the base clause with
@@ -1204,11 +1271,11 @@ PF_HD void freeze_arities(const D& db, OLV me, OLV other) {
 // gives the same clause.
 PF_HD u32 xor_lit(const u32* base, u32 parity, u32 k) { return base[k] ^ ((parity >> k) & 1u); }
 template <class D>
-PF_HD bool make_arity(const D& db, u32& parity, const u32* base, u32 size, u32 kx) {
+PF_HD bool make_arity(const D& db, u32& parity, const u32* base, u32 size, u32 kx, u32 gl = 0, u32 G = 1) {
   do { parity++; } while (__builtin_popcount(parity) & 1);
   const OLV ol = olist(db, xor_lit(base, parity, kx));
   u32 found = NOPOS, found_id = NOPOS;
-  for (u32 i = 0; i < ol.n; i++) {
+  for (u32 i = gl; i < ol.n; i += G) {
     const u32 c = ol.r[i];
     const u32 id = cid(db, c);
     if (id >= found_id) continue;
@@ -1221,13 +1288,18 @@ PF_HD bool make_arity(const D& db, u32& parity, const u32* base, u32 size, u32 k
       found_id = id;
     }
   }
+  if (G > 1) {  // the lowest id over the group, then its clause
+    const u32 m = group_min(found_id, G);
+    found = group_min(found_id == m ? found : NOPOS, G);
+  }
   if (found == NOPOS) return false;
-  melt(db, found);
+  if (gl == 0) melt(db, found);
+  group_sync();
   return true;
 }
 template <class D>
 PF_HD bool find_xor_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int& nAddedCls,
-                         u32& nAddedLits) {
+                         u32& nAddedLits, u32 gl = 0, u32 G = 1) {
   const u32 fx = FLIP(dx), v = ABS(dx);
   const OLV it = olist(db, dx), ot2 = olist(db, fx);
   for (u32 i = 0; i < it.n; i++) {
@@ -1242,14 +1314,15 @@ PF_HD bool find_xor_gate(const D& db, const Options& o, u32 dx, int nOrgCls, int
     if (kx == size) continue;  // not reached: the clause is in the list of dx
     u32 parity = 0;
     int itargets = 1 << arity;
-    while (--itargets && make_arity(db, parity, base, size, kx)) {
+    while (--itargets && make_arity(db, parity, base, size, kx, gl, G)) {
     }
-    if (itargets) freeze_arities(db, it, ot2);
+    if (itargets) freeze_arities(db, it, ot2, gl, G);
     else {
-      melt(db, ci);
+      if (gl == 0) melt(db, ci);
+      group_sync();
       nAddedCls = 0;
-      if (count_res(db, o, v, nOrgCls, it, ot2, nAddedCls, nAddedLits, true)) {
-        freeze_arities(db, it, ot2);
+      if (count_res(db, o, v, nOrgCls, it, ot2, nAddedCls, nAddedLits, true, gl, G)) {
+        freeze_arities(db, it, ot2, gl, G);
         break;
       }
       return true;
@@ -1316,7 +1389,8 @@ PF_HD bool ve_decide_gates(const DB& db, const Options& o, u32 v, u32 k, const V
 }
 // D: the pool (DB) or the variable's lists staged in an LView
 template <class D>
-PF_HD void ve_decide_rest(const D& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
+PF_HD void ve_decide_rest(const D& db, const Options& o, u32 v, u32 k, const VeDecision& d, u32 gl = 0,
+                          u32 G = 1) {
   const u32 p = V2L(v), n = NEGL(p);
   const OLV poss = olist(db, p), negs = olist(db, n);
   const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);
@@ -1325,13 +1399,13 @@ PF_HD void ve_decide_rest(const D& db, const Options& o, u32 v, u32 k, const VeD
   u32 type = VT_NONE;
   int nAddedCls = (int)d.ncls[k];
   u32 nAddedLits = 0;
-  if (find_ite_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-  else if (!nAddedCls && find_ite_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-  else if (find_xor_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-  else if (!nAddedCls && find_xor_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits)) type = VT_SUBST;
-  if (!type && !nAddedCls && !count_res(db, o, v, nOrgCls, poss, negs, nAddedCls, nAddedLits, false))
+  if (find_ite_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits, gl, G)) type = VT_SUBST;
+  else if (!nAddedCls && find_ite_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits, gl, G)) type = VT_SUBST;
+  else if (find_xor_gate(db, o, p, nOrgCls, nAddedCls, nAddedLits, gl, G)) type = VT_SUBST;
+  else if (!nAddedCls && find_xor_gate(db, o, n, nOrgCls, nAddedCls, nAddedLits, gl, G)) type = VT_SUBST;
+  if (!type && !nAddedCls && !count_res(db, o, v, nOrgCls, poss, negs, nAddedCls, nAddedLits, false, gl, G))
     type = VT_RES;
-  ve_write(db, k, d, side, type, 0, nAddedCls, nAddedLits);
+  if (gl == 0) ve_write(db, k, d, side, type, 0, nAddedCls, nAddedLits);
 }
 PF_HD void ve_decide(const DB& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
   if (!ve_decide_gates(db, o, v, k, d)) ve_decide_rest(db, o, v, k, d);

@@ -930,7 +930,13 @@ constexpr u32 LV_BLOCKS = 256 * (160 * 1024 / VR_LDS);
 // AND/OR) for every elected variable, then, staged, the variables they
 // leave open (ITE and XOR searches, resolvent count)
 // lsum[2k], lsum[2k + 1]: bounds on the clause size sums of the lists of
-// elected variable k (from the list sort)
+// elected variable k (from the list sort).  The searches' work grows with
+// the square of the list lengths, their memory linearly: a variable's
+// class is raised by VE_CLASS_BUMP (more lanes, a larger slice) over the
+// class its lists need.
+#ifndef VE_CLASS_BUMP
+#define VE_CLASS_BUMP 1
+#endif
 __global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order, u32* work,
                            const u32* lsum, u32 cap) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -940,7 +946,9 @@ __global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d,
   if (!ve_decide_gates(db, o, pv[k], k, d)) {
     const u32 p = V2L(pv[k]);
     const u64 b = lview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
-    key = LV_CLASSES - 1 - lv_class((u32)min(b, (u64)cap + 1), cap);
+    u32 c = lv_class((u32)min(b, (u64)cap + 1), cap);
+    if (c < LV_CLASSES - 1) c = min(c + VE_CLASS_BUMP, LV_CLASSES - 2);
+    key = LV_CLASSES - 1 - c;
   }
   work[LV_WORK_HDR + t] = key;
   work[LV_WORK_HDR + n + t] = k;
@@ -948,13 +956,15 @@ __global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d,
 __global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 n, u32 cap) {
   extern __shared__ u32 lds[];
   lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice, u32 gl, u32 G) {
-    if (gl) return;  // one lane per variable
     const u32 v = pv[k];
     LView lv;
+    // every lane of the group stages the same words; the searches are
+    // shared, the writes are the first lane's
     if (w && lview_stage(db, v, w, slice, lv)) {
-      ve_decide_rest(lv, o, v, k, d);
-      lview_commit(db, lv);
-    } else {
+      group_sync();
+      ve_decide_rest(lv, o, v, k, d, gl, G);
+      if (gl == 0) lview_commit(db, lv);
+    } else if (gl == 0) {
       ve_decide_rest(db, o, v, k, d);
     }
   });
@@ -969,13 +979,18 @@ void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeD
   KLAUNCH_LDS(ve_rest_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, d, work, n, cap);
 }
 // SUB, staged as the VE searches (the class of a variable from the words of
-// its SView)
+// its SView, raised by SUB_CLASS_BUMP)
+#ifndef SUB_CLASS_BUMP
+#define SUB_CLASS_BUMP 0
+#endif
 __global__ void sub_class_k(DB db, const u32* pv, const u32* idx, u32 n, u32* work, const u32* lsum, u32 cap) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const u32 k = idx ? idx[t] : t, p = V2L(pv[k]);
   const u64 b = sview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
-  work[LV_WORK_HDR + t] = LV_CLASSES - 1 - lv_class((u32)min(b, (u64)cap + 1), cap);
+  u32 c = lv_class((u32)min(b, (u64)cap + 1), cap);
+  if (c < LV_CLASSES - 1) c = min(c + SUB_CLASS_BUMP, LV_CLASSES - 2);
+  work[LV_WORK_HDR + t] = LV_CLASSES - 1 - c;
   work[LV_WORK_HDR + n + t] = k;
 }
 // take_snap: record each variable's state for the SUB re-runs just before
