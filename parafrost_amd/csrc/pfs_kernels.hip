@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <new>
 #include <vector>
@@ -35,12 +36,30 @@ struct KRec {
   const char* name;
   hipEvent_t a, b;
 };
+// Small device-to-host reads (counts, flags, a few records) land in pinned
+// staging memory: a copy into pageable memory makes the runtime stage it
+// itself, which adds tens of microseconds to every read-and-wait.  The
+// copies to their destinations happen in stream_sync, which every caller
+// runs before it looks at what it read.
+constexpr size_t PIN_BYTES = 64 << 10, PIN_SMALL = 4096;
+struct PinCopy {
+  void* dst;
+  size_t off, bytes;
+};
 struct Stream {
   hipStream_t s;
   int device;
   bool prof = false;        // record events around every kernel launch
   std::vector<KRec> recs;   // pending event pairs
+  unsigned char* pin = nullptr;  // PIN_BYTES of pinned host memory
+  size_t pin_used = 0;
+  std::vector<PinCopy> pend;     // staged reads not yet copied out
 };
+static void pin_flush(Stream* st) {
+  for (const PinCopy& c : st->pend) memcpy(c.dst, st->pin + c.off, c.bytes);
+  st->pend.clear();
+  st->pin_used = 0;
+}
 struct Event {
   hipEvent_t e;
 };
@@ -75,12 +94,14 @@ Stream* stream_create(int device) {
   Stream* st = new Stream();
   st->device = device;
   PFS_HIP(hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking));
+  if (hipHostMalloc((void**)&st->pin, PIN_BYTES, hipHostMallocDefault) != hipSuccess) st->pin = nullptr;
   return st;
 }
 void prof_enable(Stream* st, bool on) { st->prof = on; }
 void stream_bind(Stream* st) { PFS_HIP(hipSetDevice(st->device)); }
 void prof_collect(Stream* st, std::vector<KernelTime>& out) {
   PFS_HIP(hipStreamSynchronize(st->s));
+  pin_flush(st);
   std::map<std::string, size_t> idx;
   for (size_t i = 0; i < out.size(); i++) idx[out[i].name] = i;
   for (const KRec& r : st->recs) {
@@ -105,10 +126,16 @@ void stream_destroy(Stream* st) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
+  (void)hipStreamSynchronize(st->s);
+  pin_flush(st);
+  if (st->pin) (void)hipHostFree(st->pin);
   (void)hipStreamDestroy(st->s);
   delete st;
 }
-void stream_sync(Stream* st) { PFS_HIP(hipStreamSynchronize(st->s)); }
+void stream_sync(Stream* st) {
+  PFS_HIP(hipStreamSynchronize(st->s));
+  pin_flush(st);
+}
 void* dev_alloc(size_t bytes) {
   void* p = nullptr;
   if (bytes == 0) bytes = 256;
@@ -125,7 +152,15 @@ void h2d(void* dst, const void* src, size_t bytes, Stream* st) {
   if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->s));
 }
 void d2h(void* dst, const void* src, size_t bytes, Stream* st) {
-  if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st->s));
+  if (!bytes) return;
+  if (st->pin && bytes <= PIN_SMALL) {
+    if (st->pin_used + bytes > PIN_BYTES) stream_sync(st);
+    PFS_HIP(hipMemcpyAsync(st->pin + st->pin_used, src, bytes, hipMemcpyDeviceToHost, st->s));
+    st->pend.push_back(PinCopy{dst, st->pin_used, bytes});
+    st->pin_used += (bytes + 15) & ~(size_t)15;
+    return;
+  }
+  PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st->s));
 }
 void d2d(void* dst, const void* src, size_t bytes, Stream* st) {
   if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st->s));
