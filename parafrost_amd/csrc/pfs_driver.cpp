@@ -1472,8 +1472,8 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
     r++;
   }
   if (!prop()) return 0;
-  count(nC, nL);
   gc();
+  nC = ncls_;  // the compaction keeps exactly the live clauses
   if (!unassigned_ || !nC) return 1;
   return 2;
 }

@@ -148,8 +148,18 @@ void* dev_alloc(size_t bytes) {
 void dev_free(void* p) {
   if (p) (void)hipFree(p);
 }
+// small host-to-device copies go through the same pinned area (the host
+// data is copied there first; the area is reused only after stream_sync)
 void h2d(void* dst, const void* src, size_t bytes, Stream* st) {
-  if (bytes) PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->s));
+  if (!bytes) return;
+  if (st->pin && bytes <= PIN_SMALL) {
+    if (st->pin_used + bytes > PIN_BYTES) stream_sync(st);
+    memcpy(st->pin + st->pin_used, src, bytes);
+    PFS_HIP(hipMemcpyAsync(dst, st->pin + st->pin_used, bytes, hipMemcpyHostToDevice, st->s));
+    st->pin_used += (bytes + 15) & ~(size_t)15;
+    return;
+  }
+  PFS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->s));
 }
 void d2h(void* dst, const void* src, size_t bytes, Stream* st) {
   if (!bytes) return;
