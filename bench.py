@@ -39,6 +39,19 @@ OPTIONS = {"phases": 1, "bce": 1}
 REF_OPTIONS = ["--phases=1", "-bce"]
 
 
+def otp_counts(V, C):
+    """Range-by-tile counts of the occurrence-table build (otp_plan in
+    pfs_kernels.hip): ranges of 4096 literals (8192 past 12288 ranges),
+    tiles of the smallest power of two >= max(1024, 8 x ranges) clauses."""
+    n2 = 2 * V + 2
+    bits = 12 if (n2 >> 12) < 12288 else 13
+    nr = (n2 + (1 << bits) - 1) >> bits
+    ct = 1024
+    while ct < 8 * nr:
+        ct <<= 1
+    return nr * ((C + ct - 1) // ct)
+
+
 def kernel_bytes(name, rounds, cnf):
     """Algorithmic bytes per launch of a kernel, from the work measures of
     the profiled step.  A clause costs a 12-byte header (offset, size,
@@ -48,6 +61,13 @@ def kernel_bytes(name, rounds, cnf):
       hist_k     D + 8V            (read every clause, write the 2V histogram)
       scatter_k  D + 4L + 8V       (read every clause and the list offsets, write one ref per literal)
       count_k    D                 (read every clause)
+    occurrence table by literal ranges (N = range-by-tile counts, as the
+    library plans them: otp_plan):
+      otp_count_k         D + 4N   (read every clause, write the counts)
+      otp_part_k<true>    D + 4N + 8L   (read every clause and the scanned counts, write the pairs)
+      otp_part_k<false>   D + 4N + 4L   (the same with literals only)
+      otp_range_k<true>   12L + 24V     (read the pairs, write one ref per literal, offsets, sizes, counts)
+      otp_range_k<false>  4L + 8V       (read the literals, write the counts)
     per-elected-variable kernels (O_E list entries, D_E clause bytes of the
     elected variables' lists; elected variables share no clause):
       sub_k, ve_decide_k, bce_k   4 O_E + D_E
@@ -59,7 +79,18 @@ def kernel_bytes(name, rounds, cnf):
         c, l = r["clauses_before"], r["literals_before"]
         d = 12 * c + 4 * l
         oe, de = r["elected_occurs"], r["elected_bytes"]
-        if name == "hist_k":
+        n = otp_counts(V, c)
+        if name == "otp_count_k":
+            per.append(d + 4 * n)
+        elif name == "otp_part_k<true>":
+            per.append(d + 4 * n + 8 * l)
+        elif name == "otp_part_k<false>":
+            per.append(d + 4 * n + 4 * l)
+        elif name == "otp_range_k<true>":
+            per.append(12 * l + 24 * V)
+        elif name == "otp_range_k<false>":
+            per.append(4 * l + 8 * V)
+        elif name == "hist_k":
             per.append(d + 8 * V)
         elif name == "scatter_k":
             per.append(d + 4 * l + 8 * V)
@@ -540,7 +571,11 @@ def main():
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": int(traffic) if traffic else None, "traffic_scales": scales,
                     "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(b),
-                    "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3)}
+                    "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3),
+                    "modelled_kernels": {
+                        k: {"frac": round(kernel_bytes(k, rounds, cnf) / (v[0] / v[1] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "avg_launch_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
+                        for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0]) if kernel_bytes(k, rounds, cnf)}}
     handoff = bench_handoff(s, cnf)
     s.close()
     ingest = None if args.no_ingest else bench_ingest(cnf)

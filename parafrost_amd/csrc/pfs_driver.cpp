@@ -57,7 +57,7 @@ Simplifier::Simplifier(int device, size_t budget, bool keep_input)
 
 Simplifier::~Simplifier() {
   Buf* all[] = {&c_off_[0], &lits_[0], &c_off_[1], &lits_[1],
-                &ot_off_, &ot_size_, &ot_refs_, &hist_, &assign_, &vstate_, &assumed_, &vorg_, &trail_,
+                &ot_off_, &ot_size_, &ot_refs_, &hist_, &otp_, &assign_, &vstate_, &assumed_, &vorg_, &trail_,
                 &score_, &cflag_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_,
                 &def_, &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_ref_, &snap_flg_,
                 &snap_size_, &snap_sig_, &snap_lbd_, &snap_np_, &snap_nn_, &scratch_, &elits_, &unit_by_,
@@ -687,17 +687,32 @@ void Simplifier::gc() {
 // ---------------------------------------------------------------- OT (simplify.cpp:35-56)
 void Simplifier::create_ot() {
   stage_begin(S_OT);
+  prop_changed_ = false;
+  ot_counts(true);
+  stage_end();
+}
+
+void Simplifier::ot_counts(bool table) {
   const size_t n2 = 2 * (size_t)V_ + 2;
   DB db = make_db(cur_);
-  prop_changed_ = false;
   u32* hist = P<u32>(hist_);
+  OtpPlan p;
+  if (otp_plan(n2, db.ncls, &p)) {
+    mem_.ensure(otp_, (p.ncounts + 1 + 2 * (size_t)lits_end_) * 4, 0, st_);
+    mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
+    u32* counts = P<u32>(otp_);
+    u32* plit = counts + p.ncounts + 1;
+    u32* pref = table ? plit + lits_end_ : nullptr;
+    k_ot_ranges(db, p, (u32)n2, counts, plit, pref, hist, tmp_.p, st_);
+    return;
+  }
   fill_u32(hist, 0, n2, st_);
   k_hist(db, hist, st_);
+  if (!table) return;
   scan_u32(hist, db.ot_off, n2, db.ot_off + n2, tmp_.p, st_);
   d2d(db.ot_size, hist, n2 * 4, st_);
   fill_u32(hist, 0, n2, st_);
   k_scatter(db, hist, st_);
-  stage_end();
 }
 
 // ---------------------------------------------------------------- BCP (elimination.cpp:64-99)
@@ -774,10 +789,7 @@ bool Simplifier::lcve() {
   // histSimp (lcve.cpp:32): the occurrence counts of the live clauses.  The
   // occurrence-table scatter leaves exactly these counts in hist_ unless the
   // round's BCP changed clauses since.
-  if (prop_changed_) {
-    fill_u32(hist, 0, n2, st_);
-    k_hist(db, hist, st_);
-  }
+  if (prop_changed_) ot_counts(false);
   u32* sc = P<u32>(scal_);
   u64* hard = (u64*)(sc + 16);
   fill_u32((u32*)hard, INF, 2, st_);

@@ -131,6 +131,8 @@ class Simplifier {
   void gc();
   void count(uint64_t& cls, uint64_t& lits);
   void create_ot();
+  // live-literal counts into hist_; with 'table' also the occurrence table
+  void ot_counts(bool table);
   bool prop();
   bool prop_visit(uint64_t front);  // propagation from trail[front..): assignments only
   void prop_clean(uint64_t from);   // satisfied clauses deleted, false literals dropped
@@ -173,6 +175,7 @@ class Simplifier {
   Buf c_off_[2], lits_[2];  // lits_: word pool of headers + literals; c_off_: handles in clause order
   // occurrence table
   Buf ot_off_, ot_size_, ot_refs_, hist_;
+  Buf otp_;  // occurrence-table build: range counts, then (literal, handle) pairs
   // per variable
   Buf assign_, vstate_, assumed_, vorg_, trail_;
   uint64_t trail_n_ = 0, propagated_ = 0;

@@ -444,6 +444,162 @@ void k_scatter(const DB& db, u32* cursor, Stream* st) {
   KLAUNCH(scatter_k, grid_for(db.ncls, 256), 256, st, db, cursor);
 }
 
+// ---------------------------------------------------------------- OT by literal ranges
+// The same table without a global atomic per occurrence: the (literal,
+// handle) pairs of the live clauses are partitioned by literal range
+// (1 << bits literals each, counted per range and tile of clauses, the
+// counts scanned range-major), then one workgroup per range counts its
+// literals in LDS, scans them into the list offsets (the range's first
+// offset is where its partition starts, because the ranges are in literal
+// order) and places the handles with LDS cursors.  Order inside a list is
+// free, as with the per-occurrence atomics.
+constexpr int OTP_UNROLL = 4;
+constexpr u32 OTP_NONE = 0xFFFFFFFFu;  // no clause / no literal in this slot
+// calls f(literal, handle) for every literal of the live clauses of the
+// tile [c0, c1); each thread holds OTP_UNROLL clauses at a time
+template <class F>
+__device__ inline void otp_tile(const DB& db, u32 c0, u32 c1, F f) {
+  for (u32 i = c0 + threadIdx.x; i < c1; i += OTP_UNROLL * ST) {
+    u32 c[OTP_UNROLL], h[OTP_UNROLL];
+#pragma unroll
+    for (int q = 0; q < OTP_UNROLL; q++) {
+      const u32 ii = i + q * ST;
+      c[q] = ii < c1 ? db.c_off[ii] : OTP_NONE;
+    }
+#pragma unroll
+    for (int q = 0; q < OTP_UNROLL; q++) h[q] = c[q] != OTP_NONE ? db.lits[c[q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < OTP_UNROLL; q++) {
+      if (c[q] == OTP_NONE || st_deleted(hdr_flags(h[q]))) continue;
+      const u32* l = db.lits + c[q] + CL_HDR;
+      const u32 n = hdr_size(h[q]);
+      for (u32 k = 0; k < n; k++) f(l[k], c[q]);
+    }
+  }
+}
+__global__ void __launch_bounds__(ST) otp_count_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, u32* counts) {
+  extern __shared__ u32 lc[];
+  for (u32 j = threadIdx.x; j < nr; j += ST) lc[j] = 0;
+  __syncthreads();
+  const u32 c0 = blockIdx.x * ct, c1 = min(db.ncls, c0 + ct);
+  otp_tile(db, c0, c1, [&](u32 l, u32) { atomicAdd(&lc[l >> bits], 1u); });
+  __syncthreads();
+  for (u32 j = threadIdx.x; j < nr; j += ST) counts[(size_t)j * nblk + blockIdx.x] = lc[j];
+}
+// TABLE: pairs (literal, handle); otherwise literals only (pref unused)
+template <bool TABLE>
+__global__ void __launch_bounds__(ST) otp_part_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, const u32* offs,
+                                                 u32* plit, u32* pref) {
+  extern __shared__ u32 lc[];
+  for (u32 j = threadIdx.x; j < nr; j += ST) lc[j] = offs[(size_t)j * nblk + blockIdx.x];
+  __syncthreads();
+  const u32 c0 = blockIdx.x * ct, c1 = min(db.ncls, c0 + ct);
+  otp_tile(db, c0, c1, [&](u32 l, u32 c) {
+    const u32 p = atomicAdd(&lc[l >> bits], 1u);
+    plit[p] = l;
+    if (TABLE) pref[p] = c;
+  });
+}
+// one workgroup per literal range r: literals [r << bits, (r + 1) << bits)
+// whose pairs sit at [offs[r * nblk], offs[(r + 1) * nblk]) (the scan's
+// total past the last range).  Without TABLE only the counts (hist) are
+// written.
+constexpr int OTP_LD = 8;
+template <bool TABLE>
+__global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 nblk, const u32* offs,
+                                                  const u32* total, const u32* plit, const u32* pref,
+                                                  u32* hist) {
+  extern __shared__ u32 lc[];
+  __shared__ u32 lw[NW];
+  const u32 r = blockIdx.x, nl = 1u << bits, base = r << bits;
+  const u32 b0 = offs[(size_t)r * nblk];
+  const u32 b1 = r + 1 < gridDim.x ? offs[(size_t)(r + 1) * nblk] : *total;
+  for (u32 j = threadIdx.x; j < nl; j += ST) lc[j] = 0;
+  __syncthreads();
+  for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * ST) {
+    u32 l[OTP_LD];
+#pragma unroll
+    for (int q = 0; q < OTP_LD; q++) l[q] = p + q * ST < b1 ? plit[p + q * ST] : OTP_NONE;
+#pragma unroll
+    for (int q = 0; q < OTP_LD; q++)
+      if (l[q] != OTP_NONE) atomicAdd(&lc[l[q] - base], 1u);
+  }
+  __syncthreads();
+  if (!TABLE) {
+    for (u32 j = threadIdx.x; j < nl; j += ST)
+      if (base + j < n2) hist[base + j] = lc[j];
+    return;
+  }
+  // exclusive scan: each thread owns nl / ST consecutive counters
+  const u32 per = nl / ST, j0 = threadIdx.x * per;
+  u32 s = 0;
+  for (u32 k = 0; k < per; k++) s += lc[j0 + k];
+  u32 tot;
+  u32 pre = block_excl_scan(s, lw, &tot) + b0;
+  for (u32 k = 0; k < per; k++) {
+    const u32 v = lc[j0 + k];
+    lc[j0 + k] = pre;
+    pre += v;
+  }
+  __syncthreads();
+  for (u32 j = threadIdx.x; j < nl; j += ST)
+    if (base + j < n2) db.ot_off[base + j] = lc[j];
+  if (r + 1 == gridDim.x && threadIdx.x == 0) db.ot_off[n2] = *total;
+  for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * ST) {
+    u32 l[OTP_LD], c[OTP_LD];
+#pragma unroll
+    for (int q = 0; q < OTP_LD; q++) {
+      const bool in = p + q * ST < b1;
+      l[q] = in ? plit[p + q * ST] : OTP_NONE;
+      c[q] = in ? pref[p + q * ST] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < OTP_LD; q++)
+      if (l[q] != OTP_NONE) db.ot_refs[atomicAdd(&lc[l[q] - base], 1u)] = c[q];
+  }
+  __syncthreads();
+  // the cursors end at the next list's offset: size = cursor - offset
+  for (u32 j = threadIdx.x; j < nl; j += ST)
+    if (base + j < n2) {
+      const u32 sz = lc[j] - db.ot_off[base + j];
+      db.ot_size[base + j] = sz;
+      hist[base + j] = sz;
+    }
+}
+bool otp_plan(size_t n2, size_t ncls, OtpPlan* p) {
+  // LDS per workgroup: nr words in the tile kernels, 1 << bits words in
+  // the range kernel (both within 64 KiB)
+  int bits = 12;
+  if ((n2 >> bits) >= OTP_MAX_RANGES) bits = 13;
+  const size_t nr = (n2 + ((size_t)1 << bits) - 1) >> bits;
+  if (nr > OTP_MAX_RANGES || ncls == 0) return false;
+  size_t ct = 1024;
+  while (ct < 8 * nr) ct <<= 1;
+  p->bits = bits;
+  p->nr = (u32)nr;
+  p->ct = (u32)ct;
+  p->nblk = (u32)((ncls + ct - 1) / ct);
+  p->ncounts = (size_t)p->nr * p->nblk;
+  return true;
+}
+// counts: p.ncounts + 1 words; plit, pref: one word per live occurrence
+// (pref null: hist only); tmp: scan_tmp_bytes(p.ncounts)
+void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, u32* plit, u32* pref, u32* hist, void* tmp,
+                 Stream* st) {
+  const size_t lds_r = (size_t)p.nr * 4, lds_b = ((size_t)4 << p.bits);
+  KLAUNCH_LDS(otp_count_k, p.nblk, ST, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts);
+  scan_u32(counts, counts, p.ncounts, counts + p.ncounts, tmp, st);
+  if (pref) {
+    KLAUNCH_LDS(otp_part_k<true>, p.nblk, ST, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, plit, pref);
+    KLAUNCH_LDS(otp_range_k<true>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts, plit,
+                pref, hist);
+  } else {
+    KLAUNCH_LDS(otp_part_k<false>, p.nblk, ST, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, plit, pref);
+    KLAUNCH_LDS(otp_range_k<false>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts,
+                plit, pref, hist);
+  }
+}
+
 constexpr u32 SORT_SMALL = 16;
 constexpr u32 SORT_BLOCK = 2048;
 // wave-aggregated append: one atomic per wavefront per counter

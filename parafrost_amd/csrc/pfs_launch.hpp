@@ -54,6 +54,23 @@ void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Strea
 // ---- occurrence table
 void k_hist(const DB& db, u32* hist, Stream*);
 void k_scatter(const DB& db, u32* cursor, Stream*);
+// the same table built by literal ranges (no global atomic per occurrence)
+constexpr u32 OTP_MAX_RANGES = 12288;
+struct OtpPlan {
+  int bits;       // literals per range: 1 << bits
+  u32 nr;         // ranges
+  u32 ct;         // clauses per tile
+  u32 nblk;       // tiles
+  size_t ncounts; // nr * nblk
+};
+// false when the literal space needs more than OTP_MAX_RANGES ranges
+bool otp_plan(size_t n2, size_t ncls, OtpPlan* p);
+// counts: p.ncounts + 1 words; plit, pref: one word per live literal
+// occurrence; pref null: only hist is written.  Otherwise writes ot_off
+// (n2 + 1 words), ot_size, ot_refs and hist (= ot_size).  tmp:
+// scan_tmp_bytes(p.ncounts)
+void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, u32* plit, u32* pref, u32* hist, void* tmp,
+                 Stream*);
 // sort the lists of the given literals (lits[0..n)) by the total order
 // classes: 3n words of scratch for the size classes; cnt: 3 device words
 // lsum[i]: sum of the clause sizes of list lits[i] (n words)
