@@ -41,13 +41,13 @@ REF_OPTIONS = ["--phases=1", "-bce"]
 
 def otp_counts(V, C):
     """Range-by-tile counts of the occurrence-table build (otp_plan in
-    pfs_kernels.hip): ranges of 4096 literals (8192 past 12288 ranges),
-    tiles of the smallest power of two >= max(1024, 8 x ranges) clauses."""
+    pfs_kernels.hip): ranges of 4096 literals (8192 past 2048 such ranges),
+    tiles of the smallest power of two >= max(4096, 32 x ranges) clauses."""
     n2 = 2 * V + 2
-    bits = 12 if (n2 >> 12) < 12288 else 13
+    bits = 12 if (n2 >> 12) < 2048 else 13
     nr = (n2 + (1 << bits) - 1) >> bits
-    ct = 1024
-    while ct < 8 * nr:
+    ct = 4096
+    while ct < 32 * nr:
         ct <<= 1
     return nr * ((C + ct - 1) // ct)
 

@@ -698,12 +698,11 @@ void Simplifier::ot_counts(bool table) {
   u32* hist = P<u32>(hist_);
   OtpPlan p;
   if (otp_plan(n2, db.ncls, &p)) {
-    mem_.ensure(otp_, (p.ncounts + 1 + 2 * (size_t)lits_end_) * 4, 0, st_);
+    mem_.ensure(otp_, (p.ncounts + 64 + 2 * (size_t)lits_end_) * 4, 0, st_);
     mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
     u32* counts = P<u32>(otp_);
-    u32* plit = counts + p.ncounts + 1;
-    u32* pref = table ? plit + lits_end_ : nullptr;
-    k_ot_ranges(db, p, (u32)n2, counts, plit, pref, hist, tmp_.p, st_);
+    u32* pairs = counts + ((p.ncounts + 1 + 63) & ~(size_t)63);  // 256-byte aligned
+    k_ot_ranges(db, p, (u32)n2, counts, pairs, table, hist, tmp_.p, st_);
     return;
   }
   fill_u32(hist, 0, n2, st_);

@@ -454,16 +454,17 @@ void k_scatter(const DB& db, u32* cursor, Stream* st) {
 // order) and places the handles with LDS cursors.  Order inside a list is
 // free, as with the per-occurrence atomics.
 constexpr int OTP_UNROLL = 4;
+constexpr int OTP_TB = 1024;  // threads per tile workgroup
 constexpr u32 OTP_NONE = 0xFFFFFFFFu;  // no clause / no literal in this slot
 // calls f(literal, handle) for every literal of the live clauses of the
 // tile [c0, c1); each thread holds OTP_UNROLL clauses at a time
 template <class F>
 __device__ inline void otp_tile(const DB& db, u32 c0, u32 c1, F f) {
-  for (u32 i = c0 + threadIdx.x; i < c1; i += OTP_UNROLL * ST) {
+  for (u32 i = c0 + threadIdx.x; i < c1; i += OTP_UNROLL * OTP_TB) {
     u32 c[OTP_UNROLL], h[OTP_UNROLL];
 #pragma unroll
     for (int q = 0; q < OTP_UNROLL; q++) {
-      const u32 ii = i + q * ST;
+      const u32 ii = i + q * OTP_TB;
       c[q] = ii < c1 ? db.c_off[ii] : OTP_NONE;
     }
 #pragma unroll
@@ -477,27 +478,30 @@ __device__ inline void otp_tile(const DB& db, u32 c0, u32 c1, F f) {
     }
   }
 }
-__global__ void __launch_bounds__(ST) otp_count_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, u32* counts) {
+__global__ void __launch_bounds__(OTP_TB) otp_count_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, u32* counts) {
   extern __shared__ u32 lc[];
-  for (u32 j = threadIdx.x; j < nr; j += ST) lc[j] = 0;
+  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) lc[j] = 0;
   __syncthreads();
   const u32 c0 = blockIdx.x * ct, c1 = min(db.ncls, c0 + ct);
   otp_tile(db, c0, c1, [&](u32 l, u32) { atomicAdd(&lc[l >> bits], 1u); });
   __syncthreads();
-  for (u32 j = threadIdx.x; j < nr; j += ST) counts[(size_t)j * nblk + blockIdx.x] = lc[j];
+  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) counts[(size_t)j * nblk + blockIdx.x] = lc[j];
 }
-// TABLE: pairs (literal, handle); otherwise literals only (pref unused)
+// TABLE: pairs (literal, handle) in one 8-byte word each (one store per
+// occurrence); otherwise the literals alone
 template <bool TABLE>
-__global__ void __launch_bounds__(ST) otp_part_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, const u32* offs,
-                                                 u32* plit, u32* pref) {
+__global__ void __launch_bounds__(OTP_TB) otp_part_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, const u32* offs,
+                                                     void* out) {
   extern __shared__ u32 lc[];
-  for (u32 j = threadIdx.x; j < nr; j += ST) lc[j] = offs[(size_t)j * nblk + blockIdx.x];
+  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) lc[j] = offs[(size_t)j * nblk + blockIdx.x];
   __syncthreads();
   const u32 c0 = blockIdx.x * ct, c1 = min(db.ncls, c0 + ct);
   otp_tile(db, c0, c1, [&](u32 l, u32 c) {
     const u32 p = atomicAdd(&lc[l >> bits], 1u);
-    plit[p] = l;
-    if (TABLE) pref[p] = c;
+    if (TABLE)
+      ((uint2*)out)[p] = make_uint2(l, c);
+    else
+      ((u32*)out)[p] = l;
   });
 }
 // one workgroup per literal range r: literals [r << bits, (r + 1) << bits)
@@ -507,8 +511,9 @@ __global__ void __launch_bounds__(ST) otp_part_k(DB db, u32 ct, int bits, u32 nr
 constexpr int OTP_LD = 8;
 template <bool TABLE>
 __global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 nblk, const u32* offs,
-                                                  const u32* total, const u32* plit, const u32* pref,
-                                                  u32* hist) {
+                                                  const u32* total, const void* in, u32* hist) {
+  const u32* plit = (const u32*)in;
+  const uint2* pair = (const uint2*)in;
   extern __shared__ u32 lc[];
   __shared__ u32 lw[NW];
   const u32 r = blockIdx.x, nl = 1u << bits, base = r << bits;
@@ -519,7 +524,8 @@ __global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 n
   for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * ST) {
     u32 l[OTP_LD];
 #pragma unroll
-    for (int q = 0; q < OTP_LD; q++) l[q] = p + q * ST < b1 ? plit[p + q * ST] : OTP_NONE;
+    for (int q = 0; q < OTP_LD; q++)
+      l[q] = p + q * ST < b1 ? (TABLE ? pair[p + q * ST].x : plit[p + q * ST]) : OTP_NONE;
 #pragma unroll
     for (int q = 0; q < OTP_LD; q++)
       if (l[q] != OTP_NONE) atomicAdd(&lc[l[q] - base], 1u);
@@ -545,13 +551,15 @@ __global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 n
   for (u32 j = threadIdx.x; j < nl; j += ST)
     if (base + j < n2) db.ot_off[base + j] = lc[j];
   if (r + 1 == gridDim.x && threadIdx.x == 0) db.ot_off[n2] = *total;
+  // every offset is read out of LDS before any cursor moves
+  __syncthreads();
   for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * ST) {
     u32 l[OTP_LD], c[OTP_LD];
 #pragma unroll
     for (int q = 0; q < OTP_LD; q++) {
-      const bool in = p + q * ST < b1;
-      l[q] = in ? plit[p + q * ST] : OTP_NONE;
-      c[q] = in ? pref[p + q * ST] : 0u;
+      const uint2 e = p + q * ST < b1 ? pair[p + q * ST] : make_uint2(OTP_NONE, 0u);
+      l[q] = e.x;
+      c[q] = e.y;
     }
 #pragma unroll
     for (int q = 0; q < OTP_LD; q++)
@@ -568,13 +576,16 @@ __global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 n
 }
 bool otp_plan(size_t n2, size_t ncls, OtpPlan* p) {
   // LDS per workgroup: nr words in the tile kernels, 1 << bits words in
-  // the range kernel (both within 64 KiB)
+  // the range kernel (both within 64 KiB); ranges of 8192 literals once
+  // ranges of 4096 would number 2048 or more
   int bits = 12;
-  if ((n2 >> bits) >= OTP_MAX_RANGES) bits = 13;
+  if ((n2 >> bits) >= 2048) bits = 13;
   const size_t nr = (n2 + ((size_t)1 << bits) - 1) >> bits;
   if (nr > OTP_MAX_RANGES || ncls == 0) return false;
-  size_t ct = 1024;
-  while (ct < 8 * nr) ct <<= 1;
+  // tiles of at least 32 clauses per range: fewer, longer segments of
+  // pairs per range (the partition's writes), fewer counts
+  size_t ct = 4096;
+  while (ct < 32 * nr) ct <<= 1;
   p->bits = bits;
   p->nr = (u32)nr;
   p->ct = (u32)ct;
@@ -582,21 +593,21 @@ bool otp_plan(size_t n2, size_t ncls, OtpPlan* p) {
   p->ncounts = (size_t)p->nr * p->nblk;
   return true;
 }
-// counts: p.ncounts + 1 words; plit, pref: one word per live occurrence
-// (pref null: hist only); tmp: scan_tmp_bytes(p.ncounts)
-void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, u32* plit, u32* pref, u32* hist, void* tmp,
-                 Stream* st) {
+// counts: p.ncounts + 1 words; pairs: 8-byte aligned, two words per live
+// occurrence (table) or one (hist only); tmp: scan_tmp_bytes(p.ncounts)
+void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, void* pairs, bool table, u32* hist,
+                 void* tmp, Stream* st) {
   const size_t lds_r = (size_t)p.nr * 4, lds_b = ((size_t)4 << p.bits);
-  KLAUNCH_LDS(otp_count_k, p.nblk, ST, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts);
+  KLAUNCH_LDS(otp_count_k, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts);
   scan_u32(counts, counts, p.ncounts, counts + p.ncounts, tmp, st);
-  if (pref) {
-    KLAUNCH_LDS(otp_part_k<true>, p.nblk, ST, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, plit, pref);
-    KLAUNCH_LDS(otp_range_k<true>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts, plit,
-                pref, hist);
+  if (table) {
+    KLAUNCH_LDS(otp_part_k<true>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, pairs);
+    KLAUNCH_LDS(otp_range_k<true>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts, pairs,
+                hist);
   } else {
-    KLAUNCH_LDS(otp_part_k<false>, p.nblk, ST, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, plit, pref);
+    KLAUNCH_LDS(otp_part_k<false>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, pairs);
     KLAUNCH_LDS(otp_range_k<false>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts,
-                plit, pref, hist);
+                pairs, hist);
   }
 }
 

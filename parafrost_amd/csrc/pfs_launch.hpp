@@ -65,12 +65,12 @@ struct OtpPlan {
 };
 // false when the literal space needs more than OTP_MAX_RANGES ranges
 bool otp_plan(size_t n2, size_t ncls, OtpPlan* p);
-// counts: p.ncounts + 1 words; plit, pref: one word per live literal
-// occurrence; pref null: only hist is written.  Otherwise writes ot_off
-// (n2 + 1 words), ot_size, ot_refs and hist (= ot_size).  tmp:
-// scan_tmp_bytes(p.ncounts)
-void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, u32* plit, u32* pref, u32* hist, void* tmp,
-                 Stream*);
+// counts: p.ncounts + 1 words; pairs: 8-byte aligned, two words per live
+// literal occurrence with 'table', else one.  Without 'table' only hist is
+// written; with it also ot_off (n2 + 1 words), ot_size and ot_refs, and
+// hist = ot_size.  tmp: scan_tmp_bytes(p.ncounts)
+void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, void* pairs, bool table, u32* hist,
+                 void* tmp, Stream*);
 // sort the lists of the given literals (lits[0..n)) by the total order
 // classes: 3n words of scratch for the size classes; cnt: 3 device words
 // lsum[i]: sum of the clause sizes of list lits[i] (n words)
