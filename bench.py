@@ -128,7 +128,7 @@ def counter_traffic(fetch_csv, write_csv, kernel, copy_bytes=256 << 20):
                 v = float(r["Counter_Value"]) * 1024.0
                 if "copyBuffer" in name or "copy" in name.lower() and "pfs::" not in name:
                     copies.append(v)
-                if name.startswith(f"pfs::{kernel}("):
+                if name.startswith((f"pfs::{kernel}(", f"void pfs::{kernel}(")):  # templates print "void"
                     vals.append(v)
         copies.sort()
         ref = statistics.median(copies[-3:]) if copies else 0.0

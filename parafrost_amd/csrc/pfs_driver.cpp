@@ -698,7 +698,13 @@ void Simplifier::ot_counts(bool table) {
   u32* hist = P<u32>(hist_);
   OtpPlan p;
   if (otp_plan(n2, db.ncls, &p)) {
-    mem_.ensure(otp_, (p.ncounts + 64 + 2 * (size_t)lits_end_) * 4, 0, st_);
+    // sized from the pool's capacity (which grows in steps of half), not
+    // its fill, so that a formula growing request by request (serving)
+    // does not reallocate it every round
+    const size_t counts_cap = (size_t)p.nr * ((std::max<size_t>(ccap_, db.ncls) + p.ct - 1) / p.ct);
+    const size_t pairs_cap = std::max<size_t>(lcap_, lits_end_);
+    if ((p.ncounts + 64 + 2 * (size_t)lits_end_) * 4 > otp_.cap)
+      mem_.ensure(otp_, (counts_cap + 64 + 2 * pairs_cap) * 4, 0, st_);
     mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
     u32* counts = P<u32>(otp_);
     u32* pairs = counts + ((p.ncounts + 1 + 63) & ~(size_t)63);  // 256-byte aligned
