@@ -81,7 +81,9 @@ struct pfs_server {
         continue;
       }
       if (budget && running_bytes) {
-        const uint64_t need = ctx_bytes(r.ctx) + 8 * (r.lits.size() + 3 * r.off.size());
+        // growth per new pool word: the two clause blocks, and the two words
+        // of the occurrence-table build's (literal, handle) pairs
+        const uint64_t need = ctx_bytes(r.ctx) + 16 * (r.lits.size() + 3 * r.off.size());
         if (running_bytes + need > budget) continue;
       }
       if (best < 0 || r.release_ms < reqs[best]->release_ms) best = (long)i;
