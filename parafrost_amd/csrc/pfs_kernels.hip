@@ -1,8 +1,12 @@
 // HIP kernels of the simplifier for CDNA4 (gfx950, wave64).
 //
 // Work mapping:
-//   * clause-parallel stages (histogram, scatter, BCP clean-up, counting,
-//     compaction) use one thread per clause;
+//   * clause-parallel stages (BCP clean-up, counting, compaction) use one
+//     thread per clause;
+//   * the occurrence table is built by literal ranges: 1024-thread
+//     workgroups per tile of clauses count and partition the occurrences
+//     by range in LDS, then one workgroup per range counts, scans and
+//     places its lists in LDS (no global atomic per occurrence);
 //   * list-parallel stages (BCP visits, list deletions) use one workgroup
 //     per occurrence list with the threads striding the list;
 //   * per-variable stages (SUB, BVE, BCE, ERE, LCVE election) use one
