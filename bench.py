@@ -4,22 +4,34 @@ A step is one simplify call with phases=1 and BCE on, i.e. one round of
 SUB + BVE (with gate detection) + BCE followed by the stopping round's ERE
 (src/cpu/simplify.cpp:184-197), on a formula already resident in HBM: the
 device copy of the input is restored (pfs_rewind, device-to-device) and
-simplified inside the timed region.  The default workload is the
-Tseitin-encoded random circuit with 2M gates (60% AND, 30% XOR, 10% ITE,
-5,000 inputs, 1,000 unit outputs, root-propagated: about 6.8M clauses).
+simplified inside the timed region.  The default workload (the headline
+`value`) is the 10M-variable heavy-tailed k-SAT formula: 40M clauses of
+length 2 + Geometric(1/2) capped at 30, variable weights rank^(-2/3).  The
+round on it is refuted by subsumption in round 0 (the units its
+strengthening derives conflict); the reference CPU simplifier stops at the
+same place.  `config.stages_run` names the stages that ran.
 
-Three more legs follow (skip with --no-heavytail / --no-batched / --no-serving):
-* heavytail: the 10M-variable heavy-tailed formula (40M clauses), time per
-  simplify call;
-* batched: 512 formulas (log-uniform 1k-500k clauses, Tseitin or
-  heavy-tailed) resident in HBM, simplified as a loop of single-formula
-  calls and by the native batch executor; formulas/s of both, per-formula
-  p50/p99;
-* serving: incremental BMC sessions under Poisson arrivals at 10-500
-  requests/s through the native request server; p50/p99 latency and
-  throughput per rate.
+The JSON line carries:
+* roofline: the dominant kernel's algorithmic bytes per launch over its
+  average launch time (HIP events on the context's stream), and
+  roofline.round: the round's algorithmic bytes B_alg (per stage, for the
+  stages that ran; see round_bytes) over the timed step;
+* cpu_baseline: the reference CPU simplifier (compiled from its sources into
+  oracle/_ref/pfref) on the same full formula, one core per run, three runs
+  side by side on three cores, median;
+* legs (skip each with its --no-... flag):
+  tseitin: the 2M-gate Tseitin circuit (6.8M clauses), same step, with its
+    round roofline, the reference's time and output compared with the GPU's;
+  heavytail_bve / heavytail_bce_ere: the heavy-tailed formula with SUB off
+    (BVE runs and refutes) and with SUB and BVE off (BCE, ERE and GC run), so
+    that every stage is timed at 10M variables;
+  handoff, ingest, els: result download and device variable map, DIMACS
+    ingestion and equivalent-literal substitution on the Tseitin workload;
+  batched: 512 formulas resident in HBM, native batch executor against a loop
+    of single calls, every output compared;
+  serving: BMC sessions under Poisson arrivals through the request server.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tseitin|uniform3|heavytail]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload heavytail|tseitin|uniform3]
 Prints one JSON line.
 """
 import argparse
@@ -39,6 +51,43 @@ OPTIONS = {"phases": 1, "bce": 1}
 REF_OPTIONS = ["--phases=1", "-bce"]
 
 
+STAGES = ["gc", "ot", "bcp", "lcve", "sort", "sub", "ve", "bce", "ere", "count"]  # pfs_stage_name order
+
+
+def round_bytes(r, V):
+    """Algorithmic bytes of one round, per stage that ran (the round's stage
+    mask), from the round's work measures (profiled step).  C, L, V at round
+    start, D = 12 C + 4 L (a clause costs a 12-byte header and 4 bytes per
+    literal, a list entry or per-variable word 4 bytes); E elected
+    variables, O_E / D_E the occurrences and distinct clause bytes of their
+    lists, O_c / D_c the same for the election's candidates, R / S the
+    resolvent and stack bytes written, D' = 12 C' + 4 L' after the round:
+      ot     2D + 24V + 4L   (histogram D + 8V, list offsets 16V, placement D + 4L)
+      bcp    D + 12V         (only when units were pending)
+      lcve   20V + 4 O_c + D_c
+      sort   8 O_E + D_E     (the elected variables' lists: read, keys, write)
+      sub    4 O_E + D_E
+      ve     2 (4 O_E + D_E) + 56 E + R + S   (decide, slot scans, apply)
+      bce    4 O_E + D_E
+      ere    4 O_E + D_E     (the stopping round)
+      count  D'
+      gc     D + D' + 4 C'
+    A round that ends early (refuted, or the stopping round) has no C', L':
+    D' is taken as D there."""
+    C, L = r["clauses_before"], r["literals_before"]
+    D = 12 * C + 4 * L
+    ran = [STAGES[s] for s in range(len(STAGES)) if (r["stages"] >> s) & 1]
+    E, OE, DE = r["elected"], r["elected_occurs"], r["elected_bytes"]
+    Oc, Dc = r["cand_occurs"], r["cand_bytes"]
+    Ca = r["clauses_after"] or C
+    Dp = (12 * r["clauses_after"] + 4 * r["literals_after"]) if r["clauses_after"] else D
+    R, S = 4 * r["resolvent_words"], 4 * r["stack_words"]
+    model = {"ot": 2 * D + 24 * V + 4 * L, "bcp": D + 12 * V, "lcve": 20 * V + 4 * Oc + Dc, "sort": 8 * OE + DE,
+             "sub": 4 * OE + DE, "ve": 2 * (4 * OE + DE) + 56 * E + R + S, "bce": 4 * OE + DE,
+             "ere": 4 * OE + DE, "count": Dp, "gc": D + Dp + 4 * Ca}
+    return {k: int(model[k]) for k in ran}
+
+
 def otp_counts(V, C):
     """Range-by-tile counts of the occurrence-table build (otp_plan in
     pfs_kernels.hip): ranges of 4096 literals (8192 past 2048 such ranges),
@@ -52,12 +101,11 @@ def otp_counts(V, C):
     return nr * ((C + ct - 1) // ct)
 
 
-def kernel_bytes(name, rounds, cnf):
+def kernel_bytes(name, rounds, V):
     """Algorithmic bytes per launch of a kernel, from the work measures of
-    the profiled step.  A clause costs a 12-byte header (offset, size,
-    status/signature) plus 4 bytes per literal; a list entry costs 4 bytes.
+    the profiled step (averaged over the rounds the kernel ran in).
 
-    clause-parallel kernels, per round (C, L, V at round start):
+    clause-parallel kernels, per round (C, L, V at round start, D = 12C + 4L):
       hist_k     D + 8V            (read every clause, write the 2V histogram)
       scatter_k  D + 4L + 8V       (read every clause and the list offsets, write one ref per literal)
       count_k    D                 (read every clause)
@@ -68,17 +116,21 @@ def kernel_bytes(name, rounds, cnf):
       otp_part_k<false>   D + 4N + 4L   (the same with literals only)
       otp_range_k<true>   12L + 24V     (read the pairs, write one ref per literal, offsets, sizes, counts)
       otp_range_k<false>  4L + 8V       (read the literals, write the counts)
+    election (O_c occurrences in the candidates' lists, D_c their distinct
+    clause bytes):
+      lcve_classify_k     23V          (histogram 8V, list sizes 8V, states 2V; scores 4V, classes V)
+      mis_step_k          4 O_c + D_c  (the first pass: every candidate's lists and clauses)
     per-elected-variable kernels (O_E list entries, D_E clause bytes of the
     elected variables' lists; elected variables share no clause):
-      sub_k, ve_decide_k, bce_k   4 O_E + D_E
-      ve_apply_k                  4 O_E + D_E + 4 E (decisions)
+      sub_k, ve_gates_k, ve_rest_k, bce_k   4 O_E + D_E
+      ve_apply_k                            4 O_E + D_E + 4 E (decisions) + R + S
     """
-    V = cnf.maxvar
     per = []
     for r in rounds:
         c, l = r["clauses_before"], r["literals_before"]
         d = 12 * c + 4 * l
         oe, de = r["elected_occurs"], r["elected_bytes"]
+        ran = {STAGES[s] for s in range(len(STAGES)) if (r["stages"] >> s) & 1}
         n = otp_counts(V, c)
         if name == "otp_count_k":
             per.append(d + 4 * n)
@@ -96,12 +148,19 @@ def kernel_bytes(name, rounds, cnf):
             per.append(d + 4 * l + 8 * V)
         elif name == "count_k":
             per.append(d)
-        elif name in ("sub_k", "ve_decide_k", "bce_k"):
-            if oe and r["clauses_after"]:  # rounds that ran SUB/VE/BCE (not the stopping round)
+        elif name == "lcve_classify_k":
+            if "lcve" in ran:
+                per.append(23 * V)
+        elif name == "mis_step_k":
+            if "lcve" in ran and r["cand_occurs"]:
+                per.append(4 * r["cand_occurs"] + r["cand_bytes"])
+        elif name in ("sub_k", "ve_gates_k", "ve_rest_k", "bce_k"):
+            stage = {"sub_k": "sub", "ve_gates_k": "ve", "ve_rest_k": "ve", "bce_k": "bce"}[name]
+            if oe and stage in ran:
                 per.append(4 * oe + de)
         elif name == "ve_apply_k":
-            if oe and r["clauses_after"]:
-                per.append(4 * oe + de + 4 * r["elected"])
+            if oe and "ve" in ran:
+                per.append(4 * oe + de + 4 * r["elected"] + 4 * r["resolvent_words"] + 4 * r["stack_words"])
         else:
             return None
     if not per or not all(per):
@@ -263,68 +322,206 @@ def bench_els(steps=3, rounds=2):
     return out
 
 
-def cpu_baseline(cnf, name, budget_clauses=7_500_000):
-    """The reference CPU simplifier (compiled from its sources into
-    oracle/_ref/pfref) on the same workload, or a bounded sample of it, on one
-    host core; falls back to the oracle restatement when the reference
-    binary is absent."""
-    from parafrost_amd import gen
-    sample = cnf
-    note = "full workload"
-    if cnf.nclauses > budget_clauses:
-        scale = budget_clauses / cnf.nclauses
-        sample = gen.workload(name, scale=scale)
-        note = f"{name} workload scaled by {scale:.3f} ({sample.nclauses} clauses)"
+def host_info():
+    """The host's core count and CPU model (lscpu / /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_reference(cnf, extra=(), runs=3, timeout=1500, keep_output=False):
+    """The reference CPU simplifier (oracle/_ref/pfref, compiled from the
+    reference's src/cpu) on the full formula's DIMACS text: its own parser
+    and root propagation, then one simplify call with the step's options.
+    'runs' processes side by side, one pinned core each; the median of their
+    simplify times.  Returns None when the binary is absent."""
+    import statistics
     ref = os.path.join(ROOT, "oracle", "_ref", "pfref")
-    if os.path.exists(ref):
-        with tempfile.TemporaryDirectory() as d:
-            p = os.path.join(d, "w.cnf")
-            # the reference runs its own parser and root propagation on the
-            # raw formula; the prepared formula has the same clauses and units
-            gen.write_dimacs(with_units(sample), p)
-            r = subprocess.run(["taskset", "-c", "0", ref, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd")]
-                               + REF_OPTIONS, capture_output=True, text=True, timeout=600)
-            line = [x for x in r.stdout.splitlines() if x.startswith("{")][-1]
-            t = json.loads(line)["total_s"]
-            tp = json.loads(line).get("parse_s")
-        return {"value": sample.nclauses / t, "unit": "clauses/s", "cores": 1, "kind": "reference",
-                "sample": note + f"; reference CPU simplifier (ParaFROST src/cpu) round time {t:.3f} s, taskset 1 core",
-                "seconds": t, "parse_seconds": tp, "parse_sample": note}
+    if not os.path.exists(ref):
+        return None
+    ncores = os.cpu_count() or 1
+    runs = max(1, min(runs, ncores))
+    d = tempfile.mkdtemp(prefix="pfref_")
+    try:
+        p = os.path.join(d, "w.cnf")
+        from parafrost_amd import gen
+        gen.write_dimacs(with_units(cnf), p)
+        procs = []
+        for i in range(runs):
+            out = os.path.join(d, f"o{i}.pfd")
+            procs.append(subprocess.Popen(["taskset", "-c", str(i), ref, p, os.path.join(d, f"i{i}.pfd"), out]
+                                          + REF_OPTIONS + list(extra), stdout=subprocess.PIPE,
+                                          stderr=subprocess.DEVNULL, text=True))
+        times, parse = [], []
+        for pr in procs:
+            so, _ = pr.communicate(timeout=timeout)
+            line = [x for x in so.splitlines() if x.startswith("{")][-1]
+            j = json.loads(line)
+            times.append(j["total_s"])
+            if j.get("parse_s") is not None:
+                parse.append(j["parse_s"])
+        from parafrost_amd import pfd
+        result = pfd.read(os.path.join(d, "o0.pfd")) if keep_output else None
+        status = pfd.read(os.path.join(d, "o0.pfd")).status
+    finally:
+        import shutil
+        shutil.rmtree(d, ignore_errors=True)
+    med = statistics.median(times)
+    return {"seconds": med, "runs_s": [round(t, 3) for t in times], "parse_seconds":
+            statistics.median(parse) if parse else None, "status": ["UNSAT", "SAT", "UNSOLVED"][status],
+            "result": result, "runs": runs}
+
+
+def cpu_baseline(cnf, name):
+    """cpu_baseline of the JSON line: the reference on the full workload
+    (three runs side by side, one core each, median), or the oracle
+    restatement on one core when the reference binary is absent."""
+    info = host_info()
+    r = cpu_reference(cnf)
+    if r is not None:
+        return {"value": cnf.nclauses / r["seconds"], "unit": "clauses/s", "cores": 1, "kind": "reference",
+                "sample": (f"full {name} workload ({cnf.nclauses} clauses); reference CPU simplifier (ParaFROST "
+                           f"src/cpu, compiled from its sources) simplify time, median of {r['runs']} runs side by "
+                           f"side, one pinned core each; the reference's own parse excluded"),
+                "seconds": round(r["seconds"], 3), "runs_s": r["runs_s"], "status": r["status"],
+                "parse_seconds": r["parse_seconds"], **info}
     from oracle import oracle
-    r = oracle.simplify(sample.maxvar, sample.offsets, sample.lits, nassigned=int(sample.trail.shape[0]), det=0,
-                        phases=1, bce_en=1)
-    return {"value": sample.nclauses / r.seconds, "unit": "clauses/s", "cores": 1, "kind": "port",
-            "sample": note + f"; oracle restatement round time {r.seconds:.3f} s", "seconds": r.seconds}
+    t = oracle.simplify(cnf.maxvar, cnf.offsets, cnf.lits, nassigned=int(cnf.trail.shape[0]), det=0, phases=1,
+                        bce_en=1)
+    return {"value": cnf.nclauses / t.seconds, "unit": "clauses/s", "cores": 1, "kind": "port",
+            "sample": f"full {name} workload; oracle restatement simplify time {t.seconds:.3f} s, one core",
+            "seconds": round(t.seconds, 3), **info}
 
 
-def bench_single(name, steps=3, warmup=1):
-    """Another single-formula workload (default: the 10M-variable heavy-tailed
-    formula, 40M clauses): ms per simplify call with inputs resident."""
-    from parafrost_amd import gen
-    from parafrost_amd.simplifier import Simplifier
-    t0 = time.time()
-    cnf = gen.workload(name)
-    s = Simplifier(0, keep_input=True, **OPTIONS)
-    s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
-    print(f"bench: {name}: {cnf.maxvar} vars, {cnf.nclauses} clauses ({time.time() - t0:.1f}s to generate "
-          "and upload)", file=sys.stderr, flush=True)
+def step_loop(s, steps, warmup):
+    """warm-up calls, then 'steps' timed simplify calls on the resident input
+    (device copy restored inside the timed region)."""
     status = None
     for _ in range(warmup):
         s.rewind()
         status = s.simplify()
-    t = time.perf_counter()
+    sync()
+    t0 = time.perf_counter()
     for _ in range(steps):
         s.rewind()
         status = s.simplify()
-    dt = (time.perf_counter() - t) / steps
-    r = s.result(status).rounds
-    s.close()
+    sync()
+    return status, (time.perf_counter() - t0) / steps
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def profile_step(s, cnf, dt, traffic_csv=None):
+    """One event-profiled step: per-kernel times, the rounds' work measures,
+    the round roofline and the dominant kernel's roofline."""
+    s.set_option("profilegpu", 1)
+    s.rewind()
+    st = s.simplify()
+    kt = s.kernel_times()
+    stages = s.stage_times()
+    s.set_option("profilegpu", 0)
+    rounds = s.result(st).rounds
+    V = cnf.maxvar
+    per_stage = {}
+    for r in rounds:
+        for k, v in round_bytes(r, V).items():
+            per_stage[k] = per_stage.get(k, 0) + v
+    b_alg = sum(per_stage.values())
+    stages_run = [k for k in STAGES if k in per_stage]
+    dom = max(kt.items(), key=lambda kv: kv[1][0], default=None)
+    modelled = {k: v for k, v in kt.items() if kernel_bytes(k, rounds, V)}
+    roof_kernel = dom if dom and dom[0] in modelled else max(modelled.items(), key=lambda kv: kv[1][0], default=None)
+    roof = {"bound": "hbm"}
+    if roof_kernel:
+        name, (ms, n) = roof_kernel
+        avg_s = ms / n / 1e3
+        runs_in = sum(1 for r in rounds if kernel_bytes(name, [r], V))
+        b = kernel_bytes(name, rounds, V) * runs_in / n  # bytes of the rounds it ran in, spread over its launches
+        ach = b / avg_s / 1e9
+        traffic, scales = None, None
+        if traffic_csv:
+            fcsv, wcsv = traffic_csv.split(",")
+            traffic, scales = counter_traffic(fcsv, wcsv, name)
+        roof.update({"kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic else None,
+                     "traffic_scales": scales, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "algorithmic_bytes_per_launch": int(b)})
+    roof["dominant_kernel"] = dom[0] if dom else None
+    roof["dominant_kernel_ms_per_step"] = round(dom[1][0], 3) if dom else None
+    ach_round = b_alg / dt / 1e9
+    roof["round"] = {"achieved": round(ach_round, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_round / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(b_alg),
+                     "ms_per_step": round(dt * 1e3, 3), "bytes_per_stage": per_stage}
+    mk = {}
+    for k, v in sorted(modelled.items(), key=lambda kv: -kv[1][0]):
+        runs_in = sum(1 for r in rounds if kernel_bytes(k, [r], V))
+        b = kernel_bytes(k, rounds, V) * runs_in / v[1]
+        mk[k] = {"frac": round(b / (v[0] / v[1] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "avg_launch_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
+    roof["modelled_kernels"] = mk
+    print("bench: kernels " + json.dumps({k: [round(v[0], 3), v[1]] for k, v in
+                                          sorted(kt.items(), key=lambda kv: -kv[1][0])[:16]}),
+          file=sys.stderr, flush=True)
+    print("bench: stages " + json.dumps({k: [round(v[0], 3), v[1]] for k, v in stages.items()}),
+          file=sys.stderr, flush=True)
+    return roof, stages_run, rounds, kt
+
+
+def leg(cnf, name, opts, steps=3, warmup=1, extra_ref=(), ref=False, compare_ref=False):
+    """A single-formula leg: ms per step with inputs resident, its round
+    roofline, the stages that ran and (with ref) the reference's time on the
+    same formula and options; with compare_ref the GPU result is compared
+    with the reference's output (status, clauses, units as a set, stack,
+    variable states)."""
+    import numpy as np
+    from parafrost_amd.simplifier import Simplifier
+    s = Simplifier(0, keep_input=True, **opts)
+    try:
+        s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
+        status, dt = step_loop(s, steps, warmup)
+        roof, stages_run, rounds, _ = profile_step(s, cnf, dt)
+        res = s.result(status)
+    except Exception:
+        s.close()
+        raise
     out = {"workload": name, "variables": cnf.maxvar, "clauses": cnf.nclauses, "literals": cnf.nlits,
-           "ms_per_step": round(dt * 1e3, 3), "clauses_per_s": round(cnf.nclauses / dt, 1),
-           "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "rounds": len(r),
-           "sub_passes": [x["sub_passes"] for x in r]}
-    print("bench: " + json.dumps(out), file=sys.stderr, flush=True)
-    return out
+           "options": " ".join(f"{k}={v}" for k, v in opts.items()), "ms_per_step": round(dt * 1e3, 3),
+           "clauses_per_s": round(cnf.nclauses / dt, 1), "round_result": ["UNSAT", "SAT", "UNSOLVED"][status],
+           "rounds": len(rounds), "elected": [x["elected"] for x in rounds],
+           "sub_passes": [x["sub_passes"] for x in rounds], "stages_run": stages_run,
+           "round_roofline": roof["round"], "dominant_kernel": roof.get("dominant_kernel")}
+    if ref:
+        r = cpu_reference(cnf, extra_ref, runs=1, keep_output=compare_ref)
+        if r is not None:
+            out["cpu_reference_s"] = round(r["seconds"], 3)
+            out["cpu_reference_parse_s"] = r["parse_seconds"]
+            out["cpu_reference_status"] = r["status"]
+            out["speedup_vs_reference"] = round(r["seconds"] / dt, 1)
+            if compare_ref and r["result"] is not None:
+                o = r["result"]
+                same = res.status == o.status
+                if same and o.status != 0:
+                    same = (np.array_equal(res.sizes, o.sizes) and np.array_equal(res.lits, o.lits)
+                            and np.array_equal(res.flags, o.flags) and np.array_equal(res.resolved, o.resolved)
+                            and np.array_equal(res.vstate, o.vstate)
+                            and np.array_equal(np.sort(res.units), np.sort(o.units)))
+                out["identical_to_reference"] = bool(same)
+    print("bench: leg " + json.dumps(out), file=sys.stderr, flush=True)
+    return out, s
 
 
 def bench_handoff(s, cnf):
@@ -370,13 +567,13 @@ def bench_handoff(s, cnf):
     return out
 
 
-def bench_batched(n=512, workers=16, check=16):
+def bench_batched(n=512, workers=16):
     """Batched workload: n formulas (clause counts log-uniform in 1k-500k,
     Tseitin or heavy-tailed) resident in HBM, simplified once as a loop of
     plain single-formula calls and once by pfs_batch_simplify with 'workers'
     native threads (one stream per formula).  formulas/s of both, per-formula
     latency percentiles of the batched run, and a bit-exact comparison of
-    the first 'check' outputs."""
+    every output (and proof-free result arrays) with the loop's."""
     import numpy as np
     from parafrost_amd import gen
     from parafrost_amd.simplifier import Simplifier, batch_simplify
@@ -397,65 +594,85 @@ def bench_batched(n=512, workers=16, check=16):
     t = time.perf_counter()
     loop_res = [s.simplify() for s in ctx]
     loop_s = time.perf_counter() - t
-    ref = [ctx[i].result(loop_res[i]) for i in range(min(check, n))]
+    ref = [ctx[i].result(loop_res[i]) for i in range(n)]
     for s in ctx:
         s.rewind()
     t = time.perf_counter()
     res, ms = batch_simplify(ctx, workers)
     batch_s = time.perf_counter() - t
     same = 0
-    for i in range(min(check, n)):
+    for i in range(n):
         b = ctx[i].result(res[i])
         same += int(b.status == ref[i].status and np.array_equal(b.lits, ref[i].lits) and
                     np.array_equal(b.sizes, ref[i].sizes) and np.array_equal(b.resolved, ref[i].resolved) and
-                    np.array_equal(b.flags, ref[i].flags) and np.array_equal(b.vstate, ref[i].vstate))
+                    np.array_equal(b.flags, ref[i].flags) and np.array_equal(b.vstate, ref[i].vstate) and
+                    np.array_equal(b.units, ref[i].units))
     for s in ctx:
         s.close()
     out = {"formulas": n, "clauses": ncl, "workers": workers,
            "formulas_per_s": round(n / batch_s, 1), "loop_formulas_per_s": round(n / loop_s, 1),
            "speedup_vs_loop": round(loop_s / batch_s, 2), "clauses_per_s": round(ncl / batch_s, 1),
            "p50_ms": round(float(np.percentile(ms, 50)), 3), "p99_ms": round(float(np.percentile(ms, 99)), 3),
-           "batch_s": round(batch_s, 3), "loop_s": round(loop_s, 3), "identical_checked": f"{same}/{min(check, n)}"}
+           "batch_s": round(batch_s, 3), "loop_s": round(loop_s, 3), "identical": f"{same}/{n}"}
     print("bench: batched " + json.dumps(out), file=sys.stderr, flush=True)
     return out
 
 
-def bench_serving(rates=(10, 20, 50, 100, 200, 500), nsessions=16, nreq=64, workers=8):
-    """Serving workload: BMC sessions (64 latches, 2,000 gates, depth
-    uniform in 10-200 frames, 10 frames per request, 1-32 assumptions) whose
-    formulas stay resident; requests arrive as a Poisson stream at each rate
-    and are served by the request server (continuous batching over
-    'workers' native threads).  Latency = completion - arrival."""
+def bench_serving(rates=(10, 20, 50, 100, 200, 300, 500), nsessions=16, workers=8, requests=1000,
+                  low_rate_requests=200):
+    """Serving workload: 16 BMC sessions (64 latches, 2,000 gates, 10 frames
+    per request, 1-32 assumptions) whose formulas stay resident.  Every
+    session is first advanced to a depth drawn uniformly from 10-190 frames
+    (untimed), so that the measured requests act on formulas of every depth
+    the workload names; then requests (each session's next ones, up to its
+    200-frame bound, after which it starts over from a fresh depth) arrive
+    as a Poisson stream at each rate and are served by the request server
+    (continuous batching over 'workers' native threads).  Latency =
+    completion - arrival; 'requests' per rate (fewer at the two lowest
+    rates, which would otherwise take minutes)."""
     import numpy as np
     from parafrost_amd import bmc
     from parafrost_amd.simplifier import Server, Simplifier
     t0 = time.time()
-    sessions = bmc.workload(nsessions)
-    print(f"bench: serving {nsessions} sessions, {sum(len(x) for x in sessions)} requests available "
-          f"({time.time() - t0:.1f}s to generate)", file=sys.stderr, flush=True)
+    rng0 = np.random.default_rng(0x5EED0005)
     sweep = []
     for rate in rates:
+        nreq = low_rate_requests if rate < 50 else requests
         rng = np.random.default_rng(rate)
-        ctx = []
-        for _ in sessions:
+        # sessions: a circuit each, pre-advanced to a random depth; a session
+        # that reaches 200 frames starts over as a fresh session (a new
+        # context, kept until the rate's requests are done)
+        ctx, unroll = [], []
+        cur = list(range(nsessions))
+
+        def fresh():
             s = Simplifier(0, **OPTIONS)
             s.upload(0, np.zeros(1, np.uint64), np.zeros(0, np.uint32))
             ctx.append(s)
-        pending = [0] * nsessions
+            unroll.append(bmc.Unrolling(bmc.Circuit(int(rng0.integers(1 << 30)))))
+            return len(ctx) - 1
+
+        for i in range(nsessions):
+            k = fresh()
+            d0 = int(rng.integers(0, 20)) * 10
+            while unroll[k].frames < d0:
+                rq = unroll[k].request(10, rng)
+                ctx[k].append(rq.maxvar, rq.offsets, rq.lits)
+                ctx[k].assume(rq.assumed)
+                ctx[k].simplify()
+        # the measured requests, prepared ahead of the arrivals
         order = []
-        while len(order) < nreq:
-            live = [i for i in range(nsessions) if pending[i] < len(sessions[i])]
-            if not live:
-                break
-            si = int(rng.choice(live))
-            order.append((si, pending[si]))
-            pending[si] += 1
+        for _ in range(nreq):
+            slot = int(rng.integers(nsessions))
+            if unroll[cur[slot]].frames >= 200:
+                cur[slot] = fresh()
+            k = cur[slot]
+            order.append((k, unroll[k].request(10, rng), unroll[k].frames))
         times = bmc.poisson_schedule(len(order), rate, rate + 1)
         srv = Server(workers)
         tickets = []
-        for (si, q), rel in zip(order, times):
-            rq = sessions[si][q]
-            tickets.append(srv.submit(ctx[si], float(rel), rq.maxvar, rq.offsets, rq.lits, rq.assumed))
+        for (k, rq, _), rel in zip(order, times):
+            tickets.append(srv.submit(ctx[k], float(rel), rq.maxvar, rq.offsets, rq.lits, rq.assumed))
         srv.start()
         lat, ends = [], []
         for tk in tickets:
@@ -468,11 +685,15 @@ def bench_serving(rates=(10, 20, 50, 100, 200, 500), nsessions=16, nreq=64, work
         for s in ctx:
             s.close()
         span = max(ends) - float(times[0])
+        frames = np.array([f for _, _, f in order])
         row = {"rate_rps": rate, "requests": len(order), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-               "p99_ms": round(float(np.percentile(lat, 99)), 3),
-               "throughput_rps": round(len(order) / (span / 1e3), 1)}
+               "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(np.max(lat)), 3),
+               "throughput_rps": round(len(order) / (span / 1e3), 1),
+               "frames_min_median_max": [int(frames.min()), int(np.median(frames)), int(frames.max())],
+               "contexts": len(ctx)}
         sweep.append(row)
         print("bench: serving " + json.dumps(row), file=sys.stderr, flush=True)
+    print(f"bench: serving done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     return {"sessions": nsessions, "workers": workers, "frames_per_request": 10, "sweep": sweep}
 
 
@@ -481,7 +702,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="tseitin", choices=["tseitin", "uniform3", "heavytail"])
+    ap.add_argument("--workload", default="heavytail", choices=["heavytail", "tseitin", "uniform3"])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -489,11 +710,13 @@ def main():
     ap.add_argument("--traffic-csv", default=None, metavar="FETCH.csv,WRITE.csv",
                     help="rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files of this workload "
                          "(runs with --calibrate-copy) for roofline.traffic")
-    ap.add_argument("--no-heavytail", action="store_true", help="skip the heavy-tailed single-formula leg")
+    ap.add_argument("--no-tseitin", action="store_true", help="skip the Tseitin leg (and handoff, ingest)")
+    ap.add_argument("--no-stage-legs", action="store_true", help="skip the heavy-tailed BVE and BCE/ERE legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-ingest", action="store_true", help="skip the DIMACS ingestion leg")
     ap.add_argument("--no-els", action="store_true", help="skip the equivalent-literal substitution leg")
+    ap.add_argument("--no-legs", action="store_true", help="skip every leg (headline only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 256 MiB device copies first (reference kernel for counter runs)")
     args = ap.parse_args()
@@ -501,16 +724,17 @@ def main():
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(formulas shard by independent instances or elected variables; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.no_legs:
+        args.no_tseitin = args.no_stage_legs = args.no_batched = args.no_serving = True
+        args.no_ingest = args.no_els = True
 
     from parafrost_amd import gen
     from parafrost_amd.simplifier import Simplifier
 
     t0 = time.time()
     cnf = gen.workload(args.workload, scale=args.scale)
-    gen_s = time.time() - t0
     print(f"bench: workload {args.workload} scale {args.scale}: {cnf.maxvar} vars, {cnf.nclauses} clauses, "
-          f"{cnf.nlits} literals ({gen_s:.1f}s to generate)", file=sys.stderr, flush=True)
-
+          f"{cnf.nlits} literals ({time.time() - t0:.1f}s to generate)", file=sys.stderr, flush=True)
     if args.calibrate_copy:
         calibrate_copy()
     opts = dict(OPTIONS)
@@ -519,71 +743,36 @@ def main():
         opts[k] = int(v)
     s = Simplifier(0, keep_input=True, **opts)
     s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
-    status = None
-    for _ in range(args.warmup):
-        s.rewind()
-        status = s.simplify()
-    sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.rewind()
-        status = s.simplify()
-    sync()
-    dt = (time.perf_counter() - t0) / args.steps
+    status, dt = step_loop(s, args.steps, args.warmup)
     res = s.result(status)
     print(f"bench: {args.steps} steps, {dt * 1e3:.2f} ms/step, result {res.status}, "
           f"{res.sizes.shape[0]} clauses out, rounds {res.rounds}", file=sys.stderr, flush=True)
-
-    # one profiled step: per-kernel device time from HIP events on the
-    # context's stream
-    s.set_option("profilegpu", 1)
-    s.rewind()
-    s.simplify()
-    kt = s.kernel_times()
-    stages = s.stage_times()
-    s.set_option("profilegpu", 0)
-    rounds = s.result(status).rounds
-    dom = max(kt.items(), key=lambda kv: kv[1][0], default=None)
-    print("bench: kernels " + json.dumps({k: [round(v[0], 3), v[1]] for k, v in
-                                          sorted(kt.items(), key=lambda kv: -kv[1][0])[:12]}),
-          file=sys.stderr, flush=True)
-    print("bench: stages " + json.dumps({k: [round(v[0], 3), v[1]] for k, v in stages.items()}),
-          file=sys.stderr, flush=True)
-    hbm = max(((k, v) for k, v in kt.items() if kernel_bytes(k, rounds, cnf)), key=lambda kv: kv[1][0],
-              default=None)
-    roof_kernel = dom if dom and kernel_bytes(dom[0], rounds, cnf) else hbm
-    roofline = None
-    if roof_kernel:
-        name, (ms, n) = roof_kernel
-        avg_s = ms / n / 1e3
-        b = kernel_bytes(name, rounds, cnf)
-        ach = b / avg_s / 1e9
-        traffic, scales = None, None
-        if args.traffic_csv:
-            fcsv, wcsv = args.traffic_csv.split(",")
-            traffic, scales = counter_traffic(fcsv, wcsv, name)
-        roofline = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": int(traffic) if traffic else None, "traffic_scales": scales,
-                    "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(b),
-                    "dominant_kernel": dom[0], "dominant_kernel_ms_per_step": round(dom[1][0], 3),
-                    "modelled_kernels": {
-                        k: {"frac": round(kernel_bytes(k, rounds, cnf) / (v[0] / v[1] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                            "avg_launch_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
-                        for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0]) if kernel_bytes(k, rounds, cnf)}}
-    handoff = bench_handoff(s, cnf)
+    roofline, stages_run, rounds, _ = profile_step(s, cnf, dt, args.traffic_csv)
     s.close()
-    ingest = None if args.no_ingest else bench_ingest(cnf)
-    els_leg = None if args.no_els else bench_els()
 
-    heavy = None if (args.no_heavytail or args.workload == "heavytail") else bench_single("heavytail")
-    batched = None if args.no_batched else bench_batched()
-    serving = None if args.no_serving else bench_serving()
+    legs = {}
+    tcnf = None
+    if not args.no_tseitin:
+        tcnf = cnf if args.workload == "tseitin" and args.scale == 1.0 else gen.workload("tseitin")
+        legs["tseitin"], ts = leg(tcnf, "tseitin", OPTIONS, ref=not args.no_cpu_baseline, compare_ref=True)
+        try:
+            legs["handoff"] = bench_handoff(ts, tcnf)
+        finally:
+            ts.close()
+        if not args.no_ingest:
+            legs["ingest"] = bench_ingest(tcnf)
+    if not args.no_els:
+        legs["els"] = bench_els()
+    if not args.no_stage_legs and args.workload == "heavytail":
+        hv = cnf
+        for key, o, extra in (("heavytail_bve", {"sub": 0, "ve+": 0}, ["-no-sub", "-no-ve+"]),
+                              ("heavytail_bce_ere", {"ve": 0, "sub": 0, "ve+": 0}, ["-no-ve", "-no-sub", "-no-ve+"])):
+            legs[key], ls = leg(hv, key, dict(OPTIONS, **o), ref=False)
+            ls.close()
+    if not args.no_batched:
+        legs["batched"] = bench_batched()
+    if not args.no_serving:
+        legs["serving"] = bench_serving()
 
     base = None
     if not args.no_cpu_baseline:
@@ -591,9 +780,10 @@ def main():
             base = cpu_baseline(cnf, args.workload)
         except Exception as e:  # the baseline is reported, never required
             print(f"bench: cpu baseline failed: {e}", file=sys.stderr)
-    if ingest is not None and base and base.get("parse_seconds"):
-        ingest["cpu_reference_parse_s"] = base["parse_seconds"]
-        ingest["cpu_reference_parse_sample"] = base.get("parse_sample")
+    if base and "status" in base:
+        base["same_status_as_gpu"] = base["status"] == ["UNSAT", "SAT", "UNSOLVED"][status]
+    if "ingest" in legs and legs.get("tseitin", {}).get("cpu_reference_parse_s") is not None:
+        legs["ingest"]["cpu_reference_parse_s"] = legs["tseitin"]["cpu_reference_parse_s"]
     value = cnf.nclauses / dt
     out = {
         "metric": METRIC,
@@ -609,17 +799,15 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} (scale {args.scale})", "variables": cnf.maxvar,
-                   "clauses": cnf.nclauses, "literals": cnf.nlits, "options": "phases=1 bce=1 (SUB+BVE+BCE, ERE)",
-                   "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "parallelism": "single GPU"},
+                   "clauses": cnf.nclauses, "literals": cnf.nlits,
+                   "options": " ".join(f"{k}={v}" for k, v in opts.items()) + " (SUB+BVE+BCE, ERE)",
+                   "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "rounds": len(rounds),
+                   "stages_run": stages_run, "elected": [x["elected"] for x in rounds],
+                   "sub_passes": [x["sub_passes"] for x in rounds], "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": base,
-        "handoff": handoff,
-        "ingest": ingest,
-        "els": els_leg,
-        "heavytail": heavy,
-        "batched": batched,
-        "serving": serving,
     }
+    out.update(legs)
     print(json.dumps(out), flush=True)
 
 

@@ -51,11 +51,17 @@ typedef struct {
 typedef struct {
   uint32_t elected, eliminated, units, resolvents;
   uint64_t clauses_after, literals_after;
-  uint32_t mis_iterations, sub_passes, ere_passes, reserved;
+  uint32_t mis_iterations, sub_passes, ere_passes;
+  /* stages that ran in the round: bit s for pfs_stage_name(s) */
+  uint32_t stages;
   /* work measures, filled when "profilegpu" is on: live clauses and
    * literals at round start, live occurrences in the elected variables'
    * lists and their clause bytes (12-byte header + 4 bytes per literal) */
   uint64_t clauses_before, literals_before, elected_occurs, elected_bytes;
+  /* live occurrences in the election candidates' lists and the bytes of
+   * their distinct clauses ("profilegpu"); resolvent words (headers and
+   * literals) and eliminated-variable stack words written */
+  uint64_t cand_occurs, cand_bytes, resolvent_words, stack_words;
 } pfs_round_stats;
 
 /* Creates a context on a device.  Replaces the allocation part of the

@@ -19,14 +19,17 @@ class OracleOpts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "phases", "mu_pos", "mu_neg", "lits_min", "ve_clause_limit", "xor_max_arity",
         "sub_limit", "bce_limit", "ere_limit", "ere_max_resolvent", "lcve_min", "shrink_rate", "lbd_tier1",
-        "ve_en", "ve_plus_en", "sub_en", "bce_en", "ere_en", "all_en", "ve_lbound_en", "det", "proof_en")]
+        "ve_en", "ve_plus_en", "sub_en", "bce_en", "ere_en", "all_en", "ve_lbound_en", "det", "proof_en",
+        "lcve_max_occurs", "lcve_clause_max", "aggr_sort")]
 
 
 # Defaults of the reference CPU simplifier options (src/cpu/options.cpp:24-46).
 DEFAULTS = dict(phases=5, mu_pos=32, mu_neg=32, lits_min=500, ve_clause_limit=100, xor_max_arity=10,
                 sub_limit=3000, bce_limit=3000, ere_limit=3000, ere_max_resolvent=1000, lcve_min=1,
                 shrink_rate=2, lbd_tier1=2, ve_en=1, ve_plus_en=1, sub_en=1, bce_en=0, ere_en=1, all_en=0,
-                ve_lbound_en=0, det=1, proof_en=0)
+                ve_lbound_en=0, det=1, proof_en=0,
+                # GPU-build election limits and the aggressive output sort: off
+                lcve_max_occurs=2**31 - 1, lcve_clause_max=2**31 - 1, aggr_sort=0)
 
 
 def lib():

@@ -40,6 +40,7 @@
 #define protected public
 #define private public
 #include "solver.h"
+#include "simplify.h"
 #undef protected
 #undef private
 
@@ -308,6 +309,10 @@ int main(int argc, char** argv) {
     if (status) {
       s->occurs.clear(true), s->ot.clear(true);
       a = now_s(); s->countFinal(); s->shrinkSimp(); T("gc", now_s() - a);
+      // -aggresivesort: the order newBeginning gives the clauses it writes
+      // back (simplify.cpp:263, the reference's own comparator)
+      if (s->opts.aggr_cnf_sort && inf.unassigned && inf.nClauses)
+        std::stable_sort(s->scnf.data(), s->scnf.data() + s->scnf.size(), CNF_CMP_KEY());
     }
   }
   double total = now_s() - t0;

@@ -57,6 +57,9 @@ struct Opts {
   int32_t ve_en, ve_plus_en, sub_en, bce_en, ere_en, all_en, ve_lbound_en;
   int32_t det;
   int32_t proof_en;
+  // GPU-build election limits (electionsmax / lcveclausemax, src/gpu/lcve.cu:43-52, 94)
+  // and the aggressive output sort (src/cpu/simplify.cpp:263); INT32_MAX / 0: off
+  int32_t lcve_max_occurs, lcve_clause_max, aggr_sort;
 };
 
 // Clause record restating SCLAUSE (src/cpu/sclause.h:29-160); literals live
@@ -447,6 +450,11 @@ struct Oracle {
       }
     }
   }
+  bool oversize(const OL& l) {
+    for (u32 c : l)
+      if (!deleted(c) && cls[c].size > opts.lcve_clause_max) return true;
+    return false;
+  }
   bool LCVE() {  // lcve.cpp:46-81
     varReorder();
     PVs.clear();
@@ -459,8 +467,14 @@ struct Oracle {
       const u32 p = V2L(cand), n = NEG(p);
       const u32 psz = (u32)ot[p].size(), nsz = (u32)ot[n].size();
       if (ops[cand] == 0 && ons[cand] == 0) continue;
+      // electionsmax (the GPU build's lcve_k, lcve.cu:94): histogram counts
+      const u32 maxocc = (u32)opts.lcve_max_occurs;
+      if (ops[cand] > maxocc || ons[cand] > maxocc) break;
       const u32 pt = (u32)opts.mu_pos << mu_inc, nt = (u32)opts.mu_neg << mu_inc;
       if (psz >= pt && nsz >= nt) break;
+      // lcveclausemax (depFreeze_d, lcve.cu:43-52): a candidate with a live
+      // clause longer than the limit is passed over; its freezes are undone
+      if (opts.lcve_clause_max != INT32_MAX && (oversize(ot[p]) || oversize(ot[n]))) continue;
       PVs.push_back(cand);
       depFreeze(ot[p], cand, pt, nt);
       depFreeze(ot[n], cand, pt, nt);
@@ -1106,6 +1120,9 @@ struct Oracle {
       nClauses = n_cls_after, nLiterals = n_lits_after;
       shrinkSimp();
       if (!unassigned || !nClauses) return 1;
+      // aggresivesort (newBeginning, simplify.cpp:263): stable by CNF_CMP_KEY
+      if (opts.aggr_sort)
+        std::stable_sort(scnf.begin(), scnf.end(), [&](u32 x, u32 y) { return keyLess(x, y); });
       return 2;
     } catch (Unsat&) {
       return 0;
@@ -1133,6 +1150,7 @@ typedef struct {
   int32_t ve_en, ve_plus_en, sub_en, bce_en, ere_en, all_en, ve_lbound_en;
   int32_t det;
   int32_t proof_en;
+  int32_t lcve_max_occurs, lcve_clause_max, aggr_sort;
 } oracle_opts;
 
 // Runs the restated simplifier.  Input: maxvar, clauses as offsets/lits

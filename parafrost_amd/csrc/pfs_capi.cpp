@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/pfsimp.h"
@@ -26,7 +27,11 @@ struct pfs_ctx {
           xor_max_arity = 10, sub_limit = 3000, bce_limit = 3000, ere_limit = 3000,
           ere_max_resolvent = 1000, lcve_min = 1, shrink_rate = 2, lbd_tier1 = 2;
   bool ve = true, ve_plus = true, sub = true, bce = false, ere = true, all = false, velbound = false,
-       profile = false, work_order = true, proof = false;
+       profile = false, work_order = true, proof = false, aggr_sort = false, sync_always = false;
+  // limits of the GPU build's election (electionsmax, lcveclausemax); off by
+  // default so that the election is the CPU simplifier's
+  int32_t lcve_max_occurs = INT32_MAX, lcve_clause_max = INT32_MAX;
+  double lits_mul = 1.0;  // literalsmul: resolvent room at upload, as a multiple of the literals
   double map_perc = 0.2;  // mapperc, src/cpu/options.cpp:146
   int32_t decompose_min = 2;  // decomposemin, src/cpu/options.cpp:87
 };
@@ -60,6 +65,10 @@ Options normalise(const pfs_ctx* c) {
   o.lbd_tier1 = c->lbd_tier1;
   o.work_order = c->work_order;
   o.proof_en = c->proof;
+  o.lcve_max_occurs = c->lcve_max_occurs;
+  o.lcve_clause_max = c->lcve_clause_max;
+  o.aggr_sort = c->aggr_sort;
+  o.sync_always = c->sync_always;
   if (o.all_en) o.ve_en = 1, o.ve_plus_en = 1, o.bce_en = 1, o.ere_en = 1;
   if (!o.phases && (o.ve_en || o.sub_en || o.bce_en)) o.phases = 1;
   if (o.phases && !(o.ve_en || o.sub_en || o.bce_en)) o.phases = 0;
@@ -82,7 +91,7 @@ bool parse_int(const char* v, int32_t lo, int32_t hi, int32_t& out) {
 }
 
 template <class F>
-pfs_status guard(pfs_ctx* c, F&& f) {
+pfs_status guard_body(pfs_ctx* c, F&& f) {
   try {
     if (c && c->s) c->s->bind();  // the calling thread may be a worker of pfs_batch_simplify
     return f();
@@ -105,6 +114,12 @@ pfs_status guard(pfs_ctx* c, F&& f) {
     if (c) c->err = e.what();
     return PFS_DEVICE_ERROR;
   }
+}
+template <class F>
+pfs_status guard(pfs_ctx* c, F&& f) {
+  const pfs_status s = guard_body(c, std::forward<F>(f));
+  if (s != PFS_OK && c && c->s) c->s->on_error();
+  return s;
 }
 
 }  // namespace
@@ -162,6 +177,8 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"gcfreq", "collectfreq", &pfs_ctx::shrink_rate, 0, 5},
       {"lbdtier1", nullptr, &pfs_ctx::lbd_tier1, 1, INT32_MAX},
       {"decomposemin", nullptr, &pfs_ctx::decompose_min, 1, 10},
+      {"electionsmax", nullptr, &pfs_ctx::lcve_max_occurs, 1, INT32_MAX},
+      {"lcveclausemax", nullptr, &pfs_ctx::lcve_clause_max, 1, INT32_MAX},
   };
   for (const IntOpt& o : ints) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -183,6 +200,8 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"profilegpu", "profilesimp", &pfs_ctx::profile},
       {"workorder", nullptr, &pfs_ctx::work_order},
       {"proof", nullptr, &pfs_ctx::proof},
+      {"aggresivesort", nullptr, &pfs_ctx::aggr_sort},
+      {"syncalways", nullptr, &pfs_ctx::sync_always},
   };
   for (const BoolOpt& o : bools) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -194,6 +213,68 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       c->*(o.f) = v;
       return PFS_OK;
     }
+  }
+  // GPU-build heuristics that this build does not provide (src/gpu/options.cpp:32,
+  // src/gpu/options.cu:50): the rank-based MIS election (nondeterministic) and
+  // the function-table gates.  Off is accepted; on is refused with a reason.
+  if (n == "lcvefast" || n == "vefunction") {
+    bool v;
+    if (!parse_bool(value, v)) {
+      c->err = "bad value for option " + n;
+      return PFS_BAD_INPUT;
+    }
+    if (v) {
+      c->err = n == "lcvefast"
+                   ? "lcvefast=1 (rank-based MIS election) is not provided: the election is the deterministic "
+                     "sequential one (the lexicographically first independent set, computed in parallel)"
+                   : "vefunction=1 (function-table gates) is not provided: variable elimination detects EQU, "
+                     "AND/OR, ITE and XOR gates as the CPU simplifier does";
+      return PFS_BAD_INPUT;
+    }
+    return PFS_OK;
+  }
+  // GPU-build launch shaping and host-side options, accepted with their
+  // reference ranges and without effect here: stream count (one stream per
+  // context), the launch tuner's floors (this build sizes its launches from
+  // the work), the option the GPU kernels never read (subclausemax,
+  // ereextend), and the solver's own scheduling of simplify calls.
+  {
+    struct Range { const char* name; double lo, hi; bool boolean; };
+    static const Range noop[] = {
+        {"nstreams", 6, 32, false},        {"veminthreads", 2, 1024, false}, {"subminthreads", 2, 1024, false},
+        {"ereminthreads", 2, 1024, false}, {"veminblocks", 0, 1, false},     {"subminblocks", 0, 1, false},
+        {"ereminblocks", 0, 1, false},     {"subclausemax", 0, INT32_MAX, false},
+        {"ereextend", 0, 3, false},        {"solve", 0, 1, true},            {"sigma", 0, 1, true},
+        {"sigmalive", 0, 1, true},         {"sigmasleep", 0, 1, true},       {"sigmainc", 1, INT32_MAX, false},
+        {"sigmamin", 1, INT32_MAX, false},
+    };
+    for (const Range& r : noop) {
+      if (n != r.name) continue;
+      bool ok;
+      if (r.boolean) {
+        bool b;
+        ok = parse_bool(value, b);
+      } else {
+        char* end = nullptr;
+        const double d = value && *value ? strtod(value, &end) : -1.0;
+        ok = end && !*end && d >= r.lo && d <= r.hi;
+      }
+      if (!ok) {
+        c->err = "bad value for option " + n;
+        return PFS_BAD_INPUT;
+      }
+      return PFS_OK;
+    }
+  }
+  if (n == "literalsmul") {  // FP64R(0, 4), src/gpu/options.cpp:43
+    char* end = nullptr;
+    const double d = value && *value ? strtod(value, &end) : -1.0;
+    if (!end || *end || !(d >= 0.0 && d <= 4.0)) {
+      c->err = "bad value for option " + n;
+      return PFS_BAD_INPUT;
+    }
+    c->lits_mul = d;
+    return PFS_OK;
   }
   if (n == "mapperc") {
     char* end = nullptr;
@@ -220,6 +301,7 @@ pfs_status pfs_upload(pfs_ctx* c, uint32_t maxvar, uint64_t ncls, const uint64_t
     for (uint32_t i = 0; i < ntrail; i++)
       if ((trail[i] >> 1) == 0 || (trail[i] >> 1) > maxvar) throw std::invalid_argument("trail literal out of range");
     c->uploaded = false;
+    c->s->lits_mul = c->lits_mul;
     c->s->upload(maxvar, ncls, offsets, lits, flags, lbd, vstate, assumed, vorg, trail, ntrail);
     c->uploaded = true;
     c->simplified = false;
@@ -234,6 +316,7 @@ pfs_status pfs_parse_dimacs(pfs_ctx* c, const char* text, uint64_t nbytes, pfs_p
     c->uploaded = false;
     c->simplified = false;
     c->s->profile = c->profile;
+    c->s->lits_mul = c->lits_mul;
     const Simplifier::ParseInfo r = c->s->parse_dimacs(text, nbytes);
     c->uploaded = true;
     c->result = r.status == 0 ? PFS_RESULT_UNSAT : PFS_RESULT_UNSOLVED;
@@ -469,11 +552,15 @@ pfs_status pfs_round_stats_get(pfs_ctx* c, pfs_round_stats* out, uint32_t max, u
     out[i].mis_iterations = r[i].mis_iterations;
     out[i].sub_passes = r[i].sub_passes;
     out[i].ere_passes = r[i].ere_passes;
-    out[i].reserved = 0;
+    out[i].stages = r[i].stages;
     out[i].clauses_before = r[i].clauses_before;
     out[i].literals_before = r[i].literals_before;
     out[i].elected_occurs = r[i].elected_occurs;
     out[i].elected_bytes = r[i].elected_bytes;
+    out[i].cand_occurs = r[i].cand_occurs;
+    out[i].cand_bytes = r[i].cand_bytes;
+    out[i].resolvent_words = r[i].resolvent_words;
+    out[i].stack_words = r[i].stack_words;
   }
   if (n) *n = (uint32_t)r.size();
   return PFS_OK;

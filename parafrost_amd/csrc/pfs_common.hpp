@@ -60,6 +60,17 @@ struct Options {
   // variables heaviest-first
   int32_t work_order = 1;
   int32_t proof_en = 0;  // write binary DRAT lines (src/cpu/proof.cpp)
+  // limits of the GPU build's election (src/gpu/options.cpp:34-36, used by
+  // lcve_k, src/gpu/lcve.cu:94 and depFreeze_d :43-52): the election stops
+  // at the first candidate with more than lcve_max_occurs occurrences of a
+  // literal, and a candidate with a live clause longer than lcve_clause_max
+  // is passed over.  The CPU simplifier has neither (INT32_MAX: off).
+  int32_t lcve_max_occurs = INT32_MAX, lcve_clause_max = INT32_MAX;
+  // stable sort of the result by the list key (aggresivesort,
+  // src/cpu/simplify.cpp:263, src/gpu/cnf.cu:232-233)
+  int32_t aggr_sort = 0;
+  // synchronise the stream after every stage (syncalways, src/gpu/options.cu:33)
+  int32_t sync_always = 0;
 };
 
 // Device view of the clause database and per-variable state.  Root

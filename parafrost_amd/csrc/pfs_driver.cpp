@@ -65,7 +65,7 @@ Simplifier::~Simplifier() {
                 &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
                 &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_,
                 &map_, &map_assign_, &out_, &vrest_, &lsum_};
-  if (st_) stream_sync(st_);
+  stream_quiesce(st_);  // a destructor must not throw, also after a device error
   for (Buf* b : all) mem_.release(*b);
   stream_destroy(st_);
 }
@@ -95,11 +95,13 @@ uint32_t Simplifier::read_u32(const void* dptr) {
 
 void Simplifier::stage_begin(Stage s) {
   stage_calls[s]++;
+  if (!rounds_.empty()) rounds_.back().stages |= 1u << s;
   if (!profile) return;
   prof_stage_ = s;
   prof_ev_ = event_record(st_);
 }
 void Simplifier::stage_end() {
+  if (opt.sync_always) stream_sync(st_);  // syncalways: surface device errors at the stage that caused them
   if (!profile || !prof_ev_) return;
   Event* e = event_record(st_);
   stage_ms[prof_stage_] += event_ms(prof_ev_, e);
@@ -177,8 +179,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     }
     w += CL_HDR + n;
   }
-  ccap_ = std::max<uint64_t>(2 * ncls + 1024, 4096);
-  lcap_ = std::max<uint64_t>(2 * L + 4096, 16384);
+  set_capacities(ncls, L);
   alloc_state();
   cur_ = 0;
   ncls_ = ncls;
@@ -392,8 +393,7 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
   if (NC >= INF / 2 || L >= INF / 4) throw AllocFail{"formula too large for 32-bit clause ids"};
   info.read_clauses = NC;
   V_ = (u32)V;
-  ccap_ = std::max<uint64_t>(2 * (uint64_t)NC + 1024, 4096);
-  lcap_ = std::max<uint64_t>(2 * L + 4096, 16384);
+  set_capacities(NC, L);
   alloc_state();
   cur_ = 0;
   ncls_ = NC;
@@ -617,6 +617,19 @@ void Simplifier::grow_vars(uint32_t maxvar) {
 }
 
 // ---------------------------------------------------------------- helpers
+// Clause-block capacities at upload: the input plus room for resolvents of
+// lits_mul times its size (the reference's literalsmul, src/gpu/options.cpp:43,
+// applied at src/gpu/simplify.cu:84-92); later growth is by half.
+void Simplifier::set_capacities(uint64_t ncls, uint64_t words) {
+  const double m = std::min(std::max(lits_mul, 0.0), 4.0);
+  ccap_ = std::max<uint64_t>(ncls + (uint64_t)(m * (double)ncls) + 1024, 4096);
+  lcap_ = std::max<uint64_t>(words + (uint64_t)(m * (double)words) + 4096, 16384);
+  if (ccap_ >= INF / 2 || lcap_ >= INF / 4) {
+    ccap_ = std::min<uint64_t>(ccap_, INF / 2 - 1);
+    lcap_ = std::min<uint64_t>(lcap_, INF / 4 - 1);
+  }
+}
+
 void Simplifier::ensure_clause_room(uint64_t add_cls, uint64_t add_lits) {
   const uint64_t need_c = ncls_ + add_cls, need_l = lits_end_ + add_lits;
   if (need_c <= ccap_ && need_l <= lcap_) return;
@@ -692,19 +705,38 @@ void Simplifier::create_ot() {
   stage_end();
 }
 
+// Room for the range build: the range-by-tile counts and one 8-byte
+// (literal, handle) pair per literal slot of the pool (an upper bound of the
+// occurrences).  Grown by half when short, so that a formula growing request
+// by request (serving) does not reallocate it every round.  When the budget
+// cannot hold it, the build falls back to per-occurrence atomics, which need
+// no extra memory.
+bool Simplifier::otp_room(const OtpPlan& p) {
+  const size_t slots = lits_end_ >= CL_HDR * ncls_ ? lits_end_ - CL_HDR * ncls_ : lits_end_;
+  const size_t counts_words = ((p.ncounts + 1 + 63) & ~(size_t)63);
+  const size_t need = counts_words + 2 * slots + 64;
+  if (need * 4 <= otp_.cap) return true;
+  const size_t counts_cap = (size_t)p.nr * ((std::max<size_t>(ccap_, ncls_) + p.ct - 1) / p.ct);
+  const size_t want = ((counts_cap + 1 + 63) & ~(size_t)63) + 2 * (slots + slots / 2) + 64;
+  try {
+    mem_.ensure(otp_, want * 4, 0, st_);
+    return true;
+  } catch (AllocFail&) {
+  }
+  try {
+    mem_.ensure(otp_, need * 4, 0, st_);
+    return true;
+  } catch (AllocFail&) {
+  }
+  return false;
+}
+
 void Simplifier::ot_counts(bool table) {
   const size_t n2 = 2 * (size_t)V_ + 2;
   DB db = make_db(cur_);
   u32* hist = P<u32>(hist_);
   OtpPlan p;
-  if (otp_plan(n2, db.ncls, &p)) {
-    // sized from the pool's capacity (which grows in steps of half), not
-    // its fill, so that a formula growing request by request (serving)
-    // does not reallocate it every round
-    const size_t counts_cap = (size_t)p.nr * ((std::max<size_t>(ccap_, db.ncls) + p.ct - 1) / p.ct);
-    const size_t pairs_cap = std::max<size_t>(lcap_, lits_end_);
-    if ((p.ncounts + 64 + 2 * (size_t)lits_end_) * 4 > otp_.cap)
-      mem_.ensure(otp_, (counts_cap + 64 + 2 * pairs_cap) * 4, 0, st_);
+  if (otp_plan(n2, db.ncls, &p) && otp_room(p)) {
     mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
     u32* counts = P<u32>(otp_);
     u32* pairs = counts + ((p.ncounts + 1 + 63) & ~(size_t)63);  // 256-byte aligned
@@ -805,6 +837,8 @@ bool Simplifier::lcve() {
   a.cflag = P<u8>(cflag_);
   a.pt = (u32)opt.mu_pos << mu_inc_;
   a.nt = (u32)opt.mu_neg << mu_inc_;
+  a.max_occurs = opt.lcve_max_occurs == INT32_MAX ? INF : (u32)opt.lcve_max_occurs;
+  a.clause_max = opt.lcve_clause_max == INT32_MAX ? INF : (u32)opt.lcve_clause_max;
   k_lcve_classify(db, a, st_);
   k_lcve_select(db, a, P<u32>(flag_), P<u32>(cand_), P<u32>(key_), sc, tmp_.p, st_);
   u32 h[4];
@@ -825,6 +859,7 @@ bool Simplifier::lcve() {
   fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
   k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
   fill_u32(P<u32>(state_), 0, ncand, st_);
+  if (a.clause_max != INF) k_lcve_reject(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(pos_), P<u32>(state_), st_);
   fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
   // worklists of undecided candidates (the selection buffers are free now);
   // after the first pass over all candidates, passes are queued eight at a
@@ -857,6 +892,14 @@ bool Simplifier::lcve() {
   E_ = read_u32(sc);
   rounds_.back().elected = E_;
   rounds_.back().mis_iterations = iters;
+  if (profile) {
+    uint64_t w[2];
+    k_cand_work(db, P<u32>(cand_), ncand, P<u8>(mark_), (u64*)(P<u32>(scal_) + 40), st_);
+    d2h(w, P<u32>(scal_) + 40, sizeof(w), st_);
+    stream_sync(st_);
+    rounds_.back().cand_occurs = w[0];
+    rounds_.back().cand_bytes = w[1];
+  }
   stage_end();
   return E_ >= (u32)opt.lcve_min;
 }
@@ -1164,6 +1207,8 @@ bool Simplifier::ve() {
   unassigned_ -= elim;
   rounds_.back().eliminated = elim;
   rounds_.back().resolvents = tot[0];
+  rounds_.back().resolvent_words += tot[1];
+  rounds_.back().stack_words += tot[2];
   rounds_.back().units += (u32)fresh.size();
   stage_end();
   return ok;
@@ -1183,6 +1228,7 @@ void Simplifier::bce() {
     k_bce(db, opt, P<u32>(pv_), P<u32>(type_), E_, nullptr, P<u32>(stkb_), P<u32>(stack_) + stack_n_, false,
           work_order(), st_);
     stack_n_ += T;
+    rounds_.back().stack_words += T;
   }
   stream_sync(st_);
   stage_end();
@@ -1492,7 +1538,21 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
   gc();
   nC = ncls_;  // the compaction keeps exactly the live clauses
   if (!unassigned_ || !nC) return 1;
+  if (opt.aggr_sort) aggr_sort();
   return 2;
+}
+
+// aggresivesort: the result's clauses in the stable order of the list key
+// (src/cpu/simplify.cpp:263).  The sorted handles are compacted into the
+// other block, so handles grow with the clause order again.
+void Simplifier::aggr_sort() {
+  if (ncls_ < 2) return;
+  stage_begin(S_GC);
+  mem_.ensure(tmp2_, ncls_ * 4, 0, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, radix_tmp_bytes(ncls_)), 0, st_);
+  k_aggr_sort(make_db(cur_), P<u32>(tmp2_), tmp_.p, st_);
+  stage_end();
+  gc();
 }
 
 // ---------------------------------------------------------------- ELS

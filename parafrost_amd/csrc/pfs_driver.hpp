@@ -49,6 +49,10 @@ struct RoundStats {
   // work measures (filled when profiling): live clauses/literals at round
   // start, occurrences and clause bytes in the elected variables' lists
   uint64_t clauses_before = 0, literals_before = 0, elected_occurs = 0, elected_bytes = 0;
+  uint32_t stages = 0;  // bit s: stage s (enum Stage) ran in this round
+  // more work measures (profiling): occurrences and distinct clause bytes in
+  // the election candidates' lists; resolvent and stack words written
+  uint64_t cand_occurs = 0, cand_bytes = 0, resolvent_words = 0, stack_words = 0;
 };
 
 // per-stage device time (ms) and launches, when profiling is enabled
@@ -85,6 +89,10 @@ class Simplifier {
   void download_trail(uint32_t* lits);  // the root trail of the resident formula
   void rewind();                    // restore the last uploaded input (device copy)
   void bind() { stream_bind(st_); }  // select this context's device on the calling thread
+  // after an exception: wait for the stream and drop staged reads whose
+  // destinations belonged to the unwound call
+  void on_error() { stream_quiesce(st_); }
+  double lits_mul = 1.0;  // resolvent room at upload, as a multiple of the input (literalsmul)
   // incremental use: add clauses / replace the assumed variables of the
   // resident formula (the state after the last simplify)
   void append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits);
@@ -129,10 +137,13 @@ class Simplifier {
   void alloc_state();
   DB make_db(int which);
   void gc();
+  void aggr_sort();
   void count(uint64_t& cls, uint64_t& lits);
   void create_ot();
   // live-literal counts into hist_; with 'table' also the occurrence table
   void ot_counts(bool table);
+  bool otp_room(const OtpPlan& p);  // pair buffer of the range build; false: use the atomics
+  void set_capacities(uint64_t ncls, uint64_t words);
   bool prop();
   bool prop_visit(uint64_t front);  // propagation from trail[front..): assignments only
   void prop_clean(uint64_t from);   // satisfied clauses deleted, false literals dropped

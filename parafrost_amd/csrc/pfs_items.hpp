@@ -393,6 +393,20 @@ PF_HD void bcp_clean(const DB& db, u32 c) {
 // pw[u] packs the candidate position of variable u and its state
 // (position << 2 | state; NOPOS for variables that are not candidates), so
 // a neighbour costs one load; state[] holds the same states by position.
+// lcveclausemax (the GPU build's depFreeze_d, src/gpu/lcve.cu:43-52): a
+// candidate with a live clause longer than the limit is passed over (it
+// elects nothing and freezes nothing; the reference's partial freezes of a
+// passed-over candidate's positive list are not reproduced)
+PF_HD bool lcve_oversize(const DB& db, u32 v, u32 limit) {
+  for (u32 s = 0; s < 2; s++) {
+    const OLV ol = olist(db, V2L(v) | s);
+    for (u32 k = 0; k < ol.n; k++) {
+      const u32 c = ol.r[k];
+      if (!is_deleted(db, c) && csize(db, c) > limit) return true;
+    }
+  }
+  return false;
+}
 PF_HD void mis_decide(u32* state, u32* pw, u32 i, u32 v, u32 s) {
   state[i] = s;
   pw[v] = (i << 2) | s;

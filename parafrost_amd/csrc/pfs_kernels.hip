@@ -131,14 +131,32 @@ void stream_destroy(Stream* st) {
     (void)hipEventDestroy(r.b);
   }
   (void)hipStreamSynchronize(st->s);
-  pin_flush(st);
+  // staged reads still pending here belong to a call that ended with an
+  // error: their destinations may be gone, so they are dropped
+  st->pend.clear();
   if (st->pin) (void)hipHostFree(st->pin);
   (void)hipStreamDestroy(st->s);
   delete st;
 }
 void stream_sync(Stream* st) {
-  PFS_HIP(hipStreamSynchronize(st->s));
+  const hipError_t e = hipStreamSynchronize(st->s);
+  if (e != hipSuccess) {
+    // the staged reads did not complete: drop them (their destinations are
+    // locals of frames the exception unwinds), then report
+    st->pend.clear();
+    st->pin_used = 0;
+    throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " at " + __FILE__ + ":" +
+                             std::to_string(__LINE__));
+  }
   pin_flush(st);
+}
+bool stream_quiesce(Stream* st) {
+  if (!st) return true;
+  const bool ok = hipStreamSynchronize(st->s) == hipSuccess;
+  if (!ok) (void)hipGetLastError();
+  st->pend.clear();
+  st->pin_used = 0;
+  return ok;
 }
 void* dev_alloc(size_t bytes) {
   void* p = nullptr;
@@ -425,6 +443,32 @@ void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Strea
   if (passes & 1) {
     d2d(keys, ka, n * sizeof(u32), st);
     d2d(vals, va, n * sizeof(u32), st);
+  }
+}
+
+// Stable sort of the clause order by the list key (size, first literal, last
+// literal, signature): the reference's aggresivesort of the result
+// (src/cpu/simplify.cpp:263 with CNF_CMP_KEY, simplify.h:29-42; the GPU
+// build's stable sort with OLIST_CMP, src/gpu/cnf.cu:232-233).  Four stable
+// radix passes over the handles, least significant key first.
+__global__ void clause_key_k(const u32* pool, const u32* handles, u32 n, int which, u32* keys) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 c = handles[i];
+  const u32 size = hdr_size(pool[c]);
+  u32 k;
+  if (which == 0) k = pool[c + 1];                                // signature
+  else if (which == 1) k = size ? pool[c + CL_HDR + size - 1] : 0;  // last literal
+  else if (which == 2) k = size ? pool[c + CL_HDR] : 0;             // first literal
+  else k = size;
+  keys[i] = k;
+}
+void k_aggr_sort(const DB& db, u32* keys, void* tmp, Stream* st) {
+  const u32 n = db.ncls;
+  if (n < 2) return;
+  for (int which = 0; which < 4; which++) {
+    KLAUNCH(clause_key_k, grid_for(n, 256), 256, st, db.lits, db.c_off, n, which, keys);
+    radix_sort_pairs(keys, db.c_off, n, which == 3 ? 24 : 32, tmp, st);
   }
 }
 
@@ -846,17 +890,40 @@ void k_set_vstate(const DB& db, const u32* lits, u32 n, u8 state, Stream* st) {
 }
 
 // ---------------------------------------------------------------- LCVE
+__device__ unsigned long long lcve_classify_var(const DB& db, const LcveArgs& a, u32 v);
+__device__ inline unsigned long long wave_min64(unsigned long long x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long y = __shfl_xor(x, d, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+// the hard-break key is the minimum over the wavefront first, then one
+// atomic per wavefront (a global atomic per heavy candidate serialised on
+// one word: 1 ms on the 10M-variable heavy-tailed formula)
 __global__ void lcve_classify_k(DB db, LcveArgs a) {
   const u32 v = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  if (v > db.nvars) return;
+  unsigned long long hk = ~0ull;
+  if (v <= db.nvars) hk = lcve_classify_var(db, a, v);
+  hk = wave_min64(hk);
+  if ((threadIdx.x & 63) == 0 && hk != ~0ull) at_min64(a.hard, hk);
+}
+__device__ unsigned long long lcve_classify_var(const DB& db, const LcveArgs& a, u32 v) {
   const u32 ps = a.hist[V2L(v)], ns = a.hist[V2L(v) | 1];
   const u32 score = ps * ns;  // 32-bit product as in the reference
   a.score[v] = score;
   const bool cand = !db.assumed[v] && !db.vstate[v] && (ps || ns);
-  const bool brk = db.ot_size[V2L(v)] >= a.pt && db.ot_size[V2L(v) | 1] >= a.nt;
+  // break rules (lcve.cpp:61-66; the GPU build's electionsmax, lcve.cu:94)
+  const bool brk = (db.ot_size[V2L(v)] >= a.pt && db.ot_size[V2L(v) | 1] >= a.nt) || ps > a.max_occurs ||
+                   ns > a.max_occurs;
+  // a breaker that no neighbour can freeze (the freeze rule spares variables
+  // at or above both temperatures) ends the election wherever it stands
   const bool hard = brk && ps >= a.pt && ns >= a.nt;
-  if (cand && hard) at_min64(a.hard, ((u64)score << 32) | v);
-  a.cflag[v] = cand ? (brk ? 3 : 1) : 0;
+  u8 f = cand ? (brk ? 3 : 1) : 0;
+  if (f == 1 && a.clause_max != 0xFFFFFFFFu && lcve_oversize(db, v, a.clause_max)) f = 4;
+  a.cflag[v] = f;
+  return cand && hard ? ((unsigned long long)score << 32) | v : ~0ull;
 }
 void k_lcve_classify(const DB& db, const LcveArgs& a, Stream* st) {
   KLAUNCH(lcve_classify_k, grid_for(db.nvars, 256), 256, st, db, a);
@@ -891,6 +958,15 @@ __global__ void lcve_pos_k(const u32* cand, u32 n, u32* pos) {
 void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream* st) {
   if (!n) return;
   KLAUNCH(lcve_pos_k, grid_for(n, 256), 256, st, cand, n, pos);
+}
+__global__ void lcve_reject_k(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || cflag[cand[i]] != 4) return;
+  mis_decide(state, pos, i, cand[i], 2);
+}
+void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state, Stream* st) {
+  if (!n) return;
+  KLAUNCH(lcve_reject_k, grid_for(n, 256), 256, st, cand, n, cflag, pos, state);
 }
 // one election step over the undecided candidates idx[0..n) (all when idx
 // is null); the ones still undecided are appended to 'next'
@@ -1538,6 +1614,44 @@ __global__ void elected_work_k(DB db, const u32* pv, u32 n, unsigned long long* 
     atomicAdd(&out[0], occ);
     atomicAdd(&out[1], bytes);
   }
+}
+// occurrences in the election candidates' lists (O_c) with their clauses
+// marked (per handle), then the bytes of the distinct marked clauses (D_c);
+// the marks are cleared again.  Profiling only.
+__global__ void cand_occ_k(DB db, const u32* cand, u32 n, u8* mark, unsigned long long* out) {
+  unsigned long long occ = 0;
+  for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (u32 s = 0; s < 2; s++) {
+      const OLV ol = olist(db, V2L(cand[k]) | s);
+      for (u32 i = 0; i < ol.n; i++) {
+        const u32 c = ol.r[i];
+        if (st_deleted(hdr_flags(chead(db, c)[0]))) continue;
+        occ++;
+        mark[c] = 1;
+      }
+    }
+  }
+  occ = wave_sum64(occ);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&out[0], occ);
+}
+__global__ void marked_bytes_k(DB db, u8* mark, unsigned long long* out) {
+  unsigned long long bytes = 0;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < db.ncls; i += gridDim.x * blockDim.x) {
+    const u32 c = db.c_off[i];
+    if (!mark[c]) continue;
+    mark[c] = 0;
+    bytes += 12 + 4ull * csize(db, c);
+  }
+  bytes = wave_sum64(bytes);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&out[1], bytes);
+}
+void k_cand_work(const DB& db, const u32* cand, u32 n, u8* mark, u64* out2, Stream* st) {
+  fill_u32((u32*)out2, 0, 4, st);
+  if (!n) return;
+  KLAUNCH(cand_occ_k, std::min(grid_for(n, ST), RED_BLOCKS), ST, st, db, cand, n, mark, (unsigned long long*)out2);
+  if (db.ncls)
+    KLAUNCH(marked_bytes_k, std::min(grid_for(db.ncls, ST), RED_BLOCKS), ST, st, db, mark,
+            (unsigned long long*)out2);
 }
 void k_elected_work(const DB& db, const u32* pv, u32 n, u64* out2, Stream* st) {
   fill_u32((u32*)out2, 0, 4, st);

@@ -18,6 +18,9 @@ struct Stream;  // a HIP stream plus reusable pinned staging memory
 Stream* stream_create(int device);
 void stream_destroy(Stream*);
 void stream_sync(Stream*);
+// waits without throwing and drops staged reads (error paths, destructors);
+// false when the stream reported an error
+bool stream_quiesce(Stream*);
 void stream_bind(Stream*);  // make the stream's device current for the calling thread
 void* dev_alloc(size_t bytes);  // throws std::bad_alloc on failure
 void dev_free(void*);
@@ -50,6 +53,9 @@ void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream*)
 size_t radix_tmp_bytes(size_t n);
 // stable sort of (key, val) pairs by the low 'bits' bits of key
 void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Stream*);
+// stable sort of db.c_off[0..ncls) by (size, first, last literal, signature);
+// keys: ncls words, tmp: radix_tmp_bytes(ncls)
+void k_aggr_sort(const DB& db, u32* keys, void* tmp, Stream*);
 
 // ---- occurrence table
 void k_hist(const DB& db, u32* hist, Stream*);
@@ -89,14 +95,19 @@ struct LcveArgs {
   const u32* hist;  // per literal
   u32* score;       // per variable
   u64* hard;        // min (score, var) over hard breakers
-  u8* cflag;        // per variable: 1 candidate, 3 soft breaker candidate, 0 none
+  u8* cflag;        // per variable: 1 candidate, 3 soft breaker candidate, 4 passed over, 0 none
   u32 pt, nt;
+  u32 max_occurs;   // electionsmax: break past this many occurrences of a literal
+  u32 clause_max;   // lcveclausemax: pass over candidates with a longer live clause
 };
 void k_lcve_classify(const DB& db, const LcveArgs& a, Stream*);
 // compact candidates with (score, var) < hard into cand/key (var order)
 void k_lcve_select(const DB& db, const LcveArgs& a, u32* flag_tmp, u32* cand, u32* key, u32* ncand,
                    void* tmp, Stream*);
 void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream*);
+// candidates passed over for an oversize clause (cflag 4) start decided and
+// not elected: they neither block nor freeze a neighbour
+void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state, Stream*);
 // one election step over the candidates idx[0..n) (all when null); blk: per
 // candidate, an undecided earlier neighbour (NOPOS before the first step);
 // the still undecided candidates are appended to next (count in *nnext)
@@ -178,6 +189,9 @@ void k_set_stamps(u32* stamp, const u32* pairs, u32 n, bool clear, Stream*);
 void k_count(const DB& db, u64* out2, Stream*);  // live clauses, live literals
 // live occurrences and clause bytes in the elected variables' lists
 void k_elected_work(const DB& db, const u32* pv, u32 n, u64* out2, Stream*);
+// O_c and D_c of the election: occurrences in the candidates' lists and the
+// bytes of their distinct clauses (mark: per handle, zero before and after)
+void k_cand_work(const DB& db, const u32* cand, u32 n, u8* mark, u64* out2, Stream*);
 // stable compaction of the live clauses into dst (ids and pool renumbered)
 void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
                Stream*);

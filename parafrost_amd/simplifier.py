@@ -43,9 +43,11 @@ class _RoundStats(ctypes.Structure):
     _fields_ = [("elected", ctypes.c_uint32), ("eliminated", ctypes.c_uint32), ("units", ctypes.c_uint32),
                 ("resolvents", ctypes.c_uint32), ("clauses_after", ctypes.c_uint64),
                 ("literals_after", ctypes.c_uint64), ("mis_iterations", ctypes.c_uint32),
-                ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("sub_passes", ctypes.c_uint32), ("ere_passes", ctypes.c_uint32), ("stages", ctypes.c_uint32),
                 ("clauses_before", ctypes.c_uint64), ("literals_before", ctypes.c_uint64),
-                ("elected_occurs", ctypes.c_uint64), ("elected_bytes", ctypes.c_uint64)]
+                ("elected_occurs", ctypes.c_uint64), ("elected_bytes", ctypes.c_uint64),
+                ("cand_occurs", ctypes.c_uint64), ("cand_bytes", ctypes.c_uint64),
+                ("resolvent_words", ctypes.c_uint64), ("stack_words", ctypes.c_uint64)]
 
 
 class _ParseInfo(ctypes.Structure):
@@ -266,7 +268,7 @@ class Simplifier:
         rs = (_RoundStats * max(nr.value, 1))()
         n = ctypes.c_uint32()
         L.pfs_round_stats_get(self._h, rs, nr.value, ctypes.byref(n))
-        rounds = [{k: getattr(rs[i], k) for k, _ in _RoundStats._fields_ if k != "reserved"} for i in range(n.value)]
+        rounds = [{k: getattr(rs[i], k) for k, _ in _RoundStats._fields_} for i in range(n.value)]
         sizes = np.diff(off).astype(np.uint32)
         npf = ctypes.c_uint64()
         self._check(L.pfs_proof_size(self._h, ctypes.byref(npf)), "pfs_proof_size")

@@ -13,7 +13,8 @@ ORACLE_KW = {"phases": "phases", "mupos": "mu_pos", "muneg": "mu_neg", "bce": "b
              "ve": "ve_en", "ve+": "ve_plus_en", "sub": "sub_en", "ere": "ere_en", "velbound": "ve_lbound_en",
              "veclausemax": "ve_clause_limit", "xormaxarity": "xor_max_arity", "submax": "sub_limit",
              "bcemax": "bce_limit", "eremax": "ere_limit", "ereclausemax": "ere_max_resolvent",
-             "lcvemin": "lcve_min", "gcfreq": "shrink_rate", "vephasemin": "lits_min"}
+             "lcvemin": "lcve_min", "gcfreq": "shrink_rate", "vephasemin": "lits_min",
+             "electionsmax": "lcve_max_occurs", "lcveclausemax": "lcve_clause_max", "aggresivesort": "aggr_sort"}
 
 
 def oracle_kw(opts):

@@ -67,3 +67,24 @@ def test_random_formulas_proof(i):
         inp = pfd.read(os.path.join(d, "i.pfd"))
     r = oracle.simplify_state(inp, det=0, proof_en=1, **oo)
     assert r.proof == ref_proof
+
+
+@pytest.mark.parametrize("i", range(0, 60, 5))
+def test_aggressive_sort(i):
+    """-aggresivesort: the output clauses in the stable order of the list key
+    (the reference's newBeginning, src/cpu/simplify.cpp:263)."""
+    cnf, flags, lbd, ro, oo = case(i)
+    if flags is not None:
+        pytest.skip("the reference parser reads original clauses only")
+    ro = dict(ro, aggresivesort=1)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "f.cnf")
+        gen.write_dimacs(cnf, p)
+        subprocess.run([REF, p, os.path.join(d, "i.pfd"), os.path.join(d, "o.pfd")] + ref_args(ro),
+                       capture_output=True, timeout=120)
+        if not os.path.exists(os.path.join(d, "i.pfd")):
+            pytest.skip("refuted by the reference parser")
+        inp, out = pfd.read(os.path.join(d, "i.pfd")), pfd.read(os.path.join(d, "o.pfd"))
+    r = oracle.simplify_state(inp, det=0, aggr_sort=1, **oo)
+    d = diff(r, out, units_as_set=False)
+    assert not d, d
