@@ -185,17 +185,54 @@ int main(int argc, char** argv) {
   s->shrinkTop(false);
   const double parse_s = now_s() - tp0;
 
-  // --- dump the simplifier input (the clauses extract() will read)
-  {
-    ClauseDump cd;
+  // --- PFREF_LEARNT=<seed>,<percent>: turn a seeded subset of the parsed
+  // clauses into learnt clauses (markLearnt, an lbd in 1..8 and a usage in
+  // 0..2), moved from orgs to learnts in their order, as the solver's
+  // inprocessing calls see them (learnt clauses next to the originals,
+  // src/cpu/simplify.cpp:148-149).  The input dump then lists the originals
+  // followed by the learnt clauses with their flags and lbd.
+  if (const char* le = getenv("PFREF_LEARNT")) {
+    uint64_t x = strtoull(le, nullptr, 10);
+    const char* comma = strchr(le, ',');
+    const uint64_t pct = comma ? strtoull(comma + 1, nullptr, 10) : 30;
+    BCNF keep, moved;
     for (uint32_t i = 0; i < s->orgs.size(); i++) {
       const C_REF r = s->orgs[i];
-      if (s->cm.deleted(r)) continue;
-      CLAUSE& c = s->cm[r];
-      cd.sizes.push_back(c.size());
-      cd.flags.push_back(0);
-      cd.lbds.push_back(0);
-      for (int k = 0; k < c.size(); k++) cd.lits.push_back(c[k]);
+      x += 0x9E3779B97F4A7C15ull;  // SplitMix64 step
+      uint64_t z = x;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      if (!s->cm.deleted(r) && z % 100 < pct) {
+        CLAUSE& c = s->cm[r];
+        c.markLearnt();
+        c.set_lbd(int(1 + (z >> 8) % 8));
+        c.set_usage(CL_ST((z >> 16) % 3));
+        moved.push(r);
+      } else {
+        keep.push(r);
+      }
+    }
+    s->orgs.clear();
+    for (uint32_t i = 0; i < keep.size(); i++) s->orgs.push(keep[i]);
+    for (uint32_t i = 0; i < moved.size(); i++) s->learnts.push(moved[i]);
+  }
+
+  // --- dump the simplifier input (the clauses extract() will read:
+  // originals, then learnt clauses)
+  {
+    ClauseDump cd;
+    for (int pass = 0; pass < 2; pass++) {
+      const BCNF& list = pass ? s->learnts : s->orgs;
+      for (uint32_t i = 0; i < list.size(); i++) {
+        const C_REF r = list[i];
+        if (s->cm.deleted(r)) continue;
+        CLAUSE& c = s->cm[r];
+        cd.sizes.push_back(c.size());
+        cd.flags.push_back(c.learnt() ? uint8_t(1 | ((c.usage() & 3) << 4)) : 0);
+        cd.lbds.push_back(c.learnt() ? c.lbd() : 0);
+        for (int k = 0; k < c.size(); k++) cd.lits.push_back(c[k]);
+      }
     }
     std::vector<uint32_t> units;
     for (uint32_t i = 0; i < s->trail.size(); i++) units.push_back(s->trail[i]);

@@ -20,6 +20,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -39,17 +40,23 @@ OPTSETS = {
 
 
 def cases():
-    yield "u3_n60", gen.uniform_ksat(60, 256, 3, seed=11)
-    yield "u3_n300", gen.uniform_ksat(300, 1278, 3, seed=12)
-    yield "u3_n2000", gen.uniform_ksat(2000, 8520, 3, seed=13)
-    yield "u3lo_n1000", gen.uniform_ksat(1000, 3000, 3, seed=14)
-    yield "u2_n400", gen.uniform_ksat(400, 500, 2, seed=15)
-    yield "ts_g300", gen.tseitin(300, 20, 3, seed=21)
-    yield "ts_g2000", gen.tseitin(2000, 60, 10, seed=22)
-    yield "ts_g4000", gen.tseitin(4000, 100, 12, seed=23)
-    yield "ht_v300", gen.heavytail(300, 1200, seed=31)
-    yield "ht_v2000", gen.heavytail(2000, 8000, seed=32)
-    yield "ht_v3000", gen.heavytail(3000, 12000, seed=33)
+    """(name, formula, learnt): learnt None, or "<seed>,<percent>" for the
+    reference harness's PFREF_LEARNT (that share of the parsed clauses handed
+    over as learnt clauses, with lbd and usage)."""
+    yield "u3_n60", gen.uniform_ksat(60, 256, 3, seed=11), None
+    yield "u3_n300", gen.uniform_ksat(300, 1278, 3, seed=12), None
+    yield "u3_n2000", gen.uniform_ksat(2000, 8520, 3, seed=13), None
+    yield "u3lo_n1000", gen.uniform_ksat(1000, 3000, 3, seed=14), None
+    yield "u2_n400", gen.uniform_ksat(400, 500, 2, seed=15), None
+    yield "ts_g300", gen.tseitin(300, 20, 3, seed=21), None
+    yield "ts_g2000", gen.tseitin(2000, 60, 10, seed=22), None
+    yield "ts_g4000", gen.tseitin(4000, 100, 12, seed=23), None
+    yield "ht_v300", gen.heavytail(300, 1200, seed=31), None
+    yield "ht_v2000", gen.heavytail(2000, 8000, seed=32), None
+    yield "ht_v3000", gen.heavytail(3000, 12000, seed=33), None
+    yield "lt_u3_n2000", gen.uniform_ksat(2000, 8520, 3, seed=13), "41,30"
+    yield "lt_ts_g4000", gen.tseitin(4000, 100, 12, seed=23), "42,30"
+    yield "lt_ht_v3000", gen.heavytail(3000, 12000, seed=33), "43,40"
 
 
 def messy_text(seed, nvars, nclauses, k=3, units=4, unsat=None, percent=False, crlf=False):
@@ -209,8 +216,9 @@ def main_els():
         env = dict(os.environ, PFREF_ELS_OUT=out)
         if rounds:
             env["PFREF_ELS_ROUNDS"] = str(rounds)
-        r = subprocess.run([REF, cnf_path, inp, os.path.join(HERE, name + ".unused.pfd")], capture_output=True,
-                           text=True, timeout=600, env=env)
+        with tempfile.TemporaryDirectory() as side:  # the simplify output of the run is not a fixture
+            r = subprocess.run([REF, cnf_path, inp, os.path.join(side, name + ".out.pfd")], capture_output=True,
+                               text=True, timeout=600, env=env)
         os.remove(cnf_path)
         entry = {"name": name, "rounds": rounds}
         if not os.path.exists(inp):
@@ -221,6 +229,8 @@ def main_els():
         entry.update({"input": name + ".in.pfd.gz", "output": name + ".els.pfd.gz"})
         line = [x for x in r.stdout.splitlines() if x.startswith("{")]
         entry["reference"] = json.loads(line[-1]) if line else None
+        if entry["reference"]:
+            entry["reference"].pop("els_s", None)  # a timing, not part of the fixture
         manifest.append(entry)
         print(name, entry["reference"])
     with open(os.path.join(HERE, "els_manifest.json"), "w") as f:
@@ -246,7 +256,7 @@ def main():
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/pfref is not built (run oracle/build_ref.sh)")
     manifest = []
-    for name, cnf in cases():
+    for name, cnf, learnt in cases():
         cnf_path = os.path.join(HERE, name + ".cnf")
         gen.write_dimacs(cnf, cnf_path)
         inp = os.path.join(HERE, name + ".in.pfd")
@@ -256,6 +266,8 @@ def main():
             drat = os.path.join(HERE, tag + ".drat")
             mp = os.path.join(HERE, tag + ".map")
             env = dict(os.environ, PFREF_MAP_OUT=mp)
+            if learnt:
+                env["PFREF_LEARNT"] = learnt
             r = subprocess.run([REF, cnf_path, inp, out] + ropts + ["-proof", f"--proofout={drat}"],
                                capture_output=True, text=True, timeout=600, env=env)
             if not os.path.exists(out):
@@ -273,6 +285,8 @@ def main():
             gz(os.path.join(HERE, tag + ".proof"))
             entry = {"name": tag, "input": name + ".in.pfd.gz", "output": tag + ".out.pfd.gz",
                      "proof": tag + ".proof.gz", "ref_options": ropts, "options": kw}
+            if learnt:
+                entry["learnt"] = learnt
             if os.path.exists(mp):
                 gz(mp)
                 entry["map"] = tag + ".map.gz"

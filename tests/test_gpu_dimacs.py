@@ -33,7 +33,7 @@ MANIFEST = load_manifest()
 
 
 def generated_text(name):
-    cnf = dict(make_golden.cases())[name]
+    cnf = {n: c for n, c, _ in make_golden.cases()}[name]
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "f.cnf")
         gen.write_dimacs(cnf, p)
