@@ -133,3 +133,46 @@ def append_clean(assign, offsets, lits):
         else:
             kept.append(cl)
     return kept, units, refuted
+
+
+def proof_digest(proof: bytes) -> str:
+    """sha256 of a binary DRAT proof in canonical form: each run of
+    consecutive unit-addition lines sorted (the parallel propagation claims
+    units level by level, so their order within a run is free).  A line ends
+    at its 0 byte: every literal's last varint byte is nonzero."""
+    import hashlib
+    h = hashlib.sha256()
+    run = []
+    for line in proof.split(b"\0")[:-1]:
+        if line[:1] == b"a" and len(line) > 1 and all(b >= 128 for b in line[1:-1]):  # one literal
+            run.append(line)
+            continue
+        for u in sorted(run):
+            h.update(u + b"\0")
+        run = []
+        h.update(line + b"\0")
+    for u in sorted(run):
+        h.update(u + b"\0")
+    return h.hexdigest()
+
+
+def result_digest(r) -> dict:
+    """sha256 digests of a simplifier result's arrays (units as a sorted
+    set, proof in canonical form) and its sizes: the form in which
+    full-size outputs are kept as fixtures."""
+    import hashlib
+
+    def h(a, dt):
+        return hashlib.sha256(np.ascontiguousarray(np.asarray(a, dt)).tobytes()).hexdigest()
+
+    out = {"status": int(r.status)}
+    if r.status != 0:
+        out.update({"clauses": int(np.asarray(r.sizes).shape[0]), "literals": int(np.asarray(r.lits).shape[0]),
+                    "stack_words": int(np.asarray(r.resolved).shape[0]), "units": int(np.asarray(r.units).shape[0]),
+                    "sizes": h(r.sizes, np.uint32), "flags": h(r.flags, np.uint8), "lbd": h(r.lbd, np.uint32),
+                    "lits": h(r.lits, np.uint32), "resolved": h(r.resolved, np.uint32),
+                    "vstate": h(r.vstate, np.uint8), "units_set": h(np.sort(np.asarray(r.units, np.uint32)), np.uint32)})
+    if getattr(r, "proof", b""):
+        out["proof_bytes"] = len(r.proof)
+        out["proof"] = proof_digest(r.proof)
+    return out

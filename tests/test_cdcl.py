@@ -1,0 +1,205 @@
+"""The test-only CDCL solver and model checker (tests/cdcl), pinned before it
+checks simplifier outputs: its answers against truth tables and known
+families, its model extension against the Python restatement, and the
+whole check against the reference simplifier's own outputs (golden
+fixtures), with planted mistakes that it must catch."""
+import itertools
+import random
+
+import numpy as np
+import pytest
+
+from parafrost_amd import gen
+from tests import cdcl
+from tests.common import GOLDEN, extend_model, load_manifest, read_golden
+
+
+def brute_sat(n, clauses):
+    for bits in itertools.product((0, 1), repeat=n):
+        if all(any(bits[(l >> 1) - 1] == (1 - (l & 1)) for l in c) for c in clauses):
+            return True
+    return False
+
+
+def flat(clauses):
+    off = np.zeros(len(clauses) + 1, np.uint64)
+    off[1:] = np.cumsum([len(c) for c in clauses])
+    lits = np.array([l for c in clauses for l in c], np.uint32)
+    return off, lits
+
+
+@pytest.mark.parametrize("seed", range(300))
+def test_truth_tables(seed):
+    """SAT/UNSAT equals a truth-table enumeration on random formulas of up
+    to 12 variables around the 3-SAT threshold (with units, binaries,
+    duplicate literals and tautologies), and every model satisfies the
+    formula."""
+    rng = random.Random(seed)
+    n = rng.randint(1, 12)
+    m = rng.randint(1, 7 * n)
+    cls = []
+    for _ in range(m):
+        k = rng.choice([1, 2, 3, 3, 3, 4])
+        cls.append([2 * rng.randint(1, n) + rng.randint(0, 1) for _ in range(k)])
+    off, lits = flat(cls)
+    st, model = cdcl.solve(n, off, lits)
+    assert st == (cdcl.SAT if brute_sat(n, cls) else cdcl.UNSAT)
+    if st == cdcl.SAT:
+        assert cdcl.verify(off, lits, model)[0] == 0
+
+
+@pytest.mark.parametrize("holes", [3, 4, 5, 6])
+def test_pigeonhole(holes):
+    """PHP(h+1, h) is unsatisfiable; PHP(h, h) is satisfiable."""
+    for p, want in ((holes + 1, cdcl.UNSAT), (holes, cdcl.SAT)):
+        var = lambda i, j: 1 + i * holes + j  # noqa: E731
+        cls = [[2 * var(i, j) for j in range(holes)] for i in range(p)]
+        for j in range(holes):
+            for a in range(p):
+                for b in range(a + 1, p):
+                    cls.append([2 * var(a, j) + 1, 2 * var(b, j) + 1])
+        off, lits = flat(cls)
+        assert cdcl.solve(p * holes, off, lits)[0] == want
+
+
+def test_assumptions():
+    """Assumptions: (a | b), (~a | c) under ~b forces a and c; under ~b, ~c
+    the formula is unsatisfiable; the solver stays reusable."""
+    off, lits = flat([[2, 4], [3, 6]])
+    st, m = cdcl.solve(3, off, lits, assumptions=[5])
+    assert st == cdcl.SAT and m[1] == 1 and m[3] == 1
+    assert cdcl.solve(3, off, lits, assumptions=[5, 7])[0] == cdcl.UNSAT
+    assert cdcl.solve(3, off, lits, units=[5, 7])[0] == cdcl.UNSAT
+
+
+def test_planted_models():
+    """Satisfiable by construction: larger random 3-SAT formulas with a
+    planted solution (each clause keeps one true literal)."""
+    rng = np.random.default_rng(5)
+    for n in (200, 2000):
+        plant = rng.integers(0, 2, n + 1)
+        cls = []
+        for _ in range(3 * n):
+            vs = rng.choice(np.arange(1, n + 1), 3, replace=False)
+            c = [2 * int(v) + int(rng.integers(0, 2)) for v in vs]
+            if not any(plant[l >> 1] == 1 - (l & 1) for l in c):
+                c[0] ^= 1
+            cls.append(c)
+        off, lits = flat(cls)
+        st, m = cdcl.solve(n, off, lits)
+        assert st == cdcl.SAT and cdcl.verify(off, lits, m)[0] == 0
+
+
+def test_extend_matches_python_restatement():
+    """cdcl_extend equals tests.common.extend_model on the reference's
+    stacks (golden fixtures) from random starting values."""
+    rng = np.random.default_rng(1)
+    n = 0
+    for e in load_manifest():
+        _, out = read_golden(e)
+        if out.status == 0 or not out.resolved.shape[0]:
+            continue
+        vals = rng.integers(0, 2, out.maxvar + 1).astype(np.uint8)
+        py = extend_model({v: bool(vals[v]) for v in range(1, out.maxvar + 1)}, out.resolved)
+        c = vals.copy()
+        cdcl.extend(c, out.resolved)
+        assert all(bool(c[v]) == py[v] for v in range(1, out.maxvar + 1)), e["name"]
+        n += 1
+    assert n > 20
+
+
+# random 3-SAT at the satisfiability threshold: beyond the test solver
+HARD = ("u3_n300.", "u3_n2000.", "lt_u3_n2000.")
+
+
+def golden_io(e):
+    """Input (its original clauses: learnt clauses are redundant and may go
+    without a stack record), output and the input's clause offsets."""
+    inp, out = read_golden(e)
+    off, lits = cdcl.original_clauses(inp.sizes, inp.flags, inp.lits)
+    inp.lits = lits
+    return inp, out, off
+
+
+@pytest.mark.parametrize("i", range(len(load_manifest())))
+def test_reference_outputs_equisatisfiable(i):
+    """The reference simplifier's own outputs pass the check: solved, the
+    model extended over its stack satisfies the input (or both sides are
+    refuted).  This pins the checker on outputs known to be right."""
+    e = load_manifest()[i]
+    if e["name"].startswith(HARD):
+        pytest.skip("random 3-SAT at the threshold: not decided by the test solver within its limit")
+    inp, out, ioff = golden_io(e)
+    how = cdcl.check_equisat(inp.maxvar, ioff, inp.lits, inp.units, out.status, out.sizes, out.lits, out.units,
+                             out.resolved)
+    assert how
+
+
+def negate_witnesses(stack):
+    out = np.array(stack, np.uint32)
+    i = out.shape[0]
+    while i > 1:
+        k = int(out[i - 1])
+        out[i - 1 - k] ^= 1
+        i -= k + 1
+    return out
+
+
+def test_planted_mistakes_are_caught():
+    """Outputs with a planted mistake fail the check: the stack left out
+    (eliminated variables keep the solver's values) or every witness
+    negated."""
+    caught = {"no_stack": 0, "negated_witnesses": 0}
+    tried = 0
+    for e in load_manifest():
+        if e["name"].startswith(HARD):
+            continue
+        inp, out, ioff = golden_io(e)
+        if out.status != 2 or out.resolved.shape[0] < 8:
+            continue
+        tried += 1
+        for kind, bad in (("no_stack", np.zeros(0, np.uint32)), ("negated_witnesses", negate_witnesses(out.resolved))):
+            try:
+                cdcl.check_equisat(inp.maxvar, ioff, inp.lits, inp.units, out.status, out.sizes, out.lits,
+                                   out.units, bad)
+            except AssertionError:
+                caught[kind] += 1
+    assert tried > 10
+    # a mistake on a record whose clause the solver's values already satisfy
+    # goes unseen, so not every fixture catches both
+    assert caught["no_stack"] >= tried // 2 and caught["negated_witnesses"] >= tried // 2, (caught, tried)
+
+
+def test_eliminated_values_matter():
+    """On every unsolved fixture with eliminated variables, the extended
+    model with the eliminated variables' values flipped falsifies the input:
+    the check depends on what extension computes."""
+    n = 0
+    for e in load_manifest():
+        if e["name"].startswith(HARD):
+            continue
+        inp, out, ioff = golden_io(e)
+        elim = np.nonzero(out.vstate == 1)[0] if out.status == 2 else []
+        if not len(elim):
+            continue
+        st, model = cdcl.solve(out.maxvar, cdcl.offsets_of(out.sizes), out.lits,
+                               np.concatenate([inp.units, out.units]))
+        if st != cdcl.SAT:
+            continue
+        cdcl.extend(model, out.resolved)
+        assert cdcl.verify(ioff, inp.lits, model, inp.units)[0] == 0
+        model[elim] ^= 1
+        assert cdcl.verify(ioff, inp.lits, model, inp.units)[0] > 0, e["name"]
+        n += 1
+    assert n > 10
+
+
+def test_unsat_output_of_satisfiable_input_is_caught():
+    """An output that claims UNSAT for a satisfiable input fails."""
+    cnf = gen.uniform_ksat(100, 300, 3, seed=3)
+    with pytest.raises(AssertionError):
+        cdcl.check_equisat(cnf.maxvar, cnf.offsets, cnf.lits, [], 0, [], [], [], [])
+
+
+def test_golden_dir_present():
+    assert GOLDEN
