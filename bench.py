@@ -52,6 +52,9 @@ REF_OPTIONS = ["--phases=1", "-bce"]
 
 
 STAGES = ["gc", "ot", "bcp", "lcve", "sort", "sub", "ve", "bce", "ere", "count"]  # pfs_stage_name order
+# kernels that only compute the work measures of a profiled step (O_c, D_c,
+# O_E, D_E): not part of a round
+MEASURE_KERNELS = ("cand_occ_k", "marked_bytes_k", "elected_work_k")
 
 
 def round_bytes(r, V):
@@ -91,12 +94,13 @@ def round_bytes(r, V):
 def otp_counts(V, C):
     """Range-by-tile counts of the occurrence-table build (otp_plan in
     pfs_kernels.hip): ranges of 4096 literals (8192 past 2048 such ranges),
-    tiles of the smallest power of two >= max(4096, 32 x ranges) clauses."""
+    tiles of the smallest power of two >= max(4096, 32 x ranges) clauses, at
+    most 16,384."""
     n2 = 2 * V + 2
     bits = 12 if (n2 >> 12) < 2048 else 13
     nr = (n2 + (1 << bits) - 1) >> bits
     ct = 4096
-    while ct < 32 * nr:
+    while ct < 32 * nr and ct < 16384:
         ct <<= 1
     return nr * ((C + ct - 1) // ct)
 
@@ -198,6 +202,61 @@ def counter_traffic(fetch_csv, write_csv, kernel, copy_bytes=256 << 20):
     if fv is None or wv is None or not fs or not ws:
         return None, None
     return fv * fs + wv * ws, {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}
+
+
+def write_counter_summary(fetch_csv, write_csv, workload, command, copy_bytes=256 << 20):
+    """profiles/<workload>_round_counters.{json,md} from the two rocprofv3
+    --pmc runs of this script with --calibrate-copy (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch): per kernel, launches and bytes per launch,
+    each counter scaled by what it reads for the 256 MiB copies."""
+    import csv
+    import statistics
+
+    def per_kernel(path, counter):
+        vals, copies = {}, []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"]
+                v = float(r["Counter_Value"]) * 1024.0
+                if "copyBuffer" in name or ("copy" in name.lower() and "pfs::" not in name):
+                    copies.append(v)
+                    continue
+                if not name.startswith(("pfs::", "void pfs::")):
+                    continue
+                short = name.split("pfs::", 1)[1].split("(", 1)[0]
+                vals.setdefault(short, []).append(v)
+        copies.sort()
+        scale = copy_bytes / statistics.median(copies[-3:]) if copies else 1.0
+        return vals, scale
+
+    fv, fs = per_kernel(fetch_csv, "FETCH_SIZE")
+    wv, ws = per_kernel(write_csv, "WRITE_SIZE")
+    kernels = {}
+    for k in sorted(set(fv) | set(wv)):
+        f = statistics.mean(fv[k]) * fs if k in fv else 0.0
+        w = statistics.mean(wv[k]) * ws if k in wv else 0.0
+        kernels[k] = {"launches": max(len(fv.get(k, [])), len(wv.get(k, []))), "fetch_bytes_per_launch": int(f),
+                      "write_bytes_per_launch": int(w), "traffic_bytes_per_launch": int(f + w)}
+    out = {"workload": workload, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "command": command,
+           "kernels": kernels}
+    base = os.path.join(ROOT, "profiles", f"{workload}_round_counters")
+    with open(base + ".json", "w") as f:
+        json.dump(out, f, indent=1)
+    rows = sorted(kernels.items(), key=lambda kv: -kv[1]["traffic_bytes_per_launch"])
+    with open(base + ".md", "w") as f:
+        f.write(f"# HBM traffic per launch: one simplify step on the {workload} workload\n\n")
+        f.write("Commands (one MI355X; one rocprofv3 pass per counter; `$REPO` is the repository root):\n\n")
+        f.write("```\n" + command + "\n```\n\n")
+        f.write(f"Calibration: the 256 MiB device copies of `--calibrate-copy` scale FETCH_SIZE by {fs:.3f} and "
+                f"WRITE_SIZE by {ws:.3f}.  Traffic = FETCH_SIZE x {fs:.3f} + WRITE_SIZE x {ws:.3f} per launch.  "
+                "bench.py reads the .json beside this file for `roofline.traffic`.\n\n")
+        f.write("| kernel | launches | fetch MB/launch | write MB/launch | traffic MB/launch |\n|---|---:|---:|---:|---:|\n")
+        for k, v in rows:
+            f.write(f"| {k} | {v['launches']} | {v['fetch_bytes_per_launch'] / 1e6:.1f} | "
+                    f"{v['write_bytes_per_launch'] / 1e6:.1f} | {v['traffic_bytes_per_launch'] / 1e6:.1f} |\n")
+    print(f"bench: wrote {base}.json/.md", file=sys.stderr, flush=True)
 
 
 def calibrate_copy(nbytes=256 << 20, reps=3):
@@ -425,13 +484,32 @@ def sync():
         pass
 
 
-def profile_step(s, cnf, dt, traffic_csv=None):
+def committed_traffic(workload, kernel):
+    """HBM bytes per launch of 'kernel' on 'workload' from the committed
+    counter summary profiles/<workload>_round_counters.json (written from two
+    rocprofv3 --pmc runs of this script, FETCH_SIZE and WRITE_SIZE, each
+    scaled by what it reads for a device copy of known size; the .md beside
+    it gives the commands).  (None, None) when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_round_counters.json")
+    try:
+        with open(path) as f:
+            j = json.load(f)
+    except OSError:
+        return None, None
+    k = j.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return k["traffic_bytes_per_launch"], {"fetch_scale": j.get("fetch_scale"), "write_scale": j.get("write_scale"),
+                                          "source": os.path.relpath(path, ROOT)}
+
+
+def profile_step(s, cnf, dt, traffic_csv=None, workload=None):
     """One event-profiled step: per-kernel times, the rounds' work measures,
     the round roofline and the dominant kernel's roofline."""
     s.set_option("profilegpu", 1)
     s.rewind()
     st = s.simplify()
-    kt = s.kernel_times()
+    kt = {k: v for k, v in s.kernel_times().items() if k not in MEASURE_KERNELS}
     stages = s.stage_times()
     s.set_option("profilegpu", 0)
     rounds = s.result(st).rounds
@@ -456,6 +534,8 @@ def profile_step(s, cnf, dt, traffic_csv=None):
         if traffic_csv:
             fcsv, wcsv = traffic_csv.split(",")
             traffic, scales = counter_traffic(fcsv, wcsv, name)
+        elif workload:
+            traffic, scales = committed_traffic(workload, name)
         roof.update({"kernel": name, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic else None,
                      "traffic_scales": scales, "avg_launch_us": round(avg_s * 1e6, 2),
@@ -719,7 +799,14 @@ def main():
     ap.add_argument("--no-legs", action="store_true", help="skip every leg (headline only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 256 MiB device copies first (reference kernel for counter runs)")
+    ap.add_argument("--write-counter-summary", default=None, metavar="FETCH.csv,WRITE.csv",
+                    help="write profiles/<workload>_round_counters.{json,md} from two counter runs and exit")
+    ap.add_argument("--counter-command", default="", help="the commands recorded in that summary")
     args = ap.parse_args()
+    if args.write_counter_summary:
+        fcsv, wcsv = args.write_counter_summary.split(",")
+        write_counter_summary(fcsv, wcsv, args.workload, args.counter_command)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(formulas shard by independent instances or elected variables; see DESIGN.md)", file=sys.stderr)
@@ -747,7 +834,8 @@ def main():
     res = s.result(status)
     print(f"bench: {args.steps} steps, {dt * 1e3:.2f} ms/step, result {res.status}, "
           f"{res.sizes.shape[0]} clauses out, rounds {res.rounds}", file=sys.stderr, flush=True)
-    roofline, stages_run, rounds, _ = profile_step(s, cnf, dt, args.traffic_csv)
+    roofline, stages_run, rounds, _ = profile_step(s, cnf, dt, args.traffic_csv,
+                                                    args.workload if args.scale == 1.0 else None)
     s.close()
 
     legs = {}

@@ -69,6 +69,38 @@ typedef struct {
 pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out);
 void pfs_destroy(pfs_ctx* ctx);
 
+/* Device arena.  Every context's device memory is sub-allocated on the host
+ * from a pool reserved once per device (the reference's cuArena pool behind
+ * cuMM::initDeviceArena, src/gpu/memory.cu:99-166): a stable region (the
+ * per-variable arrays, never moved) and a dynamic region (clause blocks,
+ * occurrence tables, temporaries) that is defragmented by sliding the
+ * blocks of idle contexts down over the holes when an allocation finds no
+ * hole (compact_gpu_dynamic and retry, memory.cu:191-219, 360-419).  So the
+ * rounds, serving requests and batch runs call neither hipMalloc nor
+ * hipFree once the pool is reserved.
+ *
+ * pfs_arena_configure: size of the default arena's first slab and the most
+ * its slabs may reach (0: grows while the device has memory); before the
+ * device's first context only (PFS_BAD_STATE afterwards).  Without it the
+ * first slab is a quarter of the free device memory (at most 64 GiB), or
+ * PFS_ARENA_MB megabytes from the environment.
+ * pfs_arena_create: a private arena of exactly 'bytes' on one device that
+ * never grows: contexts made with pfs_create_in_arena share it, and an
+ * allocation that cannot fit even after compaction fails with
+ * PFS_ALLOC_FAIL.  Destroy it after every context in it.
+ * pfs_arena_get_stats: of a private arena, or (arena null) of the default
+ * arena of 'device'. */
+typedef struct pfs_arena pfs_arena;
+typedef struct {
+  uint64_t capacity, used, peak, largest_hole, holes, slabs;
+  uint64_t compactions, moved_blocks, moved_bytes, slab_allocs, pending;
+} pfs_arena_stats;
+pfs_status pfs_arena_configure(int32_t device, uint64_t first_slab, uint64_t limit);
+pfs_status pfs_arena_create(int32_t device, uint64_t bytes, pfs_arena** out);
+void pfs_arena_destroy(pfs_arena* arena);
+pfs_status pfs_create_in_arena(const pfs_config* cfg, pfs_arena* arena, pfs_ctx** out);
+pfs_status pfs_arena_get_stats(pfs_arena* arena, int32_t device, pfs_arena_stats* out);
+
 /* Sets a simplifier option by its reference name (src/cpu/options.cpp:24-46,
  * src/gpu/options.cpp:24-43): booleans "ve", "ve+", "sub", "bce", "ere",
  * "all", "velbound", "profilegpu"; integers "phases", "mupos", "muneg",
@@ -143,7 +175,13 @@ pfs_status pfs_decompose(pfs_ctx* ctx, uint32_t max_rounds, pfs_els_info* info);
  * clause or opposite units make info.status 0 and the next pfs_simplify
  * report UNSAT.  The root trail is pfs_download_trail; the clauses,
  * vstate and sizes are readable before pfs_simplify through
- * pfs_result_sizes, pfs_download_cnf and pfs_download_vstate. */
+ * pfs_result_sizes, pfs_download_cnf and pfs_download_vstate.
+ * With option "proof" the call records DRAT lines from the text's clauses
+ * to the resident formula (pfs_download_proof; the parser's lines,
+ * dimacs.cpp:245-305, and the top-level shrink's, shrink.cpp:122): every
+ * root unit, then per text clause the deletion of a clause that is gone or
+ * the addition of what is left of a clause that lost literals with the
+ * deletion of its text, and the empty clause for a refuted text. */
 typedef struct pfs_parse_info {
   uint32_t maxvar;          /* V of the header */
   uint32_t status;          /* 0: refuted while parsing/propagating, 2: otherwise */
@@ -234,6 +272,9 @@ pfs_status pfs_download_map(pfs_ctx* ctx, uint32_t* mapped);
  * root-assigned literal becomes trail_lit or its negation by value, a
  * dropped unassigned literal becomes 0. */
 pfs_status pfs_map_lits(pfs_ctx* ctx, uint32_t* lits, uint64_t n);
+/* The binary DRAT lines of the last pfs_simplify or pfs_parse_dimacs call
+ * with option "proof" (proof.cpp: the reference's writer): a caller that
+ * keeps one proof file appends each call's lines. */
 pfs_status pfs_proof_size(pfs_ctx* ctx, uint64_t* nbytes);
 pfs_status pfs_download_proof(pfs_ctx* ctx, uint8_t* bytes);
 pfs_status pfs_round_stats_get(pfs_ctx* ctx, pfs_round_stats* out, uint32_t max, uint32_t* n);

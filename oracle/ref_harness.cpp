@@ -95,7 +95,7 @@ void write_pfd(const std::string& path, uint32_t status, uint32_t maxvar,
 // reference's own Solver::map(true) when canMap() holds, else newBeginning().
 // Dump "PFM1": applied, maxVar, vorg[0..maxVar], trail, model.lits,
 // vstate[0..maxVar], then the clause arena (orgs, then learnts): sizes,
-// flags (bit0 learnt), lbd, literals.
+// flags (bit0 learnt, bits 4-5 usage), lbd, literals.
 void write_map(const std::string& path, uint32_t applied, Solver* s) {
   FILE* f = fopen(path.c_str(), "wb");
   if (!f) { perror("pfref: open map output"); exit(2); }
@@ -119,7 +119,7 @@ void write_map(const std::string& path, uint32_t applied, Solver* s) {
     for (uint32_t i = 0; i < list.size(); i++) {
       CLAUSE& c = s->cm[list[i]];
       cd.sizes.push_back(c.size());
-      cd.flags.push_back(c.learnt() ? 1 : 0);
+      cd.flags.push_back(c.learnt() ? uint8_t(1 | ((c.usage() & 3) << 4)) : 0);
       cd.lbds.push_back(c.learnt() ? c.lbd() : 0);
       for (int k = 0; k < c.size(); k++) cd.lits.push_back(c[k]);
     }

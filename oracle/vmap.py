@@ -109,7 +109,7 @@ def map_state(maxvar, vstate, assign, sizes, flags, lbd, lits, vorg=None, trail=
     out_lits = np.concatenate([lits[off[i]:off[i + 1]] for i in order]) if order.size else np.zeros(0, np.uint32)
     return dict(applied=int(applied), maxvar=int(maxvar), vorg=vorg, trail=trail, model_lits=model_lits,
                 vstate=vstate[: maxvar + 1], sizes=out_sizes.astype(np.uint32),
-                flags=learnt[order].astype(np.uint8), lbd=np.where(learnt[order], lbd[order], 0).astype(np.uint32),
+                flags=np.where(learnt[order], 1 | (flags[order] & 0x30), 0).astype(np.uint8), lbd=np.where(learnt[order], lbd[order], 0).astype(np.uint32),
                 lits=out_lits.astype(np.uint32))
 
 

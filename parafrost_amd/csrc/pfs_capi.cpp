@@ -7,6 +7,7 @@
 #include <atomic>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -34,6 +35,12 @@ struct pfs_ctx {
   double lits_mul = 1.0;  // literalsmul: resolvent room at upload, as a multiple of the literals
   double map_perc = 0.2;  // mapperc, src/cpu/options.cpp:146
   int32_t decompose_min = 2;  // decomposemin, src/cpu/options.cpp:87
+  // launch shaping of the occurrence-table build (no reference counterpart):
+  // log2 of the clauses per tile (0: planned from the literal ranges) and
+  // the tile-to-XCD mapping (consecutive tiles on one XCD)
+  int32_t otp_tile = 0;
+  bool otp_xcd = true;
+  int32_t mis_lds = 10240;  // LDS reserved per election-step workgroup (occupancy)
 };
 
 namespace {
@@ -69,6 +76,9 @@ Options normalise(const pfs_ctx* c) {
   o.lcve_clause_max = c->lcve_clause_max;
   o.aggr_sort = c->aggr_sort;
   o.sync_always = c->sync_always;
+  o.otp_tile = c->otp_tile;
+  o.otp_xcd = c->otp_xcd;
+  o.mis_lds = c->mis_lds;
   if (o.all_en) o.ve_en = 1, o.ve_plus_en = 1, o.bce_en = 1, o.ere_en = 1;
   if (!o.phases && (o.ve_en || o.sub_en || o.bce_en)) o.phases = 1;
   if (o.phases && !(o.ve_en || o.sub_en || o.bce_en)) o.phases = 0;
@@ -93,7 +103,10 @@ bool parse_int(const char* v, int32_t lo, int32_t hi, int32_t& out) {
 template <class F>
 pfs_status guard_body(pfs_ctx* c, F&& f) {
   try {
-    if (c && c->s) c->s->bind();  // the calling thread may be a worker of pfs_batch_simplify
+    if (c && c->s) {
+      c->s->bind();  // the calling thread may be a worker of pfs_batch_simplify
+      c->s->call_begin();
+    }
     return f();
   } catch (AllocFail& e) {
     if (c) c->err = e.what;
@@ -115,10 +128,15 @@ pfs_status guard_body(pfs_ctx* c, F&& f) {
     return PFS_DEVICE_ERROR;
   }
 }
+// Every call on a context holds the context's arena client lock: the arena
+// moves a context's blocks only while that lock is free (or, at the start of
+// a call, for the caller itself).
 template <class F>
 pfs_status guard(pfs_ctx* c, F&& f) {
+  if (!c || !c->s) return guard_body(c, std::forward<F>(f));
+  std::lock_guard<std::mutex> lk(c->s->client().busy);
   const pfs_status s = guard_body(c, std::forward<F>(f));
-  if (s != PFS_OK && c && c->s) c->s->on_error();
+  if (s != PFS_OK) c->s->on_error();
   return s;
 }
 
@@ -126,7 +144,7 @@ pfs_status guard(pfs_ctx* c, F&& f) {
 
 extern "C" {
 
-pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out) {
+static pfs_status create_in(const pfs_config* cfg, DeviceArena* arena, pfs_ctx** out) {
   if (!out) return PFS_BAD_INPUT;
   *out = nullptr;
   const int dev = cfg ? cfg->device : 0;
@@ -134,10 +152,14 @@ pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out) {
     g_create_error = "no HIP device " + std::to_string(dev);
     return PFS_NO_DEVICE;
   }
+  if (arena && arena->device() != dev) {
+    g_create_error = "the arena belongs to another device";
+    return PFS_BAD_INPUT;
+  }
   pfs_ctx* c = new pfs_ctx();
   const pfs_status s = guard(c, [&] {
     const size_t budget = cfg && cfg->mem_budget ? (size_t)cfg->mem_budget : ~(size_t)0;
-    c->s.reset(new Simplifier(dev, budget, cfg ? cfg->keep_input != 0 : false));
+    c->s.reset(new Simplifier(dev, budget, cfg ? cfg->keep_input != 0 : false, arena));
     return PFS_OK;
   });
   if (s != PFS_OK) {
@@ -149,13 +171,61 @@ pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out) {
   return PFS_OK;
 }
 
+pfs_status pfs_create(const pfs_config* cfg, pfs_ctx** out) { return create_in(cfg, nullptr, out); }
+
+pfs_status pfs_create_in_arena(const pfs_config* cfg, pfs_arena* arena, pfs_ctx** out) {
+  if (!arena) return PFS_BAD_INPUT;
+  return create_in(cfg, reinterpret_cast<DeviceArena*>(arena), out);
+}
+
 void pfs_destroy(pfs_ctx* c) {
   if (!c) return;
   try {
+    if (c->s) {
+      std::lock_guard<std::mutex> lk(c->s->client().busy);
+      c->s->free_all();  // the blocks go back to the arena while no compaction can touch them
+    }
     c->s.reset();
   } catch (...) {
   }
   delete c;
+}
+
+pfs_status pfs_arena_create(int32_t device, uint64_t bytes, pfs_arena** out) {
+  if (!out || !bytes) return PFS_BAD_INPUT;
+  *out = nullptr;
+  if (pfs::device_count() <= device || device < 0) return PFS_NO_DEVICE;
+  *out = reinterpret_cast<pfs_arena*>(DeviceArena::create_private(device, bytes));
+  return PFS_OK;
+}
+
+void pfs_arena_destroy(pfs_arena* a) { delete reinterpret_cast<DeviceArena*>(a); }
+
+pfs_status pfs_arena_configure(int32_t device, uint64_t first_slab, uint64_t limit) {
+  if (device < 0) return PFS_BAD_INPUT;
+  return DeviceArena::configure(device, first_slab, limit) ? PFS_OK : PFS_BAD_STATE;
+}
+
+pfs_status pfs_arena_get_stats(pfs_arena* a, int32_t device, pfs_arena_stats* out) {
+  if (!out) return PFS_BAD_INPUT;
+  DeviceArena* ar = a ? reinterpret_cast<DeviceArena*>(a) : nullptr;
+  if (!ar) {
+    if (pfs::device_count() <= device || device < 0) return PFS_NO_DEVICE;
+    ar = &DeviceArena::get(device);
+  }
+  const ArenaStats s = ar->stats();
+  out->capacity = s.capacity;
+  out->used = s.used;
+  out->peak = s.peak;
+  out->largest_hole = s.largest_hole;
+  out->holes = s.holes;
+  out->slabs = s.slabs;
+  out->compactions = s.compactions;
+  out->moved_blocks = s.moved_blocks;
+  out->moved_bytes = s.moved_bytes;
+  out->slab_allocs = s.slab_allocs;
+  out->pending = s.pending;
+  return PFS_OK;
 }
 
 pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
@@ -179,6 +249,8 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"decomposemin", nullptr, &pfs_ctx::decompose_min, 1, 10},
       {"electionsmax", nullptr, &pfs_ctx::lcve_max_occurs, 1, INT32_MAX},
       {"lcveclausemax", nullptr, &pfs_ctx::lcve_clause_max, 1, INT32_MAX},
+      {"otptile", nullptr, &pfs_ctx::otp_tile, 0, 20},
+      {"mislds", nullptr, &pfs_ctx::mis_lds, 0, 65536},
   };
   for (const IntOpt& o : ints) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -202,6 +274,7 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"proof", nullptr, &pfs_ctx::proof},
       {"aggresivesort", nullptr, &pfs_ctx::aggr_sort},
       {"syncalways", nullptr, &pfs_ctx::sync_always},
+      {"otpxcd", nullptr, &pfs_ctx::otp_xcd},
   };
   for (const BoolOpt& o : bools) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -315,6 +388,7 @@ pfs_status pfs_parse_dimacs(pfs_ctx* c, const char* text, uint64_t nbytes, pfs_p
   return guard(c, [&] {
     c->uploaded = false;
     c->simplified = false;
+    c->s->opt = normalise(c);  // "proof" and the propagation's options
     c->s->profile = c->profile;
     c->s->lits_mul = c->lits_mul;
     const Simplifier::ParseInfo r = c->s->parse_dimacs(text, nbytes);

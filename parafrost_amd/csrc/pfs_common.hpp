@@ -71,6 +71,16 @@ struct Options {
   int32_t aggr_sort = 0;
   // synchronise the stream after every stage (syncalways, src/gpu/options.cu:33)
   int32_t sync_always = 0;
+  // occurrence-table build by literal ranges: log2 of the clauses per tile
+  // (0: planned, 32 clauses per range and tile up to 16,384 clauses) and
+  // the tile-to-XCD mapping (consecutive tiles on one XCD, so that the
+  // partition's writes to one range meet in one L2)
+  int32_t otp_tile = 0, otp_xcd = 1;
+  // LDS bytes reserved per one-wavefront workgroup of the election's steps
+  // (occupancy; on the heavy-tailed round 0 / 10 / 20 / 32 KiB gave the
+  // first pass 3.6 / 3.8 / 6.0 / 10.2 ms and the worklist passes 1.02 /
+  // 0.89 / 0.80 / 0.79 ms)
+  int32_t mis_lds = 10240;
 };
 
 // Device view of the clause database and per-variable state.  Root

@@ -8,6 +8,7 @@
 #include <iterator>
 #include <map>
 #include <new>
+#include <set>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -18,44 +19,61 @@ const char* const kStageNames[S_NSTAGES] = {"gc", "ot", "bcp", "lcve", "sort",
 static constexpr u32 INF = 0xFFFFFFFFu;
 
 // ---------------------------------------------------------------- memory
-void DeviceMemory::ensure(Buf& b, size_t bytes, size_t keep, Stream* st) {
+void DeviceMemory::ensure(Buf& b, size_t bytes, size_t keep, Stream* st, Region r) {
   if (bytes <= b.cap) return;
-  if (used_ - b.cap + bytes > budget_)
-    throw AllocFail{"device memory budget exceeded (" + std::to_string(used_ - b.cap + bytes) + " > " +
+  if (client.used - b.cap + bytes > budget_)
+    throw AllocFail{"device memory budget exceeded (" + std::to_string(client.used - b.cap + bytes) + " > " +
                     std::to_string(budget_) + " bytes)"};
-  void* p = nullptr;
+  if (b.p && arena_->grow(&client, b.p, bytes)) {  // the space after the block was free
+    b.cap = bytes;
+    peak_ = std::max(peak_, client.used);
+    return;
+  }
+  void* old = b.p;
+  const size_t old_cap = b.cap;
   try {
-    p = dev_alloc(bytes);
+    arena_->alloc(&client, bytes, r, &b.p);  // sets b.p
   } catch (std::bad_alloc&) {
-    throw AllocFail{"device allocation of " + std::to_string(bytes) + " bytes failed"};
+    b.p = old;
+    throw AllocFail{"device arena allocation of " + std::to_string(bytes) + " bytes failed"};
   }
-  if (b.p) {
-    if (keep) d2d(p, b.p, std::min(keep, b.cap), st);
-    stream_sync(st);
-    dev_free(b.p);
-    used_ -= b.cap;
+  if (old) {
+    if (keep) d2d(b.p, old, std::min(keep, old_cap), st);
+    arena_->release(&client, old);  // returns to the pool once the copy has run
   }
-  b.p = p;
   b.cap = bytes;
-  used_ += bytes;
-  peak_ = std::max(peak_, used_);
+  peak_ = std::max(peak_, client.used);
 }
 void DeviceMemory::release(Buf& b) {
-  if (b.p) {
-    dev_free(b.p);
-    used_ -= b.cap;
-  }
+  if (b.p) arena_->release(&client, b.p);
   b.p = nullptr;
   b.cap = 0;
 }
+void DeviceMemory::take(Buf& dst, Buf& src) {
+  release(dst);
+  dst = src;
+  src = Buf{};
+  arena_->retarget(dst.p, &dst.p);
+}
+void DeviceMemory::swap(Buf& a, Buf& b) {
+  std::swap(a, b);
+  arena_->retarget(a.p, &a.p);
+  arena_->retarget(b.p, &b.p);
+}
 
 // ---------------------------------------------------------------- context
-Simplifier::Simplifier(int device, size_t budget, bool keep_input)
+Simplifier::Simplifier(int device, size_t budget, bool keep_input, DeviceArena* arena)
     : mem_(budget), keep_input_(keep_input) {
   st_ = stream_create(device);
+  mem_.attach(arena ? arena : &DeviceArena::get(device), st_);
 }
 
 Simplifier::~Simplifier() {
+  free_all();
+  stream_destroy(st_);
+}
+
+void Simplifier::free_all() {
   Buf* all[] = {&c_off_[0], &lits_[0], &c_off_[1], &lits_[1],
                 &ot_off_, &ot_size_, &ot_refs_, &hist_, &otp_, &assign_, &vstate_, &assumed_, &vorg_, &trail_,
                 &score_, &cflag_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_,
@@ -67,7 +85,6 @@ Simplifier::~Simplifier() {
                 &map_, &map_assign_, &out_, &vrest_, &lsum_};
   stream_quiesce(st_);  // a destructor must not throw, also after a device error
   for (Buf* b : all) mem_.release(*b);
-  stream_destroy(st_);
 }
 
 DB Simplifier::make_db(int w) {
@@ -120,23 +137,25 @@ void Simplifier::alloc_state() {
   mem_.ensure(ot_size_, L2 * 4, 0, st_);
   mem_.ensure(ot_refs_, lcap_ * 4, 0, st_);
   mem_.ensure(hist_, L2 * 4, 0, st_);
-  mem_.ensure(assign_, V1 * 4, 0, st_);
-  mem_.ensure(vstate_, V1, 0, st_);
-  mem_.ensure(assumed_, V1, 0, st_);
-  mem_.ensure(vorg_, V1 * 4, 0, st_);
-  mem_.ensure(trail_, (V1 + 1) * 4, 0, st_);
+  // per-variable arrays: the arena's stable region
+  const Region S = R_STABLE;
+  mem_.ensure(assign_, V1 * 4, 0, st_, S);
+  mem_.ensure(vstate_, V1, 0, st_, S);
+  mem_.ensure(assumed_, V1, 0, st_, S);
+  mem_.ensure(vorg_, V1 * 4, 0, st_, S);
+  mem_.ensure(trail_, (V1 + 1) * 4, 0, st_, S);
   for (Buf* b : {&score_, &flag_, &cand_, &key_, &pos_, &state_, &pv_, &vsz_, &vbase_, &type_, &def_,
                  &vncls_, &vnlits_, &vnstack_, &clsb_, &litb_, &stkb_, &snap_np_, &snap_nn_, &order_, &sublast_,
                  &ust_, &ulit_})
-    mem_.ensure(*b, (V1 + 1) * 4, 0, st_);
-  mem_.ensure(cflag_, V1, 0, st_);
-  mem_.ensure(elits_, 2 * (V1 + 1) * 4, 0, st_);
+    mem_.ensure(*b, (V1 + 1) * 4, 0, st_, S);
+  mem_.ensure(cflag_, V1, 0, st_, S);
+  mem_.ensure(elits_, 2 * (V1 + 1) * 4, 0, st_, S);
   mem_.ensure(unit_by_, L2 * 4, 0, st_);
-  mem_.ensure(recs_, 3 * (V1 + 1) * 4, 0, st_);
-  mem_.ensure(subaff_, 3 * (V1 + 1) * 4, 0, st_);
-  mem_.ensure(ids_, (V1 + 1) * 4, 0, st_);
-  mem_.ensure(vrest_, (2 * (V1 + 1) + 32) * 4, 0, st_);
-  mem_.ensure(lsum_, 2 * (V1 + 1) * 4, 0, st_);
+  mem_.ensure(recs_, 3 * (V1 + 1) * 4, 0, st_, S);
+  mem_.ensure(subaff_, 3 * (V1 + 1) * 4, 0, st_, S);
+  mem_.ensure(ids_, (V1 + 1) * 4, 0, st_, S);
+  mem_.ensure(vrest_, (2 * (V1 + 1) + 32) * 4, 0, st_, S);
+  mem_.ensure(lsum_, 2 * (V1 + 1) * 4, 0, st_, S);
   mem_.ensure(mark_, lcap_, 0, st_);  // per clause handle
   const size_t big = std::max<size_t>({V1 + 2, L2 + 1, ccap_});
   mem_.ensure(tmp_, std::max(scan_tmp_bytes(big), radix_tmp_bytes(V1 + 1)), 0, st_);
@@ -157,6 +176,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     if (!(flags && (flags[i] & 1))) order.push_back(i);
   for (uint64_t i = 0; i < ncls; i++)
     if (flags && (flags[i] & 1)) order.push_back(i);
+  has_learnts_ = order.size() && flags && (flags[order.back()] & 1);
   // pool words: a header per clause, then its literals
   const uint64_t L = offsets[ncls] - offsets[0] + CL_HDR * ncls;
   if (ncls >= INF / 2 || L >= INF / 4) throw AllocFail{"formula too large for 32-bit clause ids"};
@@ -407,8 +427,11 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
   iota_u32(P<u32>(vorg_), V1, st_);
   fill_u8(P<u8>(mark_), 0, lcap_, st_);
   k_prep(db, 0, st_);
-  mem_.release(lits_d);
-  mem_.release(ends_d);
+  const bool proof = opt.proof_en != 0;  // the text clauses stay for the proof lines
+  if (!proof) {
+    mem_.release(lits_d);
+    mem_.release(ends_d);
+  }
   mem_.ensure(ulit_d, ((size_t)NC + 1) * 4, 0, st_);
   mem_.ensure(first_d, V1 * 4, 0, st_);
   mem_.ensure(win_d, ((size_t)NC + 1) * 4, 0, st_);
@@ -442,19 +465,20 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
   pending_units_.clear();
   refuted_ = (h.err & LEXE_UNSAT) != 0;
   resident_ = false;
+  has_learnts_ = false;
   map_called_ = map_applied_ = remapped_ = false;
   orgmax_ = (u32)V;
   proof_n_ = 0;
   pl_.clear();
   pst_.clear();
   if (!refuted_ && ntrail) {
-    const int32_t proof = opt.proof_en;
-    opt.proof_en = 0;  // the parse's own proof lines are not part of the simplifier's proof
+    opt.proof_en = 0;  // the propagation's lines are written from the end state below
     create_ot();
     const bool ok = prop();
     opt.proof_en = proof;
     if (!ok) refuted_ = true;
   }
+  if (proof) parse_proof(P<u32>(lits_d), P<u32>(ends_d), refuted_ ? 0 : (u32)NC);
   if (!refuted_) gc();
   units_out_.clear();
   rounds_.clear();
@@ -589,11 +613,11 @@ void Simplifier::assume(const uint32_t* vars, uint32_t n) {
 // grows the per-variable state to maxvar variables, keeping its content
 void Simplifier::grow_vars(uint32_t maxvar) {
   const size_t o1 = (size_t)V_ + 1, n1 = (size_t)maxvar + 1;
-  mem_.ensure(assign_, n1 * 4, o1 * 4, st_);
-  mem_.ensure(vstate_, n1, o1, st_);
-  mem_.ensure(assumed_, n1, o1, st_);
-  mem_.ensure(vorg_, n1 * 4, o1 * 4, st_);
-  mem_.ensure(trail_, (n1 + 1) * 4, trail_n_ * 4, st_);
+  mem_.ensure(assign_, n1 * 4, o1 * 4, st_, R_STABLE);
+  mem_.ensure(vstate_, n1, o1, st_, R_STABLE);
+  mem_.ensure(assumed_, n1, o1, st_, R_STABLE);
+  mem_.ensure(vorg_, n1 * 4, o1 * 4, st_, R_STABLE);
+  mem_.ensure(trail_, (n1 + 1) * 4, trail_n_ * 4, st_, R_STABLE);
   fill_u32(P<u32>(assign_) + o1, 0, n1 - o1, st_);
   fill_u8(P<u8>(vstate_) + o1, 0, n1 - o1, st_);
   fill_u8(P<u8>(assumed_) + o1, 0, n1 - o1, st_);
@@ -697,6 +721,54 @@ void Simplifier::gc() {
   stage_end();
 }
 
+// the live clauses compacted with the originals first, then the learnt
+// clauses, each in clause order (newBeginning + extract,
+// src/cpu/simplify.cpp:251-269, :147-149)
+void Simplifier::originals_first() {
+  mem_.ensure(tmp2_, std::max<size_t>(ncls_, 1) * 4, 0, st_);
+  mem_.ensure(tmp3_, std::max<size_t>(ncls_, 1) * 4, 0, st_);
+  DB src = make_db(cur_), dst = make_db(1 - cur_);
+  u32* tot = P<u32>(scal_) + 8;
+  k_compact_originals_first(src, dst, P<u32>(tmp2_), P<u32>(tmp3_), tmp_.p, tot, st_);
+  u32 h[2];
+  d2h(h, tot, 8, st_);
+  stream_sync(st_);
+  ncls_ = h[0];
+  lits_end_ = h[1];
+  cur_ = 1 - cur_;
+  prop_changed_ = true;
+}
+
+// DRAT lines of an ingestion (pfs_parse_dimacs with "proof"): every root
+// unit of the trail in trail order (each follows by propagation from the
+// text and the units before it), then, per text clause in text order, the
+// deletion of a clause that is gone (satisfied, tautology, a unit now on
+// the trail) or the addition of what is left of a clause that lost literals
+// with the deletion of its text; a refuted text ends with the empty clause.
+// The lines describe the parser's, the propagation's and the top-level
+// shrink's changes together (dimacs.cpp:245-305, shrink.cpp:122).
+void Simplifier::parse_proof(const u32* toks, const u32* ends, u32 nc) {
+  const u32 nu = (u32)trail_n_, n = nu + nc;
+  DB db = make_db(cur_);
+  mem_.ensure(pbytes_, ((size_t)n + 1) * 8, 0, st_);
+  u32* bytes = P<u32>(pbytes_);
+  u32* off = bytes + n + 1;
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes((size_t)n + 1)), 0, st_);
+  k_parse_proof(db, toks, ends, P<u32>(trail_), nu, nc, nullptr, nullptr, bytes, st_);
+  u32* total = P<u32>(scal_) + 50;
+  scan_u32(bytes, off, n, total, tmp_.p, st_);
+  const u32 add = n ? read_u32(total) : 0;
+  const size_t all = (size_t)add + (refuted_ ? 2 : 0);
+  mem_.ensure(proof_, std::max<size_t>(all, 4096), 0, st_);
+  k_parse_proof(db, toks, ends, P<u32>(trail_), nu, nc, off, P<u8>(proof_), bytes, st_);
+  if (refuted_) {
+    const u8 empty[2] = {'a', 0};
+    h2d(P<u8>(proof_) + add, empty, 2, st_);
+  }
+  stream_sync(st_);
+  proof_n_ = all;
+}
+
 // ---------------------------------------------------------------- OT (simplify.cpp:35-56)
 void Simplifier::create_ot() {
   stage_begin(S_OT);
@@ -736,7 +808,7 @@ void Simplifier::ot_counts(bool table) {
   DB db = make_db(cur_);
   u32* hist = P<u32>(hist_);
   OtpPlan p;
-  if (otp_plan(n2, db.ncls, &p) && otp_room(p)) {
+  if (otp_plan(n2, db.ncls, &p, opt.otp_tile, opt.otp_xcd) && otp_room(p)) {
     mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
     u32* counts = P<u32>(otp_);
     u32* pairs = counts + ((p.ncounts + 1 + 63) & ~(size_t)63);  // 256-byte aligned
@@ -861,6 +933,8 @@ bool Simplifier::lcve() {
   fill_u32(P<u32>(state_), 0, ncand, st_);
   if (a.clause_max != INF) k_lcve_reject(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(pos_), P<u32>(state_), st_);
   fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
+  u32* resume = P<u32>(vsz_);                  // free until SUB: where each scan goes on
+  fill_u32(resume, 0, ncand, st_);
   // worklists of undecided candidates (the selection buffers are free now);
   // after the first pass over all candidates, passes are queued eight at a
   // time with the list lengths kept on the device (sc[4], sc[5]; sc[6]
@@ -870,14 +944,15 @@ bool Simplifier::lcve() {
   u32 iters = 0;
   if (ncand) {
     fill_u32(sc + 4, 0, 3, st_);
-    k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), wl[0], cnt[0], st_);
+    k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), resume, wl[0],
+               cnt[0], (u32)opt.mis_lds, st_);
     u32 cur = 0, bound = read_u32(cnt[0]), launched = 0;
     while (bound) {
       for (int b = 0; b < 8; b++, launched++) {
         const u32 nx = cur ^ 1;
         fill_u32(cnt[nx], 0, 1, st_);
         k_mis_step_dev(db, P<u32>(cand_), wl[cur], cnt[cur], bound, P<u32>(pos_), P<u32>(state_), P<u32>(key_),
-                       wl[nx], cnt[nx], sc + 6, st_);
+                       resume, wl[nx], cnt[nx], sc + 6, (u32)opt.mis_lds, st_);
         cur = nx;
       }
       if (launched > ncand + 8) throw DeviceFail{"LCVE election did not converge"};
@@ -892,7 +967,8 @@ bool Simplifier::lcve() {
   E_ = read_u32(sc);
   rounds_.back().elected = E_;
   rounds_.back().mis_iterations = iters;
-  if (profile) {
+  stage_end();
+  if (profile) {  // work measures, outside the stage's time
     uint64_t w[2];
     k_cand_work(db, P<u32>(cand_), ncand, P<u8>(mark_), (u64*)(P<u32>(scal_) + 40), st_);
     d2h(w, P<u32>(scal_) + 40, sizeof(w), st_);
@@ -900,7 +976,6 @@ bool Simplifier::lcve() {
     rounds_.back().cand_occurs = w[0];
     rounds_.back().cand_bytes = w[1];
   }
-  stage_end();
   return E_ >= (u32)opt.lcve_min;
 }
 
@@ -914,7 +989,8 @@ void Simplifier::sort_ot() {
   mem_.ensure(tmp2_, (6 * (size_t)E_ + 3) * 4, 0, st_);
   k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), P<u32>(lsum_), st_);
   if (opt.work_order) k_work_order(db, P<u32>(pv_), E_, P<u32>(tmp2_), P<u32>(order_), tmp_.p, st_);
-  if (profile) {
+  stage_end();
+  if (profile) {  // work measures, outside the stage's time
     uint64_t w[2];
     k_elected_work(db, P<u32>(pv_), E_, (u64*)(P<u32>(scal_) + 40), st_);
     d2h(w, P<u32>(scal_) + 40, sizeof(w), st_);
@@ -922,7 +998,6 @@ void Simplifier::sort_ot() {
     rounds_.back().elected_occurs = w[0];
     rounds_.back().elected_bytes = w[1];
   }
-  stage_end();
 }
 
 // ---------------------------------------------------------------- SUB (elimination.cpp:110-129)
@@ -1015,29 +1090,70 @@ bool Simplifier::sub() {
     for (const ProofEv& e : pev) pmap[e.k].push_back(e);
   };
 
+  // The first refutation in elected order, as the commit below meets it: a
+  // variable's own refutation (literal 0) or a unit whose variable is
+  // already assigned the other way (at the root or by an earlier variable).
+  // Variables after it cannot change the outcome (the sequential stage stops
+  // there), so re-runs are limited to the variables up to it; affected
+  // variables past it are deferred and run only if the refutation moves past
+  // them (an earlier variable's re-run can withdraw the unit it met).
+  auto first_refutation = [&]() -> u32 {
+    std::unordered_map<u32, u32> over;  // variable -> true literal + 1, over h_assign_
+    for (auto& kv : units) {
+      std::vector<std::pair<u32, u32>> ev = kv.second;
+      std::sort(ev.begin(), ev.end());
+      for (auto& e : ev) {
+        const u32 u = e.second;
+        if (!u) return kv.first;
+        auto it = over.find(ABS(u));
+        const u32 a = it != over.end() ? it->second : h_assign_[ABS(u)];
+        if (a) {
+          if (a - 1 != u) return kv.first;
+          continue;
+        }
+        over[ABS(u)] = u + 1;
+      }
+    }
+    return INF;
+  };
+
   // pass 1: every variable, nothing of this stage visible
   u32 pass = 1;
   fill_u32(P<u32>(sublast_), 1, E_, st_);
   run(work_order(), E_, {}, pass, false);
   std::vector<u32> vis_set;  // variables with a visible stage unit
+  std::set<u32> deferred;    // affected variables past the refutation, not re-run
   bool vis_ready = false;
-  while (!changes.empty()) {
+  while (!changes.empty() || !deferred.empty()) {
     if (pass > E_ + 2) throw DeviceFail{"SUB re-runs did not converge"};
-    const u32 nch = (u32)(changes.size() / 3);
-    mem_.ensure(ids_, changes.size() * 4, 0, st_);
-    h2d(ids_.p, changes.data(), changes.size() * 4, st_);
-    changes.clear();
     u32* aff = P<u32>(subaff_);
     u32* pos = aff + E_;
     u32* idx = aff + 2 * (size_t)E_;
-    fill_u32(aff, 0, E_, st_);
-    k_sub_affected(db, snap, P<u32>(ids_), nch, P<u32>(sublast_), aff, st_);
-    k_pack_flags(aff, E_, pos, idx, sc + 6, tmp_.p, st_);
-    const u32 m = read_u32(sc + 6);
+    std::vector<u32> ran;
+    if (!changes.empty()) {
+      const u32 nch = (u32)(changes.size() / 3);
+      mem_.ensure(ids_, changes.size() * 4, 0, st_);
+      h2d(ids_.p, changes.data(), changes.size() * 4, st_);
+      changes.clear();
+      fill_u32(aff, 0, E_, st_);
+      k_sub_affected(db, snap, P<u32>(ids_), nch, P<u32>(sublast_), aff, st_);
+      k_pack_flags(aff, E_, pos, idx, sc + 6, tmp_.p, st_);
+      const u32 m0 = read_u32(sc + 6);
+      ran.resize(m0);
+      if (m0) d2h(ran.data(), idx, (size_t)m0 * 4, st_);
+      stream_sync(st_);
+    }
+    // the re-runs: affected or deferred variables up to the refutation
+    const u32 kref = first_refutation();
+    for (u32 k : ran) deferred.insert(k);
+    ran.clear();
+    for (auto it = deferred.begin(); it != deferred.end() && *it <= kref;) {
+      ran.push_back(*it);
+      it = deferred.erase(it);
+    }
+    const u32 m = (u32)ran.size();
     if (!m) break;
-    std::vector<u32> ran(m);
-    d2h(ran.data(), idx, (size_t)m * 4, st_);
-    stream_sync(st_);
+    h2d(idx, ran.data(), (size_t)m * 4, st_);
     pass++;
     // visible stage units: per variable, the earliest deriving index
     if (!vis_ready) {
@@ -1245,7 +1361,10 @@ void Simplifier::ere() {
 // then replayed in elected-variable and pair order, which decides exactly
 // which clauses the sequential ERE (elimination.cpp:169-197) deletes: a pair
 // whose antecedent was deleted by an earlier pair is skipped, and a pair
-// deletes its lowest-id match that is still live.
+// deletes its lowest-id match that is still live.  The elected variables are
+// taken in chunks of fewer than 2^31 pairs (one thread per pair); a record
+// keeps a pair's ERE_K lowest matches, and the pairs with more get their
+// full match lists from a second pass.
 bool Simplifier::ere_pairs() {
   stage_begin(S_ERE);
   DB db = make_db(cur_);
@@ -1255,69 +1374,129 @@ bool Simplifier::ere_pairs() {
   u64 total = 0;
   d2h(&total, tot64, 8, st_);
   stream_sync(st_);
+  // chunks of variables [kb[c], kb[c + 1]) with fewer than 2^31 pairs each
+  std::vector<u32> kb{0};
   if (total >= (1ull << 31)) {
-    stage_end();
-    return false;
+    std::vector<u32> np(E_);
+    d2h(np.data(), vncls_.p, (size_t)E_ * 4, st_);
+    stream_sync(st_);
+    u64 acc = 0;
+    for (u32 k = 0; k < E_; k++) {
+      if (acc + np[k] >= (1ull << 31)) {
+        kb.push_back(k);
+        acc = 0;
+      }
+      acc += np[k];
+    }
   }
-  const u32 npairs = (u32)total;
+  kb.push_back(E_);
   bool ok = true;
-  if (npairs) {
-    scan_u32(P<u32>(vncls_), P<u32>(clsb_), E_, nullptr, tmp_.p, st_);
+  std::vector<u32> rec;                           // every chunk's records, k made global
+  std::unordered_map<u32, std::vector<u32>> full;  // record -> all its matches, ascending
+  if (total) {
     size_t cap = 1024;
     while (cap < 2 * (size_t)ncls_) cap <<= 1;
     mem_.ensure(ht_, cap * 8, 0, st_);
     HashTable ht{(u64*)ht_.p, (u32)(cap - 1)};
     k_ht_build(db, ht, opt.ere_max_resolvent, st_);
-    const u32 rcap = std::max<u32>(1u << 16, V_ + 1);
-    mem_.ensure(erecs_, (size_t)rcap * ERE_REC_WORDS * 4, 0, st_);
-    fill_u32(sc, 0, 2, st_);
-    ErePairs a{P<u32>(pv_), P<u32>(clsb_), E_, P<u32>(erecs_), sc, rcap, sc + 1, ht, opt.ere_max_resolvent};
-    k_ere_pairs(db, a, npairs, st_);
-    u32 h[2];
-    d2h(h, sc, 8, st_);
-    stream_sync(st_);
-    if (h[1] || h[0] > rcap) {
-      ok = false;  // more equal clauses than kept per pair: replay per variable
-    } else if (h[0]) {
-      const u32 nrec = h[0];
-      std::vector<u32> rec((size_t)nrec * ERE_REC_WORDS);
-      d2h(rec.data(), erecs_.p, rec.size() * 4, st_);
+    u32 rcap = std::max<u32>(1u << 16, V_ + 1), ocap = 1u << 12;
+    for (size_t ci = 0; ok && ci + 1 < kb.size(); ci++) {
+      const u32 k0 = kb[ci], nk = kb[ci + 1] - k0;
+      if (!nk) continue;
+      scan_u32(P<u32>(vncls_) + k0, P<u32>(clsb_), nk, sc + 34, tmp_.p, st_);
+      const u32 npairs = read_u32(sc + 34);
+      if (!npairs) continue;
+      // records and overflow indices, grown to the counts of a pass that
+      // overflowed them and the pass run again
+      u32 h[2] = {0, 0};
+      ErePairs a{};
+      for (int attempt = 0;; attempt++) {
+        try {
+          mem_.ensure(erecs_, ((size_t)rcap * ERE_REC_WORDS + ocap) * 4, 0, st_);
+        } catch (AllocFail&) {
+          ok = false;
+          break;
+        }
+        fill_u32(sc, 0, 2, st_);
+        a = ErePairs{P<u32>(pv_) + k0, P<u32>(clsb_), nk, P<u32>(erecs_), sc, rcap, sc + 1,
+                     P<u32>(erecs_) + (size_t)rcap * ERE_REC_WORDS, ocap, ht, opt.ere_max_resolvent};
+        k_ere_pairs(db, a, npairs, st_);
+        d2h(h, sc, 8, st_);
+        stream_sync(st_);
+        if (h[0] <= rcap && h[1] <= ocap) break;
+        if (attempt == 2 || h[0] >= (1u << 27)) {
+          ok = false;
+          break;
+        }
+        rcap = std::max(rcap, h[0]);
+        ocap = std::max(ocap, h[1]);
+      }
+      if (!ok || !h[0]) continue;
+      const u32 nrec = h[0], novf = h[1], first = (u32)(rec.size() / ERE_REC_WORDS);
+      rec.resize(rec.size() + (size_t)nrec * ERE_REC_WORDS);
+      u32* cr = rec.data() + (size_t)first * ERE_REC_WORDS;
+      d2h(cr, erecs_.p, (size_t)nrec * ERE_REC_WORDS * 4, st_);
       stream_sync(st_);
-      std::vector<u32> idx(nrec);
-      for (u32 i = 0; i < nrec; i++) idx[i] = i;
-      auto R = [&](u32 i, int w) { return rec[(size_t)i * ERE_REC_WORDS + w]; };
-      std::sort(idx.begin(), idx.end(), [&](u32 x, u32 y) {
-        return R(x, 0) != R(y, 0) ? R(x, 0) < R(y, 0) : R(x, 1) < R(y, 1);
-      });
-      std::unordered_map<u32, bool> dead;
-      std::vector<u32> del;
-      for (u32 i : idx) {
-        if (dead.count(R(i, 2)) || dead.count(R(i, 3))) continue;
-        const u32 nm = R(i, 4);
-        u32 target = INF;
-        for (u32 q = 0; q < nm && q < (u32)ERE_K; q++) {
+      if (novf) {
+        std::vector<u32> ovf(novf), off(novf + 1, 0);
+        d2h(ovf.data(), a.ovf, (size_t)novf * 4, st_);
+        stream_sync(st_);
+        for (u32 o = 0; o < novf; o++) off[o + 1] = off[o] + cr[(size_t)ovf[o] * ERE_REC_WORDS + 4];
+        mem_.ensure(tmp3_, ((size_t)novf + 1 + off[novf]) * 4, 0, st_);
+        u32* d_off = P<u32>(tmp3_);
+        u32* d_out = d_off + novf + 1;
+        h2d(d_off, off.data(), ((size_t)novf + 1) * 4, st_);
+        k_ere_all_matches(db, a, novf, d_off, d_out, st_);
+        std::vector<u32> m(off[novf]);
+        d2h(m.data(), d_out, m.size() * 4, st_);
+        stream_sync(st_);
+        for (u32 o = 0; o < novf; o++) {
+          std::vector<u32> l(m.begin() + off[o], m.begin() + off[o + 1]);
+          std::sort(l.begin(), l.end());
+          full[first + ovf[o]] = std::move(l);
+        }
+      }
+      for (u32 i = 0; i < nrec; i++) cr[(size_t)i * ERE_REC_WORDS] += k0;
+    }
+  }
+  const u32 nrec = (u32)(rec.size() / ERE_REC_WORDS);
+  if (ok && nrec) {
+    auto R = [&](u32 i, int w) { return rec[(size_t)i * ERE_REC_WORDS + w]; };
+    std::vector<u32> idx(nrec);
+    for (u32 i = 0; i < nrec; i++) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](u32 x, u32 y) {
+      return R(x, 0) != R(y, 0) ? R(x, 0) < R(y, 0) : R(x, 1) < R(y, 1);
+    });
+    std::unordered_map<u32, bool> dead;
+    std::vector<u32> del;
+    for (u32 i : idx) {
+      if (dead.count(R(i, 2)) || dead.count(R(i, 3))) continue;
+      const u32 nm = R(i, 4);
+      u32 target = INF;
+      if (nm > (u32)ERE_K) {
+        for (u32 c : full[i])
+          if (!dead.count(c)) {
+            target = c;
+            break;
+          }
+      } else {
+        for (u32 q = 0; q < nm; q++)
           if (!dead.count(R(i, 5 + q))) {
             target = R(i, 5 + q);
             break;
           }
-        }
-        if (target == INF) {
-          if (nm > (u32)ERE_K) { ok = false; break; }
-          continue;
-        }
-        dead[target] = true;
-        del.push_back(target);
       }
-      if (ok) {
-        for (u32 c : del) proof_line(PR_DEL, c);  // removeClause (redundancy.h:137)
-        proof_flush();
-      }
-      if (ok && !del.empty()) {
-        mem_.ensure(ids_, del.size() * 4, 0, st_);
-        h2d(ids_.p, del.data(), del.size() * 4, st_);
-        k_apply_deletions(db, P<u32>(ids_), (u32)del.size(), st_);
-        stream_sync(st_);
-      }
+      if (target == INF) continue;
+      dead[target] = true;
+      del.push_back(target);
+    }
+    for (u32 c : del) proof_line(PR_DEL, c);  // removeClause (redundancy.h:137)
+    proof_flush();
+    if (!del.empty()) {
+      mem_.ensure(ids_, del.size() * 4, 0, st_);
+      h2d(ids_.p, del.data(), del.size() * 4, st_);
+      k_apply_deletions(db, P<u32>(ids_), (u32)del.size(), st_);
+      stream_sync(st_);
     }
   }
   if (ok) rounds_.back().ere_passes = 1;
@@ -1481,8 +1660,13 @@ int Simplifier::simplify(uint32_t max_rounds) {
   proof_n_ = 0;
   pl_.clear();
   pst_.clear();
-  // a resident formula simplified before: its resolvents are no longer new
-  if (resident_) k_clear_added(make_db(cur_), st_);
+  // a resident formula simplified before: its resolvents are no longer new,
+  // and, as the reference's next call extracts the clauses it wrote back
+  // (orgs, then learnts), its originals come first
+  if (resident_) {
+    k_clear_added(make_db(cur_), st_);
+    if (has_learnts_) originals_first();
+  }
   resident_ = true;
   kernel_times.clear();
   prof_enable(st_, profile);
@@ -1574,7 +1758,20 @@ Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
     throw std::logic_error("root units are not propagated: simplify or parse first");
   ElsInfo info;
   ProfScope prof(this);
-  // the substitution's own proof lines are not recorded
+  // proof lines: written at the end from the clauses before the call (a
+  // copy of the pool: the rounds rewrite clauses in place and keep their
+  // handles until the final compaction) and after it
+  proof_n_ = 0;
+  pl_.clear();
+  pst_.clear();
+  const bool proof = opt.proof_en != 0;
+  u32 els_conflict = 0;  // with a refutation: the unit written before the empty clause
+  Buf before;
+  BufScope before_scope{mem_, {&before}};
+  if (proof) {
+    mem_.ensure(before, std::max<uint64_t>(lits_end_, 1) * 4, 0, st_);
+    d2d(before.p, lits_[cur_].p, lits_end_ * 4, st_);
+  }
   struct NoProof {
     Options& o;
     int32_t saved;
@@ -1606,7 +1803,7 @@ Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
       k_els_trim(P<u32>(fa), nf, P<u32>(off), P<u32>(adj), P<u32>(alive), P<u32>(indeg), P<u32>(outdeg),
                  P<u32>(root), P<u32>(fb), w + 1, st_);
       nf = rd(1);
-      std::swap(fa, fb);
+      mem_.swap(fa, fb);
     }
   };
   bool unsat = false;
@@ -1649,6 +1846,15 @@ Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
     d2h(ri, w + 4, 8, st_);
     stream_sync(st_);
     if (ri[0]) {
+      if (proof) {  // the literal whose negation shares its SCC (the first one)
+        std::vector<u32> rt(n2);
+        std::vector<u8> vs(n1);
+        d2h(rt.data(), root.p, n2 * 4, st_);
+        d2h(vs.data(), vstate_.p, n1, st_);
+        stream_sync(st_);
+        for (u32 l = 2; l < n2 && !els_conflict; l++)
+          if (!vs[ABS(l)] && rt[l] == rt[FLIP(l)]) els_conflict = FLIP(l);
+      }
       unsat = true;
       break;
     }
@@ -1724,6 +1930,7 @@ Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
   info.units = trail_n_ - start;
   info.stack_words = stack_n_ - stack0;
   if (unsat) {
+    if (proof) rewrite_proof(nullptr, start, true, els_conflict);
     refuted_ = true;
     info.status = 0;
     return info;
@@ -1733,10 +1940,62 @@ Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
     create_ot();
     prop_clean(start);
   }
+  if (proof) rewrite_proof(P<u32>(before), start, false, 0);
   propagated_ = trail_n_;
   prop_changed_ = true;
   gc();
   return info;
+}
+
+// DRAT lines of a call that rewrote the clauses in place (decompose): the
+// root units from trail[from..), then the new form of every clause that
+// changed, then the deletions of the old forms and of the clauses gone
+// (equivalence.cpp:250-270 writes per clause the addition and then the
+// deletion; all additions first keep every old clause, the SCC binaries
+// included, available to the later additions' unit propagation); a
+// refutation ends with the empty clause, after the unit 'pre' when it is not
+// 0 (a literal whose negation shares its SCC: the unit of its negation
+// follows by propagation along the cycle, and then the empty clause).  old:
+// the pool before the call (null: units and the empty clause only).
+void Simplifier::rewrite_proof(const u32* old, uint64_t from, bool empty, u32 pre) {
+  const u32 nu = (u32)(trail_n_ - from);
+  DB db = make_db(cur_);
+  if (!old) db.ncls = 0;
+  const size_t n = (size_t)nu + 2 * (size_t)db.ncls;
+  mem_.ensure(pbytes_, (n + 1) * 8, 0, st_);
+  u32* bytes = P<u32>(pbytes_);
+  u32* off = bytes + n + 1;
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(n + 1)), 0, st_);
+  u32* units = P<u32>(trail_) + from;
+  k_rewrite_proof(db, old, units, nu, nullptr, nullptr, bytes, st_);
+  u32* total = P<u32>(scal_) + 50;
+  scan_u32(bytes, off, n, total, tmp_.p, st_);
+  const u32 add = n ? read_u32(total) : 0;
+  u8 tail[16];
+  size_t nt = 0;
+  if (empty && pre) {
+    tail[nt++] = 'a';
+    u32 vo = 0;  // the proof numbers literals by original variable
+    d2h(&vo, P<u32>(vorg_) + ABS(pre), 4, st_);
+    stream_sync(st_);
+    u32 r = V2L(vo) | SIGN(pre);
+    while (r > 127) {
+      tail[nt++] = (u8)((r & 127) | 128);
+      r >>= 7;
+    }
+    tail[nt++] = (u8)r;
+    tail[nt++] = 0;
+  }
+  if (empty) {
+    tail[nt++] = 'a';
+    tail[nt++] = 0;
+  }
+  const size_t all = (size_t)add + nt;
+  mem_.ensure(proof_, std::max<size_t>(all, 4096), 0, st_);
+  k_rewrite_proof(db, old, units, nu, off, P<u8>(proof_), bytes, st_);
+  if (nt) h2d(P<u8>(proof_) + add, tail, nt, st_);
+  stream_sync(st_);
+  proof_n_ = all;
 }
 
 // ---------------------------------------------------------------- results
@@ -1788,10 +2047,10 @@ Simplifier::MapInfo Simplifier::map(bool force, double map_perc) {
   // per-variable state in the new numbering; the old assignments stay for map_lits
   const size_t m1 = (size_t)nV + 1;
   Buf vorg2, vstate2, assumed2, assign2;
-  mem_.ensure(vorg2, m1 * 4, 0, st_);
-  mem_.ensure(vstate2, m1, 0, st_);
-  mem_.ensure(assumed2, m1, 0, st_);
-  mem_.ensure(assign2, m1 * 4, 0, st_);
+  mem_.ensure(vorg2, m1 * 4, 0, st_, R_STABLE);
+  mem_.ensure(vstate2, m1, 0, st_, R_STABLE);
+  mem_.ensure(assumed2, m1, 0, st_, R_STABLE);
+  mem_.ensure(assign2, m1 * 4, 0, st_, R_STABLE);
   k_map_vars(m, V, P<u32>(vorg_), P<u8>(vstate_), P<u8>(assumed_), P<u32>(vorg2), P<u8>(vstate2), P<u8>(assumed2),
              P<u32>(assign2), st_);
   // trail: only 'first' survives (VMAP::mapShrinkLits, vmap.h:158-169)
@@ -1801,12 +2060,11 @@ Simplifier::MapInfo Simplifier::map(bool force, double map_perc) {
     trail_n_ = propagated_ = 1;
   }
   stream_sync(st_);
-  mem_.release(map_assign_);
-  map_assign_ = assign_;
-  assign_ = assign2;
-  std::swap(vorg_, vorg2);
-  std::swap(vstate_, vstate2);
-  std::swap(assumed_, assumed2);
+  mem_.take(map_assign_, assign_);
+  mem_.take(assign_, assign2);
+  mem_.swap(vorg_, vorg2);
+  mem_.swap(vstate_, vstate2);
+  mem_.swap(assumed_, assumed2);
   mem_.release(vorg2);
   mem_.release(vstate2);
   mem_.release(assumed2);

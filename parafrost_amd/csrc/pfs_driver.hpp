@@ -8,14 +8,17 @@
 #include <string>
 #include <vector>
 
+#include "pfs_arena.hpp"
 #include "pfs_launch.hpp"
 
 namespace pfs {
 
-// Device memory owned by a context.  Every buffer is accounted against a
-// budget (admission control for contexts that share a device); the two
-// clause-database blocks form the dynamic region that garbage collection
-// compacts into and that grows when resolvents need more room.
+// Device memory owned by a context: blocks of the device arena (one pool
+// per device, pfs_arena.hpp) accounted against the context's budget
+// (admission control for contexts that share a device).  Per-variable
+// arrays live in the arena's stable region; the clause blocks, the
+// occurrence table and the temporaries in its dynamic region, which
+// compaction slides down between calls.
 class DeviceMemory {
  public:
   explicit DeviceMemory(size_t budget) : budget_(budget) {}
@@ -23,16 +26,28 @@ class DeviceMemory {
     void* p = nullptr;
     size_t cap = 0;
   };
+  void attach(DeviceArena* arena, Stream* st) {
+    arena_ = arena;
+    client.arena = arena;
+    client.st = st;
+  }
   // make b hold at least 'bytes'; keeps the first 'keep' bytes of content
-  void ensure(Buf& b, size_t bytes, size_t keep, Stream* st);
+  void ensure(Buf& b, size_t bytes, size_t keep, Stream* st, Region r = R_DYNAMIC);
   void release(Buf& b);
-  size_t used() const { return used_; }
+  // buffer records hand blocks over only through these, so that the arena
+  // keeps the right pointer to rewrite when it moves a block
+  void take(Buf& dst, Buf& src);  // dst's block released, dst = src, src empty
+  void swap(Buf& a, Buf& b);
+  size_t used() const { return client.used; }
   size_t peak() const { return peak_; }
   size_t budget() const { return budget_; }
   void set_budget(size_t b) { budget_ = b; }
+  DeviceArena* arena() const { return arena_; }
+  ArenaClient client;
 
  private:
-  size_t budget_, used_ = 0, peak_ = 0;
+  size_t budget_, peak_ = 0;
+  DeviceArena* arena_ = nullptr;
 };
 
 struct AllocFail {
@@ -61,8 +76,11 @@ extern const char* const kStageNames[S_NSTAGES];
 
 class Simplifier {
  public:
-  Simplifier(int device, size_t budget, bool keep_input);
+  // arena: the pool the context's device memory comes from (null: the
+  // device's default arena)
+  Simplifier(int device, size_t budget, bool keep_input, DeviceArena* arena = nullptr);
   ~Simplifier();
+  void free_all();  // every device block back to the arena (the destructor does it too)
 
   Options opt;
   bool profile = false;
@@ -92,6 +110,14 @@ class Simplifier {
   // after an exception: wait for the stream and drop staged reads whose
   // destinations belonged to the unwound call
   void on_error() { stream_quiesce(st_); }
+  // the arena client: held locked while a call works on the device; at the
+  // start of a call (no device address held yet) the context's own blocks
+  // may be moved by a compaction the arena asked for
+  ArenaClient& client() { return mem_.client; }
+  void call_begin() {
+    DeviceArena* a = mem_.arena();
+    if (a && a->wants_compaction()) a->compact(&mem_.client, true);
+  }
   double lits_mul = 1.0;  // resolvent room at upload, as a multiple of the input (literalsmul)
   // incremental use: add clauses / replace the assumed variables of the
   // resident formula (the state after the last simplify)
@@ -137,6 +163,9 @@ class Simplifier {
   void alloc_state();
   DB make_db(int which);
   void gc();
+  void originals_first();
+  void parse_proof(const u32* toks, const u32* ends, u32 nc);  // the ingestion's DRAT lines
+  void rewrite_proof(const u32* old, uint64_t from, bool empty, u32 pre);  // an in-place rewrite's DRAT lines
   void aggr_sort();
   void count(uint64_t& cls, uint64_t& lits);
   void create_ot();
@@ -237,6 +266,7 @@ class Simplifier {
   int phase_ = 0;
   bool refuted_ = false;           // an empty clause was appended
   bool resident_ = false;          // the clauses come from an earlier simplify call
+  bool has_learnts_ = false;       // learnt clauses were uploaded (resolvents of them are learnt too)
   std::vector<u32> pending_units_;  // units of the clauses being appended
   bool prop_changed_ = true;  // BCP touched clauses since the occurrence table was built
   uint32_t mu_inc_ = 0;

@@ -386,13 +386,19 @@ PF_HD void bcp_clean(const DB& db, u32 c) {
 // independent set over the candidate order: a candidate is elected iff no
 // earlier candidate sharing a live clause is elected.  state: 0 undecided,
 // 1 elected, 2 frozen.  Returns 1 if the candidate stays undecided.
-// blk[i] remembers an undecided earlier neighbour found by the last scan;
-// while it stays undecided the candidate stays blocked without a rescan.
 // The elected set does not depend on the schedule (the lexicographically
-// first independent set is unique), only the number of iterations does.
+// first independent set is unique), only the number of passes does.
 // pw[u] packs the candidate position of variable u and its state
 // (position << 2 | state; NOPOS for variables that are not candidates), so
 // a neighbour costs one load; state[] holds the same states by position.
+// A scan reads the whole neighbourhood (an elected earlier neighbour
+// freezes the candidate wherever it sits) and remembers the first earlier
+// neighbour that is still undecided in blk[i] and the list entry where it
+// met it in resume[i] (sign << 31 | entry).  States only move from
+// undecided to elected or frozen, so everything before that entry stays
+// irrelevant (later or frozen neighbours); while the blocker is undecided
+// the candidate waits without a scan, an elected blocker freezes it, and a
+// frozen one makes the next scan start from the entry where it met it.
 // lcveclausemax (the GPU build's depFreeze_d, src/gpu/lcve.cu:43-52): a
 // candidate with a live clause longer than the limit is passed over (it
 // elects nothing and freezes nothing; the reference's partial freezes of a
@@ -411,7 +417,7 @@ PF_HD void mis_decide(u32* state, u32* pw, u32 i, u32 v, u32 s) {
   state[i] = s;
   pw[v] = (i << 2) | s;
 }
-PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, u32* pw, u32* state, u32* blk) {
+PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, u32* pw, u32* state, u32* blk, u32* resume) {
   if (at_load(&state[i])) return 0;
   const u32 b = blk[i];
   const u32 v = cand[i];
@@ -423,10 +429,11 @@ PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, u32* pw, u32* state, u3
       return 0;
     }
   }
-  u32 blocker = NOPOS;
-  for (u32 s = 0; s < 2; s++) {
+  const u32 r = resume[i];
+  u32 blocker = NOPOS, at = 0;
+  for (u32 s = r >> 31; s < 2; s++) {
     const OLV ol = olist(db, V2L(v) | s);
-    for (u32 k = 0; k < ol.n; k++) {
+    for (u32 k = s == (r >> 31) ? (r & 0x7FFFFFFFu) : 0u; k < ol.n; k++) {
       const u32 c = ol.r[k];
       if (is_deleted(db, c)) continue;
       const CL cl = clause(db, c);
@@ -441,12 +448,16 @@ PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, u32* pw, u32* state, u3
           mis_decide(state, pw, i, v, 2);
           return 0;
         }
-        if (su == 0 && blocker == NOPOS) blocker = pu;
+        if (su == 0 && blocker == NOPOS) {
+          blocker = pu;
+          at = (s << 31) | k;
+        }
       }
     }
   }
   if (blocker != NOPOS) {
     blk[i] = blocker;
+    resume[i] = at;
     return 1;
   }
   mis_decide(state, pw, i, v, 1);
@@ -1804,7 +1815,9 @@ struct ErePairs {
   u32* rec;
   u32* nrec;
   u32 cap;
-  u32* overflow;
+  u32* overflow;  // records with more matches than ERE_K
+  u32* ovf;       // their record indices (up to ovf_cap)
+  u32 ovf_cap;
   HashTable ht;
   int maxsize;
 };
@@ -1821,14 +1834,19 @@ PF_HD u32 ere_pair_count(const DB& db, const Options& o, u32 v) {
   if ((int)me.n > o.ere_limit || (int)other.n > o.ere_limit) return 0;
   return me.n * other.n;
 }
-// one resolvent (pair 'local' of elected variable index k)
-PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
+// The resolvent of pair 'local' of elected variable index k and every live
+// clause equal to it (found through the hash table): f(c) per match.
+// Returns false when the pair has no resolvent to look up (an antecedent
+// deleted, tautology, size limits); c1, c2 are the antecedents.
+template <class F>
+PF_HD bool ere_matches(const DB& db, const ErePairs& a, u32 k, u32 local, u32& c1, u32& c2, F f) {
   const u32 x = a.pv[k];
   OLV me, other;
   ere_lists(db, x, me, other);
   const u32 i = local / other.n, j = local % other.n;
-  const u32 c1 = me.r[i], c2 = other.r[j];
-  if (is_deleted(db, c1) || is_deleted(db, c2)) return;
+  c1 = me.r[i];
+  c2 = other.r[j];
+  if (is_deleted(db, c1) || is_deleted(db, c2)) return false;
   const CL A = clause(db, c1), B = clause(db, c2);
   u32 i1 = 0, i2 = 0, len = 0, h = HASH_SEED;
   while (i1 < A.n && i2 < B.n) {
@@ -1836,27 +1854,36 @@ PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
     const u32 v1 = ABS(l1), v2 = ABS(l2);
     if (v1 == x) i1++;
     else if (v2 == x) i2++;
-    else if ((l1 ^ l2) == 1u) return;
+    else if ((l1 ^ l2) == 1u) return false;
     else if (v1 < v2) { i1++; h = hash_step(h, l1); len++; }
     else if (v2 < v1) { i2++; h = hash_step(h, l2); len++; }
     else { i1++; i2++; h = hash_step(h, l1); len++; }
   }
   while (i1 < A.n) { const u32 l = A.l[i1++]; if (ABS(l) != x) { h = hash_step(h, l); len++; } }
   while (i2 < B.n) { const u32 l = B.l[i2++]; if (ABS(l) != x) { h = hash_step(h, l); len++; } }
-  if (!(len > 1 && (!a.maxsize || (int)len <= a.maxsize))) return;
+  if (!(len > 1 && (!a.maxsize || (int)len <= a.maxsize))) return false;
   h = hash_final(h);
   const u8 type = (is_learnt(db, c1) || is_learnt(db, c2)) ? ST_LEARNT : ST_ORIGINAL;
-  u32 ids[ERE_K];
-  u32 nm = 0;
   for (u32 s = h & a.ht.mask;; s = (s + 1) & a.ht.mask) {
     const u64 e = a.ht.slot[s];
     if (e == HT_EMPTY) break;
     if ((u32)(e >> 32) != h) continue;
     const u32 c = (u32)e;
-    const u8 f = cflags(db, c);
-    if (st_deleted(f) || csize(db, c) != len) continue;
-    if (!(st_learnt(f) || (f & ST_MASK) == type)) continue;
+    const u8 fl = cflags(db, c);
+    if (st_deleted(fl) || csize(db, c) != len) continue;
+    if (!(st_learnt(fl) || (fl & ST_MASK) == type)) continue;
     if (!merge_ere(x, A, B, clause(db, c))) continue;
+    f(c);
+  }
+  return true;
+}
+// one resolvent (pair 'local' of elected variable index k): a record with
+// its ERE_K lowest matching clause ids; a pair with more matches is listed
+// in ovf, whose full match lists ere_all_matches writes afterwards
+PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
+  u32 ids[ERE_K];
+  u32 nm = 0, c1, c2;
+  ere_matches(db, a, k, local, c1, c2, [&](u32 c) {
     // keep the ERE_K lowest ids, ascending
     u32 ins = c;
     for (u32 q = 0; q < nm && q < (u32)ERE_K; q++) {
@@ -1864,7 +1891,7 @@ PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
     }
     if (nm < (u32)ERE_K) ids[nm] = ins;
     nm++;
-  }
+  });
   if (!nm) return;
   const u32 r = at_add(a.nrec, 1u);
   if (r >= a.cap) return;
@@ -1875,7 +1902,21 @@ PF_HD void ere_pair(const DB& db, const ErePairs& a, u32 k, u32 local) {
   w[3] = c2;
   w[4] = nm;
   for (u32 q = 0; q < (u32)ERE_K; q++) w[5 + q] = q < nm ? ids[q] : 0xFFFFFFFFu;
-  if (nm > (u32)ERE_K) at_add(a.overflow, 1u);
+  if (nm > (u32)ERE_K) {
+    const u32 o = at_add(a.overflow, 1u);
+    if (o < a.ovf_cap) a.ovf[o] = r;
+  }
+}
+// every match of overflow record ovf[o] into out[off[o] ..) (off[o] + nm
+// words: the record's count)
+PF_HD void ere_all_matches(const DB& db, const ErePairs& a, u32 o, const u32* off, u32* out) {
+  const u32* w = a.rec + (u64)a.ovf[o] * ERE_REC_WORDS;
+  u32 c1, c2, n = 0;
+  const u32 base = off[o], nm = w[4];
+  ere_matches(db, a, w[0], w[1], c1, c2, [&](u32 c) {
+    if (n < nm) out[base + n] = c;
+    n++;
+  });
 }
 
 // ---------------------------------------------------------------- variable map
@@ -1998,6 +2039,83 @@ PF_HD u32 parse_dedup_clause(const DB& db, u32 c, u32* unit) {
   // dropping duplicates keeps the signature (the same set of literals)
   if (m != n) set_csize(db, c, m);
   return 0;
+}
+
+// DRAT lines of the ingestion (the parser's deletions of satisfied clauses
+// and its additions of shrunken ones, dimacs.cpp:245-305; the top-level
+// shrink's, shrink.cpp:122), written from the end state: per text clause i
+// (tokens toks[s..ends[i]) with s = ends[i - 1] + 1) whose pool clause is
+// gone, a deletion of the text clause; per text clause that lost literals
+// (duplicates, literals false at the root), an addition of what is left and
+// the deletion of the text clause.  Bytes of the lines; written to o when
+// not null.
+PF_HD u32 parse_proof_clause(const DB& db, const u32* toks, const u32* ends, u32 i, u8* o) {
+  const u32 s = i ? ends[i - 1] + 1 : 0u, n = ends[i] - s;
+  const u32 c = db.c_off[i];
+  const u32 h = db.lits[c];
+  const bool gone = st_deleted(hdr_flags(h));
+  const u32 m = hdr_size(h);
+  if (!gone && m >= n) return 0;
+  u32 b = 0;
+  if (!gone) {
+    const u32* l = clits(db, c);
+    b += 2;
+    for (u32 q = 0; q < m; q++) b += varint_bytes(proof_lit_value(db, l[q]));
+    if (o) {
+      *o++ = 'a';
+      for (u32 q = 0; q < m; q++) o = put_varint(o, proof_lit_value(db, l[q]));
+      *o++ = 0;
+    }
+  }
+  b += 2;
+  for (u32 q = 0; q < n; q++) b += varint_bytes(proof_lit_value(db, toks[s + q]));
+  if (o) {
+    *o++ = 'd';
+    for (u32 q = 0; q < n; q++) o = put_varint(o, proof_lit_value(db, toks[s + q]));
+    *o = 0;
+  }
+  return b;
+}
+// a root unit as a line of its own ('a' u)
+PF_HD u32 unit_proof_line(const DB& db, u32 u, u8* o) {
+  const u32 v = proof_lit_value(db, u);
+  if (o) {
+    *o++ = 'a';
+    o = put_varint(o, v);
+    *o = 0;
+  }
+  return 2 + varint_bytes(v);
+}
+
+// DRAT lines of a call that rewrites clauses in place (equivalent-literal
+// substitution and its top-level shrink), from the clause at handle c
+// before the call (pool 'old') and now: a clause gone now is deleted; a
+// clause whose literals changed gets its new form added (phase 0) and its
+// old form deleted (phase 1).  Bytes of the phase's line; written to o
+// when not null.
+PF_HD u32 rewrite_proof_clause(const DB& db, const u32* old, u32 c, int phase, u8* o) {
+  const u32 ho = old[c], hn = db.lits[c];
+  if (st_deleted(hdr_flags(ho))) return 0;
+  const u32 no = hdr_size(ho), nn = hdr_size(hn);
+  const u32* lo = old + c + CL_HDR;
+  const u32* ln = db.lits + c + CL_HDR;
+  const bool gone = st_deleted(hdr_flags(hn));
+  if (!gone) {
+    bool same = no == nn;
+    for (u32 q = 0; same && q < no; q++) same = lo[q] == ln[q];
+    if (same) return 0;
+  }
+  if (phase == 0 && gone) return 0;
+  const u32* l = phase == 0 ? ln : lo;
+  const u32 n = phase == 0 ? nn : no;
+  u32 b = 2;
+  for (u32 q = 0; q < n; q++) b += varint_bytes(proof_lit_value(db, l[q]));
+  if (o) {
+    *o++ = phase == 0 ? 'a' : 'd';
+    for (u32 q = 0; q < n; q++) o = put_varint(o, proof_lit_value(db, l[q]));
+    *o = 0;
+  }
+  return b;
 }
 
 // ---------------------------------------------------------------- ELS

@@ -223,6 +223,40 @@ void fill_u8(u8* dst, u8 value, size_t n, Stream* st) {
   if (!n) return;
   KLAUNCH(fill_u8_k, grid_for((n + 15) / 16, 256), 256, st, dst, value, n);
 }
+struct Marker {
+  hipEvent_t e;
+};
+Marker* marker_record(Stream* st) {
+  Marker* m = new Marker();
+  PFS_HIP(hipEventCreateWithFlags(&m->e, hipEventDisableTiming));
+  PFS_HIP(hipEventRecord(m->e, st->s));
+  return m;
+}
+bool marker_done(Marker* m) {
+  const hipError_t e = hipEventQuery(m->e);
+  if (e == hipErrorNotReady) return false;
+  if (e != hipSuccess) (void)hipGetLastError();
+  return true;
+}
+void marker_wait(Marker* m) {
+  if (hipEventSynchronize(m->e) != hipSuccess) (void)hipGetLastError();
+}
+void marker_free(Marker* m) {
+  if (!m) return;
+  (void)hipEventDestroy(m->e);
+  delete m;
+}
+bool stream_wait(Stream* st) {
+  if (hipStreamSynchronize(st->s) == hipSuccess) return true;
+  (void)hipGetLastError();
+  return false;
+}
+void dev_mem_info(size_t* fr, size_t* tot) {
+  if (hipMemGetInfo(fr, tot) != hipSuccess) {
+    (void)hipGetLastError();
+    *fr = *tot = 0;
+  }
+}
 Event* event_record(Stream* st) {
   Event* e = new Event();
   PFS_HIP(hipEventCreate(&e->e));
@@ -526,24 +560,35 @@ __device__ inline void otp_tile(const DB& db, u32 c0, u32 c1, F f) {
     }
   }
 }
-__global__ void __launch_bounds__(OTP_TB) otp_count_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, u32* counts) {
+// the tile of workgroup b: b itself, or with xcd the tiles taken in runs,
+// one run per XCD (workgroups go to the 8 XCDs round robin, so XCD x runs
+// the workgroups b = x mod 8; it gets tiles start(x) .. start(x + 1) - 1)
+__device__ inline u32 otp_tile_of(u32 b, u32 nblk, int xcd) {
+  if (!xcd) return b;
+  const u32 q = nblk >> 3, r = nblk & 7u, x = b & 7u;
+  return x * q + min(x, r) + (b >> 3);
+}
+__global__ void __launch_bounds__(OTP_TB) otp_count_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, int xcd,
+                                                      u32* counts) {
   extern __shared__ u32 lc[];
   for (u32 j = threadIdx.x; j < nr; j += OTP_TB) lc[j] = 0;
   __syncthreads();
-  const u32 c0 = blockIdx.x * ct, c1 = min(db.ncls, c0 + ct);
+  const u32 t = otp_tile_of(blockIdx.x, nblk, xcd);
+  const u32 c0 = t * ct, c1 = min(db.ncls, c0 + ct);
   otp_tile(db, c0, c1, [&](u32 l, u32) { atomicAdd(&lc[l >> bits], 1u); });
   __syncthreads();
-  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) counts[(size_t)j * nblk + blockIdx.x] = lc[j];
+  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) counts[(size_t)j * nblk + t] = lc[j];
 }
 // TABLE: pairs (literal, handle) in one 8-byte word each (one store per
 // occurrence); otherwise the literals alone
 template <bool TABLE>
-__global__ void __launch_bounds__(OTP_TB) otp_part_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, const u32* offs,
-                                                     void* out) {
+__global__ void __launch_bounds__(OTP_TB) otp_part_k(DB db, u32 ct, int bits, u32 nr, u32 nblk, int xcd,
+                                                     const u32* offs, void* out) {
   extern __shared__ u32 lc[];
-  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) lc[j] = offs[(size_t)j * nblk + blockIdx.x];
+  const u32 t = otp_tile_of(blockIdx.x, nblk, xcd);
+  for (u32 j = threadIdx.x; j < nr; j += OTP_TB) lc[j] = offs[(size_t)j * nblk + t];
   __syncthreads();
-  const u32 c0 = blockIdx.x * ct, c1 = min(db.ncls, c0 + ct);
+  const u32 c0 = t * ct, c1 = min(db.ncls, c0 + ct);
   otp_tile(db, c0, c1, [&](u32 l, u32 c) {
     const u32 p = atomicAdd(&lc[l >> bits], 1u);
     if (TABLE)
@@ -622,7 +667,7 @@ __global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 n
       hist[base + j] = sz;
     }
 }
-bool otp_plan(size_t n2, size_t ncls, OtpPlan* p) {
+bool otp_plan(size_t n2, size_t ncls, OtpPlan* p, int tile_log2, int xcd) {
   // LDS per workgroup: nr words in the tile kernels, 1 << bits words in
   // the range kernel (both within 64 KiB); ranges of 8192 literals once
   // ranges of 4096 would number 2048 or more
@@ -630,10 +675,15 @@ bool otp_plan(size_t n2, size_t ncls, OtpPlan* p) {
   if ((n2 >> bits) >= 2048) bits = 13;
   const size_t nr = (n2 + ((size_t)1 << bits) - 1) >> bits;
   if (nr > OTP_MAX_RANGES || ncls == 0) return false;
-  // tiles of at least 32 clauses per range: fewer, longer segments of
-  // pairs per range (the partition's writes), fewer counts
+  // tiles of 32 clauses per range (longer segments of pairs per range for
+  // the partition's writes, fewer counts), at most 16,384 clauses: on the
+  // heavy-tailed round (2,442 ranges) tiles of 16,384 clauses instead of
+  // 131,072 gave 2,442 workgroups instead of 306 and took the count and
+  // partition kernels from 0.56 + 2.69 to 0.38 + 2.24 ms
   size_t ct = 4096;
-  while (ct < 32 * nr) ct <<= 1;
+  while (ct < 32 * nr && ct < 16384) ct <<= 1;
+  if (tile_log2 >= 10) ct = (size_t)1 << tile_log2;
+  p->xcd = xcd;
   p->bits = bits;
   p->nr = (u32)nr;
   p->ct = (u32)ct;
@@ -646,14 +696,15 @@ bool otp_plan(size_t n2, size_t ncls, OtpPlan* p) {
 void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, void* pairs, bool table, u32* hist,
                  void* tmp, Stream* st) {
   const size_t lds_r = (size_t)p.nr * 4, lds_b = ((size_t)4 << p.bits);
-  KLAUNCH_LDS(otp_count_k, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts);
+  KLAUNCH_LDS(otp_count_k, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, p.xcd, counts);
   scan_u32(counts, counts, p.ncounts, counts + p.ncounts, tmp, st);
   if (table) {
-    KLAUNCH_LDS(otp_part_k<true>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, pairs);
+    KLAUNCH_LDS(otp_part_k<true>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, p.xcd, counts, pairs);
     KLAUNCH_LDS(otp_range_k<true>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts, pairs,
                 hist);
   } else {
-    KLAUNCH_LDS(otp_part_k<false>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, counts, pairs);
+    KLAUNCH_LDS(otp_part_k<false>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, p.xcd, counts,
+                pairs);
     KLAUNCH_LDS(otp_range_k<false>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts,
                 pairs, hist);
   }
@@ -899,15 +950,24 @@ __device__ inline unsigned long long wave_min64(unsigned long long x) {
   }
   return x;
 }
-// the hard-break key is the minimum over the wavefront first, then one
-// atomic per wavefront (a global atomic per heavy candidate serialised on
-// one word: 1 ms on the 10M-variable heavy-tailed formula)
+// the hard-break key is a minimum over the grid: per lane over a strided
+// range of variables, then over the workgroup, then one atomic per
+// workgroup into a bounded grid (one atomic per wavefront over 10M
+// variables serialised 1 ms on the one word)
 __global__ void lcve_classify_k(DB db, LcveArgs a) {
-  const u32 v = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  __shared__ unsigned long long wmin[ST / 64];
   unsigned long long hk = ~0ull;
-  if (v <= db.nvars) hk = lcve_classify_var(db, a, v);
+  for (u32 v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= db.nvars; v += gridDim.x * blockDim.x) {
+    const unsigned long long k = lcve_classify_var(db, a, v);
+    hk = k < hk ? k : hk;
+  }
   hk = wave_min64(hk);
-  if ((threadIdx.x & 63) == 0 && hk != ~0ull) at_min64(a.hard, hk);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = hk;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (u32 w = 1; w < ST / 64; w++) hk = wmin[w] < hk ? wmin[w] : hk;
+    if (hk != ~0ull) at_min64(a.hard, hk);
+  }
 }
 __device__ unsigned long long lcve_classify_var(const DB& db, const LcveArgs& a, u32 v) {
   const u32 ps = a.hist[V2L(v)], ns = a.hist[V2L(v) | 1];
@@ -926,7 +986,7 @@ __device__ unsigned long long lcve_classify_var(const DB& db, const LcveArgs& a,
   return cand && hard ? ((unsigned long long)score << 32) | v : ~0ull;
 }
 void k_lcve_classify(const DB& db, const LcveArgs& a, Stream* st) {
-  KLAUNCH(lcve_classify_k, grid_for(db.nvars, 256), 256, st, db, a);
+  KLAUNCH(lcve_classify_k, std::min(grid_for(db.nvars, ST), 2048u), ST, st, db, a);
 }
 __global__ void lcve_flag_k(DB db, LcveArgs a, u32* flag) {
   const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -969,45 +1029,42 @@ void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state
   KLAUNCH(lcve_reject_k, grid_for(n, 256), 256, st, cand, n, cflag, pos, state);
 }
 // one election step over the undecided candidates idx[0..n) (all when idx
-// is null); the ones still undecided are appended to 'next'
-// LDS reserved per one-wavefront workgroup of the MIS step: 2 wavefronts per
-// SIMD, as PV_LDS (Tseitin round: 0 / 20 / 40 KiB -> mis_step_k 3.34 / 2.76
-// / 3.18 ms)
-#ifndef MIS_LDS
-#define MIS_LDS 20480
-#endif
+// is null); the ones still undecided are appended to 'next'.  The launch
+// reserves LDS per one-wavefront workgroup only to bound occupancy (option
+// "mislds"; 20 KiB leaves 2 wavefronts per SIMD: Tseitin round 0 / 20 / 40
+// KiB -> mis_step_k 3.34 / 2.76 / 3.18 ms)
 __global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
-                           u32* next, u32* nnext) {
+                           u32* resume, u32* next, u32* nnext) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const u32 i = idx ? idx[t] : t;
-  const bool undecided = mis_step(db, i, cand, pos, state, blk) != 0;
+  const bool undecided = mis_step(db, i, cand, pos, state, blk, resume) != 0;
   wave_append(undecided, i, next, nnext);
 }
 void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
-                u32* next, u32* nnext, Stream* st) {
+                u32* resume, u32* next, u32* nnext, u32 lds, Stream* st) {
   if (!n) return;
-  KLAUNCH_LDS(mis_step_k, grid_for(n, 64), 64, MIS_LDS, st, db, cand, idx, n, pos, state, blk, next, nnext);
+  KLAUNCH_LDS(mis_step_k, grid_for(n, 64), 64, lds, st, db, cand, idx, n, pos, state, blk, resume, next, nnext);
 }
 // A worklist pass whose length is read on the device (n_in): a bounded grid
 // strides over the list, so several passes are queued without a host read;
 // passes over an empty list return at once.  iters counts the non-empty passes.
 __global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32* n_in, u32* pos, u32* state,
-                               u32* blk, u32* next, u32* nnext, u32* iters) {
+                               u32* blk, u32* resume, u32* next, u32* nnext, u32* iters) {
   const u32 n = *n_in;
   if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(iters, 1u);
   for (u32 base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {  // wave-uniform
     const u32 t = base + threadIdx.x;
     const u32 i = t < n ? idx[t] : 0u;
-    const bool undecided = t < n && mis_step(db, i, cand, pos, state, blk) != 0;
+    const bool undecided = t < n && mis_step(db, i, cand, pos, state, blk, resume) != 0;
     wave_append(undecided, i, next, nnext);
   }
 }
 void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
-                    u32* state, u32* blk, u32* next, u32* nnext, u32* iters, Stream* st) {
+                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, Stream* st) {
   if (!n_bound) return;
   const unsigned grid = std::min(grid_for(n_bound, 64), 2048u);
-  KLAUNCH_LDS(mis_step_dev_k, grid, 64, MIS_LDS, st, db, cand, idx, n_in, pos, state, blk, next, nnext, iters);
+  KLAUNCH_LDS(mis_step_dev_k, grid, 64, lds, st, db, cand, idx, n_in, pos, state, blk, resume, next, nnext, iters);
 }
 __global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1462,6 +1519,14 @@ void k_ere_pairs(const DB& db, const ErePairs& a, u32 npairs, Stream* st) {
   if (!npairs) return;
   KLAUNCH(ere_pairs_k, grid_for(npairs, 256), 256, st, db, a, npairs);
 }
+__global__ void ere_all_matches_k(DB db, ErePairs a, u32 n, const u32* off, u32* out) {
+  const u32 o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < n) ere_all_matches(db, a, o, off, out);
+}
+void k_ere_all_matches(const DB& db, const ErePairs& a, u32 n, const u32* off, u32* out, Stream* st) {
+  if (!n) return;
+  KLAUNCH(ere_all_matches_k, grid_for(n, 64), 64, st, db, a, n, off, out);
+}
 
 // ---------------------------------------------------------------- small helpers
 __global__ void assign_units_k(DB db, const u32* lits, u32 n) {
@@ -1663,21 +1728,28 @@ void k_count(const DB& db, u64* out2, Stream* st) {
   if (!db.ncls) return;
   KLAUNCH(count_k, std::min(grid_for(db.ncls, ST), RED_BLOCKS), ST, st, db, (unsigned long long*)out2);
 }
-__global__ void live_k(DB db, u32* tc, u32* tl) {
+// sel: 0 every live clause, 1 live originals only, 2 live learnt clauses only
+__device__ inline bool compact_sel(u32 h, int sel) {
+  const u8 f = hdr_flags(h);
+  if (st_deleted(f)) return false;
+  return sel == 0 || (sel == 1) == !st_learnt(f);
+}
+__global__ void live_k(DB db, u32* tc, u32* tl, int sel) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= db.ncls) return;
   const u32 h = chead(db, db.c_off[i])[0];
-  const bool live = !st_deleted(hdr_flags(h));
+  const bool live = compact_sel(h, sel);
   tc[i] = live ? 1u : 0u;
   tl[i] = live ? CL_HDR + hdr_size(h) : 0u;  // pool words
 }
-__global__ void compact_k(DB src, DB dst, const u32* tc, const u32* tl) {
+// base (device, nullable): clause index and pool word where the selection starts
+__global__ void compact_k(DB src, DB dst, const u32* tc, const u32* tl, int sel, const u32* base) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= src.ncls) return;
   const u32* a = chead(src, src.c_off[i]);
   const u32 h = a[0];
-  if (st_deleted(hdr_flags(h))) return;
-  const u32 nid = tc[i], noff = tl[i], n = hdr_size(h);
+  if (!compact_sel(h, sel)) return;
+  const u32 nid = tc[i] + (base ? base[0] : 0u), noff = tl[i] + (base ? base[1] : 0u), n = hdr_size(h);
   dst.c_off[nid] = noff;
   u32* b = dst.lits + noff;
   b[0] = make_hdr(n, (u8)(hdr_flags(h) & ~F_MOLTEN));
@@ -1688,10 +1760,33 @@ void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u3
     fill_u32(totals, 0, 2, st);
     return;
   }
-  KLAUNCH(live_k, grid_for(src.ncls, 256), 256, st, src, tmp_cls, tmp_lit);
+  KLAUNCH(live_k, grid_for(src.ncls, 256), 256, st, src, tmp_cls, tmp_lit, 0);
   scan_u32(tmp_cls, tmp_cls, src.ncls, totals, tmp, st);
   scan_u32(tmp_lit, tmp_lit, src.ncls, totals + 1, tmp, st);
-  KLAUNCH(compact_k, grid_for(src.ncls, 256), 256, st, src, dst, tmp_cls, tmp_lit);
+  KLAUNCH(compact_k, grid_for(src.ncls, 256), 256, st, src, dst, tmp_cls, tmp_lit, 0, (const u32*)nullptr);
+}
+// the live clauses with the originals first and the learnt clauses after
+// them, each part in clause order (totals[0..1]: clauses and words in all;
+// totals[2..3] scratch)
+__global__ void add_totals_k(u32* t) {
+  t[0] += t[2];
+  t[1] += t[3];
+}
+void k_compact_originals_first(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
+                               Stream* st) {
+  if (!src.ncls) {
+    fill_u32(totals, 0, 4, st);
+    return;
+  }
+  KLAUNCH(live_k, grid_for(src.ncls, 256), 256, st, src, tmp_cls, tmp_lit, 1);
+  scan_u32(tmp_cls, tmp_cls, src.ncls, totals, tmp, st);
+  scan_u32(tmp_lit, tmp_lit, src.ncls, totals + 1, tmp, st);
+  KLAUNCH(compact_k, grid_for(src.ncls, 256), 256, st, src, dst, tmp_cls, tmp_lit, 1, (const u32*)nullptr);
+  KLAUNCH(live_k, grid_for(src.ncls, 256), 256, st, src, tmp_cls, tmp_lit, 2);
+  scan_u32(tmp_cls, tmp_cls, src.ncls, totals + 2, tmp, st);
+  scan_u32(tmp_lit, tmp_lit, src.ncls, totals + 3, tmp, st);
+  KLAUNCH(compact_k, grid_for(src.ncls, 256), 256, st, src, dst, tmp_cls, tmp_lit, 2, (const u32*)totals);
+  KLAUNCH(add_totals_k, 1, 1, st, totals);
 }
 
 // ---------------------------------------------------------------- variable map
@@ -1915,6 +2010,43 @@ __global__ void parse_build_k(DB db, const u32* lits, const u32* ends, u32* err)
 void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Stream* st) {
   if (!db.ncls) return;
   KLAUNCH(parse_build_k, grid_for(db.ncls, 256), 256, st, db, lits, ends, err);
+}
+// ingestion proof: bytes per line (units first, then the text clauses),
+// then the lines at their scanned offsets
+__global__ void parse_proof_k(DB db, const u32* toks, const u32* ends, const u32* units, u32 nu, u32 nc,
+                              const u32* off, u8* out, u32* bytes) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nu + nc) return;
+  u8* o = off ? out + off[i] : nullptr;
+  const u32 b = i < nu ? unit_proof_line(db, units[i], o) : parse_proof_clause(db, toks, ends, i - nu, o);
+  if (!off) bytes[i] = b;
+}
+void k_parse_proof(const DB& db, const u32* toks, const u32* ends, const u32* units, u32 nu, u32 nc,
+                   const u32* off, u8* out, u32* bytes, Stream* st) {
+  if (!(nu + nc)) return;
+  KLAUNCH(parse_proof_k, grid_for((size_t)nu + nc, 256), 256, st, db, toks, ends, units, nu, nc, off, out, bytes);
+}
+// rewrite proof: line i < nu is unit i, then the additions (phase 0) of
+// clause indices 0..ncls, then the deletions (phase 1)
+__global__ void rewrite_proof_k(DB db, const u32* old, const u32* units, u32 nu, const u32* off, u8* out,
+                                u32* bytes) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nu + 2 * db.ncls) return;
+  u8* o = off ? out + off[i] : nullptr;
+  u32 b;
+  if (i < nu) {
+    b = unit_proof_line(db, units[i], o);
+  } else {
+    const u32 j = i - nu, phase = j >= db.ncls;
+    b = rewrite_proof_clause(db, old, db.c_off[phase ? j - db.ncls : j], (int)phase, o);
+  }
+  if (!off) bytes[i] = b;
+}
+void k_rewrite_proof(const DB& db, const u32* old, const u32* units, u32 nu, const u32* off, u8* out, u32* bytes,
+                     Stream* st) {
+  const size_t n = (size_t)nu + 2 * (size_t)db.ncls;
+  if (!n) return;
+  KLAUNCH(rewrite_proof_k, grid_for(n, 256), 256, st, db, old, units, nu, off, out, bytes);
 }
 // after the literal sort: duplicates dropped, tautologies removed, empty
 // clauses reported, unit clauses turned into unit records (toClause,

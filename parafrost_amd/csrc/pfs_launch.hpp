@@ -30,6 +30,14 @@ void d2d(void* dst, const void* src, size_t bytes, Stream*);
 void fill_u32(u32* dst, u32 value, size_t n, Stream*);  // fill kernel, no memset nodes
 void fill_u8(u8* dst, u8 value, size_t n, Stream*);
 // stage timing: events recorded on the stream
+// the position of a stream at a point in its queue (a timing-free event)
+struct Marker;
+Marker* marker_record(Stream*);
+bool marker_done(Marker*);  // the stream has passed it (no wait)
+void marker_wait(Marker*);
+void marker_free(Marker*);
+bool stream_wait(Stream*);  // waits for the stream's queued work; false on a device error
+void dev_mem_info(size_t* free_bytes, size_t* total_bytes);  // of the current device
 struct Event;
 Event* event_record(Stream*);
 float event_ms(Event* a, Event* b);  // synchronises on b
@@ -68,9 +76,11 @@ struct OtpPlan {
   u32 ct;         // clauses per tile
   u32 nblk;       // tiles
   size_t ncounts; // nr * nblk
+  int xcd;        // tiles spread by XCD: each of the 8 XCDs takes a run of consecutive tiles
 };
-// false when the literal space needs more than OTP_MAX_RANGES ranges
-bool otp_plan(size_t n2, size_t ncls, OtpPlan* p);
+// false when the literal space needs more than OTP_MAX_RANGES ranges;
+// tile_log2: clauses per tile (0: planned)
+bool otp_plan(size_t n2, size_t ncls, OtpPlan* p, int tile_log2 = 0, int xcd = 0);
 // counts: p.ncounts + 1 words; pairs: 8-byte aligned, two words per live
 // literal occurrence with 'table', else one.  Without 'table' only hist is
 // written; with it also ot_off (n2 + 1 words), ot_size and ot_refs, and
@@ -110,13 +120,16 @@ void k_lcve_positions(const u32* cand, u32 n, u32* pos, Stream*);
 void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state, Stream*);
 // one election step over the candidates idx[0..n) (all when null); blk: per
 // candidate, an undecided earlier neighbour (NOPOS before the first step);
-// the still undecided candidates are appended to next (count in *nnext)
+// resume: per candidate, the list entry its scan goes on from (0 before the
+// first step); the still undecided candidates are appended to next (count
+// in *nnext)
 // the same over the list idx[0..*n_in) (n_in on the device, at most n_bound);
 // *iters counts the passes over a non-empty list
+// lds: bytes of LDS reserved per one-wavefront workgroup (occupancy)
 void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
-                    u32* state, u32* blk, u32* next, u32* nnext, u32* iters, Stream*);
+                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, Stream*);
 void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
-                u32* next, u32* nnext, Stream*);
+                u32* resume, u32* next, u32* nnext, u32 lds, Stream*);
 void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
 void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_tmp, u32* out,
                    u32* nout, void* tmp, Stream*);
@@ -165,6 +178,8 @@ void k_apply_deletions(const DB& db, const u32* ids, u32 n, Stream*);
 void k_ht_build(const DB& db, const HashTable& ht, int maxsize, Stream*);
 void k_ere_pair_counts(const DB& db, const Options& o, const u32* pv, u32 n, u32* np, u64* total, Stream*);
 void k_ere_pairs(const DB& db, const ErePairs& a, u32 npairs, Stream*);
+// the full match lists of the n overflow records a.ovf[0..n) (offsets off[n])
+void k_ere_all_matches(const DB& db, const ErePairs& a, u32 n, const u32* off, u32* out, Stream*);
 
 // ---- small helpers
 void k_assign_units(const DB& db, const u32* lits, u32 n, Stream*);  // assign + FROZEN
@@ -195,6 +210,12 @@ void k_cand_work(const DB& db, const u32* cand, u32 n, u8* mark, u64* out2, Stre
 // stable compaction of the live clauses into dst (ids and pool renumbered)
 void k_compact(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
                Stream*);
+// the same with the live originals first and the live learnt clauses after
+// them (each part in clause order): the order in which the reference
+// extracts a formula it wrote back (orgs, then learnts; newBeginning,
+// src/cpu/simplify.cpp:251-269, and extract, :147-149).  totals: 4 words.
+void k_compact_originals_first(const DB& src, DB& dst, u32* tmp_cls, u32* tmp_lit, void* tmp, u32* totals,
+                               Stream*);
 
 // ---- variable map (Solver::map, src/cpu/vmap.cpp:61-135; VMAP, src/cpu/vmap.h)
 // mapped[0..nvars] (old variable -> new, 0 dropped) and
@@ -243,6 +264,18 @@ void k_parse_build(const DB& db, const u32* lits, const u32* ends, u32* err, Str
 // sorted clauses: duplicates dropped, tautologies and units removed
 // (ulit[i]: the unit), err bit 8: empty clause
 void k_parse_dedup(const DB& db, u32* ulit, u32* err, Stream*);
+// DRAT lines of an in-place rewrite of the clauses (old: the pool before
+// it, same handles): the nu units, then the new forms of the changed
+// clauses, then the deletions of their old forms and of the clauses gone;
+// off null: bytes per line (nu + 2 ncls) into bytes; else written at out + off[i]
+void k_rewrite_proof(const DB& db, const u32* old, const u32* units, u32 nu, const u32* off, u8* out, u32* bytes,
+                     Stream*);
+// DRAT lines of the ingestion from its end state: the nu root units, then
+// per text clause its deletion / shrunken addition (parse_proof_clause);
+// off null: the byte count of each line into bytes[nu + nc]; else the
+// lines written at out + off[i]
+void k_parse_proof(const DB& db, const u32* toks, const u32* ends, const u32* units, u32 nu, u32 nc,
+                   const u32* off, u8* out, u32* bytes, Stream*);
 // unit clauses in order: first of each variable assigned and written to
 // the trail (*total of them), opposite units set err bit 8; first needs
 // nvars+1 words, win and pos n words

@@ -28,7 +28,8 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_device_count", "pfs_batch_simplify", "pfs_append", "pfs_assume", "pfs_server_create",
            "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy", "pfs_map",
            "pfs_download_map", "pfs_map_lits", "pfs_parse_dimacs", "pfs_download_trail",
-           "pfs_decompose")
+           "pfs_decompose", "pfs_arena_configure", "pfs_arena_create", "pfs_arena_destroy",
+           "pfs_create_in_arena", "pfs_arena_get_stats")
 
 
 class PfsError(RuntimeError):
@@ -37,6 +38,12 @@ class PfsError(RuntimeError):
 
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("mem_budget", ctypes.c_uint64), ("keep_input", ctypes.c_int32)]
+
+
+class _ArenaStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("capacity", "used", "peak", "largest_hole", "holes", "slabs",
+                                               "compactions", "moved_blocks", "moved_bytes", "slab_allocs",
+                                               "pending")]
 
 
 class _RoundStats(ctypes.Structure):
@@ -123,6 +130,12 @@ def load_library():
     L.pfs_download_trail.argtypes = [vp, vp, ctypes.POINTER(u64)]
     L.pfs_decompose.argtypes = [vp, u32, ctypes.POINTER(_ElsInfo)]
     L.pfs_device_count.restype = i32
+    L.pfs_arena_configure.argtypes = [i32, u64, u64]
+    L.pfs_arena_create.argtypes = [i32, u64, ctypes.POINTER(vp)]
+    L.pfs_arena_destroy.argtypes = [vp]
+    L.pfs_arena_destroy.restype = None
+    L.pfs_create_in_arena.argtypes = [ctypes.POINTER(_Config), vp, ctypes.POINTER(vp)]
+    L.pfs_arena_get_stats.argtypes = [vp, i32, ctypes.POINTER(_ArenaStats)]
     _LIB = L
     return L
 
@@ -146,15 +159,58 @@ class Result:
     proof: bytes = b""  # binary DRAT lines (option "proof")
 
 
+def arena_stats(arena=None, device: int = 0) -> dict:
+    """pfs_arena_get_stats of a private Arena, or of the device's default arena."""
+    L = load_library()
+    out = _ArenaStats()
+    h = arena._h if arena is not None else None
+    st = L.pfs_arena_get_stats(h, device, ctypes.byref(out))
+    if st != PFS_OK:
+        raise PfsError(f"pfs_arena_get_stats: {STATUS_NAMES.get(st, st)}")
+    return {n: int(getattr(out, n)) for n, _ in _ArenaStats._fields_}
+
+
+def arena_configure(device: int, first_slab: int, limit: int = 0) -> bool:
+    """pfs_arena_configure: False once the device's default arena exists."""
+    return load_library().pfs_arena_configure(device, first_slab, limit) == PFS_OK
+
+
+class Arena:
+    """A private device arena of a fixed capacity (pfs_arena_create):
+    contexts created with arena=... share it."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        L = load_library()
+        self._L = L
+        self._h = ctypes.c_void_p()
+        st = L.pfs_arena_create(device, nbytes, ctypes.byref(self._h))
+        if st != PFS_OK:
+            raise PfsError(f"pfs_arena_create: {STATUS_NAMES.get(st, st)}")
+        self.device = device
+
+    def stats(self) -> dict:
+        return arena_stats(self, self.device)
+
+    def close(self):
+        """Destroys the arena: every context in it must be closed first."""
+        if self._h:
+            self._L.pfs_arena_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+
 class Simplifier:
     """One simplifier context (device state of one formula)."""
 
-    def __init__(self, device: int = 0, mem_budget: int = 0, keep_input: bool = False, **options):
+    def __init__(self, device: int = 0, mem_budget: int = 0, keep_input: bool = False, arena: Arena = None,
+                 **options):
         L = load_library()
         self._L = L
         self._h = ctypes.c_void_p()
         cfg = _Config(device, mem_budget, 1 if keep_input else 0)
-        st = L.pfs_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if arena is not None:
+            st = L.pfs_create_in_arena(ctypes.byref(cfg), arena._h, ctypes.byref(self._h))
+        else:
+            st = L.pfs_create(ctypes.byref(cfg), ctypes.byref(self._h))
         if st != PFS_OK:
             raise PfsError(f"pfs_create: {STATUS_NAMES.get(st, st)}: {L.pfs_last_error(None).decode()}")
         self.maxvar = 0
@@ -231,6 +287,17 @@ class Simplifier:
         info = _ElsInfo()
         self._check(self._L.pfs_decompose(self._h, rounds, ctypes.byref(info)), "pfs_decompose")
         return {k: int(getattr(info, k)) for k, _ in _ElsInfo._fields_}
+
+    def proof(self) -> bytes:
+        """The binary DRAT lines of the last call (pfs_proof_size,
+        pfs_download_proof): a simplify or an ingestion with "proof" on."""
+        n = ctypes.c_uint64()
+        self._check(self._L.pfs_proof_size(self._h, ctypes.byref(n)), "pfs_proof_size")
+        if not n.value:
+            return b""
+        pf = np.zeros(n.value, np.uint8)
+        self._check(self._L.pfs_download_proof(self._h, _ptr(pf)), "pfs_download_proof")
+        return pf.tobytes()
 
     def trail(self) -> np.ndarray:
         """The root trail of the resident formula (pfs_download_trail)."""

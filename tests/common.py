@@ -138,12 +138,25 @@ def append_clean(assign, offsets, lits):
 def proof_digest(proof: bytes) -> str:
     """sha256 of a binary DRAT proof in canonical form: each run of
     consecutive unit-addition lines sorted (the parallel propagation claims
-    units level by level, so their order within a run is free).  A line ends
-    at its 0 byte: every literal's last varint byte is nonzero."""
+    units level by level, so their order within a run is free), and, in a
+    proof that ends with the empty clause, the run of unit lines right
+    before it left out: a propagation that ends in a conflict writes the
+    units it assigned before meeting it, which depends on the order it
+    visits clauses in (the sequential propagation writes them one by one,
+    the level-parallel one stops at the level); they are implied by unit
+    propagation and carry nothing the empty clause's check needs.  A line
+    ends at its 0 byte: every literal's last varint byte is nonzero."""
     import hashlib
     h = hashlib.sha256()
     run = []
-    for line in proof.split(b"\0")[:-1]:
+    lines = proof.split(b"\0")[:-1]
+    if lines and lines[-1] == b"a":  # the empty clause: drop the unit run before it
+        k = len(lines) - 1
+        while k > 0 and lines[k - 1][:1] == b"a" and len(lines[k - 1]) > 1 and \
+                all(b >= 128 for b in lines[k - 1][1:-1]):
+            k -= 1
+        lines = lines[:k] + lines[-1:]
+    for line in lines:
         if line[:1] == b"a" and len(line) > 1 and all(b >= 128 for b in line[1:-1]):  # one literal
             run.append(line)
             continue

@@ -17,7 +17,8 @@ from tests.fuzz_cases import case as fuzz_case
 
 pytestmark = pytest.mark.gpu
 HARD = ("u3_n300.", "u3_n2000.", "lt_u3_n2000.")
-MANIFEST = [e for e in load_manifest() if not e["name"].startswith(HARD)]
+# the learnt-clause fixtures plant random learnt clauses (not implied): left out
+MANIFEST = [e for e in load_manifest() if not e["name"].startswith(HARD) and not e.get("learnt")]
 
 
 def gpu(maxvar, offsets, lits, trail=None, flags=None, lbd=None, vstate=None, **opts):
@@ -52,6 +53,8 @@ def test_golden_inputs(entry):
 @pytest.mark.parametrize("i", range(60))
 def test_fuzz_formulas(i):
     cnf, flags, lbd, ro, _ = fuzz_case(i)
+    if flags is not None:
+        pytest.skip("its learnt clauses are drawn at random, not implied: no equisatisfiability to check")
     prep, refuted = (cnf, False) if flags is not None else gen.prepare(cnf)
     if refuted:
         pytest.skip("refuted at the root")

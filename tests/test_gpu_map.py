@@ -25,13 +25,15 @@ MAPPED = [e for e in load_manifest() if "map" in e]
 
 
 def arena(res):
-    """Result clauses as the reference's arena lists: originals, then learnts."""
+    """Result clauses as the reference's arena lists: originals, then learnts
+    (flags: bit 0 learnt, bits 4-5 the learnt clause's usage)."""
     off = np.zeros(res.sizes.shape[0] + 1, np.int64)
     np.cumsum(res.sizes, out=off[1:])
     learnt = (res.flags & 1) != 0
     order = np.concatenate([np.nonzero(~learnt)[0], np.nonzero(learnt)[0]])
     lits = [res.lits[off[i]:off[i + 1]] for i in order]
-    return (res.sizes[order], learnt[order].astype(np.uint8), np.where(learnt[order], res.lbd[order], 0),
+    return (res.sizes[order], np.where(learnt[order], 1 | (res.flags[order] & 0x30), 0).astype(np.uint8),
+            np.where(learnt[order], res.lbd[order], 0),
             np.concatenate(lits) if lits else np.zeros(0, np.uint32))
 
 

@@ -83,7 +83,10 @@ def test_options_match_oracle(family, opt):
         s.close()
     kw = {"phases": 1, "bce_en": 1}
     kw.update(oracle_kw({k: v for k, v in o.items() if k not in ("literalsmul", "syncalways")}))
-    r = oracle.simplify(cnf.maxvar, cnf.offsets, cnf.lits, nassigned=int(cnf.trail.shape[0]), det=1, **kw)
+    vs = np.zeros(cnf.maxvar + 1, np.uint8)
+    vs[np.asarray(cnf.trail, np.int64) >> 1] = 2  # root-assigned variables are frozen
+    r = oracle.simplify(cnf.maxvar, cnf.offsets, cnf.lits, nassigned=int(cnf.trail.shape[0]), det=1, vstate=vs,
+                        **kw)
     d = diff(g, r)
     assert not d, d
     if g.rounds and r.elected.shape[0]:
