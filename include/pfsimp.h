@@ -272,6 +272,15 @@ pfs_status pfs_download_map(pfs_ctx* ctx, uint32_t* mapped);
  * root-assigned literal becomes trail_lit or its negation by value, a
  * dropped unassigned literal becomes 0. */
 pfs_status pfs_map_lits(pfs_ctx* ctx, uint32_t* lits, uint64_t n);
+/* pfs_map_lits for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation); no
+ * counterpart in the reference.  The literals are staged through a fixed
+ * 4 MiB device array of the library (one per device, held until the
+ * process ends) instead of a buffer of the context: the call allocates
+ * nothing, host or device.  At most 1,048,576 literals per call
+ * (PFS_BAD_INPUT beyond).  Every call on one device shares the array:
+ * calls that overlap on a device race. */
+pfs_status pfs_map_lits_noalloc(pfs_ctx* ctx, uint32_t* lits, uint64_t n);
 /* The binary DRAT lines of the last pfs_simplify or pfs_parse_dimacs call
  * with option "proof" (proof.cpp: the reference's writer): a caller that
  * keeps one proof file appends each call's lines. */

@@ -587,6 +587,14 @@ pfs_status pfs_map_lits(pfs_ctx* c, uint32_t* lits, uint64_t n) {
   });
 }
 
+pfs_status pfs_map_lits_noalloc(pfs_ctx* c, uint32_t* lits, uint64_t n) {
+  if (!c || (!lits && n)) return PFS_BAD_INPUT;
+  return guard(c, [&] {
+    c->s->map_lits_noalloc(lits, n);
+    return PFS_OK;
+  });
+}
+
 pfs_status pfs_download_stack(pfs_ctx* c, uint32_t* words) {
   if (!c || !words) return PFS_BAD_INPUT;
   if (!c->uploaded) return PFS_BAD_STATE;

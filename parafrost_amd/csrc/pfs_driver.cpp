@@ -66,6 +66,7 @@ Simplifier::Simplifier(int device, size_t budget, bool keep_input, DeviceArena* 
     : mem_(budget), keep_input_(keep_input) {
   st_ = stream_create(device);
   mem_.attach(arena ? arena : &DeviceArena::get(device), st_);
+  fixed_scratch_ = scratch_lookup(device);  // once per device; stream_create made the device current
 }
 
 Simplifier::~Simplifier() {
@@ -2106,6 +2107,23 @@ void Simplifier::map_lits(uint32_t* lits, uint64_t n) {
   VarMap m{P<u32>(map_), P<u32>(map_assign_), map_first_, map_first_true_};
   k_map_lits(m, P<u32>(tmp3_), n, st_);
   d2h(lits, tmp3_.p, n * 4, st_);
+  stream_sync(st_);
+}
+
+void Simplifier::map_lits_noalloc(uint32_t* lits, uint64_t n) {
+  if (!map_called_) throw std::logic_error("no variable map was computed");
+  if (n > SCRATCH_WORDS)
+    throw std::invalid_argument("pfs_map_lits_noalloc maps at most " + std::to_string(SCRATCH_WORDS) +
+                                " literals per call (" + std::to_string(n) + " given)");
+  for (uint64_t i = 0; i < n; i++)
+    if (lits[i] && (ABS(lits[i]) == 0 || ABS(lits[i]) > map_old_nvars_))
+      throw std::invalid_argument("literal out of range of the mapped numbering");
+  if (!map_applied_ || !n) return;
+  // the scratch is written in full before the kernel reads it: nothing to clear
+  h2d(fixed_scratch_, lits, n * 4, st_);
+  VarMap m{P<u32>(map_), P<u32>(map_assign_), map_first_, map_first_true_};
+  k_map_lits(m, fixed_scratch_, n, st_);
+  d2h(lits, fixed_scratch_, n * 4, st_);
   stream_sync(st_);
 }
 

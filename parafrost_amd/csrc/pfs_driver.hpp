@@ -150,6 +150,9 @@ class Simplifier {
   MapInfo map(bool force, double map_perc);
   void download_map(uint32_t* mapped);  // old variable count + 1 words
   void map_lits(uint32_t* lits, uint64_t n);  // VMAP::mapOrgs on caller literals
+  // the same staged through the device's fixed scratch array instead of a
+  // context buffer: no allocation of any kind; n <= SCRATCH_WORDS
+  void map_lits_noalloc(uint32_t* lits, uint64_t n);
   size_t mem_used() const { return mem_.used(); }
   size_t mem_peak() const { return mem_.peak(); }
   uint32_t nvars() const { return V_; }
@@ -204,6 +207,7 @@ class Simplifier {
   void stage_end();
 
   Stream* st_ = nullptr;
+  u32* fixed_scratch_ = nullptr;  // the device's fixed scratch array (its device address)
   DeviceMemory mem_;
   bool keep_input_;
 

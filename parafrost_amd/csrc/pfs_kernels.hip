@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 #include <stdexcept>
@@ -1873,6 +1874,33 @@ __global__ void map_lits_k(VarMap m, u32* lits, u64 n) {
 void k_map_lits(const VarMap& m, u32* lits, u64 n, Stream* st) {
   if (!n) return;
   KLAUNCH(map_lits_k, grid_for(n, 256), 256, st, m, lits, n);
+}
+
+// ---------------------------------------------------------------- fixed device scratch
+// Staging memory of the entry points that allocate nothing
+// (pfs_map_lits_noalloc): one array per device, held until the process
+// ends.  Host code never uses the array's name as an address: its device
+// address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and kept.  Every call on a device shares the array, so
+// calls that overlap on one device race.
+static __device__ __align__(256) u32 scratch[SCRATCH_WORDS];
+static std::mutex g_scratch_m;
+static std::map<int, u32*> g_scratch;
+u32* scratch_lookup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_m);
+  auto it = g_scratch.find(device);
+  if (it != g_scratch.end()) return it->second;
+  int cur = -1;
+  PFS_HIP(hipGetDevice(&cur));
+  if (cur != device) throw std::runtime_error("scratch lookup: device " + std::to_string(device) + " is not current");
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  }
+  g_scratch[device] = (u32*)p;
+  return (u32*)p;
 }
 
 // ---------------------------------------------------------------- output packing

@@ -228,6 +228,10 @@ void k_map_clauses(const DB& db, const VarMap& m, Stream*);
 void k_map_vars(const VarMap& m, u32 nvars_old, const u32* vorg, const u8* vstate, const u8* assumed, u32* vorg2,
                 u8* vstate2, u8* assumed2, u32* assign2, Stream*);
 void k_map_lits(const VarMap& m, u32* lits, u64 n, Stream*);
+// the device address of the fixed scratch array (SCRATCH_WORDS words) of
+// 'device', which must be current; looked up once per device
+constexpr size_t SCRATCH_WORDS = (size_t)1 << 20;  // 4 MiB
+u32* scratch_lookup(int device);
 // ---- output packing: offsets[ncls+1], literals, flags, lbd of the live
 // clauses in clause order (start_tmp: ncls words)
 void k_pack_cnf(const DB& db, u32* start_tmp, void* tmp, u32* total, u64* offsets, u32* lits, u8* flags, u32* lbd,

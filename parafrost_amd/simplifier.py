@@ -29,7 +29,7 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy", "pfs_map",
            "pfs_download_map", "pfs_map_lits", "pfs_parse_dimacs", "pfs_download_trail",
            "pfs_decompose", "pfs_arena_configure", "pfs_arena_create", "pfs_arena_destroy",
-           "pfs_create_in_arena", "pfs_arena_get_stats")
+           "pfs_create_in_arena", "pfs_arena_get_stats", "pfs_map_lits_noalloc")
 
 
 class PfsError(RuntimeError):
@@ -126,6 +126,7 @@ def load_library():
     L.pfs_map.argtypes = [vp, i32, ctypes.POINTER(_MapInfo)]
     L.pfs_download_map.argtypes = [vp, vp]
     L.pfs_map_lits.argtypes = [vp, vp, u64]
+    L.pfs_map_lits_noalloc.argtypes = [vp, vp, u64]
     L.pfs_parse_dimacs.argtypes = [vp, ctypes.c_char_p, u64, ctypes.POINTER(_ParseInfo)]
     L.pfs_download_trail.argtypes = [vp, vp, ctypes.POINTER(u64)]
     L.pfs_decompose.argtypes = [vp, u32, ctypes.POINTER(_ElsInfo)]
@@ -366,6 +367,13 @@ class Simplifier:
         """VMAP::mapOrgs on a copy of 'lits' (pfs_map_lits)."""
         a = np.array(lits, dtype=np.uint32, copy=True)
         self._check(self._L.pfs_map_lits(self._h, _ptr(a), a.shape[0]), "pfs_map_lits")
+        return a
+
+    def map_lits_noalloc(self, lits) -> np.ndarray:
+        """pfs_map_lits_noalloc on a copy of 'lits': the same mapping staged
+        through the library's fixed device scratch (no allocation)."""
+        a = np.array(lits, dtype=np.uint32, copy=True)
+        self._check(self._L.pfs_map_lits_noalloc(self._h, _ptr(a), a.shape[0]), "pfs_map_lits_noalloc")
         return a
 
     def stage_times(self) -> dict:
