@@ -122,24 +122,14 @@ bool DeviceArena::place(ArenaClient* c, size_t bytes, Region r, void** slot, voi
   return false;
 }
 
+static void give_range(std::map<size_t, size_t>& holes, size_t off, size_t bytes);
+
 void DeviceArena::free_block(size_t si, size_t off) {
   Slab& s = slabs_[si];
   auto b = s.blocks.find(off);
-  size_t lo = off, hi = off + b->second.size;
+  const size_t bytes = b->second.size;
   s.blocks.erase(b);
-  auto nx = s.holes.lower_bound(off);
-  if (nx != s.holes.end() && nx->first == hi) {
-    hi += nx->second;
-    nx = s.holes.erase(nx);
-  }
-  if (nx != s.holes.begin()) {
-    auto pv = std::prev(nx);
-    if (pv->first + pv->second == lo) {
-      lo = pv->first;
-      s.holes.erase(pv);
-    }
-  }
-  s.holes[lo] = hi - lo;
+  give_range(s.holes, off, bytes);
 }
 
 bool DeviceArena::find(void* p, size_t* si, size_t* off) {
@@ -259,17 +249,73 @@ uint64_t DeviceArena::compact(ArenaClient* self, bool self_movable) {
   return compact_locked(self, self_movable);
 }
 
+// takes [off, off + bytes) out of the hole that contains it
+static void take_range(std::map<size_t, size_t>& holes, size_t off, size_t bytes) {
+  auto h = std::prev(holes.upper_bound(off));
+  const size_t ho = h->first, hs = h->second;
+  holes.erase(h);
+  if (off > ho) holes[ho] = off - ho;
+  if (off + bytes < ho + hs) holes[off + bytes] = ho + hs - off - bytes;
+}
+
+// returns [off, off + bytes) to the holes, merged with its neighbours
+static void give_range(std::map<size_t, size_t>& holes, size_t off, size_t bytes) {
+  size_t lo = off, hi = off + bytes;
+  auto nx = holes.lower_bound(off);
+  if (nx != holes.end() && nx->first == hi) {
+    hi += nx->second;
+    nx = holes.erase(nx);
+  }
+  if (nx != holes.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == lo) {
+      lo = pv->first;
+      holes.erase(pv);
+    }
+  }
+  holes[lo] = hi - lo;
+}
+
+// copies a block from 'from' to 'to' on one stream; ranges that overlap are
+// copied in pieces of the distance moved, in the order that reads every
+// piece before it is overwritten (ascending for a move down, descending for
+// a move up).  False (nothing copied) when that takes more than MAX_PIECES.
+static bool copy_block(char* base, size_t from, size_t to, size_t bytes, Stream* st) {
+  const size_t gap = from > to ? from - to : to - from;
+  if (gap < bytes && (bytes + gap - 1) / gap > MAX_PIECES) return false;
+  if (gap >= bytes) {
+    d2d(base + to, base + from, bytes, st);
+  } else if (to < from) {
+    for (size_t done = 0; done < bytes; done += gap)
+      d2d(base + to + done, base + from + done, std::min(gap, bytes - done), st);
+  } else {
+    for (size_t end = bytes; end > 0; end -= std::min(gap, end)) {
+      const size_t lo = end - std::min(gap, end);
+      d2d(base + to + lo, base + from + lo, end - lo, st);
+    }
+  }
+  return true;
+}
+
+// Compaction.  Blocks of idle clients move, the caller's only when it holds
+// no device address; every other block is an obstacle that stays put.
+//   dynamic blocks, lowest first: into the lowest hole below them that
+//     holds them whole, else down over the hole right below them;
+//   stable blocks, highest first: into the top of the highest hole above
+//     them that holds them whole, else up over the hole right above them.
+// Every copy runs on one stream, in this order, so a block may land in the
+// space another block left earlier in the same compaction; the owners'
+// streams are idle, and the copies are finished before any of them runs
+// again.
 uint64_t DeviceArena::compact_locked(ArenaClient* self, bool self_movable) {
   reclaim(true);
   want_compact_ = false;
-  // the clients whose blocks may move: idle ones (their lock taken here
-  // without waiting) and the caller when it holds no device address
   std::vector<ArenaClient*> locked;
   std::map<ArenaClient*, bool> movable;
   for (Slab& s : slabs_)
     for (auto& kv : s.blocks) {
       ArenaClient* o = kv.second.owner;
-      if (!o || kv.second.region != R_DYNAMIC || movable.count(o)) continue;
+      if (!o || movable.count(o)) continue;
       if (o == self) {
         movable[o] = self_movable;
       } else if (o->busy.try_lock()) {
@@ -279,40 +325,57 @@ uint64_t DeviceArena::compact_locked(ArenaClient* self, bool self_movable) {
         movable[o] = false;
       }
     }
+  Stream* cs = self ? self->st : nullptr;
   uint64_t moved = 0;
-  std::vector<ArenaClient*> touched;
   for (Slab& s : slabs_) {
-    std::map<size_t, Block> blocks;
-    size_t cursor = 0;
-    for (auto& kv : s.blocks) {
-      const size_t off = kv.first;
-      Block b = kv.second;
-      size_t at = off;
-      if (b.owner && b.region == R_DYNAMIC && movable[b.owner] && b.slot && cursor < off) {
-        const size_t gap = off - cursor;
-        if (gap >= b.size || (b.size + gap - 1) / gap <= MAX_PIECES) {
-          for (size_t done = 0; done < b.size; done += gap)  // ascending pieces: overlap-safe
-            d2d(s.base + cursor + done, s.base + off + done, std::min(gap, b.size - done), b.owner->st);
-          at = cursor;
-          *b.slot = s.base + at;
-          moved += b.size;
-          moved_blocks_++;
-          if (std::find(touched.begin(), touched.end(), b.owner) == touched.end()) touched.push_back(b.owner);
+    auto relocate = [&](size_t from, size_t to) {
+      Block b = s.blocks[from];
+      if (!cs) cs = b.owner->st;
+      if (!copy_block(s.base, from, to, b.size, cs)) return;
+      s.blocks.erase(from);
+      give_range(s.holes, from, b.size);
+      take_range(s.holes, to, b.size);
+      s.blocks[to] = b;
+      *b.slot = s.base + to;
+      moved += b.size;
+      moved_blocks_++;
+    };
+    auto can_move = [&](const Block& b, Region r) { return b.owner && b.slot && b.region == r && movable[b.owner]; };
+    std::vector<size_t> order;
+    for (auto& kv : s.blocks)
+      if (can_move(kv.second, R_DYNAMIC)) order.push_back(kv.first);
+    for (size_t off : order) {  // ascending
+      const size_t size = s.blocks[off].size;
+      size_t to = off;
+      for (auto& h : s.holes) {
+        if (h.first >= off) break;
+        if (h.second >= size && h.first + size <= off) {
+          to = h.first;
+          break;
         }
+        if (h.first + h.second == off) to = h.first;  // the hole right below: slide
       }
-      blocks[at] = b;
-      cursor = at + b.size;
+      if (to != off) relocate(off, to);
     }
-    s.blocks.swap(blocks);
-    s.holes.clear();
-    size_t end = 0;
-    for (auto& kv : s.blocks) {
-      if (kv.first > end) s.holes[end] = kv.first - end;
-      end = kv.first + kv.second.size;
+    order.clear();
+    for (auto& kv : s.blocks)
+      if (can_move(kv.second, R_STABLE)) order.push_back(kv.first);
+    for (auto it = order.rbegin(); it != order.rend(); ++it) {  // descending
+      const size_t off = *it, size = s.blocks[off].size;
+      size_t to = off;
+      for (auto h = s.holes.rbegin(); h != s.holes.rend(); ++h) {
+        const size_t hend = h->first + h->second;
+        if (hend <= off + size) break;
+        if (h->second >= size && h->first >= off + size) {
+          to = hend - size;
+          break;
+        }
+        if (h->first == off + size) to = hend - size;  // the hole right above: slide
+      }
+      if (to != off) relocate(off, to);
     }
-    if (end < s.size) s.holes[end] = s.size - end;
   }
-  for (ArenaClient* o : touched) stream_wait(o->st);  // the moves are done before the owner runs again
+  if (moved && cs) stream_wait(cs);  // the moves are done before any owner runs again
   for (ArenaClient* o : locked) o->busy.unlock();
   compactions_++;
   moved_bytes_ += moved;

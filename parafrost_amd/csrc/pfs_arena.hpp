@@ -9,10 +9,10 @@
 // stall every other context's stream.
 //
 // Two regions per slab, as in the reference:
-//   stable  -- per-variable arrays, placed from the top of a slab and never
-//              moved;
+//   stable  -- per-variable arrays, placed from the top of a slab;
+//              compaction moves them up;
 //   dynamic -- clause blocks, occurrence tables and temporaries, placed from
-//              the bottom; compaction slides them down over the holes.
+//              the bottom; compaction moves them down.
 // A block names the pointer that holds its address (its slot); compaction
 // rewrites that pointer.  A block is moved only while its context cannot be
 // using the address: the context is between calls (its client lock is free
@@ -72,9 +72,9 @@ class DeviceArena {
   // the pointer that holds the block at p is now *slot (the block changed
   // hands between two buffer records)
   void retarget(void* p, void** slot);
-  // slides the dynamic blocks of idle clients (and of 'self' when
-  // self_movable: the caller at a point where it holds no device address)
-  // down over the holes.  Returns the bytes moved.
+  // moves the blocks of idle clients (and of 'self' when self_movable: the
+  // caller at a point where it holds no device address) over the holes,
+  // dynamic ones down and stable ones up.  Returns the bytes moved.
   uint64_t compact(ArenaClient* self, bool self_movable);
   // an allocation found no hole although the pool may hold enough in total
   bool wants_compaction() const { return want_compact_; }

@@ -2,15 +2,18 @@
 // for the CPU tests.  TEST INFRASTRUCTURE ONLY.
 //
 // The device runtime calls the arena makes are stubbed with host memory
-// (slabs from malloc, copies with memmove in queue order, stream positions
-// always passed), so the placement, coalescing, in-place growth, stream-
-// ordered release and compaction logic run unchanged.  A randomized driver
-// fills every live block with a pattern derived from its id and checks,
-// after every operation, that each block's slot still points at its own
-// pattern, that blocks never overlap and that the accounting adds up.
+// (slabs from malloc; copies queued per stream and run when a stream is
+// waited on, in a random interleaving with the other streams' queued copies,
+// as streams run concurrently on the device), so the placement, coalescing,
+// in-place growth, stream-ordered release and compaction logic run
+// unchanged.  A randomized driver fills every live block with a pattern
+// derived from its id and checks, after every operation, that each block's
+// slot still points at its own pattern, that blocks never overlap and that
+// the accounting adds up.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <random>
 #include <set>
@@ -19,12 +22,20 @@
 #include "../../parafrost_amd/csrc/pfs_arena.hpp"
 
 namespace pfs {
+struct Copy {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
 struct Stream {
   int id;
+  std::deque<Copy> q;  // queued copies, in stream order
 };
 struct Marker {
-  int unused;
+  Stream* st;
 };
+static std::vector<Stream*> g_streams;
+static std::mt19937_64 g_sched(1);
 static size_t g_dev_limit = ~(size_t)0, g_dev_used = 0;
 void* dev_alloc(size_t bytes) {
   if (g_dev_used + bytes > g_dev_limit) throw std::bad_alloc();
@@ -34,12 +45,24 @@ void* dev_alloc(size_t bytes) {
   return p;
 }
 void dev_free(void* p) { free(p); }
-void d2d(void* dst, const void* src, size_t bytes, Stream*) { memmove(dst, src, bytes); }
-Marker* marker_record(Stream*) { return new Marker(); }
-bool marker_done(Marker*) { return true; }
-void marker_wait(Marker*) {}
+void d2d(void* dst, const void* src, size_t bytes, Stream* st) { st->q.push_back(Copy{dst, src, bytes}); }
+// runs queued copies of randomly chosen streams until st has none left
+bool stream_wait(Stream* st) {
+  while (!st->q.empty()) {
+    std::vector<Stream*> busy;
+    for (Stream* s : g_streams)
+      if (!s->q.empty()) busy.push_back(s);
+    Stream* s = busy[g_sched() % busy.size()];
+    const Copy c = s->q.front();
+    s->q.pop_front();
+    memmove(c.dst, c.src, c.bytes);
+  }
+  return true;
+}
+Marker* marker_record(Stream* st) { return new Marker{st}; }
+bool marker_done(Marker* m) { return m->st->q.empty(); }
+void marker_wait(Marker* m) { stream_wait(m->st); }
 void marker_free(Marker* m) { delete m; }
-bool stream_wait(Stream*) { return true; }
 void dev_mem_info(size_t* f, size_t* t) { *f = *t = 1ull << 30; }
 }  // namespace pfs
 
@@ -94,8 +117,10 @@ int main(int argc, char** argv) {
     const int nclients = 4;
     Stream sts[nclients];
     ArenaClient cl[nclients];
+    g_streams.clear();
     for (int c = 0; c < nclients; c++) {
       sts[c].id = c;
+      g_streams.push_back(&sts[c]);
       cl[c].arena = a;
       cl[c].st = &sts[c];
     }

@@ -1,6 +1,9 @@
 """The device arena's allocator (parafrost_amd/csrc/pfs_arena.cpp) on the
 host: tests/arena/arena_host.cpp builds it with the device runtime calls
-stubbed by host memory and drives it with random allocations, releases,
+stubbed by host memory (each stream's copies queued and run, when a stream
+is waited on, interleaved at random with the other streams' queued copies,
+so compaction that lets one context's copy overwrite what another's has not
+read yet fails) and drives it with random allocations, releases,
 in-place growth and compactions across four clients in small fixed-size
 arenas (256 KiB to 2 MiB), checking after every operation that every block's slot still
 points at its own content, that no blocks overlap and that the per-client
