@@ -8,7 +8,6 @@
 #include <iterator>
 #include <map>
 #include <new>
-#include <set>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -1122,8 +1121,8 @@ bool Simplifier::sub() {
   u32 pass = 1;
   fill_u32(P<u32>(sublast_), 1, E_, st_);
   run(work_order(), E_, {}, pass, false);
-  std::vector<u32> vis_set;  // variables with a visible stage unit
-  std::set<u32> deferred;    // affected variables past the refutation, not re-run
+  std::vector<u32> vis_set;   // variables with a visible stage unit
+  std::vector<u32> deferred;  // affected variables past the refutation, not re-run (ascending)
   bool vis_ready = false;
   while (!changes.empty() || !deferred.empty()) {
     if (pass > E_ + 2) throw DeviceFail{"SUB re-runs did not converge"};
@@ -1145,13 +1144,14 @@ bool Simplifier::sub() {
       stream_sync(st_);
     }
     // the re-runs: affected or deferred variables up to the refutation
+    // (the packed list is ascending: merged into the deferred ones, then
+    // split at the refutation)
     const u32 kref = first_refutation();
-    for (u32 k : ran) deferred.insert(k);
-    ran.clear();
-    for (auto it = deferred.begin(); it != deferred.end() && *it <= kref;) {
-      ran.push_back(*it);
-      it = deferred.erase(it);
-    }
+    std::vector<u32> all(deferred.size() + ran.size());
+    all.erase(std::set_union(deferred.begin(), deferred.end(), ran.begin(), ran.end(), all.begin()), all.end());
+    const auto cut = std::upper_bound(all.begin(), all.end(), kref);
+    ran.assign(all.begin(), cut);
+    deferred.assign(cut, all.end());
     const u32 m = (u32)ran.size();
     if (!m) break;
     h2d(idx, ran.data(), (size_t)m * 4, st_);
