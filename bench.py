@@ -573,6 +573,7 @@ def leg(cnf, name, opts, steps=3, warmup=1, extra_ref=(), ref=False, compare_ref
     try:
         s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
         status, dt = step_loop(s, steps, warmup)
+        shape = s.launch_shape()
         roof, stages_run, rounds, _ = profile_step(s, cnf, dt)
         res = s.result(status)
     except Exception:
@@ -583,7 +584,7 @@ def leg(cnf, name, opts, steps=3, warmup=1, extra_ref=(), ref=False, compare_ref
            "clauses_per_s": round(cnf.nclauses / dt, 1), "round_result": ["UNSAT", "SAT", "UNSOLVED"][status],
            "rounds": len(rounds), "elected": [x["elected"] for x in rounds],
            "sub_passes": [x["sub_passes"] for x in rounds], "stages_run": stages_run,
-           "round_roofline": roof["round"], "dominant_kernel": roof.get("dominant_kernel")}
+           "round_roofline": roof["round"], "dominant_kernel": roof.get("dominant_kernel"), "launch_shape": shape}
     if ref:
         r = cpu_reference(cnf, extra_ref, runs=1, keep_output=compare_ref)
         if r is not None:
@@ -832,6 +833,7 @@ def main():
     s.upload(cnf.maxvar, cnf.offsets, cnf.lits, trail=cnf.trail)
     status, dt = step_loop(s, args.steps, args.warmup)
     res = s.result(status)
+    shape = s.launch_shape()
     print(f"bench: {args.steps} steps, {dt * 1e3:.2f} ms/step, result {res.status}, "
           f"{res.sizes.shape[0]} clauses out, rounds {res.rounds}", file=sys.stderr, flush=True)
     roofline, stages_run, rounds, _ = profile_step(s, cnf, dt, args.traffic_csv,
@@ -891,7 +893,8 @@ def main():
                    "options": " ".join(f"{k}={v}" for k, v in opts.items()) + " (SUB+BVE+BCE, ERE)",
                    "round_result": ["UNSAT", "SAT", "UNSOLVED"][status], "rounds": len(rounds),
                    "stages_run": stages_run, "elected": [x["elected"] for x in rounds],
-                   "sub_passes": [x["sub_passes"] for x in rounds], "parallelism": "single GPU"},
+                   "sub_passes": [x["sub_passes"] for x in rounds], "launch_shape": shape,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": base,
     }

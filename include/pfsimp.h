@@ -72,10 +72,11 @@ void pfs_destroy(pfs_ctx* ctx);
 /* Device arena.  Every context's device memory is sub-allocated on the host
  * from a pool reserved once per device (the reference's cuArena pool behind
  * cuMM::initDeviceArena, src/gpu/memory.cu:99-166): a stable region (the
- * per-variable arrays, never moved) and a dynamic region (clause blocks,
- * occurrence tables, temporaries) that is defragmented by sliding the
- * blocks of idle contexts down over the holes when an allocation finds no
- * hole (compact_gpu_dynamic and retry, memory.cu:191-219, 360-419).  So the
+ * per-variable arrays, placed from the top) and a dynamic region (clause
+ * blocks, occurrence tables, temporaries, placed from the bottom),
+ * defragmented by moving the blocks of idle contexts over the holes, dynamic
+ * ones down and stable ones up, when an allocation finds no hole
+ * (compact_gpu_dynamic and retry, memory.cu:191-219, 360-419).  So the
  * rounds, serving requests and batch runs call neither hipMalloc nor
  * hipFree once the pool is reserved.
  *
@@ -298,6 +299,17 @@ uint32_t pfs_stage_count(void);
 uint32_t pfs_kernel_count(pfs_ctx* ctx);
 pfs_status pfs_kernel_stats(pfs_ctx* ctx, uint32_t i, const char** name, double* total_ms,
                             uint64_t* launches);
+/* The launch shapes of the last pfs_simplify: the occurrence table's tile
+ * (log2 of the clauses per tile; 0 where the table was not built by literal
+ * ranges), the LDS bytes reserved per election-step workgroup and the lanes
+ * per election candidate, and whether that call chose them by timing the
+ * candidates (option "autotune": the first call of a formula class with at
+ * least 2^20 clauses measures every tile, lane count and reservation not
+ * set by "otptile" / "mislanes" / "mislds"; later calls of the class reuse
+ * the fastest).  Replaces the reference's launch tuner
+ * (src/gpu/grid.cuh:137-229); results do not depend on the shapes. */
+pfs_status pfs_launch_shape(pfs_ctx* ctx, int32_t* otp_tile_log2, int32_t* mis_lds, int32_t* mis_lanes,
+                            int32_t* tuned);
 pfs_status pfs_memory(pfs_ctx* ctx, uint64_t* used, uint64_t* peak);
 
 /* Batched simplification (no reference counterpart: the reference

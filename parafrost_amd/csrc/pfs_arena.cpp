@@ -221,7 +221,7 @@ bool DeviceArena::grow(ArenaClient* c, void* p, size_t bytes) {
   return true;
 }
 
-void DeviceArena::release(ArenaClient* c, void* p) {
+void DeviceArena::release(ArenaClient* c, void* p, bool idle) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(m_);
   size_t si, off;
@@ -230,11 +230,31 @@ void DeviceArena::release(ArenaClient* c, void* p) {
   if (!b.owner) return;  // already released
   used_ -= b.size;
   c->used -= b.size;
+  if (idle) {
+    free_block(si, off);
+    return;
+  }
   // the owner's queued work may still read the block: it returns to the
   // holes once the stream has passed this point
   b.owner = nullptr;
   b.slot = nullptr;
-  pending_.push_back(Pending{marker_record(c->st), si, off});
+  pending_.push_back(Pending{marker_record(c->st), c->st, si, off});
+}
+
+void DeviceArena::drain(Stream* st) {
+  std::lock_guard<std::mutex> lk(m_);
+  size_t j = 0;
+  for (size_t i = 0; i < pending_.size(); i++) {
+    Pending& p = pending_[i];
+    if (p.st == st) {
+      marker_wait(p.marker);
+      marker_free(p.marker);
+      free_block(p.slab, p.off);
+    } else {
+      pending_[j++] = p;
+    }
+  }
+  pending_.resize(j);
 }
 
 void DeviceArena::retarget(void* p, void** slot) {

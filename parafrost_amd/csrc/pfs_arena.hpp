@@ -68,7 +68,12 @@ class DeviceArena {
   void* alloc(ArenaClient* c, size_t bytes, Region r, void** slot);
   // grows the block at p in place to 'bytes' when the space after it is free
   bool grow(ArenaClient* c, void* p, size_t bytes);
-  void release(ArenaClient* c, void* p);
+  // returns a block; 'idle': the owner's stream has no queued work (the
+  // block is free at once, without a stream marker)
+  void release(ArenaClient* c, void* p, bool idle = false);
+  // returns every pending release made on stream st (idle) to the holes, so
+  // that no marker outlives the stream it was recorded on
+  void drain(Stream* st);
   // the pointer that holds the block at p is now *slot (the block changed
   // hands between two buffer records)
   void retarget(void* p, void** slot);
@@ -96,6 +101,7 @@ class DeviceArena {
   };
   struct Pending {
     Marker* marker;  // the owner's stream position at the release
+    Stream* st;      // that stream
     size_t slab, off;
   };
   explicit DeviceArena(int device) : device_(device) {}

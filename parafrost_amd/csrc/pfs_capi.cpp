@@ -40,7 +40,9 @@ struct pfs_ctx {
   // the tile-to-XCD mapping (consecutive tiles on one XCD)
   int32_t otp_tile = 0;
   bool otp_xcd = true;
-  int32_t mis_lds = 10240;  // LDS reserved per election-step workgroup (occupancy)
+  int32_t mis_lds = -1;  // LDS reserved per election-step workgroup (occupancy; -1: tuned, else 10240)
+  int32_t mis_lanes = 0;  // lanes per election candidate (0: tuned, else 1)
+  bool autotune = true;  // launch-shape tuner (Options::autotune)
 };
 
 namespace {
@@ -79,6 +81,8 @@ Options normalise(const pfs_ctx* c) {
   o.otp_tile = c->otp_tile;
   o.otp_xcd = c->otp_xcd;
   o.mis_lds = c->mis_lds;
+  o.mis_lanes = c->mis_lanes;
+  o.autotune = c->autotune;
   if (o.all_en) o.ve_en = 1, o.ve_plus_en = 1, o.bce_en = 1, o.ere_en = 1;
   if (!o.phases && (o.ve_en || o.sub_en || o.bce_en)) o.phases = 1;
   if (o.phases && !(o.ve_en || o.sub_en || o.bce_en)) o.phases = 0;
@@ -250,13 +254,18 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"electionsmax", nullptr, &pfs_ctx::lcve_max_occurs, 1, INT32_MAX},
       {"lcveclausemax", nullptr, &pfs_ctx::lcve_clause_max, 1, INT32_MAX},
       {"otptile", nullptr, &pfs_ctx::otp_tile, 0, 20},
-      {"mislds", nullptr, &pfs_ctx::mis_lds, 0, 65536},
+      {"mislds", nullptr, &pfs_ctx::mis_lds, -1, 65536},
+      {"mislanes", nullptr, &pfs_ctx::mis_lanes, 0, 8},
   };
   for (const IntOpt& o : ints) {
     if (n == o.a || (o.b && n == o.b)) {
       int32_t v;
       if (!parse_int(value, o.lo, o.hi, v)) {
         c->err = "bad value for option " + n;
+        return PFS_BAD_INPUT;
+      }
+      if (n == "mislanes" && (v & (v - 1))) {
+        c->err = "mislanes must be 0 (tuned), 1, 2, 4 or 8";
         return PFS_BAD_INPUT;
       }
       c->*(o.f) = v;
@@ -275,6 +284,7 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"aggresivesort", nullptr, &pfs_ctx::aggr_sort},
       {"syncalways", nullptr, &pfs_ctx::sync_always},
       {"otpxcd", nullptr, &pfs_ctx::otp_xcd},
+      {"autotune", nullptr, &pfs_ctx::autotune},
   };
   for (const BoolOpt& o : bools) {
     if (n == o.a || (o.b && n == o.b)) {
@@ -668,6 +678,16 @@ pfs_status pfs_kernel_stats(pfs_ctx* c, uint32_t i, const char** name, double* t
   if (name) *name = k.name.c_str();
   if (total_ms) *total_ms = k.ms;
   if (launches) *launches = k.launches;
+  return PFS_OK;
+}
+
+pfs_status pfs_launch_shape(pfs_ctx* c, int32_t* otp_tile_log2, int32_t* mis_lds, int32_t* mis_lanes,
+                            int32_t* tuned) {
+  if (!c) return PFS_BAD_INPUT;
+  if (otp_tile_log2) *otp_tile_log2 = c->s->used_tile();
+  if (mis_lds) *mis_lds = c->s->used_lds();
+  if (mis_lanes) *mis_lanes = c->s->used_lanes();
+  if (tuned) *tuned = c->s->tuned_now() ? 1 : 0;
   return PFS_OK;
 }
 

@@ -79,9 +79,20 @@ struct Options {
   // LDS bytes reserved per one-wavefront workgroup of the election's steps
   // (occupancy; on the heavy-tailed round 0 / 10 / 20 / 32 KiB gave the
   // first pass 3.6 / 3.8 / 6.0 / 10.2 ms and the worklist passes 1.02 /
-  // 0.89 / 0.80 / 0.79 ms)
-  int32_t mis_lds = 10240;
+  // 0.89 / 0.80 / 0.79 ms; the Tseitin round is fastest at 20 KiB);
+  // -1: tuned, else 10240
+  int32_t mis_lds = -1;
+  // lanes per candidate in the election steps (1, 2, 4, 8; 0: tuned, else 1)
+  int32_t mis_lanes = 0;
+  // launch-shape tuner: on the first call of a formula class (clause count,
+  // variable count and mean clause size, each bucketed) with at least
+  // TUNE_MIN_CLAUSES clauses, the occurrence-table tile and the election's
+  // LDS reservation that are not set are chosen by timing the candidates on
+  // that call's first round; later calls of the class reuse the choice.
+  // Results do not depend on either.
+  int32_t autotune = 1;
 };
+constexpr uint64_t TUNE_MIN_CLAUSES = 1ull << 20;
 
 // Device view of the clause database and per-variable state.  Root
 // assignments are kept per variable (assign[v] = true literal + 1) so that

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <iterator>
 #include <map>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <unordered_map>
@@ -43,8 +44,8 @@ void DeviceMemory::ensure(Buf& b, size_t bytes, size_t keep, Stream* st, Region 
   b.cap = bytes;
   peak_ = std::max(peak_, client.used);
 }
-void DeviceMemory::release(Buf& b) {
-  if (b.p) arena_->release(&client, b.p);
+void DeviceMemory::release(Buf& b, bool idle) {
+  if (b.p) arena_->release(&client, b.p, idle);
   b.p = nullptr;
   b.cap = 0;
 }
@@ -62,7 +63,7 @@ void DeviceMemory::swap(Buf& a, Buf& b) {
 
 // ---------------------------------------------------------------- context
 Simplifier::Simplifier(int device, size_t budget, bool keep_input, DeviceArena* arena)
-    : mem_(budget), keep_input_(keep_input) {
+    : device_(device), mem_(budget), keep_input_(keep_input) {
   st_ = stream_create(device);
   mem_.attach(arena ? arena : &DeviceArena::get(device), st_);
   fixed_scratch_ = scratch_lookup(device);  // once per device; stream_create made the device current
@@ -83,8 +84,14 @@ void Simplifier::free_all() {
                 &in_lits_, &in_assign_, &in_vstate_, &in_trail_buf_, &proof_, &precs_,
                 &pstamp_, &plines_, &pbytes_, &subu_, &subaff_, &sublast_, &ust_, &ulit_,
                 &map_, &map_assign_, &out_, &vrest_, &lsum_};
-  stream_quiesce(st_);  // a destructor must not throw, also after a device error
-  for (Buf* b : all) mem_.release(*b);
+  // a destructor must not throw, also after a device error.  Once the
+  // stream has finished (or failed) its work, the blocks are free at once,
+  // and the releases still pending on it are returned too: no stream
+  // marker (a HIP event) outlives the stream it was recorded on, which is
+  // destroyed next
+  stream_quiesce(st_);
+  for (Buf* b : all) mem_.release(*b, true);
+  if (mem_.arena()) mem_.arena()->drain(st_);
 }
 
 DB Simplifier::make_db(int w) {
@@ -260,6 +267,16 @@ void Simplifier::save_input() {
   d2d(in_assign_.p, assign_.p, ((size_t)V_ + 1) * 4, st_);
   d2d(in_vstate_.p, vstate_.p, (size_t)V_ + 1, st_);
   d2d(in_trail_buf_.p, trail_.p, (size_t)ntrail * 4, st_);
+  // the live counts of the kept input: a simplify right after this or
+  // after a rewind starts from them instead of counting the pool again
+  u64* d = (u64*)P<u8>(scal_);
+  k_count(make_db(cur_), d, st_);
+  u64 h[2];
+  d2h(h, d, 16, st_);
+  stream_sync(st_);
+  in_live_cls_ = h[0];
+  in_live_lits_ = h[1];
+  counts_known_ = true;
 }
 
 Simplifier::ProfScope::ProfScope(Simplifier* x) : s(x) {
@@ -520,6 +537,7 @@ void Simplifier::rewind() {
   resident_ = false;
   std::fill(h_assign_.begin(), h_assign_.end(), 0u);
   for (u32 u : in_units_) h_assign_[ABS(u)] = u + 1;
+  counts_known_ = true;
 }
 
 // ---------------------------------------------------------------- incremental use
@@ -529,6 +547,7 @@ void Simplifier::rewind() {
 // root are dropped, root-satisfied clauses skipped, units assigned (they are
 // propagated by the next simplify), an empty clause refutes.
 void Simplifier::append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets, const uint32_t* lits) {
+  counts_known_ = false;
   if (maxvar < V_) throw std::invalid_argument("append cannot reduce the number of variables");
   if (maxvar > V_) grow_vars(maxvar);
   std::vector<u32> words, off;
@@ -601,6 +620,7 @@ void Simplifier::append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
 // Replaces the set of assumed variables (never eliminated; the reference's
 // incremental assumptions, src/cpu/assume.cpp, and lcve.cpp's 'iassumed').
 void Simplifier::assume(const uint32_t* vars, uint32_t n) {
+  counts_known_ = false;
   std::vector<u8> as((size_t)V_ + 1, 0);
   for (uint32_t i = 0; i < n; i++) {
     if (vars[i] == 0 || vars[i] > V_) throw std::invalid_argument("assumed variable out of range");
@@ -770,6 +790,42 @@ void Simplifier::parse_proof(const u32* toks, const u32* ends, u32 nc) {
 }
 
 // ---------------------------------------------------------------- OT (simplify.cpp:35-56)
+// ---------------------------------------------------------------- launch-shape tuning
+// One table per process: formula class -> tuned shapes.  The class is the
+// device and the binary logarithms of the clause and variable counts, with
+// the mean clause size in eighths.
+static std::mutex g_tune_m;
+static std::map<uint64_t, Tuned> g_tuned;
+static uint64_t ilog2(uint64_t x) { return x ? 63 - __builtin_clzll(x) : 0; }
+
+void Simplifier::tune_begin(uint64_t nC, uint64_t nL) {
+  tune_key_ = ((uint64_t)device_ << 48) | (ilog2(nC) << 32) | (ilog2(V_) << 16) |
+              std::min<uint64_t>(nC ? 8 * nL / nC : 0, 0xFFFF);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_m);
+    auto it = g_tuned.find(tune_key_);
+    tune_ = it != g_tuned.end() ? it->second : Tuned{};
+  }
+  const bool big = opt.autotune && nC >= TUNE_MIN_CLAUSES;
+  tune_ot_ = big && opt.otp_tile == 0 && !tune_.ot;
+  tune_mis_ = big && (opt.mis_lds < 0 || opt.mis_lanes == 0) && !tune_.mis;
+  tuned_now_ = tune_ot_ || tune_mis_;
+  used_tile_ = 0;
+  used_lds_ = lds_choice();
+  used_lanes_ = lanes_choice();
+}
+void Simplifier::tune_store() {
+  std::lock_guard<std::mutex> lk(g_tune_m);
+  Tuned& t = g_tuned[tune_key_];
+  if (tune_.ot) t.ot = true, t.otp_tile = tune_.otp_tile;
+  if (tune_.mis) t.mis = true, t.mis_lds = tune_.mis_lds, t.mis_lanes = tune_.mis_lanes;
+}
+int32_t Simplifier::tile_choice() const { return opt.otp_tile > 0 ? opt.otp_tile : tune_.ot ? tune_.otp_tile : 0; }
+int32_t Simplifier::lds_choice() const {
+  return opt.mis_lds >= 0 ? opt.mis_lds : tune_.mis ? tune_.mis_lds : 10240;
+}
+int32_t Simplifier::lanes_choice() const { return opt.mis_lanes > 0 ? opt.mis_lanes : tune_.mis ? tune_.mis_lanes : 1; }
+
 void Simplifier::create_ot() {
   stage_begin(S_OT);
   prop_changed_ = false;
@@ -808,13 +864,45 @@ void Simplifier::ot_counts(bool table) {
   DB db = make_db(cur_);
   u32* hist = P<u32>(hist_);
   OtpPlan p;
-  if (otp_plan(n2, db.ncls, &p, opt.otp_tile, opt.otp_xcd) && otp_room(p)) {
+  auto build = [&](int tile) {
+    if (!otp_plan(n2, db.ncls, &p, tile, opt.otp_xcd) || !otp_room(p)) return false;
     mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
     u32* counts = P<u32>(otp_);
     u32* pairs = counts + ((p.ncounts + 1 + 63) & ~(size_t)63);  // 256-byte aligned
     k_ot_ranges(db, p, (u32)n2, counts, pairs, table, hist, tmp_.p, st_);
+    used_tile_ = (int32_t)ilog2(p.ct);
+    return true;
+  };
+  if (table && tune_ot_) {
+    // every candidate tile builds the whole table, twice over in turn (each
+    // build is complete, the last one stands); the fastest is kept for the
+    // class
+    tune_ot_ = false;
+    const int tiles[4] = {11, 12, 13, 14};
+    float best[4] = {0, 0, 0, 0};
+    bool ok[4] = {false, false, false, false}, any = false;
+    for (int rep = 0; rep < 2; rep++)
+      for (int j = 0; j < 4; j++) {
+        Event* a = event_record(st_);
+        const bool built = build(tiles[j]);
+        Event* b = event_record(st_);
+        const float ms = event_ms(a, b);
+        event_free(a);
+        event_free(b);
+        if (built) best[j] = ok[j] ? std::min(best[j], ms) : ms, ok[j] = true;
+      }
+    for (int j = 0; j < 4; j++)
+      if (ok[j] && (!any || best[j] < best[tune_.otp_tile - tiles[0]]))
+        tune_.otp_tile = tiles[j], any = true;
+    if (any) {
+      tune_.ot = true;
+      tune_store();
+      return;
+    }
+  } else if (build(tile_choice())) {
     return;
   }
+  used_tile_ = 0;
   fill_u32(hist, 0, n2, st_);
   k_hist(db, hist, st_);
   if (!table) return;
@@ -928,37 +1016,86 @@ bool Simplifier::lcve() {
   radix_sort_pairs(P<u32>(key_), P<u32>(cand_), ncand, bits, tmp_.p, st_);
   // positions share a word with the candidate's state (position << 2)
   if (ncand >= (1u << 30)) throw DeviceFail{"LCVE: too many candidates for the packed positions"};
-  fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
-  k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
-  fill_u32(P<u32>(state_), 0, ncand, st_);
-  if (a.clause_max != INF) k_lcve_reject(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(pos_), P<u32>(state_), st_);
-  fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
-  u32* resume = P<u32>(vsz_);                  // free until SUB: where each scan goes on
-  fill_u32(resume, 0, ncand, st_);
+  u32* resume = P<u32>(vsz_);  // free until SUB: where each scan goes on
   // worklists of undecided candidates (the selection buffers are free now);
   // after the first pass over all candidates, passes are queued eight at a
   // time with the list lengths kept on the device (sc[4], sc[5]; sc[6]
   // counts the passes that had work)
   u32* wl[2] = {P<u32>(flag_), P<u32>(score_)};
   u32* cnt[2] = {sc + 4, sc + 5};
-  u32 iters = 0;
-  if (ncand) {
+  // the election from scratch with 'lds' bytes reserved per step
+  // workgroup; the passes it took
+  auto elect = [&](u32 lds, u32 lanes) -> u32 {
+    fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
+    k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
+    fill_u32(P<u32>(state_), 0, ncand, st_);
+    if (a.clause_max != INF) k_lcve_reject(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(pos_), P<u32>(state_), st_);
+    fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
+    fill_u32(resume, 0, ncand, st_);
+    if (!ncand) return 0u;
     fill_u32(sc + 4, 0, 3, st_);
     k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), resume, wl[0],
-               cnt[0], (u32)opt.mis_lds, st_);
+               cnt[0], lds, lanes, st_);
     u32 cur = 0, bound = read_u32(cnt[0]), launched = 0;
     while (bound) {
       for (int b = 0; b < 8; b++, launched++) {
         const u32 nx = cur ^ 1;
         fill_u32(cnt[nx], 0, 1, st_);
         k_mis_step_dev(db, P<u32>(cand_), wl[cur], cnt[cur], bound, P<u32>(pos_), P<u32>(state_), P<u32>(key_),
-                       resume, wl[nx], cnt[nx], sc + 6, (u32)opt.mis_lds, st_);
+                       resume, wl[nx], cnt[nx], sc + 6, lds, lanes, st_);
         cur = nx;
       }
       if (launched > ncand + 8) throw DeviceFail{"LCVE election did not converge"};
       bound = read_u32(cnt[cur]);
     }
-    iters = 1 + read_u32(sc + 6);
+    return 1 + read_u32(sc + 6);
+  };
+  u32 iters = 0;
+  if (tune_mis_ && ncand) {
+    // the election is the same set whatever the shape (the
+    // lexicographically first independent set is unique): each candidate
+    // shape runs it in full, twice over in turn, the last run stands.  The
+    // lane count is chosen first (at the default or set reservation), then
+    // the reservation at that lane count.
+    tune_mis_ = false;
+    auto fastest = [&](const std::vector<std::pair<u32, u32>>& shapes) {
+      std::vector<float> best(shapes.size(), 0.f);
+      for (int rep = 0; rep < 2; rep++)
+        for (size_t j = 0; j < shapes.size(); j++) {
+          Event* e0 = event_record(st_);
+          iters = elect(shapes[j].first, shapes[j].second);
+          Event* e1 = event_record(st_);
+          const float ms = event_ms(e0, e1);
+          event_free(e0);
+          event_free(e1);
+          best[j] = rep ? std::min(best[j], ms) : ms;
+        }
+      size_t pick = 0;
+      for (size_t j = 1; j < shapes.size(); j++)
+        if (best[j] < best[pick]) pick = j;
+      return shapes[pick];
+    };
+    const u32 lds0 = (u32)(opt.mis_lds >= 0 ? opt.mis_lds : 10240);
+    std::vector<std::pair<u32, u32>> shapes;
+    if (opt.mis_lanes > 0) shapes.push_back({lds0, (u32)opt.mis_lanes});
+    else
+      for (u32 g : {1u, 2u, 4u, 8u}) shapes.push_back({lds0, g});
+    const u32 lanes = fastest(shapes).second;
+    shapes.clear();
+    if (opt.mis_lds >= 0) shapes.push_back({lds0, lanes});
+    else
+      for (u32 l : {0u, 10240u, 20480u}) shapes.push_back({l, lanes});
+    const std::pair<u32, u32> pick = fastest(shapes);
+    tune_.mis_lds = (int32_t)pick.first;
+    tune_.mis_lanes = (int32_t)pick.second;
+    tune_.mis = true;
+    tune_store();
+    used_lanes_ = tune_.mis_lanes;
+    used_lds_ = tune_.mis_lds;
+  } else {
+    used_lds_ = lds_choice();
+    used_lanes_ = lanes_choice();
+    iters = elect((u32)used_lds_, (u32)used_lanes_);
   }
   fill_u32(sc, 0, 1, st_);
   fill_u32(sc + 1, ncand, 1, st_);
@@ -1687,7 +1824,14 @@ int Simplifier::simplify_rounds(uint32_t max_rounds) {
   phase_ = 0;
   mu_inc_ = 0;
   uint64_t nC, nL;
-  count(nC, nL);
+  if (counts_known_) {  // the kept input, untouched since it was counted
+    nC = in_live_cls_;
+    nL = in_live_lits_;
+  } else {
+    count(nC, nL);
+  }
+  counts_known_ = false;
+  tune_begin(nC, nL);
   int64_t litsbefore = (int64_t)nL, diff = INT64_MAX;
   u32 r = 0;
   while (nC && nL) {
@@ -1754,6 +1898,7 @@ void Simplifier::aggr_sort() {
 // last round the clauses are shrunk at the top level (rootify/shrinkTop)
 // and compacted.
 Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
+  counts_known_ = false;
   if (refuted_) throw std::logic_error("the formula is refuted");
   if (!pending_units_.empty() || trail_n_ != propagated_)
     throw std::logic_error("root units are not propagated: simplify or parse first");
@@ -2020,6 +2165,7 @@ void Simplifier::download_cnf(uint64_t* offsets, uint32_t* lits, uint8_t* flags,
 
 // ---------------------------------------------------------------- variable map
 Simplifier::MapInfo Simplifier::map(bool force, double map_perc) {
+  counts_known_ = false;
   if (refuted_) throw std::logic_error("the formula is refuted");
   if (!pending_units_.empty() || trail_n_ != propagated_)
     throw std::logic_error("root units are not propagated: simplify before mapping");

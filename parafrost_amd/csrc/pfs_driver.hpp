@@ -33,7 +33,7 @@ class DeviceMemory {
   }
   // make b hold at least 'bytes'; keeps the first 'keep' bytes of content
   void ensure(Buf& b, size_t bytes, size_t keep, Stream* st, Region r = R_DYNAMIC);
-  void release(Buf& b);
+  void release(Buf& b, bool idle = false);  // idle: the stream has no queued work
   // buffer records hand blocks over only through these, so that the arena
   // keeps the right pointer to rewrite when it moves a block
   void take(Buf& dst, Buf& src);  // dst's block released, dst = src, src empty
@@ -73,6 +73,12 @@ struct RoundStats {
 // per-stage device time (ms) and launches, when profiling is enabled
 enum Stage { S_GC, S_OT, S_BCP, S_LCVE, S_SORT, S_SUB, S_VE, S_BCE, S_ERE, S_COUNT, S_NSTAGES };
 extern const char* const kStageNames[S_NSTAGES];
+
+// a formula class's tuned launch shapes (0 / -1 and false: not measured)
+struct Tuned {
+  int32_t otp_tile = 0, mis_lds = -1, mis_lanes = 0;
+  bool ot = false, mis = false;
+};
 
 class Simplifier {
  public:
@@ -153,6 +159,13 @@ class Simplifier {
   // the same staged through the device's fixed scratch array instead of a
   // context buffer: no allocation of any kind; n <= SCRATCH_WORDS
   void map_lits_noalloc(uint32_t* lits, uint64_t n);
+  // the occurrence-table tile (log2 clauses; 0 where the table was not
+  // built by ranges) and election LDS bytes of the last simplify, and
+  // whether it measured them
+  int32_t used_tile() const { return used_tile_; }
+  int32_t used_lds() const { return used_lds_; }
+  int32_t used_lanes() const { return used_lanes_; }
+  bool tuned_now() const { return tuned_now_; }
   size_t mem_used() const { return mem_.used(); }
   size_t mem_peak() const { return mem_.peak(); }
   uint32_t nvars() const { return V_; }
@@ -206,6 +219,7 @@ class Simplifier {
   void stage_begin(Stage s);
   void stage_end();
 
+  int device_ = 0;
   Stream* st_ = nullptr;
   u32* fixed_scratch_ = nullptr;  // the device's fixed scratch array (its device address)
   DeviceMemory mem_;
@@ -246,6 +260,20 @@ class Simplifier {
   // pristine input copy
   Buf in_c_off_, in_lits_, in_assign_, in_vstate_;
   uint64_t in_ncls_ = 0, in_nlits_ = 0, in_trail_ = 0;
+  uint64_t in_live_cls_ = 0, in_live_lits_ = 0;  // live clauses and literals of the kept input
+  // launch-shape tuning (Options::autotune): the class's choices (ot, mis:
+  // known), whether this call's first round measures them, and what the
+  // last call used
+  Tuned tune_;
+  uint64_t tune_key_ = 0;
+  bool tune_ot_ = false, tune_mis_ = false, tuned_now_ = false;
+  int32_t used_tile_ = 0, used_lds_ = 0, used_lanes_ = 1;
+  void tune_begin(uint64_t nC, uint64_t nL);
+  void tune_store();
+  int32_t tile_choice() const;
+  int32_t lds_choice() const;
+  int32_t lanes_choice() const;
+  bool counts_known_ = false;  // the pool is the kept input as counted (upload or rewind since)
   uint32_t in_unassigned_ = 0;
   Buf in_trail_buf_;
   // variable map of the last map() call: old -> new variables, the old

@@ -417,73 +417,6 @@ PF_HD void mis_decide(u32* state, u32* pw, u32 i, u32 v, u32 s) {
   state[i] = s;
   pw[v] = (i << 2) | s;
 }
-// list entries and literals are read in groups (MIS_B clauses, then 4
-// literals of a clause at a time), every load of a group issued before any
-// of its results is used, so a lane keeps several loads in flight instead
-// of one chain of dependent loads per clause; the results are checked in
-// list order, so the blocker and resume entry are the ones a one-by-one
-// scan finds
-constexpr u32 MIS_B = 4;
-PF_HD u32 mis_step(const DB& db, u32 i, const u32* cand, u32* pw, u32* state, u32* blk, u32* resume) {
-  if (at_load(&state[i])) return 0;
-  const u32 b = blk[i];
-  const u32 v = cand[i];
-  if (b != NOPOS) {
-    const u32 sb = at_load(&state[b]);
-    if (sb == 0) return 1;
-    if (sb == 1) {
-      mis_decide(state, pw, i, v, 2);
-      return 0;
-    }
-  }
-  const u32 r = resume[i];
-  u32 blocker = NOPOS, at = 0;
-  for (u32 s = r >> 31; s < 2; s++) {
-    const OLV ol = olist(db, V2L(v) | s);
-    for (u32 k0 = s == (r >> 31) ? (r & 0x7FFFFFFFu) : 0u; k0 < ol.n; k0 += MIS_B) {
-      u32 c[MIS_B], h[MIS_B];
-#pragma unroll
-      for (u32 e = 0; e < MIS_B; e++) c[e] = k0 + e < ol.n ? ol.r[k0 + e] : NOPOS;
-#pragma unroll
-      for (u32 e = 0; e < MIS_B; e++) h[e] = c[e] != NOPOS ? db.lits[c[e]] : make_hdr(0, ST_DELETED);
-#pragma unroll
-      for (u32 e = 0; e < MIS_B; e++) {
-        if (st_deleted(hdr_flags(h[e]))) continue;
-        const u32* l = db.lits + c[e] + CL_HDR;
-        const u32 n = hdr_size(h[e]);
-        for (u32 q = 0; q < n; q += 4) {
-          u32 u[4], w[4];
-#pragma unroll
-          for (u32 j = 0; j < 4; j++) u[j] = q + j < n ? ABS(l[q + j]) : v;
-#pragma unroll
-          for (u32 j = 0; j < 4; j++) w[j] = u[j] != v ? at_load(&pw[u[j]]) : NOPOS;
-#pragma unroll
-          for (u32 j = 0; j < 4; j++) {
-            const u32 pu = w[j] >> 2;  // NOPOS: past every candidate
-            if (pu >= i) continue;
-            const u32 su = w[j] & 3u;
-            if (su == 1) {
-              mis_decide(state, pw, i, v, 2);
-              return 0;
-            }
-            if (su == 0 && blocker == NOPOS) {
-              blocker = pu;
-              at = (s << 31) | (k0 + e);
-            }
-          }
-        }
-      }
-    }
-  }
-  if (blocker != NOPOS) {
-    blk[i] = blocker;
-    resume[i] = at;
-    return 1;
-  }
-  mis_decide(state, pw, i, v, 1);
-  return 0;
-}
-
 // ---------------------------------------------------------------- shared helpers
 PF_HD bool sub_sig(u32 A, u32 B) { return !(A & ~B); }  // simplify.h:84
 PF_HD bool selfsub_sig(u32 A, u32 B) {                   // simplify.h:86-90

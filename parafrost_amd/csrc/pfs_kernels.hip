@@ -294,7 +294,8 @@ constexpr int NW = ST / 64;      // wavefronts per workgroup
 
 // exclusive scan of one value per thread over the workgroup; *total gets
 // the workgroup sum.  'lw' holds NW words of LDS.
-__device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) {
+template <int W>  // wavefronts per workgroup
+__device__ inline u32 block_excl_scan_w(u32 v, u32* lw, u32* total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   u32 x = v;
 #pragma unroll
@@ -306,7 +307,7 @@ __device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) {
   __syncthreads();
   u32 pre = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < NW; w++) {
+  for (int w = 0; w < W; w++) {
     const u32 s = lw[w];
     pre += (w < wid) ? s : 0u;
     tot += s;
@@ -315,6 +316,7 @@ __device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) {
   *total = tot;
   return pre + x - v;
 }
+__device__ inline u32 block_excl_scan(u32 v, u32* lw, u32* total) { return block_excl_scan_w<NW>(v, lw, total); }
 
 // order-free compaction: each thread holds the keep bits of SIPT
 // consecutive items; the workgroup reserves its slots with one atomic and
@@ -394,10 +396,42 @@ __global__ void scan_down_k(const u32* in, u32* out, size_t n, const u32* part) 
     pre += v[i];
   }
 }
+// short scans in one launch: one workgroup of 1024 threads walks the input
+// in tiles of 8192 items with a running carry (each thread reads its items
+// before the tile's barriers and writes them after, so in == out is fine)
+constexpr int S1_T = 1024;
+constexpr size_t S1_MAX = 4 * (size_t)S1_T * SIPT;  // at most four tiles
+__global__ void __launch_bounds__(S1_T) scan_one_k(const u32* in, u32* out, size_t n, u32* total) {
+  __shared__ u32 lw[S1_T / 64];
+  u32 carry = 0;
+  for (size_t base = 0; base < n; base += (size_t)S1_T * SIPT) {
+    const size_t b = base + (size_t)threadIdx.x * SIPT;
+    u32 v[SIPT];
+    u32 s = 0;
+#pragma unroll
+    for (int i = 0; i < SIPT; i++) {
+      v[i] = (b + i < n) ? in[b + i] : 0u;
+      s += v[i];
+    }
+    u32 tot;
+    u32 pre = block_excl_scan_w<S1_T / 64>(s, lw, &tot) + carry;
+#pragma unroll
+    for (int i = 0; i < SIPT; i++) {
+      if (b + i < n) out[b + i] = pre;
+      pre += v[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
 size_t scan_tmp_bytes(size_t n) { return (grid_for(n, STILE) + 64) * sizeof(u32); }
 void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream* st) {
   if (n == 0) {
     if (total) fill_u32(total, 0, 1, st);
+    return;
+  }
+  if (n <= S1_MAX) {
+    KLAUNCH(scan_one_k, 1, S1_T, st, in, out, n, total);
     return;
   }
   const unsigned nb = grid_for(n, STILE);
@@ -724,24 +758,16 @@ __device__ inline void wave_append(bool pred, u32 val, u32* arr, u32* cnt) {
   base = __shfl(base, leader, 64);
   if (pred) arr[base + __popcll(mask & ((1ull << lane) - 1))] = val;
 }
-// collects the positions of the lists longer than SORT_SMALL (few: elected
-// variables have at least one short list) for the workgroup and long-list
-// sorts
-__global__ void sort_classify_k(DB db, const u32* lits, u32 n, u32* big, u32* cnt) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  const u32 lit = i < n ? lits[i] : 0;
-  const u32 len = i < n ? db.ot_size[lit] : 0;
-  wave_append(len > SORT_SMALL, i, big, cnt);
-}
-// 16 lanes per list: bitonic network over wave64 shuffles, keys in registers.
-// The sorts also leave in lsum[g] the sum of the clause sizes of list g (a
-// bound for later stages: the lists of elected variables only shrink).
-__global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum) {
-  const u32 g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-  const u32 j = threadIdx.x & 15;
-  const u32 lit = g < n ? lits[g] : 0;
-  const u32 len0 = g < n ? db.ot_size[lit] : 0;
-  const u32 small = len0 <= SORT_SMALL ? len0 : 0;  // longer lists go to sort_block_k
+// Lists of LO < len <= W entries (every length up to W when LO is 0)
+// sorted by a group of W lanes: bitonic network over wave64 shuffles, keys
+// in registers.  Also leaves in lsum[g] the sum of the clause sizes of such
+// a list g (a bound for later stages: the lists of elected variables only
+// shrink).  Returns the list's length.
+template <u32 W, u32 LO>
+__device__ inline u32 sort_group(const DB& db, u32 lit, bool valid, u32 j, u32 g, u32* lsum) {
+  const u32 len0 = valid ? db.ot_size[lit] : 0;
+  const bool mine = valid && len0 <= W && (LO == 0 || len0 > LO);
+  const u32 small = mine ? len0 : 0;
   const u32 len = small >= 2 ? small : 0;
   u32* r = db.ot_refs + (small ? db.ot_off[lit] : 0);
   u64 a = ~0ull, b = ~0ull;
@@ -756,10 +782,10 @@ __global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum) {
     }
   }
 #pragma unroll
-  for (u32 d = 8; d > 0; d >>= 1) sz += __shfl_xor(sz, d, 64);
-  if (j == 0 && g < n && len0 <= SORT_SMALL) lsum[g] = sz;
+  for (u32 d = W / 2; d > 0; d >>= 1) sz += __shfl_xor(sz, d, 64);
+  if (j == 0 && mine) lsum[g] = sz;
 #pragma unroll
-  for (u32 k = 2; k <= 16; k <<= 1) {
+  for (u32 k = 2; k <= W; k <<= 1) {
 #pragma unroll
     for (u32 d = k >> 1; d > 0; d >>= 1) {
       const u64 pa = __shfl_xor(a, d, 64), pb = __shfl_xor(b, d, 64);
@@ -775,6 +801,19 @@ __global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum) {
     }
   }
   if (j < len) r[j] = id;
+  return len0;
+}
+// Every list of up to SORT_SMALL entries by a 16-lane group; the longer
+// ones (few) are collected for the workgroup and long-list sorts.  (Tried:
+// 4- or 8-lane groups for the shortest lists with a worklist or a second
+// pass for the rest, and a bounded grid striding over the lists; each was
+// slower on the heavy-tailed round's 6.1M lists.)
+__global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum, u32* big, u32* cnt) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 g = t / SORT_SMALL, j = t % SORT_SMALL;
+  const bool valid = g < n;
+  const u32 len = sort_group<SORT_SMALL, 0>(db, valid ? lits[g] : 0u, valid, j, g, lsum);
+  wave_append(j == 0 && len > SORT_SMALL, g, big, cnt);
 }
 // bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK);
 // big: positions in lits
@@ -875,13 +914,12 @@ __global__ void sort_large_k(DB db, const u32* lits, const u32* big, const u32* 
 void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum,
                   Stream* st) {
   if (!n) return;
-  u32* big = classes;
+  u32* big = classes;  // n words: lists longer than SORT_SMALL
   fill_u32(cnt, 0, 1, st);
   // short lists (the bulk) are sorted in place by 16-lane groups, longer
-  // ones are collected and sorted per workgroup (or by the long-list merge)
-  KLAUNCH(sort_small_k, grid_for((size_t)n * 16, 256), 256, st, db, lits, n, lsum);
-  KLAUNCH(sort_classify_k, grid_for(n, 256), 256, st, db, lits, n, big, cnt);
-  // the long lists are few; both launches read their count on the device
+  // ones are collected and sorted per workgroup or by the long-list merge;
+  // those launches read their count on the device
+  KLAUNCH(sort_small_k, grid_for((size_t)n * SORT_SMALL, 256), 256, st, db, lits, n, lsum, big, cnt);
   KLAUNCH(sort_block_k, std::min(grid_for(n, 1), 1024u), 256, st, db, lits, big, cnt, lsum);
   KLAUNCH(sort_large_k, std::min(grid_for(n, 64), 64u), 64, st, db, lits, big, cnt, scratch, lsum);
 }
@@ -1029,44 +1067,145 @@ void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state
   if (!n) return;
   KLAUNCH(lcve_reject_k, grid_for(n, 256), 256, st, cand, n, cflag, pos, state);
 }
+// One election step by a group of G lanes per candidate (G a power of two,
+// groups aligned in the wavefront): the lanes share out the entries of the
+// candidate's two lists (positive, then negative: one combined index) and
+// combine their findings, an elected earlier neighbour anywhere freezes
+// the candidate, else the earliest entry holding an undecided earlier
+// neighbour is its blocker's entry.  The outcome is the one-lane scan's
+// (mis_step); G lanes keep G chains of loads in flight per candidate.
+// Values that steer the group's control flow are read by its first lane and
+// broadcast, so every lane of a group takes the same path.  Returns 1 if
+// the candidate stays undecided (in every lane of the group).
+template <u32 G>
+__device__ inline u32 mis_step_group(const DB& db, u32 i, bool valid, const u32* cand, u32* pw, u32* state, u32* blk,
+                                     u32* resume) {
+  const int lane = threadIdx.x & 63, lead = lane & ~(int)(G - 1), j = lane - lead;
+  u32 s0 = (valid && j == 0) ? at_load(&state[i]) : 0u;
+  s0 = __shfl(s0, lead, 64);
+  if (!valid || s0) return 0;
+  const u32 v = cand[i];
+  const u32 b = blk[i];
+  if (b != NOPOS) {
+    u32 sb = j == 0 ? at_load(&state[b]) : 0u;
+    sb = __shfl(sb, lead, 64);
+    if (sb == 0) return 1;
+    if (sb == 1) {
+      if (j == 0) mis_decide(state, pw, i, v, 2);
+      return 0;
+    }
+  }
+  const u32 r = resume[i];
+  const OLV o0 = olist(db, V2L(v)), o1 = olist(db, V2L(v) | 1);
+  const u32 n0 = o0.n, ntot = o0.n + o1.n;
+  const u32 start = (r >> 31) ? n0 + (r & 0x7FFFFFFFu) : r;
+  u32 first = NOPOS, bpos = NOPOS;  // this lane's earliest entry with an undecided earlier neighbour
+  bool frozen = false;
+  for (u32 e = start + j; e < ntot && !frozen; e += G) {
+    const u32 c = e < n0 ? o0.r[e] : o1.r[e - n0];
+    const u32 h = db.lits[c];
+    if (st_deleted(hdr_flags(h))) continue;
+    const u32* l = db.lits + c + CL_HDR;
+    const u32 n = hdr_size(h);
+    for (u32 q = 0; q < n && !frozen; q += 4) {
+      u32 u[4], w[4];
+#pragma unroll
+      for (u32 k = 0; k < 4; k++) u[k] = q + k < n ? ABS(l[q + k]) : v;
+#pragma unroll
+      for (u32 k = 0; k < 4; k++) w[k] = u[k] != v ? at_load(&pw[u[k]]) : NOPOS;
+#pragma unroll
+      for (u32 k = 0; k < 4; k++) {
+        const u32 pu = w[k] >> 2;  // NOPOS: past every candidate
+        if (pu >= i) continue;
+        const u32 su = w[k] & 3u;
+        if (su == 1) frozen = true;
+        if (su == 0 && first == NOPOS) first = e, bpos = pu;
+      }
+    }
+  }
+  u32 fz = frozen ? 1u : 0u;
+  u64 m = ((u64)first << 32) | bpos;
+#pragma unroll
+  for (u32 d = 1; d < G; d <<= 1) {
+    fz |= __shfl_xor(fz, d, 64);
+    const u64 o = __shfl_xor(m, d, 64);
+    m = o < m ? o : m;
+  }
+  if (fz) {
+    if (j == 0) mis_decide(state, pw, i, v, 2);
+    return 0;
+  }
+  const u32 e = (u32)(m >> 32);
+  if (e != NOPOS) {
+    if (j == 0) {
+      blk[i] = (u32)m;
+      resume[i] = e < n0 ? e : ((1u << 31) | (e - n0));
+    }
+    return 1;
+  }
+  if (j == 0) mis_decide(state, pw, i, v, 1);
+  return 0;
+}
+
 // one election step over the undecided candidates idx[0..n) (all when idx
-// is null); the ones still undecided are appended to 'next'.  The launch
-// reserves LDS per one-wavefront workgroup only to bound occupancy (option
-// "mislds"; 20 KiB leaves 2 wavefronts per SIMD: Tseitin round 0 / 20 / 40
-// KiB -> mis_step_k 3.34 / 2.76 / 3.18 ms)
+// is null), G lanes per candidate; the ones still undecided are appended
+// to 'next'.  The launch reserves LDS per one-wavefront workgroup only to
+// bound occupancy (option "mislds"; 20 KiB leaves 2 wavefronts per SIMD:
+// Tseitin round 0 / 20 / 40 KiB -> mis_step_k 3.34 / 2.76 / 3.18 ms)
+template <u32 G>
 __global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
                            u32* resume, u32* next, u32* nnext) {
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const u32 i = idx ? idx[t] : t;
-  const bool undecided = mis_step(db, i, cand, pos, state, blk, resume) != 0;
-  wave_append(undecided, i, next, nnext);
-}
-void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
-                u32* resume, u32* next, u32* nnext, u32 lds, Stream* st) {
-  if (!n) return;
-  KLAUNCH_LDS(mis_step_k, grid_for(n, 64), 64, lds, st, db, cand, idx, n, pos, state, blk, resume, next, nnext);
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 q = (u32)(t / G);
+  const bool valid = q < n;
+  const u32 i = valid ? (idx ? idx[q] : q) : 0u;
+  const bool undecided = mis_step_group<G>(db, i, valid, cand, pos, state, blk, resume) != 0;
+  wave_append(valid && (threadIdx.x & (G - 1)) == 0 && undecided, i, next, nnext);
 }
 // A worklist pass whose length is read on the device (n_in): a bounded grid
 // strides over the list, so several passes are queued without a host read;
 // passes over an empty list return at once.  iters counts the non-empty passes.
+template <u32 G>
 __global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32* n_in, u32* pos, u32* state,
                                u32* blk, u32* resume, u32* next, u32* nnext, u32* iters) {
   const u32 n = *n_in;
   if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(iters, 1u);
-  for (u32 base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {  // wave-uniform
-    const u32 t = base + threadIdx.x;
-    const u32 i = t < n ? idx[t] : 0u;
-    const bool undecided = t < n && mis_step(db, i, cand, pos, state, blk, resume) != 0;
-    wave_append(undecided, i, next, nnext);
+  const u64 m = (u64)n * G;
+  for (u64 base = (u64)blockIdx.x * blockDim.x; base < m; base += (u64)gridDim.x * blockDim.x) {  // wave-uniform
+    const u64 t = base + threadIdx.x;
+    const u32 q = (u32)(t / G);
+    const bool valid = q < n;
+    const u32 i = valid ? idx[q] : 0u;
+    const bool undecided = mis_step_group<G>(db, i, valid, cand, pos, state, blk, resume) != 0;
+    wave_append(valid && (threadIdx.x & (G - 1)) == 0 && undecided, i, next, nnext);
   }
 }
-void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
-                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, Stream* st) {
-  if (!n_bound) return;
-  const unsigned grid = std::min(grid_for(n_bound, 64), 2048u);
-  KLAUNCH_LDS(mis_step_dev_k, grid, 64, lds, st, db, cand, idx, n_in, pos, state, blk, resume, next, nnext, iters);
+#define MIS_LANES(G, call) \
+  switch (G) {             \
+    case 2: call(2); break; \
+    case 4: call(4); break; \
+    case 8: call(8); break; \
+    default: call(1); break; \
+  }
+void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
+                u32* resume, u32* next, u32* nnext, u32 lds, u32 lanes, Stream* st) {
+  if (!n) return;
+#define MIS_FIRST(G) \
+  KLAUNCH_LDS(mis_step_k<G>, grid_for((size_t)n * G, 64), 64, lds, st, db, cand, idx, n, pos, state, blk, resume, next, nnext)
+  MIS_LANES(lanes, MIS_FIRST)
+#undef MIS_FIRST
 }
+void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
+                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, u32 lanes,
+                    Stream* st) {
+  if (!n_bound) return;
+#define MIS_DEV(G)                                                                                           \
+  KLAUNCH_LDS(mis_step_dev_k<G>, std::min(grid_for((size_t)n_bound * G, 64), 2048u), 64, lds, st, db, cand, idx, \
+              n_in, pos, state, blk, resume, next, nnext, iters)
+  MIS_LANES(lanes, MIS_DEV)
+#undef MIS_DEV
+}
+#undef MIS_LANES
 __global__ void mis_break_k(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && state[i] == 1 && cflag[cand[i]] == 3) atomicMin(brk, i);

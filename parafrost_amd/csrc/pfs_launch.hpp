@@ -125,11 +125,13 @@ void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state
 // in *nnext)
 // the same over the list idx[0..*n_in) (n_in on the device, at most n_bound);
 // *iters counts the passes over a non-empty list
-// lds: bytes of LDS reserved per one-wavefront workgroup (occupancy)
+// lds: bytes of LDS reserved per one-wavefront workgroup (occupancy);
+// lanes: lanes per candidate (1, 2, 4 or 8)
 void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
-                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, Stream*);
+                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, u32 lanes,
+                    Stream*);
 void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
-                u32* resume, u32* next, u32* nnext, u32 lds, Stream*);
+                u32* resume, u32* next, u32* nnext, u32 lds, u32 lanes, Stream*);
 void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
 void k_mis_collect(const u32* cand, u32 n, const u32* state, u32 brk, u32* flag_tmp, u32* out,
                    u32* nout, void* tmp, Stream*);

@@ -29,7 +29,7 @@ SYMBOLS = ("pfs_create", "pfs_destroy", "pfs_set_option", "pfs_upload", "pfs_rew
            "pfs_server_submit", "pfs_server_start", "pfs_server_wait", "pfs_server_destroy", "pfs_map",
            "pfs_download_map", "pfs_map_lits", "pfs_parse_dimacs", "pfs_download_trail",
            "pfs_decompose", "pfs_arena_configure", "pfs_arena_create", "pfs_arena_destroy",
-           "pfs_create_in_arena", "pfs_arena_get_stats", "pfs_map_lits_noalloc")
+           "pfs_create_in_arena", "pfs_arena_get_stats", "pfs_map_lits_noalloc", "pfs_launch_shape")
 
 
 class PfsError(RuntimeError):
@@ -107,6 +107,8 @@ def load_library():
     L.pfs_stage_name.restype = ctypes.c_char_p
     L.pfs_stage_count.restype = u32
     L.pfs_memory.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.pfs_launch_shape.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                   ctypes.POINTER(i32)]
     L.pfs_kernel_count.argtypes = [vp]
     L.pfs_kernel_count.restype = u32
     L.pfs_kernel_stats.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
@@ -393,6 +395,15 @@ class Simplifier:
             L.pfs_kernel_stats(self._h, i, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(n))
             out[name.value.decode()] = (ms.value, n.value)
         return out
+
+    def launch_shape(self) -> dict:
+        """pfs_launch_shape: the occurrence-table tile (log2 clauses), the
+        election's LDS bytes and lanes per candidate the last simplify used,
+        and whether it tuned them."""
+        t, l, g, tuned = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._L.pfs_launch_shape(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(g),
+                                             ctypes.byref(tuned)), "pfs_launch_shape")
+        return {"otp_tile_log2": t.value, "mis_lds": l.value, "mis_lanes": g.value, "tuned": bool(tuned.value)}
 
     def memory(self) -> tuple[int, int]:
         used, peak = ctypes.c_uint64(), ctypes.c_uint64()

@@ -149,7 +149,9 @@ int main(int argc, char** argv) {
         auto it = live.begin();
         std::advance(it, rng() % live.size());
         Live* l = it->second;
-        a->release(&cl[l->client], l->p);
+        const bool idle = rng() % 4 == 0;  // the owner waited for its stream: freed at once
+        if (idle) stream_wait(&sts[l->client]);
+        a->release(&cl[l->client], l->p, idle);
         live.erase(it);
         delete l;
       } else if (kind == 8) {  // grow a block in place (or not)
@@ -163,6 +165,8 @@ int main(int argc, char** argv) {
           l->size = nb;
           grows++;
         }
+      } else if (rng() % 4 == 0) {  // client c's pending releases returned (its stream is going away)
+        a->drain(&sts[c]);
       } else {  // compaction with client c at a safe point (every client idle here)
         moved += a->compact(&cl[c], true);
         compactions++;

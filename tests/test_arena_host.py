@@ -4,7 +4,8 @@ stubbed by host memory (each stream's copies queued and run, when a stream
 is waited on, interleaved at random with the other streams' queued copies,
 so compaction that lets one context's copy overwrite what another's has not
 read yet fails) and drives it with random allocations, releases,
-in-place growth and compactions across four clients in small fixed-size
+in-place growth, releases on idle streams, drains of a stream's pending
+releases and compactions across four clients in small fixed-size
 arenas (256 KiB to 2 MiB), checking after every operation that every block's slot still
 points at its own content, that no blocks overlap and that the per-client
 accounting adds up; built with AddressSanitizer and UBSan."""
