@@ -1024,19 +1024,23 @@ bool Simplifier::lcve() {
   u32* wl[2] = {P<u32>(flag_), P<u32>(score_)};
   u32* cnt[2] = {sc + 4, sc + 5};
   // the election from scratch with 'lds' bytes reserved per step
-  // workgroup; the passes it took
-  auto elect = [&](u32 lds, u32 lanes) -> u32 {
+  // workgroup and 'lanes' per candidate; sc[6] counts the worklist passes
+  // that had work
+  auto elect = [&](u32 lds, u32 lanes) {
     fill_u32(P<u32>(pos_), NOPOS, (size_t)V_ + 1, st_);
     k_lcve_positions(P<u32>(cand_), ncand, P<u32>(pos_), st_);
     fill_u32(P<u32>(state_), 0, ncand, st_);
     if (a.clause_max != INF) k_lcve_reject(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(pos_), P<u32>(state_), st_);
     fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
     fill_u32(resume, 0, ncand, st_);
-    if (!ncand) return 0u;
     fill_u32(sc + 4, 0, 3, st_);
+    if (!ncand) return;
     k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), resume, wl[0],
                cnt[0], lds, lanes, st_);
-    u32 cur = 0, bound = read_u32(cnt[0]), launched = 0;
+    // the first batch of worklist passes is sized for every candidate (its
+    // grid is at the cap either way on large rounds); later batches for the
+    // list length read after the previous batch
+    u32 cur = 0, bound = ncand, launched = 0;
     while (bound) {
       for (int b = 0; b < 8; b++, launched++) {
         const u32 nx = cur ^ 1;
@@ -1048,9 +1052,7 @@ bool Simplifier::lcve() {
       if (launched > ncand + 8) throw DeviceFail{"LCVE election did not converge"};
       bound = read_u32(cnt[cur]);
     }
-    return 1 + read_u32(sc + 6);
   };
-  u32 iters = 0;
   if (tune_mis_ && ncand) {
     // the election is the same set whatever the shape (the
     // lexicographically first independent set is unique): each candidate
@@ -1063,7 +1065,7 @@ bool Simplifier::lcve() {
       for (int rep = 0; rep < 2; rep++)
         for (size_t j = 0; j < shapes.size(); j++) {
           Event* e0 = event_record(st_);
-          iters = elect(shapes[j].first, shapes[j].second);
+          elect(shapes[j].first, shapes[j].second);
           Event* e1 = event_record(st_);
           const float ms = event_ms(e0, e1);
           event_free(e0);
@@ -1095,15 +1097,18 @@ bool Simplifier::lcve() {
   } else {
     used_lds_ = lds_choice();
     used_lanes_ = lanes_choice();
-    iters = elect((u32)used_lds_, (u32)used_lanes_);
+    elect((u32)used_lds_, (u32)used_lanes_);
   }
   fill_u32(sc, 0, 1, st_);
   fill_u32(sc + 1, ncand, 1, st_);
   k_mis_break(P<u32>(cand_), ncand, P<u8>(cflag_), P<u32>(state_), sc + 1, st_);
   k_mis_collect(P<u32>(cand_), ncand, P<u32>(state_), 0, P<u32>(flag_), P<u32>(pv_), sc, tmp_.p, st_);
-  E_ = read_u32(sc);
+  u32 w7[7];  // sc[0]: elected, sc[6]: worklist passes with work
+  d2h(w7, sc, sizeof(w7), st_);
+  stream_sync(st_);
+  E_ = w7[0];
   rounds_.back().elected = E_;
-  rounds_.back().mis_iterations = iters;
+  rounds_.back().mis_iterations = ncand ? 1 + w7[6] : 0;
   stage_end();
   if (profile) {  // work measures, outside the stage's time
     uint64_t w[2];
@@ -1413,11 +1418,17 @@ bool Simplifier::ve() {
   ap.proof = proof_out(tot[0] + S + V_ + 64);
   fill_u32(sc + 6, 0, 1, st_);
   k_ve_apply(db, P<u32>(pv_), E_, d, ap, P<u32>(vrest_), P<u32>(lsum_), tmp_.p, st_);
-  if (read_u32(sc + 6)) throw DeviceFail{"BVE resolvent writes disagree with the counted space"};
+  // eliminated variables (markEliminated, solver.h:535-541), counted now so
+  // that one read brings it with the unit records' count and the error flag
+  k_count_nonzero(P<u32>(type_), E_, sc + 7, st_);
+  u32 w5[5];  // sc[3]: unit records, sc[6]: write error, sc[7]: eliminated
+  d2h(w5, sc + 3, sizeof(w5), st_);
+  stream_sync(st_);
+  if (w5[3]) throw DeviceFail{"BVE resolvent writes disagree with the counted space"};
   ncls_ += tot[0];
   lits_end_ += tot[1];
   stack_n_ += tot[2];
-  const u32 nrec = read_u32(sc + 3);
+  const u32 nrec = w5[0], elim = w5[4];
   if (nrec > V_ + 1) throw DeviceFail{"BVE unit record buffer overflow"};
   bool ok = true;
   std::vector<u32> fresh;
@@ -1455,9 +1466,6 @@ bool Simplifier::ve() {
   if (ok) commit_units(fresh);
   proof_events(pev);
   proof_flush();
-  // eliminated variables (markEliminated, solver.h:535-541)
-  k_count_nonzero(P<u32>(type_), E_, sc + 7, st_);
-  const u32 elim = read_u32(sc + 7);
   unassigned_ -= elim;
   rounds_.back().eliminated = elim;
   rounds_.back().resolvents = tot[0];
@@ -1542,7 +1550,7 @@ bool Simplifier::ere_pairs() {
       const u32 k0 = kb[ci], nk = kb[ci + 1] - k0;
       if (!nk) continue;
       scan_u32(P<u32>(vncls_) + k0, P<u32>(clsb_), nk, sc + 34, tmp_.p, st_);
-      const u32 npairs = read_u32(sc + 34);
+      const u32 npairs = kb.size() == 2 ? (u32)total : read_u32(sc + 34);  // one chunk: the total
       if (!npairs) continue;
       // records and overflow indices, grown to the counts of a pass that
       // overflowed them and the pass run again
