@@ -105,6 +105,13 @@ def otp_counts(V, C):
     return nr * ((C + ct - 1) // ct)
 
 
+def norm_kernel(name):
+    """A kernel's name as the library's profiler reports it: the profilers'
+    unsigned template arguments ("mis_step_k<1u>") without the suffix."""
+    import re
+    return re.sub(r"<(\d+)u>", r"<\1>", name)
+
+
 def kernel_bytes(name, rounds, V):
     """Algorithmic bytes per launch of a kernel, from the work measures of
     the profiled step (averaged over the rounds the kernel ran in).
@@ -129,6 +136,8 @@ def kernel_bytes(name, rounds, V):
       sub_k, ve_gates_k, ve_rest_k, bce_k   4 O_E + D_E
       ve_apply_k                            4 O_E + D_E + 4 E (decisions) + R + S
     """
+    if name.startswith(("mis_step_k<", "mis_step_dev_k<")):  # any lanes per candidate
+        name = name.split("<", 1)[0]
     per = []
     for r in rounds:
         c, l = r["clauses_before"], r["literals_before"]
@@ -191,7 +200,7 @@ def counter_traffic(fetch_csv, write_csv, kernel, copy_bytes=256 << 20):
                 v = float(r["Counter_Value"]) * 1024.0
                 if "copyBuffer" in name or "copy" in name.lower() and "pfs::" not in name:
                     copies.append(v)
-                if name.startswith((f"pfs::{kernel}(", f"void pfs::{kernel}(")):  # templates print "void"
+                if "pfs::" in name and norm_kernel(name.split("pfs::", 1)[1].split("(", 1)[0]) == kernel:
                     vals.append(v)
         copies.sort()
         ref = statistics.median(copies[-3:]) if copies else 0.0
@@ -496,7 +505,7 @@ def committed_traffic(workload, kernel):
             j = json.load(f)
     except OSError:
         return None, None
-    k = j.get("kernels", {}).get(kernel)
+    k = {norm_kernel(n): v for n, v in j.get("kernels", {}).items()}.get(kernel)
     if not k:
         return None, None
     return k["traffic_bytes_per_launch"], {"fetch_scale": j.get("fetch_scale"), "write_scale": j.get("write_scale"),

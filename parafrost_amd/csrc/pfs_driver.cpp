@@ -215,6 +215,7 @@ void Simplifier::upload(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
   h2d(lits_[0].p, h_lits.data(), L * 4, st_);
   // per-variable state
   h_assign_.assign((size_t)V_ + 1, 0);
+  h_touched_.clear();
   in_units_.assign(trail, trail + ntrail);
   for (u32 i = 0; i < ntrail; i++) h_assign_[ABS(trail[i])] = trail[i] + 1;
   std::vector<u8> vs((size_t)V_ + 1, 0), as((size_t)V_ + 1, 0);
@@ -469,6 +470,7 @@ Simplifier::ParseInfo Simplifier::parse_dimacs(const char* text, uint64_t n) {
 
   // ---- the state of a fresh upload, then root propagation and the shrink
   h_assign_.assign(V1, 0);
+  h_touched_.clear();
   std::vector<u32> tr(ntrail);
   if (ntrail) d2h(tr.data(), trail_.p, (size_t)ntrail * 4, st_);
   stream_sync(st_);
@@ -535,7 +537,15 @@ void Simplifier::rewind() {
   units_out_.clear();
   refuted_ = false;
   resident_ = false;
-  std::fill(h_assign_.begin(), h_assign_.end(), 0u);
+  // the host assignments back to the input's: only the entries set since
+  // (clearing all 10M entries of the heavy-tailed formula showed as a
+  // 0.36 ms gap before each call's first kernel)
+  if (h_touched_.size() < h_assign_.size() / 8) {
+    for (u32 v : h_touched_) h_assign_[v] = 0;
+  } else {
+    std::fill(h_assign_.begin(), h_assign_.end(), 0u);
+  }
+  h_touched_.clear();
   for (u32 u : in_units_) h_assign_[ABS(u)] = u + 1;
   counts_known_ = true;
 }
@@ -579,7 +589,7 @@ void Simplifier::append(uint32_t maxvar, uint64_t ncls, const uint64_t* offsets,
     if (cl.size() == 1) {
       const u32 u = cl[0];
       if (h_assign_[ABS(u)]) continue;  // assigned by an earlier clause of this batch
-      h_assign_[ABS(u)] = u + 1;
+      set_host_assign(u);
       pending_units_.push_back(u);
       continue;
     }
@@ -955,7 +965,7 @@ bool Simplifier::prop_visit(uint64_t front) {
     k_set_vstate(db, trail + start_new, (u32)nnew, VS_FROZEN, st_);
     stream_sync(st_);
     for (u32 u : fresh) {
-      h_assign_[ABS(u)] = u + 1;
+      set_host_assign(u);
       units_out_.push_back(u);
       proof_line(PR_UNIT, u);  // enqueueUnit -> learnUnit (solver.h:430-441, 475-479)
     }
@@ -1206,8 +1216,15 @@ bool Simplifier::sub() {
       for (const ProofEv& e : pev) pmap[e.k].push_back(e);
       return;
     }
-    // unit changes of the variables that ran
-    for (u32 k : ran) {
+    // unit changes of the variables that ran ('ran' is ascending); only a
+    // variable with units before or after can change
+    std::vector<u32> keys;
+    for (auto& kv : units) keys.push_back(kv.first);
+    for (auto& kv : fresh) keys.push_back(kv.first);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (u32 k : keys) {
+      if (!std::binary_search(ran.begin(), ran.end(), k)) continue;
       std::vector<u32> before, after;
       auto it = units.find(k);
       if (it != units.end())
@@ -1225,7 +1242,10 @@ bool Simplifier::sub() {
       for (u32 l : diff) changes.insert(changes.end(), {k, l, pass});
       if (jt != fresh.end()) units[k] = jt->second;
       else if (it != units.end()) units.erase(it);
-      pmap.erase(k);
+    }
+    for (auto it = pmap.begin(); it != pmap.end();) {
+      if (std::binary_search(ran.begin(), ran.end(), it->first)) it = pmap.erase(it);
+      else ++it;
     }
     std::vector<ProofEv> pev;
     proof_collect(pev, INF);
@@ -1353,7 +1373,7 @@ bool Simplifier::sub() {
         continue;
       }
       pev.push_back({kv.first, e.first, PR_UNIT, u});
-      h_assign_[ABS(u)] = u + 1;
+      set_host_assign(u);
       committed.push_back(u);
       units_out_.push_back(u);
       unassigned_--;
@@ -1457,7 +1477,7 @@ bool Simplifier::ve() {
         continue;
       }
       pev.push_back({rec[3 * i], rec[3 * i + 1], PR_UNIT, u});
-      h_assign_[ABS(u)] = u + 1;
+      set_host_assign(u);
       fresh.push_back(u);
       units_out_.push_back(u);
       unassigned_--;
@@ -2051,7 +2071,7 @@ Simplifier::ElsInfo Simplifier::decompose(uint32_t max_rounds) {
         d2h(us.data(), P<u32>(uval), (size_t)nunits * 4, st_);
         stream_sync(st_);
         for (u32 u : us) {
-          h_assign_[ABS(u)] = u + 1;
+          set_host_assign(u);
           units_out_.push_back(u);
         }
         trail_n_ += nunits;
@@ -2228,6 +2248,7 @@ Simplifier::MapInfo Simplifier::map(bool force, double map_perc) {
     if (ABS(u) == map_first_) units.push_back(map_first_true_);
   units_out_.swap(units);
   h_assign_.assign(m1, 0);
+  h_touched_.clear();
   if (map_first_) h_assign_[ABS(map_first_true_)] = map_first_true_ + 1;
   V_ = nV;
   remapped_ = true;

@@ -292,6 +292,11 @@ class Simplifier {
 
   // host mirrors
   std::vector<uint32_t> h_assign_;  // per variable
+  std::vector<uint32_t> h_touched_;  // variables assigned in h_assign_ since it was last set in full
+  void set_host_assign(u32 lit) {
+    h_assign_[ABS(lit)] = lit + 1;
+    h_touched_.push_back(ABS(lit));
+  }
   std::vector<uint32_t> units_out_;  // new root units in trail order
   std::vector<uint32_t> in_units_;
   uint32_t unassigned_ = 0;
