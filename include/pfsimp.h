@@ -115,7 +115,22 @@ pfs_status pfs_arena_get_stats(pfs_arena* arena, int32_t device, pfs_arena_stats
  * "mapperc" (real in [0, 1], default 0.2) is the inactive share above which
  * pfs_map renumbers.
  * "workorder" (boolean, default on; no reference counterpart) launches the
- * per-variable kernels heaviest lists first; results do not depend on it. */
+ * per-variable kernels heaviest lists first; results do not depend on it.
+ * GPU-build options (src/gpu/options.cpp:24-43, options.cu:36-60):
+ * "electionsmax" and "lcveclausemax" cap the election (default: no cap, as
+ * the CPU simplifier); "aggresivesort" sorts the result's clauses by the
+ * list key; "syncalways" synchronises after every stage; "lcvefast" and
+ * "vefunction" are accepted off and refused on (PFS_BAD_INPUT with the
+ * reason: the rank-based election and the function-table gates are not
+ * provided); "nstreams", "*minthreads", "*minblocks", "subclausemax",
+ * "ereextend" and the solver's "sigma*" / "solve" options are range-checked
+ * and have no effect here.
+ * Launch shapes (no reference counterpart; results do not depend on them):
+ * "otptile" (0-20: log2 clauses per occurrence-table tile, 0 tuned or
+ * planned), "otpxcd" (boolean, default on: consecutive tiles on one XCD),
+ * "mislanes" (0, 1, 2, 4, 8: lanes per election candidate, 0 tuned),
+ * "mislds" (-1-65536: LDS bytes reserved per election workgroup, -1 tuned),
+ * "autotune" (boolean, default on: see pfs_launch_shape). */
 pfs_status pfs_set_option(pfs_ctx* ctx, const char* name, const char* value);
 
 /* Hands over the root-level clause database, as the host solver has it after
