@@ -109,7 +109,8 @@ def norm_kernel(name):
     """A kernel's name as the library's profiler reports it: the profilers'
     unsigned template arguments ("mis_step_k<1u>") without the suffix."""
     import re
-    return re.sub(r"<(\d+)u>", r"<\1>", name)
+    name = name.strip("()")
+    return re.sub(r"(\d+)u([,>])", r"\1\2", name)
 
 
 def kernel_bytes(name, rounds, V):
@@ -136,7 +137,8 @@ def kernel_bytes(name, rounds, V):
       sub_k, ve_gates_k, ve_rest_k, bce_k   4 O_E + D_E
       ve_apply_k                            4 O_E + D_E + 4 E (decisions) + R + S
     """
-    if name.startswith(("mis_step_k<", "mis_step_dev_k<")):  # any lanes per candidate
+    name = norm_kernel(name)
+    if name.startswith(("mis_step_k<", "mis_step_dev_k<")):  # any launch shape
         name = name.split("<", 1)[0]
     per = []
     for r in rounds:
@@ -200,7 +202,7 @@ def counter_traffic(fetch_csv, write_csv, kernel, copy_bytes=256 << 20):
                 v = float(r["Counter_Value"]) * 1024.0
                 if "copyBuffer" in name or "copy" in name.lower() and "pfs::" not in name:
                     copies.append(v)
-                if "pfs::" in name and norm_kernel(name.split("pfs::", 1)[1].split("(", 1)[0]) == kernel:
+                if "pfs::" in name and norm_kernel(name.split("pfs::", 1)[1].split("(", 1)[0]) == norm_kernel(kernel):
                     vals.append(v)
         copies.sort()
         ref = statistics.median(copies[-3:]) if copies else 0.0
@@ -505,7 +507,7 @@ def committed_traffic(workload, kernel):
             j = json.load(f)
     except OSError:
         return None, None
-    k = {norm_kernel(n): v for n, v in j.get("kernels", {}).items()}.get(kernel)
+    k = {norm_kernel(n): v for n, v in j.get("kernels", {}).items()}.get(norm_kernel(kernel))
     if not k:
         return None, None
     return k["traffic_bytes_per_launch"], {"fetch_scale": j.get("fetch_scale"), "write_scale": j.get("write_scale"),
@@ -518,7 +520,7 @@ def profile_step(s, cnf, dt, traffic_csv=None, workload=None):
     s.set_option("profilegpu", 1)
     s.rewind()
     st = s.simplify()
-    kt = {k: v for k, v in s.kernel_times().items() if k not in MEASURE_KERNELS}
+    kt = {norm_kernel(k): v for k, v in s.kernel_times().items() if k not in MEASURE_KERNELS}
     stages = s.stage_times()
     s.set_option("profilegpu", 0)
     rounds = s.result(st).rounds
