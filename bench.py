@@ -122,10 +122,11 @@ def kernel_bytes(name, rounds, V):
       scatter_k  D + 4L + 8V       (read every clause and the list offsets, write one ref per literal)
       count_k    D                 (read every clause)
     occurrence table by literal ranges (N = range-by-tile counts, as the
-    library plans them: otp_plan):
-      otp_count_k         D + 4N   (read every clause, write the counts)
-      otp_part_k<true>    D + 4N + 8L   (read every clause and the scanned counts, write the pairs)
-      otp_part_k<false>   D + 4N + 4L   (the same with literals only)
+    library plans them: otp_plan; the tile kernels read a clause's handle,
+    its header word and its literals: T = 8C + 4L):
+      otp_count_k         T + 4N   (read every clause, write the counts)
+      otp_part_k<true>    T + 4N + 8L   (read every clause and the scanned counts, write the pairs)
+      otp_part_k<false>   T + 4N + 4L   (the same with literals only)
       otp_range_k<true>   12L + 24V     (read the pairs, write one ref per literal, offsets, sizes, counts)
       otp_range_k<false>  4L + 8V       (read the literals, write the counts)
     election (O_c occurrences in the candidates' lists, D_c their distinct
@@ -144,15 +145,16 @@ def kernel_bytes(name, rounds, V):
     for r in rounds:
         c, l = r["clauses_before"], r["literals_before"]
         d = 12 * c + 4 * l
+        tl = 8 * c + 4 * l
         oe, de = r["elected_occurs"], r["elected_bytes"]
         ran = {STAGES[s] for s in range(len(STAGES)) if (r["stages"] >> s) & 1}
         n = otp_counts(V, c)
         if name == "otp_count_k":
-            per.append(d + 4 * n)
+            per.append(tl + 4 * n)
         elif name == "otp_part_k<true>":
-            per.append(d + 4 * n + 8 * l)
+            per.append(tl + 4 * n + 8 * l)
         elif name == "otp_part_k<false>":
-            per.append(d + 4 * n + 4 * l)
+            per.append(tl + 4 * n + 4 * l)
         elif name == "otp_range_k<true>":
             per.append(12 * l + 24 * V)
         elif name == "otp_range_k<false>":

@@ -128,6 +128,8 @@ pfs_status pfs_arena_get_stats(pfs_arena* arena, int32_t device, pfs_arena_stats
  * Launch shapes (no reference counterpart; results do not depend on them):
  * "otptile" (0-20: log2 clauses per occurrence-table tile, 0 tuned or
  * planned), "otpxcd" (boolean, default on: consecutive tiles on one XCD),
+ * "otpranges" (boolean, default on: the occurrence table by literal ranges;
+ * off: one atomic per occurrence, the build used past 12,288 ranges),
  * "mislanes" (0, 1, 2, 4, 8: lanes per election candidate, 0 tuned),
  * "mislds" (-1-65536: LDS bytes reserved per election workgroup, -1 tuned),
  * "autotune" (boolean, default on: see pfs_launch_shape). */

@@ -40,6 +40,7 @@ struct pfs_ctx {
   // the tile-to-XCD mapping (consecutive tiles on one XCD)
   int32_t otp_tile = 0;
   bool otp_xcd = true;
+  bool otp_ranges = true;  // the occurrence table by literal ranges (off: one atomic per occurrence)
   int32_t mis_lds = -1;  // LDS reserved per election-step workgroup (occupancy; -1: tuned, else 10240)
   int32_t mis_lanes = 0;  // lanes per election candidate (0: tuned, else 1)
   bool autotune = true;  // launch-shape tuner (Options::autotune)
@@ -80,6 +81,7 @@ Options normalise(const pfs_ctx* c) {
   o.sync_always = c->sync_always;
   o.otp_tile = c->otp_tile;
   o.otp_xcd = c->otp_xcd;
+  o.otp_ranges = c->otp_ranges;
   o.mis_lds = c->mis_lds;
   o.mis_lanes = c->mis_lanes;
   o.autotune = c->autotune;
@@ -284,6 +286,7 @@ pfs_status pfs_set_option(pfs_ctx* c, const char* name, const char* value) {
       {"aggresivesort", nullptr, &pfs_ctx::aggr_sort},
       {"syncalways", nullptr, &pfs_ctx::sync_always},
       {"otpxcd", nullptr, &pfs_ctx::otp_xcd},
+      {"otpranges", nullptr, &pfs_ctx::otp_ranges},
       {"autotune", nullptr, &pfs_ctx::autotune},
   };
   for (const BoolOpt& o : bools) {

@@ -76,6 +76,9 @@ struct Options {
   // the tile-to-XCD mapping (consecutive tiles on one XCD, so that the
   // partition's writes to one range meet in one L2)
   int32_t otp_tile = 0, otp_xcd = 1;
+  // the table by literal ranges (1) or by one atomic per occurrence (0,
+  // hist_k / scatter_k: also the build past OTP_MAX_RANGES ranges)
+  int32_t otp_ranges = 1;
   // LDS bytes reserved per one-wavefront workgroup of the election's steps
   // (occupancy; on the heavy-tailed round 0 / 10 / 20 / 32 KiB gave the
   // first pass 3.6 / 3.8 / 6.0 / 10.2 ms and the worklist passes 1.02 /

@@ -875,7 +875,7 @@ void Simplifier::ot_counts(bool table) {
   u32* hist = P<u32>(hist_);
   OtpPlan p;
   auto build = [&](int tile) {
-    if (!otp_plan(n2, db.ncls, &p, tile, opt.otp_xcd) || !otp_room(p)) return false;
+    if (!opt.otp_ranges || !otp_plan(n2, db.ncls, &p, tile, opt.otp_xcd) || !otp_room(p)) return false;
     mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(p.ncounts)), 0, st_);
     u32* counts = P<u32>(otp_);
     u32* pairs = counts + ((p.ncounts + 1 + 63) & ~(size_t)63);  // 256-byte aligned
@@ -883,7 +883,7 @@ void Simplifier::ot_counts(bool table) {
     used_tile_ = (int32_t)ilog2(p.ct);
     return true;
   };
-  if (table && tune_ot_) {
+  if (table && tune_ot_ && opt.otp_ranges) {
     // every candidate tile builds the whole table, twice over in turn (each
     // build is complete, the last one stands); the fastest is kept for the
     // class

@@ -1,6 +1,8 @@
 """The option surface of the reference's GPU build (src/gpu/options.cpp:24-43,
 src/gpu/options.cu:36-60) through the C ABI, and parity of the options that
-change results (electionsmax, lcveclausemax, aggresivesort) with the oracle."""
+change results (electionsmax, lcveclausemax, aggresivesort) with the oracle;
+the occurrence table built by one atomic per occurrence (otpranges=0: the
+build past 12,288 literal ranges) and with other tiles also matches it."""
 import numpy as np
 import pytest
 
@@ -63,7 +65,7 @@ FORMULAS = {
 }
 LIMITS = [{"electionsmax": 8}, {"electionsmax": 40}, {"lcveclausemax": 2}, {"lcveclausemax": 3},
           {"aggresivesort": 1}, {"aggresivesort": 1, "all": 1, "phases": 3}, {"literalsmul": 0},
-          {"syncalways": 1}]
+          {"syncalways": 1}, {"otpranges": 0}, {"otptile": 10, "otpxcd": 0}]
 
 
 @pytest.mark.parametrize("family", sorted(FORMULAS))
@@ -82,7 +84,9 @@ def test_options_match_oracle(family, opt):
     finally:
         s.close()
     kw = {"phases": 1, "bce_en": 1}
-    kw.update(oracle_kw({k: v for k, v in o.items() if k not in ("literalsmul", "syncalways")}))
+    # launch shapes and stream behaviour have no oracle counterpart (results do not depend on them)
+    shaping = ("literalsmul", "syncalways", "otpranges", "otptile", "otpxcd")
+    kw.update(oracle_kw({k: v for k, v in o.items() if k not in shaping}))
     vs = np.zeros(cnf.maxvar + 1, np.uint8)
     vs[np.asarray(cnf.trail, np.int64) >> 1] = 2  # root-assigned variables are frozen
     r = oracle.simplify(cnf.maxvar, cnf.offsets, cnf.lits, nassigned=int(cnf.trail.shape[0]), det=1, vstate=vs,
