@@ -1572,17 +1572,26 @@ void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n
 }
 // heaviest-first order of the elected variables: key = 0xFFFF - min(|poss| +
 // |negs|, 0xFFFF), stable radix sort on 16 bits
+// the launch order's key: list occurrences on an 8-bit log-linear scale
+// (exact below 64, then 16 steps per power of two up to 2^18), descending,
+// so the sort takes two radix passes instead of four; ties keep the elected
+// order
+__device__ inline u32 work_code(u32 w) {
+  if (w < 64) return w;
+  const u32 e = 31 - __clz(w);
+  return min(255u, 64 + (e - 6) * 16 + ((w >> (e - 4)) & 15u));
+}
 __global__ void work_keys_k(DB db, const u32* pv, u32 n, u32* key, u32* idx) {
   const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const u32 w = db.ot_size[V2L(pv[k])] + db.ot_size[V2L(pv[k]) | 1];
-  key[k] = 0xFFFFu - (w < 0xFFFFu ? w : 0xFFFFu);
+  key[k] = 255u - work_code(w);
   idx[k] = k;
 }
 void k_work_order(const DB& db, const u32* pv, u32 n, u32* key, u32* order, void* tmp, Stream* st) {
   if (!n) return;
   KLAUNCH(work_keys_k, grid_for(n, 256), 256, st, db, pv, n, key, order);
-  radix_sort_pairs(key, order, n, 16, tmp, st);
+  radix_sort_pairs(key, order, n, 8, tmp, st);
 }
 __global__ void ere_k(DB db, Options o, const u32* pv, u32 from, u32 n, EreOut out) {
   const u32 k = from + blockIdx.x * blockDim.x + threadIdx.x;

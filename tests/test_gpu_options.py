@@ -2,7 +2,8 @@
 src/gpu/options.cu:36-60) through the C ABI, and parity of the options that
 change results (electionsmax, lcveclausemax, aggresivesort) with the oracle;
 the occurrence table built by one atomic per occurrence (otpranges=0: the
-build past 12,288 literal ranges) and with other tiles also matches it."""
+build past 12,288 literal ranges) and with other tiles, and the election with
+2, 4 or 8 lanes per candidate and other LDS reservations, also match it."""
 import numpy as np
 import pytest
 
@@ -65,7 +66,8 @@ FORMULAS = {
 }
 LIMITS = [{"electionsmax": 8}, {"electionsmax": 40}, {"lcveclausemax": 2}, {"lcveclausemax": 3},
           {"aggresivesort": 1}, {"aggresivesort": 1, "all": 1, "phases": 3}, {"literalsmul": 0},
-          {"syncalways": 1}, {"otpranges": 0}, {"otptile": 10, "otpxcd": 0}]
+          {"syncalways": 1}, {"otpranges": 0}, {"otptile": 10, "otpxcd": 0}, {"mislanes": 2},
+          {"mislanes": 8, "mislds": 0}, {"mislanes": 4, "mislds": 20480, "autotune": 0}]
 
 
 @pytest.mark.parametrize("family", sorted(FORMULAS))
@@ -85,7 +87,7 @@ def test_options_match_oracle(family, opt):
         s.close()
     kw = {"phases": 1, "bce_en": 1}
     # launch shapes and stream behaviour have no oracle counterpart (results do not depend on them)
-    shaping = ("literalsmul", "syncalways", "otpranges", "otptile", "otpxcd")
+    shaping = ("literalsmul", "syncalways", "otpranges", "otptile", "otpxcd", "mislanes", "mislds", "autotune")
     kw.update(oracle_kw({k: v for k, v in o.items() if k not in shaping}))
     vs = np.zeros(cnf.maxvar + 1, np.uint8)
     vs[np.asarray(cnf.trail, np.int64) >> 1] = 2  # root-assigned variables are frozen
