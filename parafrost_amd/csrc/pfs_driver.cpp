@@ -1138,8 +1138,10 @@ void Simplifier::sort_ot() {
   DB db = make_db(cur_);
   k_pv_lits(P<u32>(pv_), E_, P<u32>(elits_), st_);
   mem_.ensure(scratch_, std::max<size_t>(lits_end_, 6 * (size_t)E_ + 3) * 4, 0, st_);
-  mem_.ensure(tmp2_, (6 * (size_t)E_ + 3) * 4, 0, st_);
-  k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), P<u32>(lsum_), st_);
+  mem_.ensure(tmp2_, (10 * (size_t)E_ + 3) * 4, 0, st_);
+  mem_.ensure(tmp_, std::max(tmp_.cap, scan_tmp_bytes(2 * (size_t)E_)), 0, st_);
+  k_sort_lists(db, P<u32>(elits_), 2 * E_, P<u32>(tmp2_), P<u32>(scal_) + 24, P<u32>(scratch_), P<u32>(lsum_),
+               tmp_.p, st_);
   if (opt.work_order) k_work_order(db, P<u32>(pv_), E_, P<u32>(tmp2_), P<u32>(order_), tmp_.p, st_);
   stage_end();
   if (profile) {  // work measures, outside the stage's time

@@ -803,17 +803,36 @@ __device__ inline u32 sort_group(const DB& db, u32 lit, bool valid, u32 j, u32 g
   if (j < len) r[j] = id;
   return len0;
 }
-// Every list of up to SORT_SMALL entries by a 16-lane group; the longer
-// ones (few) are collected for the workgroup and long-list sorts.  (Tried:
-// 4- or 8-lane groups for the shortest lists with a worklist or a second
-// pass for the rest, and a bounded grid striding over the lists; each was
-// slower on the heavy-tailed round's 6.1M lists.)
-__global__ void sort_small_k(DB db, const u32* lits, u32 n, u32* lsum, u32* big, u32* cnt) {
+// Every list once by a 4-lane group (wavefront count and dependent loads
+// per list, not the sorting, bound these passes): the lists of at most
+// SORT_TINY entries, most of an election's, are sorted there; the others
+// are flagged (fmid: up to SORT_SMALL entries, flong: beyond), compacted by
+// scans and sorted by 16-lane groups, per workgroup or by the long-list
+// merge.  (Tried: one 16-lane pass over every list, 1.09 ms on the
+// heavy-tailed round's 6.1M lists; a worklist filled with one atomic per
+// wavefront on a shared counter; a second pass over every list that skips
+// the short ones; a bounded grid striding over the lists.)
+constexpr u32 SORT_TINY = 4;
+__global__ void sort_tiny_k(DB db, const u32* lits, u32 n, u32* lsum, u32* fmid, u32* flong) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  const u32 g = t / SORT_SMALL, j = t % SORT_SMALL;
+  const u32 g = t / SORT_TINY, j = t % SORT_TINY;
   const bool valid = g < n;
-  const u32 len = sort_group<SORT_SMALL, 0>(db, valid ? lits[g] : 0u, valid, j, g, lsum);
-  wave_append(j == 0 && len > SORT_SMALL, g, big, cnt);
+  const u32 len = sort_group<SORT_TINY, 0>(db, valid ? lits[g] : 0u, valid, j, g, lsum);
+  if (valid && j == 0) {
+    fmid[g] = len > SORT_TINY && len <= SORT_SMALL ? 1u : 0u;
+    flong[g] = len > SORT_SMALL ? 1u : 0u;
+  }
+}
+// the lists mid[0..*nmid), 16 lanes each; a bounded grid strides over them
+// (the loop bound is the same for every lane of a wavefront)
+__global__ void sort_mid_k(DB db, const u32* lits, const u32* mid, const u32* nmid, u32* lsum) {
+  const u64 m = (u64)*nmid * SORT_SMALL;
+  for (u64 base = (u64)blockIdx.x * blockDim.x; base < m; base += (u64)gridDim.x * blockDim.x) {
+    const u64 t = base + threadIdx.x;
+    const bool valid = t < m;
+    const u32 g = valid ? mid[t / SORT_SMALL] : 0u;
+    sort_group<SORT_SMALL, SORT_TINY>(db, valid ? lits[g] : 0u, valid, (u32)(t % SORT_SMALL), g, lsum);
+  }
 }
 // bitonic sort of one list per workgroup in LDS (16 < len <= SORT_BLOCK);
 // big: positions in lits
@@ -912,14 +931,18 @@ __global__ void sort_large_k(DB db, const u32* lits, const u32* big, const u32* 
     sort_large_list(db, lits, big[i], scratch, lsum);
 }
 void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum,
-                  Stream* st) {
+                  void* tmp, Stream* st) {
   if (!n) return;
-  u32* big = classes;  // n words: lists longer than SORT_SMALL
-  fill_u32(cnt, 0, 1, st);
-  // short lists (the bulk) are sorted in place by 16-lane groups, longer
-  // ones are collected and sorted per workgroup or by the long-list merge;
-  // those launches read their count on the device
-  KLAUNCH(sort_small_k, grid_for((size_t)n * SORT_SMALL, 256), 256, st, db, lits, n, lsum, big, cnt);
+  u32* big = classes;           // lists longer than SORT_SMALL
+  u32* mid = classes + n;       // lists of SORT_TINY < len <= SORT_SMALL
+  u32* fmid = classes + 2 * (size_t)n;
+  u32* flong = classes + 3 * (size_t)n;
+  u32* pos = classes + 4 * (size_t)n;
+  KLAUNCH(sort_tiny_k, grid_for((size_t)n * SORT_TINY, 256), 256, st, db, lits, n, lsum, fmid, flong);
+  k_pack_flags(fmid, n, pos, mid, cnt + 1, tmp, st);
+  k_pack_flags(flong, n, pos, big, cnt, tmp, st);
+  // the collected lists' launches read their counts on the device
+  KLAUNCH(sort_mid_k, std::min(grid_for((size_t)n * SORT_SMALL, 256), 4096u), 256, st, db, lits, mid, cnt + 1, lsum);
   KLAUNCH(sort_block_k, std::min(grid_for(n, 1), 1024u), 256, st, db, lits, big, cnt, lsum);
   KLAUNCH(sort_large_k, std::min(grid_for(n, 64), 64u), 64, st, db, lits, big, cnt, scratch, lsum);
 }

@@ -88,9 +88,11 @@ bool otp_plan(size_t n2, size_t ncls, OtpPlan* p, int tile_log2 = 0, int xcd = 0
 void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, void* pairs, bool table, u32* hist,
                  void* tmp, Stream*);
 // sort the lists of the given literals (lits[0..n)) by the total order
-// classes: 3n words of scratch for the size classes; cnt: 3 device words
+// classes: 5n words of scratch for the size classes; cnt: 3 device words;
+// tmp: scan_tmp_bytes(n)
 // lsum[i]: sum of the clause sizes of list lits[i] (n words)
-void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum, Stream*);
+void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, u32* scratch, u32* lsum, void* tmp,
+                  Stream*);
 
 // ---- BCP
 void k_bcp_visit(const DB& db, const u32* front, u32 nfront, u32* next, u32* nnext, u32* conflict,
