@@ -948,26 +948,41 @@ void k_sort_lists(const DB& db, const u32* lits, u32 n, u32* classes, u32* cnt, 
 }
 
 // ---------------------------------------------------------------- BCP
+// Kernels that walk the occurrence lists of a few literals (units): one
+// row of workgroups per literal (blockIdx.y) strides over its list, so that
+// a handful of hot literals (lists of 10^4-10^5 entries on heavy-tailed
+// formulas) does not leave the device to one workgroup each.  The launcher
+// aims at about 4096 workgroups in all and at most 65535 literals per
+// launch (the grid's y extent).
+template <class L>
+void launch_per_literal(u32 n, L launch) {
+  for (u32 c0 = 0; c0 < n; c0 += 65535) {
+    const u32 m = std::min<u32>(n - c0, 65535);
+    launch(dim3(std::max<u32>(1, std::min<u32>(64, 4096 / m)), m), c0);
+  }
+}
+__device__ inline u32 list_lane() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline u32 list_stride() { return gridDim.x * blockDim.x; }
 __global__ void bcp_visit_k(DB db, const u32* front, u32* next, u32* nnext, u32* conflict) {
-  const u32 t = front[blockIdx.x];
+  const u32 t = front[blockIdx.y];
   const OLV ol = olist(db, FLIP(t));
-  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) bcp_visit(db, ol.r[i], next, nnext, conflict);
+  for (u32 i = list_lane(); i < ol.n; i += list_stride()) bcp_visit(db, ol.r[i], next, nnext, conflict);
 }
 void k_bcp_visit(const DB& db, const u32* front, u32 nfront, u32* next, u32* nnext, u32* conflict,
                  Stream* st) {
-  if (!nfront) return;
-  KLAUNCH(bcp_visit_k, nfront, 256, st, db, front, next, nnext, conflict);
+  launch_per_literal(nfront, [&](dim3 grid, u32 c0) {
+    KLAUNCH(bcp_visit_k, grid, 256, st, db, front + c0, next, nnext, conflict);
+  });
 }
 __global__ void bcp_mark_k(DB db, const u32* lits, u8* mark) {
-  const u32 t = lits[blockIdx.x];
+  const u32 t = lits[blockIdx.y];
   for (u32 q = 0; q < 2; q++) {
     const OLV ol = olist(db, t ^ q);
-    for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) mark[ol.r[i]] = 1;
+    for (u32 i = list_lane(); i < ol.n; i += list_stride()) mark[ol.r[i]] = 1;
   }
 }
 void k_bcp_mark(const DB& db, const u32* lits, u32 n, u8* mark, Stream* st) {
-  if (!n) return;
-  KLAUNCH(bcp_mark_k, n, 256, st, db, lits, mark);
+  launch_per_literal(n, [&](dim3 grid, u32 c0) { KLAUNCH(bcp_mark_k, grid, 256, st, db, lits + c0, mark); });
 }
 // mark: per handle
 __global__ void bcp_clean_k(DB db, u8* mark) {
@@ -1303,18 +1318,21 @@ void k_sub_restore(const DB& db, const u32* pv, const u32* idx, u32 n, const Sub
 // one workgroup per unit change (elected index, literal, pass): flag the
 // owners of the clauses in the literal's list that come after the deriving
 // variable and last ran no later than that pass
+// one row of workgroups per change (its unit's list: launch_per_literal)
 __global__ void sub_affected_k(DB db, SubSnap s, const u32* ch, const u32* lastrun, u32* aff) {
-  const u32 k_unit = ch[3 * (u64)blockIdx.x], u = ch[3 * (u64)blockIdx.x + 1], pass = ch[3 * (u64)blockIdx.x + 2];
+  const u64 c = 3 * (u64)blockIdx.y;
+  const u32 k_unit = ch[c], u = ch[c + 1], pass = ch[c + 2];
   const OLV ol = olist(db, u);
-  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) {
+  for (u32 i = list_lane(); i < ol.n; i += list_stride()) {
     const u32 o = sub_affected_owner(s, ol.r[i], k_unit, lastrun, pass);
     if (o != NOPOS) aff[o] = 1;
   }
 }
 void k_sub_affected(const DB& db, const SubSnap& s, const u32* changes, u32 nch, const u32* lastrun, u32* aff,
                     Stream* st) {
-  if (!nch) return;
-  KLAUNCH(sub_affected_k, nch, 256, st, db, s, changes, lastrun, aff);
+  launch_per_literal(nch, [&](dim3 grid, u32 c0) {
+    KLAUNCH(sub_affected_k, grid, 256, st, db, s, changes + 3 * (size_t)c0, lastrun, aff);
+  });
 }
 __global__ void set_at_k(u32* dst, const u32* idx, u32 n, u32 value) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1348,13 +1366,12 @@ void k_pack_flags(const u32* flags, u32 n, u32* pos, u32* out, u32* count, void*
   KLAUNCH(pack_flags_k, grid_for(n, 256), 256, st, flags, pos, n, out);
 }
 __global__ void oblivion_k(DB db, const u32* lits) {
-  const u32 t = lits[blockIdx.x];
+  const u32 t = lits[blockIdx.y];
   const OLV ol = olist(db, t);
-  for (u32 i = threadIdx.x; i < ol.n; i += blockDim.x) set_deleted(db, ol.r[i]);
+  for (u32 i = list_lane(); i < ol.n; i += list_stride()) set_deleted(db, ol.r[i]);
 }
 void k_oblivion(const DB& db, const u32* lits, u32 n, Stream* st) {
-  if (!n) return;
-  KLAUNCH(oblivion_k, n, 256, st, db, lits);
+  launch_per_literal(n, [&](dim3 grid, u32 c0) { KLAUNCH(oblivion_k, grid, 256, st, db, lits + c0); });
 }
 // Per-variable searches over a variable's own two occurrence lists (SUB,
 // the VE decisions that are not settled by the gate checks) re-read those
