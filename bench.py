@@ -556,9 +556,17 @@ def profile_step(s, cnf, dt, traffic_csv=None, workload=None):
     roof["dominant_kernel"] = dom[0] if dom else None
     roof["dominant_kernel_ms_per_step"] = round(dom[1][0], 3) if dom else None
     ach_round = b_alg / dt / 1e9
+    # every kernel of the profiled step (the work-measure kernels left out):
+    # what the timed step spends beyond it is host reads of counts,
+    # launch-to-launch latency and the rewind's device copies.  The profiled
+    # step records events around every launch, which stretches its kernels
+    # a little, so on a round with almost no time outside kernels the
+    # difference can come out slightly negative.
+    kernel_ms = sum(v[0] for v in kt.values())
     roof["round"] = {"achieved": round(ach_round, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_round / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(b_alg),
-                     "ms_per_step": round(dt * 1e3, 3), "bytes_per_stage": per_stage}
+                     "ms_per_step": round(dt * 1e3, 3), "kernel_ms_per_step": round(kernel_ms, 3),
+                     "outside_kernels_ms": round(dt * 1e3 - kernel_ms, 3), "bytes_per_stage": per_stage}
     mk = {}
     for k, v in sorted(modelled.items(), key=lambda kv: -kv[1][0]):
         runs_in = sum(1 for r in rounds if kernel_bytes(k, [r], V))

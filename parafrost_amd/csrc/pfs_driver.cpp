@@ -1029,10 +1029,12 @@ bool Simplifier::lcve() {
   u32* resume = P<u32>(vsz_);  // free until SUB: where each scan goes on
   // worklists of undecided candidates (the selection buffers are free now);
   // after the first pass over all candidates, passes are queued eight at a
-  // time with the list lengths kept on the device (sc[4], sc[5]; sc[6]
+  // time with the list lengths kept on the device in a ring of three
+  // counters (sc[12..14]: pass p reads ring[p % 3], appends to
+  // ring[(p + 1) % 3] and clears ring[(p + 2) % 3] for the next pass; sc[6]
   // counts the passes that had work)
   u32* wl[2] = {P<u32>(flag_), P<u32>(score_)};
-  u32* cnt[2] = {sc + 4, sc + 5};
+  u32* ring[3] = {sc + 12, sc + 13, sc + 14};
   // the election from scratch with 'lds' bytes reserved per step
   // workgroup and 'lanes' per candidate; sc[6] counts the worklist passes
   // that had work
@@ -1044,23 +1046,22 @@ bool Simplifier::lcve() {
     fill_u32(P<u32>(key_), NOPOS, ncand, st_);  // the sort keys are consumed: blockers
     fill_u32(resume, 0, ncand, st_);
     fill_u32(sc + 4, 0, 3, st_);
+    fill_u32(sc + 12, 0, 3, st_);
     if (!ncand) return;
     k_mis_step(db, P<u32>(cand_), nullptr, ncand, P<u32>(pos_), P<u32>(state_), P<u32>(key_), resume, wl[0],
-               cnt[0], lds, lanes, st_);
+               ring[0], lds, lanes, st_);
     // the first batch of worklist passes is sized for every candidate (its
     // grid is at the cap either way on large rounds); later batches for the
     // list length read after the previous batch
-    u32 cur = 0, bound = ncand, launched = 0;
+    u32 p = 0, bound = ncand;
     while (bound) {
-      for (int b = 0; b < 8; b++, launched++) {
-        const u32 nx = cur ^ 1;
-        fill_u32(cnt[nx], 0, 1, st_);
-        k_mis_step_dev(db, P<u32>(cand_), wl[cur], cnt[cur], bound, P<u32>(pos_), P<u32>(state_), P<u32>(key_),
-                       resume, wl[nx], cnt[nx], sc + 6, lds, lanes, st_);
-        cur = nx;
+      for (int b = 0; b < 8; b++, p++) {
+        k_mis_step_dev(db, P<u32>(cand_), wl[p & 1], ring[p % 3], bound, P<u32>(pos_), P<u32>(state_),
+                       P<u32>(key_), resume, wl[(p + 1) & 1], ring[(p + 1) % 3], sc + 6, ring[(p + 2) % 3], lds,
+                       lanes, st_);
       }
-      if (launched > ncand + 8) throw DeviceFail{"LCVE election did not converge"};
-      bound = read_u32(cnt[cur]);
+      if (p > ncand + 8) throw DeviceFail{"LCVE election did not converge"};
+      bound = read_u32(ring[p % 3]);
     }
   };
   if (tune_mis_ && ncand) {

@@ -1203,11 +1203,17 @@ __global__ void mis_step_k(DB db, const u32* cand, const u32* idx, u32 n, u32* p
 // A worklist pass whose length is read on the device (n_in): a bounded grid
 // strides over the list, so several passes are queued without a host read;
 // passes over an empty list return at once.  iters counts the non-empty passes.
+// zero: the counter the next pass appends to (the input counter of the
+// pass before this one, so nothing reads it any more), cleared here instead
+// of by a launch of its own before every pass
 template <u32 G>
 __global__ void mis_step_dev_k(DB db, const u32* cand, const u32* idx, const u32* n_in, u32* pos, u32* state,
-                               u32* blk, u32* resume, u32* next, u32* nnext, u32* iters) {
+                               u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32* zero) {
   const u32 n = *n_in;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(iters, 1u);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *zero = 0;
+    if (n) atomicAdd(iters, 1u);
+  }
   const u64 m = (u64)n * G;
   for (u64 base = (u64)blockIdx.x * blockDim.x; base < m; base += (u64)gridDim.x * blockDim.x) {  // wave-uniform
     const u64 t = base + threadIdx.x;
@@ -1234,12 +1240,12 @@ void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, 
 #undef MIS_FIRST
 }
 void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
-                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, u32 lanes,
-                    Stream* st) {
+                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32* zero, u32 lds,
+                    u32 lanes, Stream* st) {
   if (!n_bound) return;
 #define MIS_DEV(G)                                                                                           \
   KLAUNCH_LDS(mis_step_dev_k<G>, std::min(grid_for((size_t)n_bound * G, 64), 2048u), 64, lds, st, db, cand, idx, \
-              n_in, pos, state, blk, resume, next, nnext, iters)
+              n_in, pos, state, blk, resume, next, nnext, iters, zero)
   MIS_LANES(lanes, MIS_DEV)
 #undef MIS_DEV
 }

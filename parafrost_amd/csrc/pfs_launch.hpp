@@ -126,12 +126,13 @@ void k_lcve_reject(const u32* cand, u32 n, const u8* cflag, u32* pos, u32* state
 // first step); the still undecided candidates are appended to next (count
 // in *nnext)
 // the same over the list idx[0..*n_in) (n_in on the device, at most n_bound);
-// *iters counts the passes over a non-empty list
+// *iters counts the passes over a non-empty list; *zero is cleared (the
+// counter the next pass appends to)
 // lds: bytes of LDS reserved per one-wavefront workgroup (occupancy);
 // lanes: lanes per candidate (1, 2, 4 or 8)
 void k_mis_step_dev(const DB& db, const u32* cand, const u32* idx, const u32* n_in, u32 n_bound, u32* pos,
-                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32 lds, u32 lanes,
-                    Stream*);
+                    u32* state, u32* blk, u32* resume, u32* next, u32* nnext, u32* iters, u32* zero, u32 lds,
+                    u32 lanes, Stream*);
 void k_mis_step(const DB& db, const u32* cand, const u32* idx, u32 n, u32* pos, u32* state, u32* blk,
                 u32* resume, u32* next, u32* nnext, u32 lds, u32 lanes, Stream*);
 void k_mis_break(const u32* cand, u32 n, const u8* cflag, const u32* state, u32* brk, Stream*);
