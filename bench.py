@@ -135,7 +135,7 @@ def kernel_bytes(name, rounds, V):
       mis_step_k          4 O_c + D_c  (the first pass: every candidate's lists and clauses)
     per-elected-variable kernels (O_E list entries, D_E clause bytes of the
     elected variables' lists; elected variables share no clause):
-      sub_k, ve_gates_k, ve_rest_k, bce_k   4 O_E + D_E
+      sub_k, ve_decide_k, bce_k   4 O_E + D_E
       ve_apply_k                            4 O_E + D_E + 4 E (decisions) + R + S
     """
     name = norm_kernel(name)
@@ -171,8 +171,8 @@ def kernel_bytes(name, rounds, V):
         elif name == "mis_step_k":
             if "lcve" in ran and r["cand_occurs"]:
                 per.append(4 * r["cand_occurs"] + r["cand_bytes"])
-        elif name in ("sub_k", "ve_gates_k", "ve_rest_k", "bce_k"):
-            stage = {"sub_k": "sub", "ve_gates_k": "ve", "ve_rest_k": "ve", "bce_k": "bce"}[name]
+        elif name in ("sub_k", "ve_decide_k", "bce_k"):
+            stage = {"sub_k": "sub", "ve_decide_k": "ve", "bce_k": "bce"}[name]
             if oe and stage in ran:
                 per.append(4 * oe + de)
         elif name == "ve_apply_k":

@@ -116,6 +116,14 @@ PF_HD u32 csize(const LView& v, u32 c) { return hdr_size(v.w[c + 1]); }
 PF_HD u8 cflags(const LView& v, u32 c) { return hdr_flags(v.w[c + 1]); }
 PF_HD void set_cflags(const LView& v, u32 c, u8 f) { v.w[c + 1] = (v.w[c + 1] & CL_MAX_SIZE) | ((u32)f << 24); }
 PF_HD u32* clits(const LView& v, u32 c) { return v.w + c + 2; }
+// the clause's signature (not staged: the OR of MAPHASH over its literals,
+// as the pool's signature word holds it)
+PF_HD u32 csig(const LView& v, u32 c) {
+  u32 sig = 0;
+  const u32* l = v.w + c + 2;
+  for (u32 q = 0; q < hdr_size(v.w[c + 1]); q++) sig |= MAPHASH(l[q]);
+  return sig;
+}
 PF_HD CL clause(const LView& v, u32 c) { return CL{clits(v, c), csize(v, c)}; }
 PF_HD bool is_original(const LView& v, u32 c) { return st_original(cflags(v, c)); }
 PF_HD bool is_molten(const LView& v, u32 c) { return (cflags(v, c) & F_MOLTEN) != 0; }
@@ -1050,14 +1058,16 @@ PF_HD bool count_res(const D& db, const Options& o, u32 x, int clsbefore, OLV me
   }
   return false;
 }
-PF_HD void freeze_binaries(const DB& db, OLV ol) {  // simplify.h:204-211
+template <class D>
+PF_HD void freeze_binaries(const D& db, OLV ol) {  // simplify.h:204-211
   for (u32 i = 0; i < ol.n; i++) {
     const u32 c = ol.r[i];
     if (is_original(db, c) && csize(db, c) == 2) freeze(db, c);
   }
 }
 // equivalence.h:89-131
-PF_HD u32 find_sfanin(const DB& db, u32 gate_out, OLV ol) {
+template <class D>
+PF_HD u32 find_sfanin(const D& db, u32 gate_out, OLV ol) {
   u32 imp = 0;
   int nImps = 0;
   for (u32 i = 0; i < ol.n; i++) {
@@ -1072,7 +1082,8 @@ PF_HD u32 find_sfanin(const DB& db, u32 gate_out, OLV ol) {
   }
   return imp;
 }
-PF_HD u32 find_equ_gate(const DB& db, u32 p) {
+template <class D>
+PF_HD u32 find_equ_gate(const D& db, u32 p) {
   const u32 n = NEGL(p);
   const OLV poss = olist(db, p), negs = olist(db, n);
   u32 first = find_sfanin(db, p, poss);
@@ -1094,7 +1105,8 @@ PF_HD u32 find_equ_gate(const DB& db, u32 p) {
   return 0;
 }
 // and.h:25-86; 'out' is per-variable scratch
-PF_HD bool find_ao_gate(const DB& db, const Options& o, u32 dx, int nOrgCls, u32* out, int& nAddedCls,
+template <class D>
+PF_HD bool find_ao_gate(const D& db, const Options& o, u32 dx, int nOrgCls, u32* out, int& nAddedCls,
                         u32& nAddedLits) {
   const u32 x = ABS(dx);
   const OLV it = olist(db, dx);
@@ -1336,7 +1348,8 @@ PF_HD void ve_write(const D& db, u32 k, const VeDecision& d, OLV side, u32 type,
   d.nstack[k] = type ? stack_words(db, side) : 0;
 }
 // returns false when the variable is left to ve_decide_rest
-PF_HD bool ve_decide_gates(const DB& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
+template <class D>
+PF_HD bool ve_decide_gates(const D& db, const Options& o, u32 v, u32 k, const VeDecision& d) {
   const u32 p = V2L(v), n = NEGL(p);
   const OLV poss = olist(db, p), negs = olist(db, n);
   const u32 pOrgs = count_orgs(db, poss), nOrgs = count_orgs(db, negs);

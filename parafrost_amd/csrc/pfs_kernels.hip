@@ -1472,35 +1472,39 @@ constexpr u32 LV_BLOCKS = 256 * (160 * 1024 / VR_LDS);
 #ifndef VE_CLASS_BUMP
 #define VE_CLASS_BUMP 1
 #endif
-__global__ void ve_gates_k(DB db, Options o, const u32* pv, u32 n, VeDecision d, const u32* order, u32* work,
-                           const u32* lsum, u32 cap) {
+__global__ void ve_class_k(DB db, const u32* pv, u32 n, const u32* order, u32* work, const u32* lsum, u32 cap) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const u32 k = order ? order[t] : t;
-  u32 key = LV_CLASSES;  // settled by the gate checks
-  if (!ve_decide_gates(db, o, pv[k], k, d)) {
-    const u32 p = V2L(pv[k]);
-    const u64 b = lview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
-    u32 c = lv_class((u32)min(b, (u64)cap + 1), cap);
-    if (c < LV_CLASSES - 1) c = min(c + VE_CLASS_BUMP, LV_CLASSES - 2);
-    key = LV_CLASSES - 1 - c;
-  }
-  work[LV_WORK_HDR + t] = key;
+  const u32 p = V2L(pv[k]);
+  const u64 b = lview_bound(db.ot_size[p] + db.ot_size[p | 1], (u64)lsum[2 * k] + lsum[2 * k + 1]);
+  u32 c = lv_class((u32)min(b, (u64)cap + 1), cap);
+  if (c < LV_CLASSES - 1) c = min(c + VE_CLASS_BUMP, LV_CLASSES - 2);
+  work[LV_WORK_HDR + t] = LV_CLASSES - 1 - c;
   work[LV_WORK_HDR + n + t] = k;
 }
-__global__ void ve_rest_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 n, u32 cap) {
+// every elected variable on an LDS copy of its lists: the gate checks
+// (pure, equivalence, AND/OR) by the group's first lane, then, for the
+// variables they leave open, the ITE and XOR searches and the resolvent
+// count shared by the group (the checks read each clause of the lists
+// once, into LDS, instead of once per check from the pool)
+__global__ void ve_decide_k(DB db, Options o, const u32* pv, VeDecision d, const u32* work, u32 n, u32 cap) {
   extern __shared__ u32 lds[];
   lv_chunks(work, n, cap, lds, [&](u32 k, u32* w, u32 slice, u32 gl, u32 G) {
     const u32 v = pv[k];
     LView lv;
-    // every lane of the group stages the same words; the searches are
-    // shared, the writes are the first lane's
+    // every lane of the group stages the same words; the writes are the
+    // first lane's
     if (w && lview_stage(db, v, w, slice, lv)) {
       group_sync();
-      ve_decide_rest(lv, o, v, k, d, gl, G);
+      u32 open = 0;
+      if (gl == 0) open = ve_decide_gates(lv, o, v, k, d) ? 0u : 1u;
+      open = __shfl(open, (int)((threadIdx.x & 63) - gl), 64);
+      group_sync();  // the first lane's marks in the copy are seen by the group
+      if (open) ve_decide_rest(lv, o, v, k, d, gl, G);
       if (gl == 0) lview_commit(db, lv);
     } else if (gl == 0) {
-      ve_decide_rest(db, o, v, k, d);
+      if (!ve_decide_gates(db, o, v, k, d)) ve_decide_rest(db, o, v, k, d);
     }
   });
 }
@@ -1509,9 +1513,9 @@ void k_ve_decide(const DB& db, const Options& o, const u32* pv, u32 n, const VeD
   if (!n) return;
   const u32 cap = VR_LDS / 4;
   fill_u32(work, n, LV_WORK_HDR, st);
-  KLAUNCH_LDS(ve_gates_k, grid_for(n, VB), VB, PV_LDS, st, db, o, pv, n, d, order, work, lsum, cap);
+  KLAUNCH(ve_class_k, grid_for(n, 256), 256, st, db, pv, n, order, work, lsum, cap);
   lv_sort(work, n, tmp, st);
-  KLAUNCH_LDS(ve_rest_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, d, work, n, cap);
+  KLAUNCH_LDS(ve_decide_k, std::min<u32>(grid_for(n, VB), LV_BLOCKS), VB, VR_LDS, st, db, o, pv, d, work, n, cap);
 }
 // SUB, staged as the VE searches (the class of a variable from the words of
 // its SView, raised by SUB_CLASS_BUMP)
