@@ -185,67 +185,59 @@ def kernel_bytes(name, rounds, V):
     return sum(per) / len(per)
 
 
+def read_counter_csv(path, counter, copy_bytes=256 << 20, reps=3):
+    """One rocprofv3 --pmc CSV of this script run with --calibrate-copy:
+    ({kernel short name: [bytes per dispatch, ...]}, scale).  The counter
+    reads KiB per dispatch; the scale is copy_bytes over what it read for
+    the calibration copies, which are the first 'reps' copy dispatches in
+    dispatch order (--calibrate-copy runs before anything else copies; the
+    workload's own copies later can be larger and must not be taken)."""
+    import csv
+    import statistics
+    vals, copies = {}, []
+    with open(path) as f:
+        rows = [r for r in csv.DictReader(f) if r["Counter_Name"] == counter]
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    for r in rows:
+        name = r["Kernel_Name"]
+        v = float(r["Counter_Value"]) * 1024.0
+        if "copyBuffer" in name or ("copy" in name.lower() and "pfs::" not in name):
+            copies.append(v)
+            continue
+        if not name.startswith(("pfs::", "void pfs::")):
+            continue
+        short = name.split("pfs::", 1)[1].split("(", 1)[0]
+        vals.setdefault(short, []).append(v)
+    ref = statistics.median(copies[:reps]) if len(copies) >= reps else 0.0
+    return vals, (copy_bytes / ref if ref else None)
+
+
 def counter_traffic(fetch_csv, write_csv, kernel, copy_bytes=256 << 20):
     """HBM bytes per launch of 'kernel' from two rocprofv3 --pmc runs of this
     script with --calibrate-copy (FETCH_SIZE, then WRITE_SIZE; KiB per
-    dispatch).  Each counter is scaled by what it reads for the 256 MiB
-    device copy of --calibrate-copy (the three largest copy dispatches).
-    Returns (bytes per launch, {"fetch_scale": f, "write_scale": w})."""
-    import csv
+    dispatch), each counter scaled by what it reads for the 256 MiB
+    calibration copies.  Returns (bytes per launch, {"fetch_scale": f,
+    "write_scale": w})."""
     import statistics
-
-    def per_kernel(path, counter):
-        vals, copies = [], []
-        with open(path) as f:
-            for r in csv.DictReader(f):
-                if r["Counter_Name"] != counter:
-                    continue
-                name = r["Kernel_Name"]
-                v = float(r["Counter_Value"]) * 1024.0
-                if "copyBuffer" in name or "copy" in name.lower() and "pfs::" not in name:
-                    copies.append(v)
-                if "pfs::" in name and norm_kernel(name.split("pfs::", 1)[1].split("(", 1)[0]) == norm_kernel(kernel):
-                    vals.append(v)
-        copies.sort()
-        ref = statistics.median(copies[-3:]) if copies else 0.0
-        return (sum(vals) / len(vals) if vals else None), (copy_bytes / ref if ref else None)
-
-    fv, fs = per_kernel(fetch_csv, "FETCH_SIZE")
-    wv, ws = per_kernel(write_csv, "WRITE_SIZE")
-    if fv is None or wv is None or not fs or not ws:
+    fv, fs = read_counter_csv(fetch_csv, "FETCH_SIZE", copy_bytes)
+    wv, ws = read_counter_csv(write_csv, "WRITE_SIZE", copy_bytes)
+    f = [v for k, l in fv.items() if norm_kernel(k) == norm_kernel(kernel) for v in l]
+    w = [v for k, l in wv.items() if norm_kernel(k) == norm_kernel(kernel) for v in l]
+    if not f or not w or not fs or not ws:
         return None, None
-    return fv * fs + wv * ws, {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}
+    return statistics.mean(f) * fs + statistics.mean(w) * ws, {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}
 
 
 def write_counter_summary(fetch_csv, write_csv, workload, command, copy_bytes=256 << 20):
     """profiles/<workload>_round_counters.{json,md} from the two rocprofv3
     --pmc runs of this script with --calibrate-copy (FETCH_SIZE, then
     WRITE_SIZE; KiB per dispatch): per kernel, launches and bytes per launch,
-    each counter scaled by what it reads for the 256 MiB copies."""
-    import csv
+    each counter scaled by what it reads for the 256 MiB calibration copies."""
     import statistics
-
-    def per_kernel(path, counter):
-        vals, copies = {}, []
-        with open(path) as f:
-            for r in csv.DictReader(f):
-                if r["Counter_Name"] != counter:
-                    continue
-                name = r["Kernel_Name"]
-                v = float(r["Counter_Value"]) * 1024.0
-                if "copyBuffer" in name or ("copy" in name.lower() and "pfs::" not in name):
-                    copies.append(v)
-                    continue
-                if not name.startswith(("pfs::", "void pfs::")):
-                    continue
-                short = name.split("pfs::", 1)[1].split("(", 1)[0]
-                vals.setdefault(short, []).append(v)
-        copies.sort()
-        scale = copy_bytes / statistics.median(copies[-3:]) if copies else 1.0
-        return vals, scale
-
-    fv, fs = per_kernel(fetch_csv, "FETCH_SIZE")
-    wv, ws = per_kernel(write_csv, "WRITE_SIZE")
+    fv, fs = read_counter_csv(fetch_csv, "FETCH_SIZE", copy_bytes)
+    wv, ws = read_counter_csv(write_csv, "WRITE_SIZE", copy_bytes)
+    if not fs or not ws:
+        raise SystemExit("bench: no calibration copies in the counter CSVs (run with --calibrate-copy)")
     kernels = {}
     for k in sorted(set(fv) | set(wv)):
         f = statistics.mean(fv[k]) * fs if k in fv else 0.0
