@@ -637,55 +637,61 @@ __global__ void __launch_bounds__(OTP_TB) otp_part_k(DB db, u32 ct, int bits, u3
 // total past the last range).  Without TABLE only the counts (hist) are
 // written.
 constexpr int OTP_LD = 8;
+// threads per range workgroup: 1024 (two workgroups per CU) against 256
+// (five, bound by the 32 KiB of counters) took the heavy-tailed round's
+// range kernel from 2.04 to 1.50 ms and Tseitin's from 0.35 to 0.28 ms:
+// more loads in flight per range and fewer table slices open at once per
+// XCD; reserving 64 KiB of LDS on top changed nothing
+constexpr int OTP_RTB = 1024;
 template <bool TABLE>
-__global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 nblk, const u32* offs,
+__global__ void __launch_bounds__(OTP_RTB) otp_range_k(DB db, int bits, u32 n2, u32 nblk, const u32* offs,
                                                   const u32* total, const void* in, u32* hist) {
   const u32* plit = (const u32*)in;
   const uint2* pair = (const uint2*)in;
   extern __shared__ u32 lc[];
-  __shared__ u32 lw[NW];
+  __shared__ u32 lw[OTP_RTB / 64];
   const u32 r = blockIdx.x, nl = 1u << bits, base = r << bits;
   const u32 b0 = offs[(size_t)r * nblk];
   const u32 b1 = r + 1 < gridDim.x ? offs[(size_t)(r + 1) * nblk] : *total;
-  for (u32 j = threadIdx.x; j < nl; j += ST) lc[j] = 0;
+  for (u32 j = threadIdx.x; j < nl; j += OTP_RTB) lc[j] = 0;
   __syncthreads();
-  for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * ST) {
+  for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * OTP_RTB) {
     u32 l[OTP_LD];
 #pragma unroll
     for (int q = 0; q < OTP_LD; q++)
-      l[q] = p + q * ST < b1 ? (TABLE ? pair[p + q * ST].x : plit[p + q * ST]) : OTP_NONE;
+      l[q] = p + q * OTP_RTB < b1 ? (TABLE ? pair[p + q * OTP_RTB].x : plit[p + q * OTP_RTB]) : OTP_NONE;
 #pragma unroll
     for (int q = 0; q < OTP_LD; q++)
       if (l[q] != OTP_NONE) atomicAdd(&lc[l[q] - base], 1u);
   }
   __syncthreads();
   if (!TABLE) {
-    for (u32 j = threadIdx.x; j < nl; j += ST)
+    for (u32 j = threadIdx.x; j < nl; j += OTP_RTB)
       if (base + j < n2) hist[base + j] = lc[j];
     return;
   }
-  // exclusive scan: each thread owns nl / ST consecutive counters
-  const u32 per = nl / ST, j0 = threadIdx.x * per;
+  // exclusive scan: each thread owns nl / OTP_RTB consecutive counters
+  const u32 per = nl / OTP_RTB, j0 = threadIdx.x * per;
   u32 s = 0;
   for (u32 k = 0; k < per; k++) s += lc[j0 + k];
   u32 tot;
-  u32 pre = block_excl_scan(s, lw, &tot) + b0;
+  u32 pre = block_excl_scan_w<OTP_RTB / 64>(s, lw, &tot) + b0;
   for (u32 k = 0; k < per; k++) {
     const u32 v = lc[j0 + k];
     lc[j0 + k] = pre;
     pre += v;
   }
   __syncthreads();
-  for (u32 j = threadIdx.x; j < nl; j += ST)
+  for (u32 j = threadIdx.x; j < nl; j += OTP_RTB)
     if (base + j < n2) db.ot_off[base + j] = lc[j];
   if (r + 1 == gridDim.x && threadIdx.x == 0) db.ot_off[n2] = *total;
   // every offset is read out of LDS before any cursor moves
   __syncthreads();
-  for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * ST) {
+  for (u32 p = b0 + threadIdx.x; p < b1; p += OTP_LD * OTP_RTB) {
     u32 l[OTP_LD], c[OTP_LD];
 #pragma unroll
     for (int q = 0; q < OTP_LD; q++) {
-      const uint2 e = p + q * ST < b1 ? pair[p + q * ST] : make_uint2(OTP_NONE, 0u);
+      const uint2 e = p + q * OTP_RTB < b1 ? pair[p + q * OTP_RTB] : make_uint2(OTP_NONE, 0u);
       l[q] = e.x;
       c[q] = e.y;
     }
@@ -695,7 +701,7 @@ __global__ void __launch_bounds__(ST) otp_range_k(DB db, int bits, u32 n2, u32 n
   }
   __syncthreads();
   // the cursors end at the next list's offset: size = cursor - offset
-  for (u32 j = threadIdx.x; j < nl; j += ST)
+  for (u32 j = threadIdx.x; j < nl; j += OTP_RTB)
     if (base + j < n2) {
       const u32 sz = lc[j] - db.ot_off[base + j];
       db.ot_size[base + j] = sz;
@@ -735,12 +741,12 @@ void k_ot_ranges(const DB& db, const OtpPlan& p, u32 n2, u32* counts, void* pair
   scan_u32(counts, counts, p.ncounts, counts + p.ncounts, tmp, st);
   if (table) {
     KLAUNCH_LDS(otp_part_k<true>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, p.xcd, counts, pairs);
-    KLAUNCH_LDS(otp_range_k<true>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts, pairs,
+    KLAUNCH_LDS(otp_range_k<true>, p.nr, OTP_RTB, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts, pairs,
                 hist);
   } else {
     KLAUNCH_LDS(otp_part_k<false>, p.nblk, OTP_TB, lds_r, st, db, p.ct, p.bits, p.nr, p.nblk, p.xcd, counts,
                 pairs);
-    KLAUNCH_LDS(otp_range_k<false>, p.nr, ST, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts,
+    KLAUNCH_LDS(otp_range_k<false>, p.nr, OTP_RTB, lds_b, st, db, p.bits, n2, p.nblk, counts, counts + p.ncounts,
                 pairs, hist);
   }
 }
