@@ -1626,8 +1626,6 @@ void k_bce(const DB& db, const Options& o, const u32* pv, const u32* type, u32 n
   if (!n) return;
   KLAUNCH(bce_k, grid_for(n, VB), VB, st, db, o, pv, type, n, words, base, stack, count, order);
 }
-// heaviest-first order of the elected variables: key = 0xFFFF - min(|poss| +
-// |negs|, 0xFFFF), stable radix sort on 16 bits
 // the launch order's key: list occurrences on an 8-bit log-linear scale
 // (exact below 64, then 16 steps per power of two up to 2^18), descending,
 // so the sort takes two radix passes instead of four; ties keep the elected
