@@ -442,59 +442,101 @@ void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream* 
 }
 
 // ---------------------------------------------------------------- stable radix sort
-constexpr int RB = 4, RBUCK = 1 << RB;
+// LSD passes of up to 8 bits.  A pass counts each tile's digits (rs_count_k),
+// scans the digit-major counts, then every tile sorts itself by the digit
+// in LDS with stable one-bit splits and writes each digit's run to its
+// place (rs_scatter_k), so the stores go out in runs rather than one
+// scattered word per item.
+constexpr int RS_T = 256, RS_IPT = 8, RS_TILE = RS_T * RS_IPT;  // threads, items per thread, items per tile
+constexpr int RB = 8, RBUCK = 1 << RB;
+static_assert(RS_T == RBUCK, "one digit bin per thread");
 
-__global__ void rs_count_k(const u32* keys, size_t n, int shift, u32* counts, u32 nb) {
+__global__ void __launch_bounds__(RS_T) rs_count_k(const u32* keys, size_t n, int shift, u32 mask, u32* counts,
+                                                  u32 nb) {
   __shared__ u32 h[RBUCK];
-  if (threadIdx.x < RBUCK) h[threadIdx.x] = 0;
+  h[threadIdx.x] = 0;  // RS_T == RBUCK
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
+  const size_t base = (size_t)blockIdx.x * RS_TILE;
 #pragma unroll
-  for (int i = 0; i < SIPT; i++)
-    if (base + i < n) atomicAdd(&h[(keys[base + i] >> shift) & (RBUCK - 1)], 1u);
+  for (int i = 0; i < RS_IPT; i++) {
+    const size_t j = base + (size_t)i * RS_T + threadIdx.x;
+    if (j < n) atomicAdd(&h[(keys[j] >> shift) & mask], 1u);
+  }
   __syncthreads();
-  if (threadIdx.x < RBUCK) counts[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+  counts[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
-// each thread owns SIPT consecutive items, so ranks follow input order
-__global__ void rs_scatter_k(const u32* kin, const u32* vin, u32* kout, u32* vout, size_t n, int shift,
-                             const u32* offs, u32 nb) {
-  __shared__ u32 lw[NW];
-  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
-  u32 k[SIPT], v[SIPT];
-  u32 cnt[RBUCK];
+__global__ void __launch_bounds__(RS_T) rs_scatter_k(const u32* kin, const u32* vin, u32* kout, u32* vout,
+                                                    size_t n, int shift, int width, const u32* offs, u32 nb) {
+  __shared__ u32 sk[2][RS_TILE], sv[2][RS_TILE];
+  __shared__ u32 start[RBUCK];
+  __shared__ u32 lw[RS_T / 64];
+  const u32 mask = (1u << width) - 1u;
+  const size_t base = (size_t)blockIdx.x * RS_TILE;
+  const u32 cnt = (u32)min((size_t)RS_TILE, n - base);
+  // the tile in input order; slots past the input take the largest digit
+  // and stay behind every item (they come last and the splits are stable)
 #pragma unroll
-  for (int d = 0; d < RBUCK; d++) cnt[d] = 0;
+  for (int i = 0; i < RS_IPT; i++) {
+    const u32 j = i * RS_T + threadIdx.x;
+    sk[0][j] = j < cnt ? kin[base + j] : (mask << shift);
+    sv[0][j] = j < cnt ? vin[base + j] : 0u;
+  }
+  start[threadIdx.x] = 0;
+  __syncthreads();
+  // one stable split per digit bit: thread t moves items t*IPT .. t*IPT+IPT-1
+  int b = 0;
+  for (int bit = 0; bit < width; bit++, b ^= 1) {
+    const u32 j0 = threadIdx.x * RS_IPT;
+    u32 k[RS_IPT], v[RS_IPT], z = 0;
 #pragma unroll
-  for (int i = 0; i < SIPT; i++) {
-    if (base + i < n) {
-      k[i] = kin[base + i];
-      v[i] = vin[base + i];
-      cnt[(k[i] >> shift) & (RBUCK - 1)]++;
+    for (int i = 0; i < RS_IPT; i++) {
+      k[i] = sk[b][j0 + i];
+      v[i] = sv[b][j0 + i];
+      z += ((k[i] >> (shift + bit)) & 1u) ^ 1u;
     }
-  }
-  u32 pre[RBUCK];
+    u32 zt;
+    u32 zp = block_excl_scan_w<RS_T / 64>(z, lw, &zt);
+    u32 op = j0 - zp;  // ones before this thread's items
 #pragma unroll
-  for (int d = 0; d < RBUCK; d++) {
-    u32 tot;
-    pre[d] = block_excl_scan(cnt[d], lw, &tot) + offs[(size_t)d * nb + blockIdx.x];
+    for (int i = 0; i < RS_IPT; i++) {
+      const u32 dst = ((k[i] >> (shift + bit)) & 1u) ? zt + op++ : zp++;
+      sk[b ^ 1][dst] = k[i];
+      sv[b ^ 1][dst] = v[i];
+    }
+    __syncthreads();
   }
+  // where each digit's run starts in the sorted tile
 #pragma unroll
-  for (int i = 0; i < SIPT; i++) {
-    if (base + i < n) {
-      const u32 d = (k[i] >> shift) & (RBUCK - 1);
-      const u32 dst = pre[d]++;
-      kout[dst] = k[i];
-      vout[dst] = v[i];
+  for (int i = 0; i < RS_IPT; i++) {
+    const u32 j = i * RS_T + threadIdx.x;
+    if (j < cnt) atomicAdd(&start[(sk[b][j] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  u32 tot;
+  const u32 c = start[threadIdx.x];
+  const u32 pre = block_excl_scan_w<RS_T / 64>(c, lw, &tot);
+  start[threadIdx.x] = pre;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RS_IPT; i++) {
+    const u32 j = i * RS_T + threadIdx.x;
+    if (j < cnt) {
+      const u32 key = sk[b][j], d = (key >> shift) & mask;
+      const u32 dst = offs[(size_t)d * nb + blockIdx.x] + (j - start[d]);
+      kout[dst] = key;
+      vout[dst] = sv[b][j];
     }
   }
 }
 size_t radix_tmp_bytes(size_t n) {
-  const size_t nb = grid_for(n ? n : 1, STILE);
+  const size_t nb = grid_for(n ? n : 1, RS_TILE);
   return 2 * n * sizeof(u32) + RBUCK * nb * sizeof(u32) + scan_tmp_bytes(RBUCK * nb) + 256;
 }
+// sorts by the low 'bits' bits of the keys (higher bits are ignored)
 void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Stream* st) {
   if (n < 2 || bits <= 0) return;
-  const u32 nb = grid_for(n, STILE);
+  bits = std::min(bits, 32);
+  const u32 nb = grid_for(n, RS_TILE);
   u32* k2 = (u32*)tmp;
   u32* v2 = k2 + n;
   u32* counts = v2 + n;
@@ -502,9 +544,11 @@ void radix_sort_pairs(u32* keys, u32* vals, size_t n, int bits, void* tmp, Strea
   u32 *ka = keys, *va = vals, *kb = k2, *vb = v2;
   int passes = 0;
   for (int shift = 0; shift < bits; shift += RB) {
-    KLAUNCH(rs_count_k, nb, ST, st, ka, n, shift, counts, nb);
+    const int width = std::min(RB, bits - shift);
+    const u32 mask = (1u << width) - 1u;
+    KLAUNCH(rs_count_k, nb, RS_T, st, ka, n, shift, mask, counts, nb);
     scan_u32(counts, counts, (size_t)RBUCK * nb, nullptr, stmp, st);
-    KLAUNCH(rs_scatter_k, nb, ST, st, ka, va, kb, vb, n, shift, counts, nb);
+    KLAUNCH(rs_scatter_k, nb, RS_T, st, ka, va, kb, vb, n, shift, width, counts, nb);
     u32* t = ka; ka = kb; kb = t;
     t = va; va = vb; vb = t;
     passes++;
