@@ -343,13 +343,45 @@ __device__ inline unsigned long long wave_sum64(unsigned long long x) {
   for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
   return x;
 }
-__global__ void scan_reduce_k(const u32* in, size_t n, u32* part) {
-  __shared__ u32 lw[NW];
-  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
-  u32 s = 0;
+// Long scans: each workgroup covers SCAN_SUB tiles of STILE items (one
+// partial sum per workgroup, so the single-workgroup pass over the partials
+// stays short); a thread's SIPT consecutive items move as two 16-byte
+// loads and stores when the arrays are 16-byte aligned (VEC).
+constexpr int SCAN_SUB = 4;
+constexpr size_t SBLK = (size_t)STILE * SCAN_SUB;
+template <bool VEC>
+__device__ inline void load_items(const u32* in, size_t b, size_t n, u32 (&v)[SIPT]) {
+  if (VEC && b + SIPT <= n) {
+    const uint4 x = *(const uint4*)(in + b), y = *(const uint4*)(in + b + 4);
+    v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w, v[4] = y.x, v[5] = y.y, v[6] = y.z, v[7] = y.w;
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < SIPT; i++) v[i] = (b + i < n) ? in[b + i] : 0u;
+}
+template <bool VEC>
+__device__ inline void store_items(u32* out, size_t b, size_t n, const u32 (&v)[SIPT]) {
+  if (VEC && b + SIPT <= n) {
+    *(uint4*)(out + b) = make_uint4(v[0], v[1], v[2], v[3]);
+    *(uint4*)(out + b + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < SIPT; i++)
-    if (base + i < n) s += in[base + i];
+    if (b + i < n) out[b + i] = v[i];
+}
+static_assert(SIPT == 8, "two 16-byte moves per thread");
+template <bool VEC>
+__global__ void scan_reduce_k(const u32* in, size_t n, u32* part) {
+  __shared__ u32 lw[NW];
+  u32 s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_SUB; q++) {
+    u32 v[SIPT];
+    load_items<VEC>(in, (size_t)blockIdx.x * SBLK + (size_t)q * STILE + (size_t)threadIdx.x * SIPT, n, v);
+#pragma unroll
+    for (int i = 0; i < SIPT; i++) s += v[i];
+  }
   u32 tot;
   block_excl_scan(s, lw, &tot);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
@@ -378,22 +410,30 @@ __global__ void scan_top_k(u32* part, size_t nb, u32* total) {
   }
   if (threadIdx.x == 0 && total) *total = carry;
 }
+// each thread reads its items before the tile's barriers and writes them
+// after, so in == out is fine
+template <bool VEC>
 __global__ void scan_down_k(const u32* in, u32* out, size_t n, const u32* part) {
   __shared__ u32 lw[NW];
-  const size_t base = (size_t)blockIdx.x * STILE + (size_t)threadIdx.x * SIPT;
-  u32 v[SIPT];
-  u32 s = 0;
+  u32 carry = part[blockIdx.x];
+#pragma unroll 1
+  for (int q = 0; q < SCAN_SUB; q++) {
+    const size_t base = (size_t)blockIdx.x * SBLK + (size_t)q * STILE + (size_t)threadIdx.x * SIPT;
+    u32 v[SIPT];
+    load_items<VEC>(in, base, n, v);
+    u32 s = 0;
 #pragma unroll
-  for (int i = 0; i < SIPT; i++) {
-    v[i] = (base + i < n) ? in[base + i] : 0u;
-    s += v[i];
-  }
-  u32 tot;
-  u32 pre = block_excl_scan(s, lw, &tot) + part[blockIdx.x];
+    for (int i = 0; i < SIPT; i++) s += v[i];
+    u32 tot;
+    u32 pre = block_excl_scan(s, lw, &tot) + carry;
 #pragma unroll
-  for (int i = 0; i < SIPT; i++) {
-    if (base + i < n) out[base + i] = pre;
-    pre += v[i];
+    for (int i = 0; i < SIPT; i++) {
+      const u32 x = v[i];
+      v[i] = pre;
+      pre += x;
+    }
+    store_items<VEC>(out, base, n, v);
+    carry += tot;
   }
 }
 // short scans in one launch: one workgroup of 1024 threads walks the input
@@ -424,7 +464,7 @@ __global__ void __launch_bounds__(S1_T) scan_one_k(const u32* in, u32* out, size
   }
   if (threadIdx.x == 0 && total) *total = carry;
 }
-size_t scan_tmp_bytes(size_t n) { return (grid_for(n, STILE) + 64) * sizeof(u32); }
+size_t scan_tmp_bytes(size_t n) { return (grid_for(n, SBLK) + 64) * sizeof(u32); }
 void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream* st) {
   if (n == 0) {
     if (total) fill_u32(total, 0, 1, st);
@@ -434,11 +474,17 @@ void scan_u32(const u32* in, u32* out, size_t n, u32* total, void* tmp, Stream* 
     KLAUNCH(scan_one_k, 1, S1_T, st, in, out, n, total);
     return;
   }
-  const unsigned nb = grid_for(n, STILE);
+  const unsigned nb = grid_for(n, SBLK);
   u32* part = (u32*)tmp;
-  KLAUNCH(scan_reduce_k, nb, ST, st, in, n, part);
-  KLAUNCH(scan_top_k, 1, ST, st, part, nb, total);
-  KLAUNCH(scan_down_k, nb, ST, st, in, out, n, part);
+  if ((((uintptr_t)in | (uintptr_t)out) & 15) == 0) {
+    KLAUNCH(scan_reduce_k<true>, nb, ST, st, in, n, part);
+    KLAUNCH(scan_top_k, 1, ST, st, part, nb, total);
+    KLAUNCH(scan_down_k<true>, nb, ST, st, in, out, n, part);
+  } else {
+    KLAUNCH(scan_reduce_k<false>, nb, ST, st, in, n, part);
+    KLAUNCH(scan_top_k, 1, ST, st, part, nb, total);
+    KLAUNCH(scan_down_k<false>, nb, ST, st, in, out, n, part);
+  }
 }
 
 // ---------------------------------------------------------------- stable radix sort
