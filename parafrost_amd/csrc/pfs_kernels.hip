@@ -1537,15 +1537,26 @@ __device__ inline void lv_chunks(const u32* work, u32 n, u32 cap, u32* lds, F f)
     chunks += (start[key + 1] - start[key] + L - 1) / L;
   }
   for (u32 J = blockIdx.x; J < chunks; J += gridDim.x) {
-    u32 key = 0, j = J;
-    for (; key < LV_CLASSES - 1; key++) {
-      const u32 L = lv_class_lanes(LV_CLASSES - 1 - key), m = (start[key + 1] - start[key] + L - 1) / L;
-      if (j < m) break;
-      j -= m;
+    // the key of chunk J and its range, found by a loop over constant
+    // indices so that start[] stays in registers (a loop that stops at the
+    // key indexes it at run time, which puts the array in scratch memory)
+    u32 key = LV_CLASSES - 1, j = J, s0 = start[LV_CLASSES - 1], s1 = start[LV_CLASSES];
+    bool found = false;
+#pragma unroll
+    for (u32 q = 0; q < LV_CLASSES - 1; q++) {
+      const u32 L = lv_class_lanes(LV_CLASSES - 1 - q), m = (start[q + 1] - start[q] + L - 1) / L;
+      if (!found) {
+        if (j < m) {
+          key = q, s0 = start[q], s1 = start[q + 1];
+          found = true;
+        } else {
+          j -= m;
+        }
+      }
     }
     const u32 c = LV_CLASSES - 1 - key, L = lv_class_lanes(c), G = 64 / L;
-    const u32 slot = lane / G, r = start[key] + j * L + slot;
-    if (r < start[key + 1])
+    const u32 slot = lane / G, r = s0 + j * L + slot;
+    if (r < s1)
       f(placed[r], c < LV_CLASSES - 1 ? lds + slot * (cap / L) : nullptr, cap / L, lane % G, G);
     __syncthreads();  // the next chunk reuses the LDS
   }
